@@ -1,0 +1,128 @@
+/*
+ * pattern.h -- C ABI of libpattern, the MI355X (gfx950) engines behind the
+ * IPDPS'15 pattern framework's API.
+ *
+ * Every entry point returns an int status (PAT_OK or a negative PAT_ERR_*
+ * code); pat_last_error() returns the calling thread's last message.  Sizes
+ * are int64_t, handles are opaque, buffers are plain pointers.  Entry points
+ * named *_dev take device pointers and an optional hipStream_t passed as
+ * void* (NULL = the null stream); handle-level entry points take host
+ * buffers, own device copies, and never modify caller memory.
+ *
+ * Each declaration names the reference interface it replaces
+ * (file:line in lcchen008/IPDPS15-Pattern).
+ */
+#ifndef PATTERN_H_
+#define PATTERN_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAT_OK 0
+#define PAT_ERR_INVALID -1     /* bad argument or size */
+#define PAT_ERR_HIP -2         /* a HIP runtime call failed */
+#define PAT_ERR_NOMEM -3       /* device allocation failed */
+#define PAT_ERR_BUDGET -4      /* job does not fit the HBM budget */
+#define PAT_ERR_UNSUPPORTED -5 /* functor index / variant not available */
+#define PAT_ERR_STATE -6       /* call out of order (e.g. run before upload) */
+
+/* Last error message of the calling thread ("" when none). */
+const char *pat_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Runtime environment
+ * ---------------------------------------------------------------------- */
+
+/* Replaces RuntimeInit (kmeans/include/runtime.cpp:4-8; MPI_Init there):
+ * selects the HIP device and creates the library's streams. */
+int pat_init(int device);
+/* Replaces RuntimeFinalize (kmeans/include/runtime.cpp:10-13). */
+int pat_finalize(void);
+/* Library version, build target and device name into buf. */
+int pat_info(char *buf, int64_t buf_len);
+/* Blocks until all work queued by the library on the current device ends. */
+int pat_device_sync(void);
+
+/* ------------------------------------------------------------------------
+ * Seeded workload generators (host code, deterministic for any thread
+ * count; the app drivers' data set-up, e.g. kmeans/km.cu:28-65,
+ * kmeans/moldyn.cu:637-741, miniMD/setup.cpp:61-147).
+ * ---------------------------------------------------------------------- */
+
+/* Halo-inclusive (nx+2)(ny+2)(nz+2) FP32 grid, x fastest; interior uniform
+ * in [0,1) from seed, halo 0. */
+int pat_gen_heat3d(int64_t nx, int64_t ny, int64_t nz, uint64_t seed, float *grid);
+/* FCC lattice of ncx*ncy*ncz unit cells at reduced density rho, atoms
+ * numbered as miniMD create_atoms visits lattice sites (miniMD/setup.cpp:
+ * 92-113), each coordinate jittered uniformly in [-jitter, jitter) and
+ * wrapped into [0, L).  pos: 4*ncx*ncy*ncz x 3.  box_out: L per axis (3). */
+int pat_gen_fcc(int64_t ncx, int64_t ncy, int64_t ncz, double rho, double jitter, uint64_t seed, float *pos,
+                double *box_out);
+/* Maxwell velocities at temperature t (unit mass, k_B = 1): Gaussian
+ * components, zero centre-of-mass motion, rescaled so that
+ * sum v^2 / (3(N-1)) = t (miniMD create_velocity, miniMD/setup.cpp:150-190). */
+int pat_gen_velocities(int64_t n, double t, uint64_t seed, float *vel);
+/* Half list of all pairs i<j whose minimum-image distance is below cut
+ * (cell-list build; the moldyn driver's RebuildNeighbor,
+ * kmeans/moldyn.cu:552-594, is the O(N^2) form).  Pairs come out in (i, j)
+ * order, or shuffled with shuffle_seed when shuffle != 0.  Call with
+ * edges == NULL to get the count in *num_edges; otherwise *num_edges is the
+ * capacity on input and the count on output. */
+int pat_gen_pairs(int64_t n, const float *pos, double box, double cut, int shuffle, uint64_t shuffle_seed,
+                  int32_t *edges, int64_t *num_edges);
+/* Gaussian mixture: ncomp centres uniform in [lo, hi]^d from seed, each
+ * point picks a component uniformly and adds sigma * N(0, 1) per axis.
+ * x: n x d row-major. */
+int pat_gen_gmm(int64_t n, int d, int ncomp, double sigma, double lo, double hi, uint64_t seed, float *x);
+
+/* ------------------------------------------------------------------------
+ * Stencil engine (replaces compute_cuda_internal, kmeans/lib/compute_cuda.cu:
+ * 30-100 / heat_3D/lib/compute_cuda.cu:30-105, driven by StencilRuntime,
+ * kmeans/lib/stencil_runtime.h:13-93)
+ * ---------------------------------------------------------------------- */
+
+/* Stencil functor indices (stencil_function_table, heat_3D/stencil.h:119). */
+#define PAT_STENCIL_HEAT 0 /* u' = u + alpha*(sum6 - 6u), parameter = float alpha */
+
+/* Kernel variants. */
+#define PAT_STENCIL_AUTO 0    /* autotuned choice */
+#define PAT_STENCIL_LDS25D 1  /* LDS-staged 2.5D z-marching tiles */
+#define PAT_STENCIL_GENERIC 2 /* one thread per cell calling the functor */
+
+/* nsteps ping-pong heat sweeps over device grids a, b (halo-inclusive,
+ * x fastest, both halos already 0).  *result_in_b = 1 when the final field
+ * is in b.  bx, by, zc: tile (x threads, y rows, z-chunk), 0 = tuned. */
+int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, int64_t nz, int nsteps, float alpha,
+                       int variant, int bx, int by, int zc, int *result_in_b, void *stream);
+
+typedef struct pat_stencil pat_stencil;
+/* StencilCreator<T>(...) + CreateRuntime + StencilInit
+ * (heat_3D/lib/stencil_creator.h:22-30, kmeans/lib/stencil_runtime.cu:34-67):
+ * a device-resident double-buffered grid of the given interior size
+ * (x, y, z), halo width 'width', element size unit_size (4 = FP32). */
+int pat_stencil_create(int ndims, const int64_t *interior, int width, int unit_size, const void *param,
+                       int64_t param_size, pat_stencil **out);
+/* set_stencil_idx (kmeans/lib/stencil_runtime.h:36). */
+int pat_stencil_set_idx(pat_stencil *s, int idx);
+/* Selects the kernel variant and tile (0 = tuned). */
+int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc);
+/* Copies a halo-inclusive host grid in (the app writes grid()->data(),
+ * heat_3D/main.cpp:72-75). */
+int pat_stencil_upload(pat_stencil *s, const void *host);
+/* StencilBegin for nsteps iterations (kmeans/lib/stencil_runtime.cu:576-614). */
+int pat_stencil_run(pat_stencil *s, int nsteps);
+/* Copies the current halo-inclusive grid out. */
+int pat_stencil_download(pat_stencil *s, void *host);
+/* Device pointer of the current grid. */
+int pat_stencil_device_ptr(pat_stencil *s, void **dptr);
+/* StencilFinalize + delete. */
+int pat_stencil_destroy(pat_stencil *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATTERN_H_ */

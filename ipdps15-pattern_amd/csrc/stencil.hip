@@ -1,0 +1,283 @@
+// Stencil engine for gfx950: the 7-point heat sweep of the heat3D workload.
+//
+// Replaces the reference's compute_cuda_internal (kmeans/lib/compute_cuda.cu:
+// 30-100; parameterised in heat_3D/lib/compute_cuda.cu:30-105), where 14
+// persistent blocks stride over 32^3 tiles, decode each cell with three
+// div/mods and call the functor through a __constant__ function-pointer
+// table.  Here the heat functor (heat_3D/stencil.h:94-117 shape; update
+// u' = u + alpha*(s - 6u), s = ((((x- + x+) + y-) + y+) + z-) + z+) is bound
+// at compile time and swept by LDS-staged 2.5D tiles:
+//   * a workgroup owns a BX x (BY*RY) tile of the x-y plane and marches a
+//     chunk of zc planes; each thread keeps its column's z-1, z, z+1 values
+//     in registers (RY rows per thread) so every input cell is read from HBM
+//     once per sweep;
+//   * the current plane (plus a one-cell x/y halo) is staged in LDS so the
+//     four in-plane neighbours come from LDS;
+//   * rows of 64 consecutive x cells per wavefront give fully coalesced loads
+//     and stores;
+//   * tiles are ordered so that consecutive tile indices land on the same XCD
+//     (the dispatcher deals workgroups round-robin over the 8 XCDs), keeping
+//     neighbouring tiles' shared halo rows in one XCD's L2.
+// Grid layout is the reference's (GET_FLOAT3, heat_3D/lib/macro.h:70):
+// halo-inclusive (nx+2)(ny+2)(nz+2), x fastest.
+// Compiled with -ffp-contract=off: each cell's arithmetic rounds exactly as
+// written, so results are bit-identical to a plain C sweep of the same rule.
+#include "pat_common.hpp"
+
+namespace {
+
+constexpr int kXcds = 8;
+
+// Maps a hardware workgroup id to a tile id so that the tiles one XCD gets
+// are contiguous in tile order.
+__device__ __forceinline__ int xcd_swizzle(int bid, int nblocks)
+{
+    const int per = nblocks / kXcds;  // full rounds of kXcds
+    if (bid >= per * kXcds) return bid;  // ragged tail keeps its id
+    return (bid % kXcds) * per + bid / kXcds;
+}
+
+template <int BX, int BY, int RY>
+__global__ __launch_bounds__(BX *BY) void heat_lds25d(const float *__restrict__ in, float *__restrict__ out, int nx,
+                                                      int ny, int nz, int zc, float alpha, int tiles_x, int tiles_y,
+                                                      int tiles_z)
+{
+    constexpr int TH = BY * RY;
+    __shared__ float plane[TH + 2][BX + 2];
+
+    const int nblocks = tiles_x * tiles_y * tiles_z;
+    const int t = xcd_swizzle(blockIdx.x, nblocks);
+    const int bx = t % tiles_x;
+    const int by = (t / tiles_x) % tiles_y;
+    const int bz = t / (tiles_x * tiles_y);
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int sx = nx + 2;
+    const int64_t sxy = (int64_t)sx * (ny + 2);
+
+    const int x = bx * BX + tx + 1;
+    const bool xin = x <= nx;
+    const int xl = xin ? x : nx + 1;  // out-of-range lanes read the halo cell
+
+    const int z0 = bz * zc + 1;
+    const int z1 = min(z0 + zc, nz + 1);
+
+    int yrow[RY];
+    bool act[RY];
+    int64_t id[RY];
+    float below[RY], cur[RY], above[RY];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const int y = by * TH + ty * RY + r + 1;
+        act[r] = xin && y <= ny;
+        yrow[r] = y <= ny ? y : ny + 1;
+        id[r] = xl + (int64_t)sx * (yrow[r] + (int64_t)(ny + 2) * z0);
+        below[r] = in[id[r] - sxy];
+        cur[r] = in[id[r]];
+        above[r] = in[id[r] + sxy];
+    }
+
+    for (int z = z0; z < z1; z++) {
+#pragma unroll
+        for (int r = 0; r < RY; r++) plane[ty * RY + r + 1][tx + 1] = cur[r];
+        // x halo columns of this tile's rows
+        if (tx == 0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) plane[ty * RY + r + 1][0] = in[id[r] - 1];
+        }
+        if (tx == BX - 1) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) plane[ty * RY + r + 1][BX + 1] = in[id[r] + 1];
+        }
+        // y halo rows
+        if (ty == 0) plane[0][tx + 1] = in[id[0] - sx];
+        if (ty == BY - 1) plane[TH + 1][tx + 1] = in[id[RY - 1] + sx];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            if (act[r]) {
+                const int ly = ty * RY + r + 1;
+                float s = plane[ly][tx] + plane[ly][tx + 2];
+                s = s + plane[ly - 1][tx + 1];
+                s = s + plane[ly + 1][tx + 1];
+                s = s + below[r];
+                s = s + above[r];
+                const float c = cur[r];
+                out[id[r]] = c + alpha * (s - 6.0f * c);
+            }
+        }
+        __syncthreads();
+        const bool more = z + 1 < z1;
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            below[r] = cur[r];
+            cur[r] = above[r];
+            id[r] += sxy;
+            if (more) above[r] = in[id[r] + sxy];
+        }
+    }
+}
+
+// One thread per interior cell, all seven loads from global memory (through
+// the caches): the shape of the reference's per-cell kernel, kept as the
+// plain baseline and as an independent check of the tiled kernel.
+__global__ __launch_bounds__(256) void heat_naive(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                                  int nz, float alpha)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int x = (int)(g % nx) + 1;
+    const int y = (int)((g / nx) % ny) + 1;
+    const int z = (int)(g / ((int64_t)nx * ny)) + 1;
+    const int sx = nx + 2;
+    const int64_t sxy = (int64_t)sx * (ny + 2);
+    const int64_t id = x + (int64_t)sx * (y + (int64_t)(ny + 2) * z);
+    const float c = in[id];
+    float s = in[id - 1] + in[id + 1];
+    s = s + in[id - sx];
+    s = s + in[id + sx];
+    s = s + in[id - sxy];
+    s = s + in[id + sxy];
+    out[id] = c + alpha * (s - 6.0f * c);
+}
+
+struct TileShape {
+    int bx, by, ry;
+};
+
+// Instantiated tile shapes (BX threads in x, BY thread rows, RY rows per thread).
+constexpr TileShape kShapes[] = {{64, 4, 1}, {64, 4, 2}, {64, 4, 4}, {64, 8, 1}, {64, 8, 2}, {64, 2, 4}, {64, 16, 1},
+                                 {32, 8, 2}, {128, 2, 2}, {128, 4, 1}};
+
+using HeatKernel = void (*)(const float *, float *, int, int, int, int, float, int, int, int);
+
+HeatKernel heat_kernel_for(int bx, int by, int ry)
+{
+#define PAT_SHAPE(X, Y, R)                                                                                             \
+    if (bx == X && by == Y && ry == R) return heat_lds25d<X, Y, R>;
+    PAT_SHAPE(64, 4, 1)
+    PAT_SHAPE(64, 4, 2)
+    PAT_SHAPE(64, 4, 4)
+    PAT_SHAPE(64, 8, 1)
+    PAT_SHAPE(64, 8, 2)
+    PAT_SHAPE(64, 2, 4)
+    PAT_SHAPE(64, 16, 1)
+    PAT_SHAPE(32, 8, 2)
+    PAT_SHAPE(128, 2, 2)
+    PAT_SHAPE(128, 4, 1)
+#undef PAT_SHAPE
+    return nullptr;
+}
+
+// Default tile for a grid: 64-wide rows, 4x2 thread rows, and a z-chunk that
+// yields enough workgroups (>= ~4 per CU on 256 CUs) without making chunks so
+// short that the two extra halo planes per chunk dominate.
+void default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, int *ry, int *zc)
+{
+    *bx = 64;
+    *by = 4;
+    *ry = 2;
+    const int64_t txy = ((nx + 63) / 64) * ((ny + 7) / 8);
+    int64_t want_z = (2048 + txy - 1) / txy;  // chunks needed for ~2048 workgroups
+    int64_t c = (nz + want_z - 1) / want_z;
+    if (c < 8) c = 8;
+    if (c > nz) c = nz;
+    *zc = (int)c;
+}
+
+}  // namespace
+
+namespace pat {
+
+int heat3d_sweep(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, int variant, int bx,
+                 int by, int ry, int zc, hipStream_t stream)
+{
+    if (variant == PAT_STENCIL_GENERIC) {
+        const int64_t n = nx * ny * nz;
+        hipLaunchKernelGGL(heat_naive, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in, out, (int)nx,
+                           (int)ny, (int)nz, alpha);
+        PAT_LAUNCH_CHECK();
+        return PAT_OK;
+    }
+    HeatKernel k = heat_kernel_for(bx, by, ry);
+    PAT_REQUIRE(k, PAT_ERR_UNSUPPORTED, "heat3d: tile %dx%dx%d not instantiated", bx, by, ry);
+    PAT_REQUIRE(zc > 0, PAT_ERR_INVALID, "heat3d: z-chunk must be positive");
+    const int tiles_x = (int)((nx + bx - 1) / bx);
+    const int tiles_y = (int)((ny + by * ry - 1) / (by * ry));
+    const int tiles_z = (int)((nz + zc - 1) / zc);
+    const int64_t nblocks = (int64_t)tiles_x * tiles_y * tiles_z;
+    PAT_REQUIRE(nblocks < (1LL << 31), PAT_ERR_INVALID, "heat3d: grid too large");
+    hipLaunchKernelGGL(k, dim3((unsigned)nblocks), dim3(bx, by), 0, stream, in, out, (int)nx, (int)ny, (int)nz, zc,
+                       alpha, tiles_x, tiles_y, tiles_z);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, int *ry, int *zc)
+{
+    default_tile(nx, ny, nz, bx, by, ry, zc);
+}
+
+int heat3d_shapes(const int **triples)
+{
+    static int flat[sizeof(kShapes) / sizeof(kShapes[0]) * 3];
+    int n = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+    for (int i = 0; i < n; i++) {
+        flat[3 * i] = kShapes[i].bx;
+        flat[3 * i + 1] = kShapes[i].by;
+        flat[3 * i + 2] = kShapes[i].ry;
+    }
+    *triples = flat;
+    return n;
+}
+
+}  // namespace pat
+
+extern "C" int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, int64_t nz, int nsteps, float alpha,
+                                  int variant, int bx, int by, int zc, int *result_in_b, void *stream)
+{
+    PAT_REQUIRE(a && b && a != b, PAT_ERR_INVALID, "pat_heat3d_run_dev: need two distinct device grids");
+    PAT_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nsteps >= 0, PAT_ERR_INVALID, "pat_heat3d_run_dev: bad sizes");
+    PAT_REQUIRE((nx + 2) * (ny + 2) * (nz + 2) < (1LL << 40), PAT_ERR_INVALID, "pat_heat3d_run_dev: grid too large");
+    PAT_REQUIRE(variant == PAT_STENCIL_AUTO || variant == PAT_STENCIL_LDS25D || variant == PAT_STENCIL_GENERIC,
+                PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: unknown variant %d", variant);
+    int dbx, dby, dry, dzc;
+    default_tile(nx, ny, nz, &dbx, &dby, &dry, &dzc);
+    // 'by' at the C ABI is the tile height in rows; split it into thread rows
+    // and rows per thread using the instantiated shapes.
+    int tbx = bx > 0 ? bx : dbx;
+    int tby = dby, tr = dry;
+    if (by > 0) {
+        bool found = false;
+        for (const auto &s : kShapes)
+            if (s.bx == tbx && s.by * s.ry == by) {
+                tby = s.by;
+                tr = s.ry;
+                found = true;
+                break;
+            }
+        PAT_REQUIRE(found, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: tile %dx%d not instantiated", tbx, by);
+    } else if (bx > 0 && bx != dbx) {
+        bool found = false;
+        for (const auto &s : kShapes)
+            if (s.bx == tbx) {
+                tby = s.by;
+                tr = s.ry;
+                found = true;
+                break;
+            }
+        PAT_REQUIRE(found, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: tile width %d not instantiated", tbx);
+    }
+    int tzc = zc > 0 ? zc : dzc;
+    hipStream_t st = (hipStream_t)stream;
+    float *src = a, *dst = b;
+    for (int s = 0; s < nsteps; s++) {
+        PAT_TRY(pat::heat3d_sweep(src, dst, nx, ny, nz, alpha, variant, tbx, tby, tr, tzc, st));
+        float *t = src;
+        src = dst;
+        dst = t;
+    }
+    if (result_in_b) *result_in_b = (src == b) ? 1 : 0;
+    return PAT_OK;
+}

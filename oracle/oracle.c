@@ -1,0 +1,644 @@
+/*
+ * oracle.c -- CPU restatement of the IPDPS'15 pattern framework's three
+ * per-partition compute engines (stencil sweep, irregular edge reduction,
+ * generalized reduction for kmeans).
+ *
+ * THIS FILE IS TEST INFRASTRUCTURE.  It is the checker the HIP engines are
+ * compared against.  Only the test suite, the smoke check and the CPU-baseline
+ * leg of the benchmark may load it; the product library never links or calls
+ * it, and nothing here is a fallback for the GPU path.
+ *
+ * Every function cites the reference file:line whose arithmetic it restates.
+ * Build rules: -O2 -ffp-contract=off (no fused multiply-add, so each float
+ * expression rounds exactly as written, like the reference's x86 CPU path).
+ *
+ * Pinning (what checks each function beyond this file; see tests/):
+ *   heat3d_*      closed-form sine-eigenmode decay, linearity, axis symmetry,
+ *                 brute-force FP64 on tiny grids (test_oracle_stencil.py)
+ *   lj_*          two-particle closed form, perfect-FCC zero force, Newton's
+ *                 third law, edge-order invariance, and miniMD's own
+ *                 Force::compute built from the reference sources
+ *                 (oracle/_ref, test_oracle_lj.py)
+ *   neighbor_*    miniMD's own Neighbor::build (oracle/_ref) and O(N^2) brute
+ *                 force (test_oracle_lj.py)
+ *   kmeans_*      brute-force FP64 argmin, hand-built ties, label recovery,
+ *                 SSE monotonicity (test_oracle_kmeans.py)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define IDX3(x, y, z, sx, sy) ((int64_t)(x) + (int64_t)(sx) * ((int64_t)(y) + (int64_t)(sy) * (int64_t)(z)))
+
+/* ======================================================================
+ * Stencil engine: heat3D
+ * ====================================================================== */
+
+/*
+ * One explicit heat step over the interior of a halo-inclusive grid.
+ * Layout follows GET_FLOAT3 (heat_3D/lib/macro.h:70): element (i=z, j=y, k=x)
+ * lives at in + size[0]*j + i*size[0]*size[1] + k, size = halo-inclusive
+ * extents, so x is the fastest index.  The functor shape (one output cell from
+ * the 7-point star) is heat_3D/stencil.h:94-117; the update rule is the heat3D
+ * workload's own: u' = u + alpha*(s - 6u) with
+ * s = ((((x- + x+) + y-) + y+) + z-) + z+, evaluated in that order.
+ * The halo is Dirichlet: it is copied through unchanged (stays 0).
+ */
+void orc_heat3d_step_f32(const float *in, float *out, int nx, int ny, int nz, float alpha)
+{
+    const int sx = nx + 2, sy = ny + 2;
+    const int64_t sxy = (int64_t)sx * sy;
+    const int64_t total = sxy * (nz + 2);
+    /* halo stays as in the input (Dirichlet) */
+    memcpy(out, in, sizeof(float) * total);
+    for (int z = 1; z <= nz; z++)
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++) {
+                int64_t id = IDX3(x, y, z, sx, sy);
+                float c = in[id];
+                float s = in[id - 1] + in[id + 1];
+                s = s + in[id - sx];
+                s = s + in[id + sx];
+                s = s + in[id - sxy];
+                s = s + in[id + sxy];
+                out[id] = c + alpha * (s - 6.0f * c);
+            }
+}
+
+void orc_heat3d_step_f64(const double *in, double *out, int nx, int ny, int nz, double alpha)
+{
+    const int sx = nx + 2, sy = ny + 2;
+    const int64_t sxy = (int64_t)sx * sy;
+    const int64_t total = sxy * (nz + 2);
+    memcpy(out, in, sizeof(double) * total);
+    for (int z = 1; z <= nz; z++)
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++) {
+                int64_t id = IDX3(x, y, z, sx, sy);
+                double c = in[id];
+                double s = in[id - 1] + in[id + 1];
+                s = s + in[id - sx];
+                s = s + in[id + sx];
+                s = s + in[id - sxy];
+                s = s + in[id + sxy];
+                out[id] = c + alpha * (s - 6.0 * c);
+            }
+}
+
+/*
+ * nsteps of ping-pong (the double buffer of kmeans/lib/grid.h:7-73 with the
+ * Swap() that the reference never calls, grid.h:61-65).  Returns 0 when the
+ * result is in a, 1 when it is in b.
+ */
+int orc_heat3d_run_f32(float *a, float *b, int nx, int ny, int nz, int nsteps, float alpha)
+{
+    float *src = a, *dst = b;
+    for (int s = 0; s < nsteps; s++) {
+        orc_heat3d_step_f32(src, dst, nx, ny, nz, alpha);
+        float *t = src; src = dst; dst = t;
+    }
+    return src == a ? 0 : 1;
+}
+
+int orc_heat3d_run_f64(double *a, double *b, int nx, int ny, int nz, int nsteps, double alpha)
+{
+    double *src = a, *dst = b;
+    for (int s = 0; s < nsteps; s++) {
+        orc_heat3d_step_f64(src, dst, nx, ny, nz, alpha);
+        double *t = src; src = dst; dst = t;
+    }
+    return src == a ? 0 : 1;
+}
+
+/*
+ * Threaded CPU path in the shape of the reference's CPU stencil engine:
+ * the interior is cut into 16^3 tiles (CPU tile size, kmeans/lib/common.h:17;
+ * tiling in kmeans/lib/data_util.cpp:224-307) and the tile list is split
+ * statically over the threads (compute_tiles, kmeans/lib/compute_cpu.cpp:
+ * 69-126), each tile swept cell by cell (process_a_tile, :35-67, with the
+ * inner offsets reset per row as the reference intends).  Same arithmetic as
+ * orc_heat3d_step_f32, so results are bit-identical to the serial path.
+ */
+int orc_heat3d_run_f32_tiled(float *a, float *b, int nx, int ny, int nz, int nsteps, float alpha, int nthreads)
+{
+    const int T = 16;
+    const int sx = nx + 2, sy = ny + 2;
+    const int64_t sxy = (int64_t)sx * sy;
+    const int64_t total = sxy * (nz + 2);
+    const int tx = (nx + T - 1) / T, ty = (ny + T - 1) / T, tz = (nz + T - 1) / T;
+    const int ntiles = tx * ty * tz;
+    float *src = a, *dst = b;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    /* halo planes of both buffers are the same Dirichlet zeros */
+    memcpy(dst, src, sizeof(float) * total);
+    for (int s = 0; s < nsteps; s++) {
+        #pragma omp parallel num_threads(nthreads)
+        {
+            int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+            int per = (ntiles + nt - 1) / nt;
+            int t0 = tid * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+            for (int t = t0; t < t1; t++) {
+                int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
+                int x0 = 1 + bx * T, y0 = 1 + by * T, z0 = 1 + bz * T;
+                int x1 = x0 + T <= nx + 1 ? x0 + T : nx + 1;
+                int y1 = y0 + T <= ny + 1 ? y0 + T : ny + 1;
+                int z1 = z0 + T <= nz + 1 ? z0 + T : nz + 1;
+                for (int z = z0; z < z1; z++)
+                    for (int y = y0; y < y1; y++)
+                        for (int x = x0; x < x1; x++) {
+                            int64_t id = IDX3(x, y, z, sx, sy);
+                            float c = src[id];
+                            float sm = src[id - 1] + src[id + 1];
+                            sm = sm + src[id - sx];
+                            sm = sm + src[id + sx];
+                            sm = sm + src[id - sxy];
+                            sm = sm + src[id + sxy];
+                            dst[id] = c + alpha * (sm - 6.0f * c);
+                        }
+            }
+        }
+        float *t = src; src = dst; dst = t;
+    }
+    return src == a ? 0 : 1;
+}
+
+/* ======================================================================
+ * Irregular engine: Lennard-Jones edge reduction
+ * ====================================================================== */
+
+/*
+ * Pair force of the moldyn functor (kmeans/irregular.h:51-72): minimum-image
+ * displacement d = x_i - x_j with box side L (:55-60), rd = |d|^2, and for
+ * rd < rc^2: rrd = 1/rd, rrd2 = rrd^2, rrd4 = rrd2^2, rrd6 = rrd2*rrd4,
+ * rrd7 = rrd6*rrd, r148 = rrd7 - 0.5*rrd4.  The reference omits the 48*eps
+ * prefactor (it folds it into the integrator constants, kmeans/moldyn.cu:
+ * 396-398); here the force is written out in full, F = 48*r148*d for
+ * eps = sigma = 1.  pbc = 0 drops the minimum image (miniMD map1,
+ * miniMD/irregular.h:127-139, which has no periodic images).
+ * Returns 1 and the force-on-i components when the pair is inside the cutoff.
+ */
+static inline int pair_force_f32(const float *pi, const float *pj, float L, float hL, int pbc, float rc2,
+                                 float *fx, float *fy, float *fz, float *phi)
+{
+    float dx = pi[0] - pj[0];
+    float dy = pi[1] - pj[1];
+    float dz = pi[2] - pj[2];
+    if (pbc) {
+        if (dx < -hL) dx += L;
+        if (dy < -hL) dy += L;
+        if (dz < -hL) dz += L;
+        if (dx > hL) dx -= L;
+        if (dy > hL) dy -= L;
+        if (dz > hL) dz -= L;
+    }
+    float rd = dx * dx + dy * dy;
+    rd = rd + dz * dz;
+    if (!(rd < rc2)) return 0;
+    float rrd = 1.0f / rd;
+    float rrd2 = rrd * rrd;
+    float rrd4 = rrd2 * rrd2;
+    float rrd6 = rrd2 * rrd4;
+    float rrd7 = rrd6 * rrd;
+    float r148 = rrd7 - 0.5f * rrd4;
+    float F = 48.0f * r148;
+    *fx = dx * F;
+    *fy = dy * F;
+    *fz = dz * F;
+    if (phi) {
+        float rrd3 = rrd2 * rrd;
+        *phi = 4.0f * (rrd6 - rrd3);
+    }
+    return 1;
+}
+
+static inline int pair_force_f64(const double *pi, const double *pj, double L, double hL, int pbc, double rc2,
+                                 double *fx, double *fy, double *fz, double *phi)
+{
+    double dx = pi[0] - pj[0];
+    double dy = pi[1] - pj[1];
+    double dz = pi[2] - pj[2];
+    if (pbc) {
+        if (dx < -hL) dx += L;
+        if (dy < -hL) dy += L;
+        if (dz < -hL) dz += L;
+        if (dx > hL) dx -= L;
+        if (dy > hL) dy -= L;
+        if (dz > hL) dz -= L;
+    }
+    double rd = dx * dx + dy * dy;
+    rd = rd + dz * dz;
+    if (!(rd < rc2)) return 0;
+    double rrd = 1.0 / rd;
+    double rrd2 = rrd * rrd;
+    double rrd4 = rrd2 * rrd2;
+    double rrd6 = rrd2 * rrd4;
+    double rrd7 = rrd6 * rrd;
+    double r148 = rrd7 - 0.5 * rrd4;
+    double F = 48.0 * r148;
+    *fx = dx * F;
+    *fy = dy * F;
+    *fz = dz * F;
+    if (phi) {
+        double rrd3 = rrd2 * rrd;
+        *phi = 4.0 * (rrd6 - rrd3);
+    }
+    return 1;
+}
+
+/*
+ * Serial scatter over a half edge list, f[i] += F*d, f[j] -= F*d, in edge
+ * order: the serial semantics of miniMD Force::compute (miniMD/force.cpp:
+ * 88-116) and of moldyn's commented-out scatter (kmeans/irregular.h:94-100),
+ * which the GPU engine's LDS hash reduction is meant to equal.
+ * pos, f: N x 3 row-major.  Returns the potential energy sum_pairs 4(r^-12 - r^-6).
+ */
+double orc_lj_force_f32(int64_t N, int64_t E, const int32_t *edges, const float *pos, float L, int pbc,
+                        float rc2, float *f)
+{
+    const float hL = 0.5f * L;
+    double epot = 0.0;
+    memset(f, 0, sizeof(float) * 3 * N);
+    for (int64_t e = 0; e < E; e++) {
+        int i = edges[2 * e], j = edges[2 * e + 1];
+        float fx, fy, fz, phi;
+        if (pair_force_f32(pos + 3 * (int64_t)i, pos + 3 * (int64_t)j, L, hL, pbc, rc2, &fx, &fy, &fz, &phi)) {
+            f[3 * (int64_t)i + 0] += fx;
+            f[3 * (int64_t)i + 1] += fy;
+            f[3 * (int64_t)i + 2] += fz;
+            f[3 * (int64_t)j + 0] -= fx;
+            f[3 * (int64_t)j + 1] -= fy;
+            f[3 * (int64_t)j + 2] -= fz;
+            epot += phi;
+        }
+    }
+    return epot;
+}
+
+double orc_lj_force_f64(int64_t N, int64_t E, const int32_t *edges, const double *pos, double L, int pbc,
+                        double rc2, double *f)
+{
+    const double hL = 0.5 * L;
+    double epot = 0.0;
+    memset(f, 0, sizeof(double) * 3 * N);
+    for (int64_t e = 0; e < E; e++) {
+        int i = edges[2 * e], j = edges[2 * e + 1];
+        double fx, fy, fz, phi;
+        if (pair_force_f64(pos + 3 * (int64_t)i, pos + 3 * (int64_t)j, L, 0.5 * L, pbc, rc2, &fx, &fy, &fz, &phi)) {
+            f[3 * (int64_t)i + 0] += fx;
+            f[3 * (int64_t)i + 1] += fy;
+            f[3 * (int64_t)i + 2] += fz;
+            f[3 * (int64_t)j + 0] -= fx;
+            f[3 * (int64_t)j + 1] -= fy;
+            f[3 * (int64_t)j + 2] -= fz;
+            epot += phi;
+        }
+    }
+    (void)hL;
+    return epot;
+}
+
+/*
+ * Threaded CPU path in the shape of the reference's CPU irregular engine
+ * (compute_cpu, kmeans/irregular/cpu_kernel.cu:13-70): threads take chunks
+ * of edges (CPU_BLOCK_SIZE, kmeans/irregular/parameters.h:8, here dynamic
+ * chunks of 2^16) into thread-private node accumulators, which are merged
+ * after the join (the reference's merge is racy and partition-0-only,
+ * cpu_kernel.cu:65 / irregular_runtime.cu:418-425; fixed here).
+ * scratch: nthreads * 3N floats.
+ */
+void orc_lj_force_f32_threads(int64_t N, int64_t E, const int32_t *edges, const float *pos, float L, int pbc,
+                              float rc2, float *f, float *scratch, int nthreads)
+{
+    const float hL = 0.5f * L;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        int tid = omp_get_thread_num();
+        float *acc = scratch + (int64_t)tid * 3 * N;
+        memset(acc, 0, sizeof(float) * 3 * N);
+        #pragma omp for schedule(dynamic, 65536)
+        for (int64_t e = 0; e < E; e++) {
+            int i = edges[2 * e], j = edges[2 * e + 1];
+            float fx, fy, fz;
+            if (pair_force_f32(pos + 3 * (int64_t)i, pos + 3 * (int64_t)j, L, hL, pbc, rc2, &fx, &fy, &fz, 0)) {
+                acc[3 * (int64_t)i + 0] += fx;
+                acc[3 * (int64_t)i + 1] += fy;
+                acc[3 * (int64_t)i + 2] += fz;
+                acc[3 * (int64_t)j + 0] -= fx;
+                acc[3 * (int64_t)j + 1] -= fy;
+                acc[3 * (int64_t)j + 2] -= fz;
+            }
+        }
+        int nt = omp_get_num_threads();
+        #pragma omp for schedule(static)
+        for (int64_t m = 0; m < 3 * N; m++) {
+            float s = 0.0f;
+            for (int t = 0; t < nt; t++) s += scratch[(int64_t)t * 3 * N + m];
+            f[m] = s;
+        }
+    }
+}
+
+/*
+ * Velocity-Verlet with unit mass and periodic wrap into [0, L), the
+ * integrator the moldyn and miniMD workloads run around the force
+ * reduction (moldyn's own update is a leapfrog with folded constants,
+ * kmeans/moldyn.cu:339-403; miniMD's is commented out, miniMD/ljs.cu:178-184).
+ * Per node and per step, in this order:
+ *   v = v + hdt*f;  x = x + dt*v;  wrap(x);  f = F(x);  v = v + hdt*f
+ * with hdt = 0.5f*dt rounded once.  Starts by computing f = F(x0).
+ * pos, vel updated in place; f returns the last force.  Returns epot of the
+ * last force evaluation.
+ */
+static inline float wrap_f32(float x, float L)
+{
+    if (x < 0.0f) x += L;
+    else if (x >= L) x -= L;
+    return x;
+}
+
+double orc_md_run_f32(int64_t N, int64_t E, const int32_t *edges, float *pos, float *vel, float *f, float L,
+                      int pbc, float rc2, float dt, int nsteps, int nthreads, float *scratch)
+{
+    const float hdt = 0.5f * dt;
+    double epot = 0.0;
+    if (nthreads == 1 || !scratch) epot = orc_lj_force_f32(N, E, edges, pos, L, pbc, rc2, f);
+    else orc_lj_force_f32_threads(N, E, edges, pos, L, pbc, rc2, f, scratch, nthreads);
+    for (int s = 0; s < nsteps; s++) {
+        for (int64_t m = 0; m < 3 * N; m++) {
+            vel[m] = vel[m] + hdt * f[m];
+            float x = pos[m] + dt * vel[m];
+            pos[m] = pbc ? wrap_f32(x, L) : x;
+        }
+        if (nthreads == 1 || !scratch) epot = orc_lj_force_f32(N, E, edges, pos, L, pbc, rc2, f);
+        else orc_lj_force_f32_threads(N, E, edges, pos, L, pbc, rc2, f, scratch, nthreads);
+        for (int64_t m = 0; m < 3 * N; m++) vel[m] = vel[m] + hdt * f[m];
+    }
+    return epot;
+}
+
+static inline double wrap_f64(double x, double L)
+{
+    if (x < 0.0) x += L;
+    else if (x >= L) x -= L;
+    return x;
+}
+
+double orc_md_run_f64(int64_t N, int64_t E, const int32_t *edges, double *pos, double *vel, double *f, double L,
+                      int pbc, double rc2, double dt, int nsteps)
+{
+    const double hdt = 0.5 * dt;
+    double epot = orc_lj_force_f64(N, E, edges, pos, L, pbc, rc2, f);
+    for (int s = 0; s < nsteps; s++) {
+        for (int64_t m = 0; m < 3 * N; m++) {
+            vel[m] = vel[m] + hdt * f[m];
+            double x = pos[m] + dt * vel[m];
+            pos[m] = pbc ? wrap_f64(x, L) : x;
+        }
+        epot = orc_lj_force_f64(N, E, edges, pos, L, pbc, rc2, f);
+        for (int64_t m = 0; m < 3 * N; m++) vel[m] = vel[m] + hdt * f[m];
+    }
+    return epot;
+}
+
+/*
+ * Brute-force O(N^2) half list: every pair i<j with minimum-image distance
+ * squared below cut2, emitted in (i, j) lexicographic order.  The moldyn
+ * driver's RebuildNeighbor (kmeans/moldyn.cu:552-594) is this O(N^2) loop.
+ * Returns the pair count; writes at most cap pairs when edges is non-NULL.
+ */
+int64_t orc_pairs_brute(int64_t N, const float *pos, float L, int pbc, float cut2, int32_t *edges, int64_t cap)
+{
+    const float hL = 0.5f * L;
+    int64_t n = 0;
+    for (int64_t i = 0; i < N; i++)
+        for (int64_t j = i + 1; j < N; j++) {
+            float dx = pos[3 * i] - pos[3 * j];
+            float dy = pos[3 * i + 1] - pos[3 * j + 1];
+            float dz = pos[3 * i + 2] - pos[3 * j + 2];
+            if (pbc) {
+                if (dx < -hL) dx += L;
+                if (dy < -hL) dy += L;
+                if (dz < -hL) dz += L;
+                if (dx > hL) dx -= L;
+                if (dy > hL) dy -= L;
+                if (dz > hL) dz -= L;
+            }
+            float r2 = dx * dx + dy * dy;
+            r2 = r2 + dz * dz;
+            if (r2 < cut2) {
+                if (edges && n < cap) {
+                    edges[2 * n] = (int32_t)i;
+                    edges[2 * n + 1] = (int32_t)j;
+                }
+                n++;
+            }
+        }
+    return n;
+}
+
+/* ======================================================================
+ * Generalized-reduction engine: kmeans
+ * ====================================================================== */
+
+/*
+ * Assignment of kmeans map1 (kmeans/regular.h:84-103), generalised from
+ * D=3/K=40 to any D, K: for each centre i in order,
+ * dist = sum_d (c_d - x_d)^2 accumulated in dimension order in FP32,
+ * dist = sqrt(dist), and the strict comparison dist < min_dist starting from
+ * min_dist = 65536*65, so the lowest index wins ties.  Writes the key and
+ * the emitted min_dist (value[4] of map1, :110-115).
+ */
+void orc_kmeans_assign_f32(int64_t N, int D, int K, const float *x, const float *c, int32_t *assign,
+                           float *mindist, int nthreads)
+{
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel for schedule(dynamic, 10000) num_threads(nthreads)
+    for (int64_t p = 0; p < N; p++) {
+        const float *xp = x + p * D;
+        unsigned int key = 0;
+        float min_dist = 65536 * 65;
+        for (int i = 0; i < K; i++) {
+            const float *ci = c + (int64_t)i * D;
+            float dist = 0;
+            for (int d = 0; d < D; d++) {
+                float t = ci[d] - xp[d];
+                dist = dist + t * t;
+            }
+            dist = sqrtf(dist);
+            if (dist < min_dist) {
+                min_dist = dist;
+                key = i;
+            }
+        }
+        assign[p] = (int32_t)key;
+        if (mindist) mindist[p] = min_dist;
+    }
+}
+
+/* FP64 twin of the assignment, for error statements. */
+void orc_kmeans_assign_f64(int64_t N, int D, int K, const float *x, const double *c, int32_t *assign,
+                           double *mindist, int nthreads)
+{
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel for schedule(dynamic, 10000) num_threads(nthreads)
+    for (int64_t p = 0; p < N; p++) {
+        const float *xp = x + p * D;
+        int key = 0;
+        double min_dist = 65536.0 * 65;
+        for (int i = 0; i < K; i++) {
+            const double *ci = c + (int64_t)i * D;
+            double dist = 0;
+            for (int d = 0; d < D; d++) {
+                double t = ci[d] - (double)xp[d];
+                dist = dist + t * t;
+            }
+            dist = sqrt(dist);
+            if (dist < min_dist) {
+                min_dist = dist;
+                key = i;
+            }
+        }
+        assign[p] = key;
+        if (mindist) mindist[p] = min_dist;
+    }
+}
+
+/*
+ * Fixed-point scale for order-independent centroid sums.  The reference's
+ * reduce1 (kmeans/regular.h:121-136) adds FP32 values in whatever order the
+ * locked hash inserts happen, which is not reproducible; here each coordinate
+ * is rounded once to an integer multiple of 2^-s and summed as int64, so any
+ * summation order gives the same bits.  s is chosen so that N * max|x| * 2^s
+ * stays below 2^62.  Returns s.
+ */
+int orc_kmeans_fixed_shift(int64_t N, float maxabs)
+{
+    int eN = 0, eM = 0;
+    while ((1LL << eN) < N) eN++;
+    double m = maxabs > 0 ? maxabs : 1.0f;
+    frexp(m, &eM); /* m < 2^eM */
+    int s = 62 - eN - eM;
+    if (s > 60) s = 60;
+    return s;
+}
+
+static inline int64_t to_fixed(float v, double scale)
+{
+    return (int64_t)rint((double)v * scale);
+}
+
+/*
+ * Reduction object of kmeans map1/reduce1 (values {x_0..x_{D-1}, 1, min_dist},
+ * kmeans/regular.h:110-115, summed per key by reduce1, :121-136) with an
+ * integer count and fixed-point sums.  In the shape of the reference's CPU
+ * regular engine (compute_cpu_reg, kmeans/regular/cpu_kernel.cu:17-81):
+ * threads take chunks of units under a shared counter into thread-private
+ * reduction objects, merged after the join (the reference merges racily
+ * while other threads still map, cpu_kernel.cu:77-78; fixed here).
+ * sums: K*D int64, counts: K int64, dsum: K int64 (min_dist in fixed point
+ * with dshift).
+ */
+void orc_kmeans_accumulate(int64_t N, int D, int K, const float *x, const int32_t *assign,
+                           const float *mindist, int shift, int dshift, int64_t *sums, int64_t *counts,
+                           int64_t *dsum, int nthreads)
+{
+    const double scale = ldexp(1.0, shift), dscale = ldexp(1.0, dshift);
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    memset(sums, 0, sizeof(int64_t) * K * D);
+    memset(counts, 0, sizeof(int64_t) * K);
+    if (dsum) memset(dsum, 0, sizeof(int64_t) * K);
+    #pragma omp parallel num_threads(nthreads)
+    {
+        int64_t *ls = (int64_t *)calloc((size_t)K * D, sizeof(int64_t));
+        int64_t *lc = (int64_t *)calloc((size_t)K, sizeof(int64_t));
+        int64_t *ld = (int64_t *)calloc((size_t)K, sizeof(int64_t));
+        #pragma omp for schedule(dynamic, 10000)
+        for (int64_t p = 0; p < N; p++) {
+            int k = assign[p];
+            const float *xp = x + p * D;
+            for (int d = 0; d < D; d++) ls[(int64_t)k * D + d] += to_fixed(xp[d], scale);
+            lc[k] += 1;
+            if (mindist) ld[k] += to_fixed(mindist[p], dscale);
+        }
+        #pragma omp critical
+        {
+            for (int64_t m = 0; m < (int64_t)K * D; m++) sums[m] += ls[m];
+            for (int k = 0; k < K; k++) counts[k] += lc[k];
+            if (dsum)
+                for (int k = 0; k < K; k++) dsum[k] += ld[k];
+        }
+        free(ls);
+        free(lc);
+        free(ld);
+    }
+}
+
+/*
+ * Lloyd update, which the reference leaves out (km.cu runs one map pass and
+ * RegularGetOutput is empty, kmeans/regular/regular_runtime.cu:288-291):
+ * c_k = (float)(((double)sum_k * 2^-s) / (double)count_k); a cluster with no
+ * points keeps its previous centre.
+ */
+void orc_kmeans_update(int D, int K, const int64_t *sums, const int64_t *counts, int shift, float *c)
+{
+    const double inv = ldexp(1.0, -shift);
+    for (int k = 0; k < K; k++) {
+        if (counts[k] == 0) continue;
+        double n = (double)counts[k];
+        for (int d = 0; d < D; d++) c[(int64_t)k * D + d] = (float)(((double)sums[(int64_t)k * D + d] * inv) / n);
+    }
+}
+
+/*
+ * Full Lloyd loop: centres start as the first K points (the workload's
+ * initialisation), then iters x (assign, accumulate, update).  Final
+ * assignment is of the last iteration (made against the centres that
+ * iteration started from).  sse_out (optional, iters entries) receives
+ * sum_p min_dist_p^2 in FP64 per iteration.
+ */
+void orc_kmeans_lloyd(int64_t N, int D, int K, const float *x, int iters, int shift, float *c, int32_t *assign,
+                      double *sse_out, int nthreads)
+{
+    int64_t *sums = (int64_t *)malloc(sizeof(int64_t) * K * D);
+    int64_t *counts = (int64_t *)malloc(sizeof(int64_t) * K);
+    float *md = (float *)malloc(sizeof(float) * N);
+    for (int it = 0; it < iters; it++) {
+        orc_kmeans_assign_f32(N, D, K, x, c, assign, md, nthreads);
+        if (sse_out) {
+            double s = 0;
+            for (int64_t p = 0; p < N; p++) s += (double)md[p] * md[p];
+            sse_out[it] = s;
+        }
+        orc_kmeans_accumulate(N, D, K, x, assign, 0, shift, 0, sums, counts, 0, nthreads);
+        orc_kmeans_update(D, K, sums, counts, shift, c);
+    }
+    free(sums);
+    free(counts);
+    free(md);
+}
+
+/* FP64 Lloyd loop with plain double sums, for error statements. */
+void orc_kmeans_lloyd_f64(int64_t N, int D, int K, const float *x, int iters, double *c, int32_t *assign,
+                          int nthreads)
+{
+    double *sums = (double *)malloc(sizeof(double) * K * D);
+    int64_t *counts = (int64_t *)malloc(sizeof(int64_t) * K);
+    for (int it = 0; it < iters; it++) {
+        orc_kmeans_assign_f64(N, D, K, x, c, assign, 0, nthreads);
+        memset(sums, 0, sizeof(double) * K * D);
+        memset(counts, 0, sizeof(int64_t) * K);
+        for (int64_t p = 0; p < N; p++) {
+            int k = assign[p];
+            for (int d = 0; d < D; d++) sums[(int64_t)k * D + d] += x[p * D + d];
+            counts[k]++;
+        }
+        for (int k = 0; k < K; k++)
+            if (counts[k])
+                for (int d = 0; d < D; d++) c[(int64_t)k * D + d] = sums[(int64_t)k * D + d] / counts[k];
+    }
+    free(sums);
+    free(counts);
+}

@@ -1,0 +1,41 @@
+"""The C ABI: libpattern.so loads, and exports every entry point
+include/*.h declares (no compute calls; runs without a GPU)."""
+import glob
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(pat_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported(pa):
+    names = declared()
+    assert len(names) >= 10
+    L = pa.lib()
+    for n in sorted(names):
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+
+
+def test_binding_covers_header(pa):
+    assert declared() == set(pa.SIGNATURES), declared() ^ set(pa.SIGNATURES)
+
+
+def test_error_path_without_gpu(pa):
+    # argument validation runs before any device call
+    import ctypes
+    rc = pa.lib().pat_heat3d_run_dev(None, None, 4, 4, 4, 1, 0.1, 0, 0, 0, 0, None, None)
+    assert rc == -1
+    assert b"distinct" in pa.lib().pat_last_error()
+    h = ctypes.c_void_p()
+    interior = (ctypes.c_int64 * 3)(4, 4, 4)
+    assert pa.lib().pat_stencil_create(2, interior, 1, 4, None, 0, ctypes.byref(h)) == -5
+    assert pa.lib().pat_stencil_create(3, interior, 2, 4, None, 0, ctypes.byref(h)) == -5

@@ -1,0 +1,91 @@
+"""GPU parity of the stencil engine (through libpattern's C ABI) vs the oracle.
+
+Bar: bit-exact against the FP32 oracle (same rule, same operation order, no
+contraction on either side).  The FP64 error is reported for the heat3D
+workload (128^3, seed 42, 100 steps).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_dev(pa, torch, g, nsteps, **kw):
+    nz, ny, nx = (s - 2 for s in g.shape)
+    a = torch.from_numpy(g).cuda()
+    b = a.clone()
+    out = pa.heat3d_run_dev(a, b, nx, ny, nz, nsteps, **kw)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (63, 7, 5), (65, 9, 17), (130, 3, 2), (64, 16, 33), (200, 33, 12)])
+def test_lds25d_bitexact_ragged(pa, orc, gpu, shape):
+    nx, ny, nz = shape
+    g = pa.gen_heat3d(nx, ny, nz, seed=11)
+    ref = orc.heat3d(g, 3)
+    got = run_dev(pa, gpu, g, 3, variant=pa.PAT_STENCIL_LDS25D)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("bx,by,zc", [(64, 4, 1), (64, 8, 3), (64, 16, 8), (64, 8, 64), (32, 16, 5), (128, 4, 2),
+                                      (128, 4, 7)])
+def test_lds25d_tiles(pa, orc, gpu, bx, by, zc):
+    nx, ny, nz = 97, 41, 23
+    g = pa.gen_heat3d(nx, ny, nz, seed=12)
+    ref = orc.heat3d(g, 4)
+    got = run_dev(pa, gpu, g, 4, variant=pa.PAT_STENCIL_LDS25D, bx=bx, by=by, zc=zc)
+    assert np.array_equal(got, ref)
+
+
+def test_naive_bitexact(pa, orc, gpu):
+    g = pa.gen_heat3d(45, 30, 20, seed=13)
+    ref = orc.heat3d(g, 4)
+    got = run_dev(pa, gpu, g, 4, variant=pa.PAT_STENCIL_GENERIC)
+    assert np.array_equal(got, ref)
+
+
+def test_heat3d_workload_128(pa, orc, gpu):
+    """heat3D workload: 128^3, seed 42, 100 steps; bit-exact vs FP32 oracle."""
+    n = 128
+    g = pa.gen_heat3d(n, n, n, seed=42)
+    ref = orc.heat3d(g, 100, threads=8)
+    s = pa.Stencil(n, n, n, alpha=0.1)
+    s.upload(g)
+    s.run(100)
+    got = s.download()
+    s.close()
+    assert np.array_equal(got, ref)
+    ref64 = orc.heat3d(g.astype(np.float64), 100, dtype=np.float64)
+    err = np.max(np.abs(got - ref64)) / np.max(np.abs(ref64))
+    print(f"heat3d 128^3 x100: Linf={np.max(np.abs(got)):.6e} rel err vs FP64 {err:.3e}")
+    assert err < 1e-5
+
+
+def test_handle_graph_replay(pa, orc, gpu):
+    nx, ny, nz = 70, 50, 40
+    g = pa.gen_heat3d(nx, ny, nz, seed=14)
+    s = pa.Stencil(nx, ny, nz)
+    s.upload(g)
+    s.run(7)
+    s.run(7)  # replays the captured graph from the other buffer
+    s.run(6)
+    got = s.download()
+    s.close()
+    assert np.array_equal(got, orc.heat3d(g, 20))
+
+
+def test_eigenmode_full_size_384(pa, gpu):
+    """384^3 (over the 256 MB last-level cache): closed-form decay of a sine mode."""
+    n, steps, alpha = 384, 10, 0.1
+    k = np.sin(np.pi * np.arange(n + 2) / (n + 1)).astype(np.float64)
+    t = gpu.from_numpy(k).cuda()
+    g = (t[:, None, None] * t[None, :, None] * t[None, None, :]).float().contiguous()
+    g[0].zero_(); g[-1].zero_(); g[:, 0].zero_(); g[:, -1].zero_(); g[:, :, 0].zero_(); g[:, :, -1].zero_()
+    b = g.clone()
+    out = pa.heat3d_run_dev(g, b, n, n, n, steps, alpha)
+    lam = 1 + alpha * 3 * (2 * np.cos(np.pi / (n + 1)) - 2)
+    g0 = (t[:, None, None] * t[None, :, None] * t[None, None, :])
+    g0[0] = 0; g0[-1] = 0; g0[:, 0] = 0; g0[:, -1] = 0; g0[:, :, 0] = 0; g0[:, :, -1] = 0
+    err = (out.double() - g0 * lam ** steps).abs().max().item()
+    assert err < 1e-5
