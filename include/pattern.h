@@ -47,6 +47,30 @@ int pat_info(char *buf, int64_t buf_len);
 int pat_device_sync(void);
 
 /* ------------------------------------------------------------------------
+ * Device memory, streams and events for host harnesses (no counterpart in
+ * the reference, which moves data with cudaMemcpy/mapped host memory inside
+ * its runtimes; exposed so that callers in other languages can keep inputs
+ * resident and time work without linking a second HIP runtime).
+ * ---------------------------------------------------------------------- */
+int pat_dev_malloc(void **ptr, int64_t bytes);
+int pat_dev_free(void *ptr);
+int pat_memcpy_h2d(void *dst, const void *src, int64_t bytes, void *stream);
+int pat_memcpy_d2h(void *dst, const void *src, int64_t bytes, void *stream);
+int pat_memcpy_d2d(void *dst, const void *src, int64_t bytes, void *stream);
+int pat_memset_dev(void *ptr, int value, int64_t bytes, void *stream);
+/* Default-priority non-blocking stream. */
+int pat_stream_create(void **stream);
+int pat_stream_destroy(void *stream);
+int pat_stream_sync(void *stream);
+int pat_event_create(void **event);
+int pat_event_destroy(void *event);
+int pat_event_record(void *event, void *stream);
+/* Milliseconds between two recorded events (waits for 'end'). */
+int pat_event_elapsed_ms(void *start, void *end, float *ms);
+/* Device properties: CU count, total HBM bytes, free HBM bytes. */
+int pat_device_query(int *num_cus, int64_t *total_bytes, int64_t *free_bytes);
+
+/* ------------------------------------------------------------------------
  * Seeded workload generators (host code, deterministic for any thread
  * count; the app drivers' data set-up, e.g. kmeans/km.cu:28-65,
  * kmeans/moldyn.cu:637-741, miniMD/setup.cpp:61-147).

@@ -84,6 +84,123 @@ int pat_device_sync(void)
     return PAT_OK;
 }
 
+int pat_dev_malloc(void **ptr, int64_t bytes)
+{
+    PAT_REQUIRE(ptr && bytes >= 0, PAT_ERR_INVALID, "pat_dev_malloc: bad arguments");
+    *ptr = nullptr;
+    if (bytes == 0) return PAT_OK;
+    hipError_t e = hipMalloc(ptr, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        pat::set_error("pat_dev_malloc: %lld bytes: %s", (long long)bytes, hipGetErrorString(e));
+        return PAT_ERR_NOMEM;
+    }
+    return PAT_OK;
+}
+
+int pat_dev_free(void *ptr)
+{
+    if (ptr) PAT_HIP(hipFree(ptr));
+    return PAT_OK;
+}
+
+int pat_memcpy_h2d(void *dst, const void *src, int64_t bytes, void *stream)
+{
+    PAT_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), PAT_ERR_INVALID, "pat_memcpy_h2d: bad arguments");
+    if (bytes == 0) return PAT_OK;
+    PAT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    PAT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_memcpy_d2h(void *dst, const void *src, int64_t bytes, void *stream)
+{
+    PAT_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), PAT_ERR_INVALID, "pat_memcpy_d2h: bad arguments");
+    if (bytes == 0) return PAT_OK;
+    PAT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PAT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_memcpy_d2d(void *dst, const void *src, int64_t bytes, void *stream)
+{
+    PAT_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), PAT_ERR_INVALID, "pat_memcpy_d2d: bad arguments");
+    if (bytes == 0) return PAT_OK;
+    PAT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_memset_dev(void *ptr, int value, int64_t bytes, void *stream)
+{
+    PAT_REQUIRE(bytes >= 0 && (bytes == 0 || ptr), PAT_ERR_INVALID, "pat_memset_dev: bad arguments");
+    if (bytes == 0) return PAT_OK;
+    PAT_HIP(hipMemsetAsync(ptr, value, (size_t)bytes, (hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_stream_create(void **stream)
+{
+    PAT_REQUIRE(stream, PAT_ERR_INVALID, "pat_stream_create: null argument");
+    hipStream_t s;
+    PAT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return PAT_OK;
+}
+
+int pat_stream_destroy(void *stream)
+{
+    if (stream) PAT_HIP(hipStreamDestroy((hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_stream_sync(void *stream)
+{
+    PAT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_event_create(void **event)
+{
+    PAT_REQUIRE(event, PAT_ERR_INVALID, "pat_event_create: null argument");
+    hipEvent_t e;
+    PAT_HIP(hipEventCreate(&e));
+    *event = e;
+    return PAT_OK;
+}
+
+int pat_event_destroy(void *event)
+{
+    if (event) PAT_HIP(hipEventDestroy((hipEvent_t)event));
+    return PAT_OK;
+}
+
+int pat_event_record(void *event, void *stream)
+{
+    PAT_REQUIRE(event, PAT_ERR_INVALID, "pat_event_record: null event");
+    PAT_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return PAT_OK;
+}
+
+int pat_event_elapsed_ms(void *start, void *end, float *ms)
+{
+    PAT_REQUIRE(start && end && ms, PAT_ERR_INVALID, "pat_event_elapsed_ms: null argument");
+    PAT_HIP(hipEventSynchronize((hipEvent_t)end));
+    PAT_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return PAT_OK;
+}
+
+int pat_device_query(int *num_cus, int64_t *total_bytes, int64_t *free_bytes)
+{
+    hipDeviceProp_t p;
+    PAT_HIP(hipGetDeviceProperties(&p, pat::g_device));
+    if (num_cus) *num_cus = p.multiProcessorCount;
+    size_t fr = 0, tot = 0;
+    PAT_HIP(hipMemGetInfo(&fr, &tot));
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    return PAT_OK;
+}
+
 int pat_info(char *buf, int64_t buf_len)
 {
     PAT_REQUIRE(buf && buf_len > 0, PAT_ERR_INVALID, "pat_info: null buffer");

@@ -39,6 +39,20 @@ SIGNATURES = {
     "pat_finalize": (c_int, []),
     "pat_info": (c_int, [ctypes.c_char_p, c_i64]),
     "pat_device_sync": (c_int, []),
+    "pat_dev_malloc": (c_int, [ctypes.POINTER(c_vp), c_i64]),
+    "pat_dev_free": (c_int, [c_vp]),
+    "pat_memcpy_h2d": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "pat_memcpy_d2h": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "pat_memcpy_d2d": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "pat_memset_dev": (c_int, [c_vp, c_int, c_i64, c_vp]),
+    "pat_stream_create": (c_int, [ctypes.POINTER(c_vp)]),
+    "pat_stream_destroy": (c_int, [c_vp]),
+    "pat_stream_sync": (c_int, [c_vp]),
+    "pat_event_create": (c_int, [ctypes.POINTER(c_vp)]),
+    "pat_event_destroy": (c_int, [c_vp]),
+    "pat_event_record": (c_int, [c_vp, c_vp]),
+    "pat_event_elapsed_ms": (c_int, [c_vp, c_vp, ctypes.POINTER(c_float)]),
+    "pat_device_query": (c_int, [c_intp, c_i64p, c_i64p]),
     "pat_gen_heat3d": (c_int, [c_i64, c_i64, c_i64, c_u64, c_vp]),
     "pat_gen_fcc": (c_int, [c_i64, c_i64, c_i64, c_double, c_double, c_u64, c_vp, c_dp]),
     "pat_gen_velocities": (c_int, [c_i64, c_double, c_u64, c_vp]),
@@ -89,20 +103,128 @@ def call(name, *args):
 
 
 def ptr(a):
-    """Address of a numpy array or torch tensor (host or device)."""
+    """Address of a numpy array (host) or a DeviceArray (device)."""
     if a is None:
         return None
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"]
         return a.ctypes.data
-    assert a.is_contiguous()
-    return a.data_ptr()
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return int(a)
 
 
 def stream_ptr(stream=None):
-    import torch
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return s.cuda_stream
+    return stream.h if isinstance(stream, Stream) else stream
+
+
+def init(device=0):
+    call("pat_init", device)
+
+
+class DeviceArray:
+    """A device allocation made through the C ABI (hipMalloc in libpattern)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        h = c_vp()
+        call("pat_dev_malloc", ctypes.byref(h), self.nbytes)
+        self.ptr = h.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        call("pat_memcpy_h2d", d.ptr, a.ctypes.data, d.nbytes, stream_ptr(stream))
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype, stream=None):
+        d = cls(shape, dtype)
+        call("pat_memset_dev", d.ptr, 0, d.nbytes, stream_ptr(stream))
+        call("pat_stream_sync", stream_ptr(stream))
+        return d
+
+    def numpy(self, stream=None):
+        out = np.empty(self.shape, dtype=self.dtype)
+        call("pat_memcpy_d2h", out.ctypes.data, self.ptr, self.nbytes, stream_ptr(stream))
+        return out
+
+    def copy_from(self, other, stream=None):
+        assert other.nbytes == self.nbytes
+        call("pat_memcpy_d2d", self.ptr, other.ptr, self.nbytes, stream_ptr(stream))
+
+    def clone(self, stream=None):
+        d = DeviceArray(self.shape, self.dtype)
+        d.copy_from(self, stream)
+        call("pat_stream_sync", stream_ptr(stream))
+        return d
+
+    def free(self):
+        if self.ptr:
+            call("pat_dev_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        h = c_vp()
+        call("pat_stream_create", ctypes.byref(h))
+        self.h = h.value
+
+    def sync(self):
+        call("pat_stream_sync", self.h)
+
+    def __del__(self):
+        try:
+            call("pat_stream_destroy", self.h)
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events recorded on the stream the timed kernels run on."""
+
+    def __init__(self, stream=None):
+        self.stream = stream_ptr(stream)
+        a, b = c_vp(), c_vp()
+        call("pat_event_create", ctypes.byref(a))
+        call("pat_event_create", ctypes.byref(b))
+        self.a, self.b = a.value, b.value
+
+    def start(self):
+        call("pat_event_record", self.a, self.stream)
+
+    def stop(self):
+        call("pat_event_record", self.b, self.stream)
+        ms = c_float(0)
+        call("pat_event_elapsed_ms", self.a, self.b, ctypes.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            call("pat_event_destroy", self.a)
+            call("pat_event_destroy", self.b)
+        except Exception:
+            pass
+
+
+def sync(stream=None):
+    call("pat_stream_sync", stream_ptr(stream))
+
+
+def device_query():
+    cus, tot, fr = c_int(0), c_i64(0), c_i64(0)
+    call("pat_device_query", ctypes.byref(cus), ctypes.byref(tot), ctypes.byref(fr))
+    return cus.value, tot.value, fr.value
 
 
 # ---------------------------------------------------------------- generators

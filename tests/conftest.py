@@ -29,9 +29,9 @@ def orc():
 
 @pytest.fixture(scope="session")
 def gpu(pa):
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    torch.cuda.set_device(0)
-    pa.call("pat_init", 0)
-    return torch
+    """Initialises device 0 through the C ABI.  Skips only on a machine with
+    no GPU driver at all; on a GPU machine a failing init is an error."""
+    if not os.path.exists("/dev/kfd"):
+        pytest.skip("no GPU driver on this machine")
+    pa.init(0)
+    return pa
