@@ -138,12 +138,97 @@ int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
 int pat_stencil_upload(pat_stencil *s, const void *host);
 /* StencilBegin for nsteps iterations (kmeans/lib/stencil_runtime.cu:576-614). */
 int pat_stencil_run(pat_stencil *s, int nsteps);
+/* nsteps sweeps launched one by one with HIP events around each kernel:
+ * total_ms = first kernel start to last kernel end, kernel_ms[k] = sweep k
+ * (array of nsteps, may be NULL).  Blocking. */
+int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *kernel_ms);
 /* Copies the current halo-inclusive grid out. */
 int pat_stencil_download(pat_stencil *s, void *host);
 /* Device pointer of the current grid. */
 int pat_stencil_device_ptr(pat_stencil *s, void **dptr);
 /* StencilFinalize + delete. */
 int pat_stencil_destroy(pat_stencil *s);
+
+/* ------------------------------------------------------------------------
+ * Irregular-reduction engine (replaces irregular compute_gpu,
+ * kmeans/irregular/gpu_kernel.cu:21-103, with the LDS/device hash reduction
+ * objects ros.cu/rog.cu, driven by IrregularRuntime,
+ * kmeans/irregular/irregular_runtime.h:12-112)
+ * ---------------------------------------------------------------------- */
+
+/* Edge functor indices (map_function_table). */
+#define PAT_IR_LJ_MINIMAGE 0 /* moldyn map, kmeans/irregular.h:26-107: minimum image;
+                                param = struct para {double side; double cutoffRadius;} */
+#define PAT_IR_LJ_OPEN 1     /* miniMD map1, miniMD/irregular.h:109-163: no images;
+                                param = double cutforcesq */
+/* Kernel variants. */
+#define PAT_IR_CSR 0    /* by-destination CSR, wavefront-segmented row sums (default) */
+#define PAT_IR_ATOMIC 1 /* edge-parallel, float atomics into both endpoints */
+
+/* Bytes of device workspace pat_ir_csr_build_dev needs. */
+int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes);
+/* Reorders a half edge list (int32 pairs i, j) once into a CSR by
+ * destination: row r lists every neighbour of r (both directions), sorted.
+ * rowptr: num_nodes+1, cols: 2*num_edges.  Replaces the host Partitioner /
+ * Reorder (kmeans/irregular/partitioner.cpp:20-499, reorder.cpp:13-281). */
+int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int32_t *rowptr, int32_t *cols,
+                         void *work, int64_t work_bytes, void *stream);
+/* LJ forces from float4 positions (x, y, z, unused) into float4 forces over
+ * the CSR; lanes_per_row in {4, 8, 16, 32, 64}; pbc selects the minimum
+ * image in a cubic box of side 'box'. */
+int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
+                         int pbc, float cutoff, int lanes_per_row, float *force4, void *stream);
+/* The same forces from the half edge list with float atomics (zeroes force4
+ * first). */
+int pat_lj_force_atomic_dev(const float *pos4, const int32_t *edges, int64_t num_edges, int64_t num_nodes, float box,
+                            int pbc, float cutoff, float *force4, void *stream);
+
+typedef struct pat_ir pat_ir;
+/* IrregularRuntime(num_edges, num_nodes, edges, ...) + IrregularInit
+ * (irregular_runtime.h:71-111, irregular_runtime.cu:17-193): uploads the
+ * half edge list and builds the CSR on the device.  The caller's arrays are
+ * not modified (the reference permutes them in place). */
+int pat_ir_create(int64_t num_edges, int64_t num_nodes, const int32_t *edges, int map_idx, const void *param,
+                  int64_t param_size, pat_ir **out);
+/* set_map_idx (irregular_runtime.h:110). */
+int pat_ir_set_map_idx(pat_ir *h, int idx);
+/* The functor parameter (the runtime's 'parameter' argument). */
+int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size);
+int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row);
+/* reset_node_data (irregular_runtime.cu:427-439): node coordinates,
+ * node_elm_size 12 (float3) or 24 (double3, moldyn's point_data). */
+int pat_ir_reset_node_data(pat_ir *h, const void *node_data, int node_elm_size);
+/* IrregularStart (irregular_runtime.cu:311-391): one reduction pass. */
+int pat_ir_start(pat_ir *h);
+/* get_reduction_result (irregular_runtime.cu:393-411): N float3 values
+ * (VALUE, kmeans/irregular/parameters.h:19-28) in the caller's node order. */
+int pat_ir_get_result(pat_ir *h, float *out);
+int pat_ir_destroy(pat_ir *h);
+
+/* Device-resident molecular dynamics around the LJ reduction: the moldyn
+ * and miniMD drivers' step loops (kmeans/moldyn.cu:331-403,
+ * miniMD/ljs.cu:171-184) as velocity-Verlet with unit mass:
+ * v += dt/2 f; x += dt v; wrap; f = F(x); v += dt/2 f.  The CSR variant
+ * fuses the update into the force kernel; run() replays a captured
+ * hipGraph.  pos, vel: N x 3 FP32 (vel may be NULL = 0). */
+typedef struct pat_md pat_md;
+int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t num_edges, const int32_t *edges,
+                  double box, int pbc, double cutoff, double dt, pat_md **out);
+int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row);
+/* nsteps steps, blocking. */
+int pat_md_run(pat_md *h, int nsteps);
+/* nsteps steps queued on 'stream' (NULL = the handle's stream), not waited for. */
+int pat_md_run_async(pat_md *h, int nsteps, void *stream);
+/* nsteps steps launched one by one on the handle's stream with HIP events
+ * around each force kernel: total_ms = whole run, kernel_ms[k] = step k's
+ * force kernel (array of nsteps, may be NULL).  Blocking. */
+int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms);
+int pat_md_get(pat_md *h, float *pos, float *vel, float *force);
+/* Potential energy sum_pairs 4(r^-12 - r^-6) and kinetic energy sum v^2/2. */
+int pat_md_energy(pat_md *h, double *epot, double *ekin);
+/* Current device buffers (float4 per node). */
+int pat_md_device_buffers(pat_md *h, void **pos4, void **vel4, void **force4);
+int pat_md_destroy(pat_md *h);
 
 #ifdef __cplusplus
 }

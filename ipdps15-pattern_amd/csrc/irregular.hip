@@ -1,0 +1,474 @@
+// Irregular-reduction engine for gfx950: the Lennard-Jones edge loop of the
+// moldyn and miniMD workloads, reducing per-edge forces into node arrays.
+//
+// Replaces the reference's irregular compute_gpu (kmeans/irregular/
+// gpu_kernel.cu:21-103): 14 persistent blocks, each grabbing one LDS-sized
+// partition of edges (built on the host by Partitioner/Reorder,
+// kmeans/irregular/partitioner.cpp:20-499, reorder.cpp:13-281), calling the
+// edge functor (kmeans/irregular.h:26-107) which inserts +F and -F into an
+// LDS open-addressing hash (ros.cu:10-68), then merging the hash into a
+// device hash (rog.cu:11-100).
+//
+// Here the edges are reordered once on the device into a by-destination CSR
+// (every half edge (i, j) appears in row i and in row j): a degree histogram,
+// an exclusive scan, a scatter, and a per-row sort that makes the layout --
+// and so every later force sum -- deterministic.  The force kernel then gives
+// each row to a group of LPR lanes of a wavefront: the lanes stride the row's
+// neighbour list, each computing the pair force with exactly the functor's
+// arithmetic, and the group reduces its partial sums with cross-lane
+// butterflies, so a node's update is a wavefront-segmented reduction with no
+// atomics and one coalesced write per node.  The velocity-Verlet update is
+// fused into that write.  The plain edge-parallel kernel with float atomics
+// (the shape of the reference's scatter) is kept as the baseline.
+//
+// Pair arithmetic (per pair, identical in the oracle): d = x_i - x_j,
+// minimum image (moldyn; miniMD's map1 has none), rd = (dx^2 + dy^2) + dz^2,
+// for rd < rc^2: rrd = 1/rd, rrd2 = rrd^2, rrd4 = rrd2^2, rrd6 = rrd2*rrd4,
+// rrd7 = rrd6*rrd, F = 48*(rrd7 - 0.5*rrd4) (the 48*eps factor the
+// reference folds into its integrator, kmeans/moldyn.cu:396-398), force on
+// i = d*F.  For the reversed pair the displacement is exactly -d, so each
+// row entry is exactly the negation the serial scatter applies.
+#include "pat_common.hpp"
+#include "pat_device.hpp"
+
+namespace {
+
+using pat::kWave;
+
+// ------------------------------------------------------------------ CSR build
+
+__global__ void csr_degree(const int2 *__restrict__ edges, int64_t E, int *__restrict__ deg)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        int2 ij = edges[e];
+        atomicAdd(&deg[ij.x], 1);
+        atomicAdd(&deg[ij.y], 1);
+    }
+}
+
+constexpr int kScanBlock = 1024;
+
+// Block-level exclusive scan of in[0..n) into out, block totals into sums.
+__global__ __launch_bounds__(kScanBlock) void scan_blocks(const int *__restrict__ in, int *__restrict__ out, int64_t n,
+                                                          int *__restrict__ sums)
+{
+    __shared__ int wsum[kScanBlock / kWave];
+    const int64_t g = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int v = g < n ? in[g] : 0;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < kScanBlock / kWave ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            int y = __shfl_up(s, o, kWave);
+            if (lane >= o) s += y;
+        }
+        if (lane < kScanBlock / kWave) wsum[lane] = s;
+    }
+    __syncthreads();
+    const int base = w > 0 ? wsum[w - 1] : 0;
+    if (g < n) out[g] = base + x - v;  // exclusive
+    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = base + x;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add(int *__restrict__ out, int64_t n, const int *__restrict__ offs)
+{
+    const int64_t g = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    if (g < n) out[g] += offs[blockIdx.x];
+}
+
+__global__ void csr_scatter(const int2 *__restrict__ edges, int64_t E, int *__restrict__ cursor, int *__restrict__ tmp)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        int2 ij = edges[e];
+        int p = atomicAdd(&cursor[ij.x], 1);
+        tmp[p] = ij.y;
+        int q = atomicAdd(&cursor[ij.y], 1);
+        tmp[q] = ij.x;
+    }
+}
+
+// One wavefront per row: rank sort of the row's columns (ties broken by
+// position), writing the sorted row from tmp into cols.
+__global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowptr, int64_t N, const int *__restrict__ tmp,
+                                                    int *__restrict__ cols)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= N) return;
+    const int b = rowptr[row], e = rowptr[row + 1];
+    for (int k = b + lane; k < e; k += kWave) {
+        const int v = tmp[k];
+        int rank = 0;
+        for (int m = b; m < e; m++) {
+            const int u = tmp[m];
+            rank += (u < v) || (u == v && m < k);
+        }
+        cols[b + rank] = v;
+    }
+}
+
+// ------------------------------------------------------------------ force
+
+struct LJParams {
+    float L, hL;   // box side and half side (pbc)
+    float rc2;     // cutoff squared
+    float dt, hdt; // time step and half step (hdt = 0.5f*dt rounded once)
+    int pbc;
+};
+
+// Force on i from j (both float4 positions); returns 0 beyond the cutoff.
+__device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams &p, float &fx, float &fy, float &fz)
+{
+    float dx = pi.x - pj.x;
+    float dy = pi.y - pj.y;
+    float dz = pi.z - pj.z;
+    if (p.pbc) {
+        dx = pat::min_image(dx, p.L, p.hL);
+        dy = pat::min_image(dy, p.L, p.hL);
+        dz = pat::min_image(dz, p.L, p.hL);
+    }
+    float rd = dx * dx + dy * dy;
+    rd = rd + dz * dz;
+    if (rd < p.rc2) {
+        const float rrd = 1.0f / rd;
+        const float rrd2 = rrd * rrd;
+        const float rrd4 = rrd2 * rrd2;
+        const float rrd6 = rrd2 * rrd4;
+        const float rrd7 = rrd6 * rrd;
+        const float F = 48.0f * (rrd7 - 0.5f * rrd4);
+        fx = dx * F;
+        fy = dy * F;
+        fz = dz * F;
+    } else {
+        fx = fy = fz = 0.0f;
+    }
+}
+
+// Potential of one pair (4(r^-12 - r^-6)), 0 beyond the cutoff.
+__device__ __forceinline__ float pair_phi(float4 pi, float4 pj, const LJParams &p)
+{
+    float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    if (p.pbc) {
+        dx = pat::min_image(dx, p.L, p.hL);
+        dy = pat::min_image(dy, p.L, p.hL);
+        dz = pat::min_image(dz, p.L, p.hL);
+    }
+    float rd = dx * dx + dy * dy;
+    rd = rd + dz * dz;
+    if (!(rd < p.rc2)) return 0.0f;
+    const float rrd = 1.0f / rd;
+    const float rrd2 = rrd * rrd;
+    const float rrd3 = rrd2 * rrd;
+    const float rrd4 = rrd2 * rrd2;
+    const float rrd6 = rrd2 * rrd4;
+    return 4.0f * (rrd6 - rrd3);
+}
+
+// Epilogue modes of the CSR force kernel.
+enum : int {
+    kForceOnly = 0,   // write f
+    kKick = 1,        // f; v += hdt*f
+    kKickDrift = 2,   // f; v += hdt*f; v += hdt*f; x_out = wrap(x + dt*v)
+};
+
+// Rows are handled by groups of LPR lanes; a 256-thread block holds
+// 256/LPR rows.  Rows are dealt to blocks XCD-contiguously.
+template <int LPR>
+__global__ __launch_bounds__(256) void lj_csr(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                              const int *__restrict__ cols, int N, LJParams p, float4 *__restrict__ force,
+                                              float4 *__restrict__ vel, float4 *__restrict__ pos_out, int mode)
+{
+    constexpr int RPB = 256 / LPR;
+    const int blk = pat::xcd_swizzle(blockIdx.x, gridDim.x);
+    const int row = blk * RPB + threadIdx.x / LPR;
+    const int l = threadIdx.x % LPR;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < N) {
+        pi = pos[row];
+        const int b = rowptr[row], e = rowptr[row + 1];
+        for (int k = b + l; k < e; k += LPR) {
+            const float4 pj = pos[cols[k]];
+            float ax, ay, az;
+            pair_force(pi, pj, p, ax, ay, az);
+            fx = fx + ax;
+            fy = fy + ay;
+            fz = fz + az;
+        }
+    }
+    // butterfly reduction inside the LPR-lane group
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        fx = fx + __shfl_xor(fx, o, LPR);
+        fy = fy + __shfl_xor(fy, o, LPR);
+        fz = fz + __shfl_xor(fz, o, LPR);
+    }
+    if (row < N && l == 0) {
+        force[row] = make_float4(fx, fy, fz, 0.0f);
+        if (mode != kForceOnly) {
+            float4 v = vel[row];
+            v.x = v.x + p.hdt * fx;
+            v.y = v.y + p.hdt * fy;
+            v.z = v.z + p.hdt * fz;
+            if (mode == kKickDrift) {
+                v.x = v.x + p.hdt * fx;
+                v.y = v.y + p.hdt * fy;
+                v.z = v.z + p.hdt * fz;
+                float4 x;
+                x.x = pi.x + p.dt * v.x;
+                x.y = pi.y + p.dt * v.y;
+                x.z = pi.z + p.dt * v.z;
+                x.w = 0.0f;
+                if (p.pbc) {
+                    x.x = pat::wrap_coord(x.x, p.L);
+                    x.y = pat::wrap_coord(x.y, p.L);
+                    x.z = pat::wrap_coord(x.z, p.L);
+                }
+                pos_out[row] = x;
+            }
+            vel[row] = v;
+        }
+    }
+}
+
+// Per-node half potential 0.5*sum_j phi(i, j) over the full CSR row.
+__global__ __launch_bounds__(256) void lj_csr_energy(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ cols, int N, LJParams p,
+                                                     double *__restrict__ node_e)
+{
+    const int row = blockIdx.x * 4 + threadIdx.x / 64;
+    const int l = threadIdx.x % 64;
+    double acc = 0.0;
+    if (row < N) {
+        const float4 pi = pos[row];
+        for (int k = rowptr[row] + l; k < rowptr[row + 1]; k += 64) acc += (double)pair_phi(pi, pos[cols[k]], p);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (row < N && l == 0) node_e[row] = 0.5 * acc;
+}
+
+// Plain baseline: one thread per half edge, float atomics into both nodes.
+__global__ __launch_bounds__(256) void lj_atomic(const float4 *__restrict__ pos, const int2 *__restrict__ edges,
+                                                 int64_t E, LJParams p, float4 *__restrict__ force)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        const int2 ij = edges[e];
+        float fx, fy, fz;
+        pair_force(pos[ij.x], pos[ij.y], p, fx, fy, fz);
+        if (fx != 0.0f || fy != 0.0f || fz != 0.0f) {
+            float *fi = reinterpret_cast<float *>(force + ij.x);
+            float *fj = reinterpret_cast<float *>(force + ij.y);
+            atomicAdd(fi + 0, fx);
+            atomicAdd(fi + 1, fy);
+            atomicAdd(fi + 2, fz);
+            atomicAdd(fj + 0, -fx);
+            atomicAdd(fj + 1, -fy);
+            atomicAdd(fj + 2, -fz);
+        }
+    }
+}
+
+__global__ void fill_zero4(float4 *__restrict__ a, int N)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Node-wise velocity-Verlet pieces for the plain path.
+__global__ void vv_kick_drift(float4 *__restrict__ pos, float4 *__restrict__ vel, const float4 *__restrict__ force,
+                              int N, LJParams p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 v = vel[i], f = force[i], x = pos[i];
+    v.x = v.x + p.hdt * f.x;
+    v.y = v.y + p.hdt * f.y;
+    v.z = v.z + p.hdt * f.z;
+    x.x = x.x + p.dt * v.x;
+    x.y = x.y + p.dt * v.y;
+    x.z = x.z + p.dt * v.z;
+    if (p.pbc) {
+        x.x = pat::wrap_coord(x.x, p.L);
+        x.y = pat::wrap_coord(x.y, p.L);
+        x.z = pat::wrap_coord(x.z, p.L);
+    }
+    vel[i] = v;
+    pos[i] = x;
+}
+
+__global__ void vv_kick(float4 *__restrict__ vel, const float4 *__restrict__ force, int N, LJParams p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 v = vel[i], f = force[i];
+    v.x = v.x + p.hdt * f.x;
+    v.y = v.y + p.hdt * f.y;
+    v.z = v.z + p.hdt * f.z;
+    vel[i] = v;
+}
+
+}  // namespace
+
+namespace pat {
+
+struct LJConfig {
+    float box;
+    int pbc;
+    float rc2;
+    float dt;
+};
+
+static LJParams make_params(float box, int pbc, float rc2, float dt)
+{
+    LJParams p;
+    p.L = box;
+    p.hL = 0.5f * box;
+    p.rc2 = rc2;
+    p.dt = dt;
+    p.hdt = 0.5f * dt;
+    p.pbc = pbc;
+    return p;
+}
+
+int64_t csr_workspace_bytes(int64_t N, int64_t E)
+{
+    const int64_t nb = (N + 1 + kScanBlock - 1) / kScanBlock;
+    const int64_t nb2 = (nb + kScanBlock - 1) / kScanBlock;
+    // deg/cursor (N+1), block sums (nb), their scan (nb), second level (nb2 x 2), tmp cols (2E)
+    return (int64_t)sizeof(int) * ((N + 1) + 2 * nb + 2 * nb2 + 2 + 2 * E) + 256;
+}
+
+// Exclusive scan of n ints (n <= kScanBlock^3) using ws for block sums.
+static int scan_exclusive(const int *in, int *out, int64_t n, int *ws, hipStream_t st)
+{
+    const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    int *sums = ws, *sums_scan = ws + nb;
+    hipLaunchKernelGGL(scan_blocks, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, out, n, sums);
+    PAT_LAUNCH_CHECK();
+    if (nb > 1) {
+        const int64_t nb2 = (nb + kScanBlock - 1) / kScanBlock;
+        PAT_REQUIRE(nb2 <= kScanBlock, PAT_ERR_INVALID, "scan: %lld elements exceed the two-level scan",
+                    (long long)n);
+        int *s2 = sums_scan + nb, *s2s = s2 + nb2;
+        hipLaunchKernelGGL(scan_blocks, dim3((unsigned)nb2), dim3(kScanBlock), 0, st, sums, sums_scan, nb, s2);
+        PAT_LAUNCH_CHECK();
+        if (nb2 > 1) {
+            hipLaunchKernelGGL(scan_blocks, dim3(1), dim3(kScanBlock), 0, st, s2, s2s, nb2, s2s + nb2);
+            PAT_LAUNCH_CHECK();
+            hipLaunchKernelGGL(scan_add, dim3((unsigned)nb2), dim3(kScanBlock), 0, st, sums_scan, nb, s2s);
+            PAT_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(scan_add, dim3((unsigned)nb), dim3(kScanBlock), 0, st, out, n, sums_scan);
+        PAT_LAUNCH_CHECK();
+    }
+    return PAT_OK;
+}
+
+int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *cols, void *work,
+              int64_t work_bytes, hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1 && E >= 0 && 2 * E < (1LL << 31), PAT_ERR_INVALID,
+                "csr_build: sizes out of range (N=%lld, E=%lld)", (long long)N, (long long)E);
+    PAT_REQUIRE(work_bytes >= csr_workspace_bytes(N, E), PAT_ERR_INVALID, "csr_build: workspace too small");
+    const int64_t nb = (N + 1 + kScanBlock - 1) / kScanBlock;
+    int *deg = (int *)work;
+    int *ws = deg + (N + 1);
+    int *tmp = ws + 2 * nb + 2 * ((nb + kScanBlock - 1) / kScanBlock) + 2;
+    PAT_HIP(hipMemsetAsync(deg, 0, sizeof(int) * (N + 1), st));
+    const unsigned grid = (unsigned)std::min<int64_t>((E + 255) / 256 + 1, 65536);
+    if (E > 0) {
+        hipLaunchKernelGGL(csr_degree, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg);
+        PAT_LAUNCH_CHECK();
+    }
+    PAT_TRY(scan_exclusive(deg, rowptr, N + 1, ws, st));
+    // cursor = rowptr (reuse deg)
+    PAT_HIP(hipMemcpyAsync(deg, rowptr, sizeof(int) * (N + 1), hipMemcpyDeviceToDevice, st));
+    if (E > 0) {
+        hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp);
+        PAT_LAUNCH_CHECK();
+        const unsigned g2 = (unsigned)((N * kWave + 255) / 256);
+        hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols);
+        PAT_LAUNCH_CHECK();
+    }
+    return PAT_OK;
+}
+
+using CsrKernel = void (*)(const float4 *, const int *, const int *, int, LJParams, float4 *, float4 *, float4 *, int);
+
+static CsrKernel csr_kernel_for(int lpr)
+{
+    switch (lpr) {
+    case 4: return lj_csr<4>;
+    case 8: return lj_csr<8>;
+    case 16: return lj_csr<16>;
+    case 32: return lj_csr<32>;
+    case 64: return lj_csr<64>;
+    default: return nullptr;
+    }
+}
+
+int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
+                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out, hipStream_t st)
+{
+    CsrKernel k = csr_kernel_for(lpr);
+    PAT_REQUIRE(k, PAT_ERR_UNSUPPORTED, "lj_force_csr: lanes per row %d not in {4,8,16,32,64}", lpr);
+    PAT_REQUIRE(N > 0 && N < (1LL << 31), PAT_ERR_INVALID, "lj_force_csr: bad node count");
+    const int rpb = 256 / lpr;
+    const int64_t blocks = (N + rpb - 1) / rpb;
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, st, pos, (const int *)rowptr, (const int *)cols, (int)N,
+                       make_params(box, pbc, rc2, dt), force, vel, pos_out, mode);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int lj_energy_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
+                  float rc2, double *node_e, hipStream_t st)
+{
+    hipLaunchKernelGGL(lj_csr_energy, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, pos, (const int *)rowptr,
+                       (const int *)cols, (int)N, make_params(box, pbc, rc2, 0.f), node_e);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t N, float box, int pbc, float rc2,
+                    float4 *force, hipStream_t st)
+{
+    hipLaunchKernelGGL(fill_zero4, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, force, (int)N);
+    PAT_LAUNCH_CHECK();
+    if (E > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>((E + 255) / 256, 1 << 20);
+        hipLaunchKernelGGL(lj_atomic, dim3(grid), dim3(256), 0, st, pos, (const int2 *)edges, E,
+                           make_params(box, pbc, rc2, 0.f), force);
+        PAT_LAUNCH_CHECK();
+    }
+    return PAT_OK;
+}
+
+int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(vv_kick_drift, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, pos, vel, force, (int)N,
+                       make_params(box, pbc, 0.f, dt));
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st)
+{
+    hipLaunchKernelGGL(vv_kick, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, vel, force, (int)N,
+                       make_params(1.f, 0, 0.f, dt));
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+}  // namespace pat

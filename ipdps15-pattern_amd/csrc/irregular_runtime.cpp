@@ -1,0 +1,562 @@
+// Handle-level irregular runtime: the device-resident form of
+// IrregularRuntime (kmeans/irregular/irregular_runtime.h:12-112) plus the
+// fused molecular-dynamics loop the moldyn/miniMD drivers run around it.
+//
+// The reference's IrregularInit permutes the caller's coordinates, edges and
+// node data in place (irregular_runtime.cu:57-69), re-partitions on the host
+// at iterations 0 and 1, exchanges halo nodes through MPI every step and
+// copies the device hash back for the host to integrate.  Here the caller's
+// arrays are never modified: edges are uploaded once and turned into a
+// by-destination CSR on the device, node data stays resident, and results
+// come back in the caller's node order.
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "pat_common.hpp"
+
+namespace pat {
+int64_t csr_workspace_bytes(int64_t N, int64_t E);
+int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *cols, void *work,
+              int64_t work_bytes, hipStream_t st);
+int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
+                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out,
+                 hipStream_t st);
+int lj_energy_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
+                  float rc2, double *node_e, hipStream_t st);
+int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t N, float box, int pbc, float rc2,
+                    float4 *force, hipStream_t st);
+int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
+                      hipStream_t st);
+int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
+}  // namespace pat
+
+namespace {
+
+constexpr int kForceOnly = 0, kKick = 1, kKickDrift = 2;
+
+// Default lanes per CSR row: the smallest power of two covering about half
+// the mean degree, within [4, 32].
+int default_lpr(int64_t N, int64_t E)
+{
+    double mean = N > 0 ? 2.0 * (double)E / (double)N : 0.0;
+    int l = 4;
+    while (l < 32 && l < mean / 4.0) l *= 2;
+    return l;
+}
+
+// Device state shared by the IrregularRuntime handle and the MD handle.
+struct IrState {
+    int64_t N = 0, E = 0;
+    int32_t *edges = nullptr;   // half list (plain kernel)
+    int32_t *rowptr = nullptr;  // CSR by destination
+    int32_t *cols = nullptr;
+    float *pos[2] = {nullptr, nullptr};  // float4 per node
+    float *vel = nullptr;
+    float *force = nullptr;
+    double *node_e = nullptr;
+    hipStream_t stream = nullptr;
+    int variant = PAT_IR_CSR;
+    int lpr = 0;
+
+    int alloc(int64_t n, int64_t e, const int32_t *host_edges, bool md)
+    {
+        N = n;
+        E = e;
+        PAT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        auto am = [](void **p, size_t b) -> int {
+            if (b == 0) b = 16;
+            if (hipMalloc(p, b) != hipSuccess) {
+                (void)hipGetLastError();
+                pat::set_error("irregular: hipMalloc of %zu bytes failed", b);
+                return PAT_ERR_NOMEM;
+            }
+            return PAT_OK;
+        };
+        PAT_TRY(am((void **)&edges, sizeof(int32_t) * 2 * (size_t)E));
+        PAT_TRY(am((void **)&rowptr, sizeof(int32_t) * (size_t)(N + 1)));
+        PAT_TRY(am((void **)&cols, sizeof(int32_t) * 2 * (size_t)E));
+        PAT_TRY(am((void **)&pos[0], sizeof(float) * 4 * (size_t)N));
+        PAT_TRY(am((void **)&force, sizeof(float) * 4 * (size_t)N));
+        PAT_TRY(am((void **)&node_e, sizeof(double) * (size_t)N));
+        if (md) {
+            PAT_TRY(am((void **)&pos[1], sizeof(float) * 4 * (size_t)N));
+            PAT_TRY(am((void **)&vel, sizeof(float) * 4 * (size_t)N));
+        }
+        if (E > 0) PAT_HIP(hipMemcpyAsync(edges, host_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, stream));
+        const int64_t wb = pat::csr_workspace_bytes(N, E);
+        void *work = nullptr;
+        PAT_TRY(am(&work, (size_t)wb));
+        int rc = pat::csr_build(edges, E, N, rowptr, cols, work, wb, stream);
+        hipError_t se = hipStreamSynchronize(stream);
+        (void)hipFree(work);
+        if (rc != PAT_OK) return rc;
+        PAT_HIP(se);
+        lpr = default_lpr(N, E);
+        return PAT_OK;
+    }
+
+    void release()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : {(void *)edges, (void *)rowptr, (void *)cols, (void *)pos[0], (void *)pos[1], (void *)vel,
+                        (void *)force, (void *)node_e})
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int upload3(float *dst4, const void *src, int elm)
+    {
+        std::vector<float> h(4 * (size_t)N);
+        if (elm == 12) {
+            const float *s = (const float *)src;
+            for (int64_t i = 0; i < N; i++) {
+                h[4 * i] = s[3 * i];
+                h[4 * i + 1] = s[3 * i + 1];
+                h[4 * i + 2] = s[3 * i + 2];
+                h[4 * i + 3] = 0.0f;
+            }
+        } else if (elm == 24) {
+            const double *s = (const double *)src;
+            for (int64_t i = 0; i < N; i++) {
+                h[4 * i] = (float)s[3 * i];
+                h[4 * i + 1] = (float)s[3 * i + 1];
+                h[4 * i + 2] = (float)s[3 * i + 2];
+                h[4 * i + 3] = 0.0f;
+            }
+        } else {
+            pat::set_error("irregular: node element size %d not supported (12 = float3, 24 = double3)", elm);
+            return PAT_ERR_UNSUPPORTED;
+        }
+        PAT_HIP(hipMemcpyAsync(dst4, h.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice, stream));
+        PAT_HIP(hipStreamSynchronize(stream));
+        return PAT_OK;
+    }
+
+    int download3(float *dst, const float *src4)
+    {
+        std::vector<float> h(4 * (size_t)N);
+        PAT_HIP(hipMemcpyAsync(h.data(), src4, sizeof(float) * 4 * N, hipMemcpyDeviceToHost, stream));
+        PAT_HIP(hipStreamSynchronize(stream));
+        for (int64_t i = 0; i < N; i++) {
+            dst[3 * i] = h[4 * i];
+            dst[3 * i + 1] = h[4 * i + 1];
+            dst[3 * i + 2] = h[4 * i + 2];
+        }
+        return PAT_OK;
+    }
+
+    int force_eval(const float *p4, float box, int pbc, float rc2)
+    {
+        if (variant == PAT_IR_ATOMIC)
+            return pat::lj_force_atomic((const float4 *)p4, edges, E, N, box, pbc, rc2, (float4 *)force, stream);
+        return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, lpr, kForceOnly,
+                                 (float4 *)force, nullptr, nullptr, stream);
+    }
+};
+
+int check_sizes(int64_t num_edges, int64_t num_nodes, const int32_t *edges)
+{
+    PAT_REQUIRE(num_nodes > 0 && num_nodes < (1LL << 31) - 1, PAT_ERR_INVALID, "irregular: node count %lld out of range",
+                (long long)num_nodes);
+    PAT_REQUIRE(num_edges >= 0 && 2 * num_edges < (1LL << 31), PAT_ERR_INVALID,
+                "irregular: edge count %lld out of range", (long long)num_edges);
+    PAT_REQUIRE(num_edges == 0 || edges, PAT_ERR_INVALID, "irregular: null edge list");
+    for (int64_t e = 0; e < 2 * num_edges; e++)
+        PAT_REQUIRE(edges[e] >= 0 && edges[e] < num_nodes, PAT_ERR_INVALID,
+                    "irregular: edge %lld references node %d outside [0, %lld)", (long long)(e / 2), edges[e],
+                    (long long)num_nodes);
+    return PAT_OK;
+}
+
+}  // namespace
+
+struct pat_ir {
+    IrState st;
+    int map_idx = PAT_IR_LJ_MINIMAGE;
+    float box = 0.f;
+    int pbc = 1;
+    float rc2 = 0.f;
+    bool have_nodes = false;
+    bool have_result = false;
+};
+
+struct pat_md {
+    IrState st;
+    float box = 0.f;
+    int pbc = 1;
+    float rc2 = 0.f;
+    float dt = 0.f;
+    int cur = 0;  // position buffer holding the current coordinates
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+
+    void drop_graphs()
+    {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
+
+    // kick-drift from the stored force, then nsteps force evaluations; the
+    // last step ends with the closing half kick.
+    int enqueue(int nsteps, int start)
+    {
+        hipStream_t s = st.stream;
+        if (st.variant == PAT_IR_ATOMIC) {
+            for (int k = 0; k < nsteps; k++) {
+                PAT_TRY(pat::vv_kick_drift_dev((float4 *)st.pos[start], (float4 *)st.vel, (const float4 *)st.force,
+                                               st.N, box, pbc, dt, s));
+                PAT_TRY(pat::lj_force_atomic((const float4 *)st.pos[start], st.edges, st.E, st.N, box, pbc, rc2,
+                                             (float4 *)st.force, s));
+                PAT_TRY(pat::vv_kick_dev((float4 *)st.vel, (const float4 *)st.force, st.N, dt, s));
+            }
+            return PAT_OK;
+        }
+        PAT_TRY(pat::vv_kick_drift_dev((float4 *)st.pos[start], (float4 *)st.vel, (const float4 *)st.force, st.N,
+                                       box, pbc, dt, s));
+        int c = start;
+        for (int k = 0; k < nsteps; k++) {
+            const bool last = k == nsteps - 1;
+            PAT_TRY(pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.lpr,
+                                      last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
+                                      (float4 *)st.pos[1 - c], s));
+            if (!last) c = 1 - c;
+        }
+        return PAT_OK;
+    }
+};
+
+extern "C" {
+
+// ------------------------------------------------------------ device level
+
+int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes)
+{
+    PAT_REQUIRE(bytes && num_nodes > 0 && num_edges >= 0, PAT_ERR_INVALID, "pat_ir_csr_workspace: bad arguments");
+    *bytes = pat::csr_workspace_bytes(num_nodes, num_edges);
+    return PAT_OK;
+}
+
+int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int32_t *rowptr, int32_t *cols,
+                         void *work, int64_t work_bytes, void *stream)
+{
+    PAT_REQUIRE(rowptr && (num_edges == 0 || (edges && cols)) && work, PAT_ERR_INVALID,
+                "pat_ir_csr_build_dev: null argument");
+    return pat::csr_build(edges, num_edges, num_nodes, rowptr, cols, work, work_bytes, (hipStream_t)stream);
+}
+
+int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
+                         int pbc, float cutoff, int lanes_per_row, float *force4, void *stream)
+{
+    PAT_REQUIRE(pos4 && rowptr && cols && force4, PAT_ERR_INVALID, "pat_lj_force_csr_dev: null argument");
+    PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_csr_dev: box must exceed twice the cutoff");
+    return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc, cutoff * cutoff, 0.f,
+                             lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pat_lj_force_atomic_dev(const float *pos4, const int32_t *edges, int64_t num_edges, int64_t num_nodes, float box,
+                            int pbc, float cutoff, float *force4, void *stream)
+{
+    PAT_REQUIRE(pos4 && force4 && (num_edges == 0 || edges), PAT_ERR_INVALID, "pat_lj_force_atomic_dev: null argument");
+    PAT_REQUIRE(num_nodes > 0 && num_nodes < (1LL << 31), PAT_ERR_INVALID, "pat_lj_force_atomic_dev: bad node count");
+    PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_atomic_dev: box must exceed twice the cutoff");
+    return pat::lj_force_atomic((const float4 *)pos4, edges, num_edges, num_nodes, box, pbc, cutoff * cutoff,
+                                (float4 *)force4, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------- IrregularRuntime
+
+int pat_ir_create(int64_t num_edges, int64_t num_nodes, const int32_t *edges, int map_idx, const void *param,
+                  int64_t param_size, pat_ir **out)
+{
+    PAT_REQUIRE(out, PAT_ERR_INVALID, "pat_ir_create: null output");
+    PAT_TRY(check_sizes(num_edges, num_nodes, edges));
+    auto *h = new pat_ir();
+    int rc = pat_ir_set_map_idx(h, map_idx);
+    if (rc == PAT_OK) rc = pat_ir_set_param(h, param, param_size);
+    if (rc == PAT_OK) rc = h->st.alloc(num_nodes, num_edges, edges, false);
+    if (rc != PAT_OK) {
+        h->st.release();
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PAT_OK;
+}
+
+int pat_ir_set_map_idx(pat_ir *h, int idx)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_map_idx: null handle");
+    PAT_REQUIRE(idx == PAT_IR_LJ_MINIMAGE || idx == PAT_IR_LJ_OPEN, PAT_ERR_UNSUPPORTED,
+                "pat_ir_set_map_idx: functor index %d not in the table", idx);
+    h->map_idx = idx;
+    h->pbc = idx == PAT_IR_LJ_MINIMAGE;
+    return PAT_OK;
+}
+
+int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size)
+{
+    PAT_REQUIRE(h && param, PAT_ERR_INVALID, "pat_ir_set_param: null argument");
+    if (h->map_idx == PAT_IR_LJ_MINIMAGE) {
+        // struct para {double side; double cutoffRadius;} (kmeans/moldyn.h:39-43)
+        PAT_REQUIRE(param_size == 2 * (int64_t)sizeof(double), PAT_ERR_INVALID,
+                    "pat_ir_set_param: moldyn parameter is {double side, double cutoffRadius}");
+        double side, rc;
+        memcpy(&side, param, sizeof(double));
+        memcpy(&rc, (const char *)param + sizeof(double), sizeof(double));
+        PAT_REQUIRE(side > 2 * rc && rc > 0, PAT_ERR_INVALID, "pat_ir_set_param: box must exceed twice the cutoff");
+        h->box = (float)side;
+        h->rc2 = (float)rc * (float)rc;
+    } else {
+        // double cutforcesq (miniMD/irregular.h:111)
+        PAT_REQUIRE(param_size == (int64_t)sizeof(double), PAT_ERR_INVALID,
+                    "pat_ir_set_param: miniMD parameter is double cutforcesq");
+        double c2;
+        memcpy(&c2, param, sizeof(double));
+        PAT_REQUIRE(c2 > 0, PAT_ERR_INVALID, "pat_ir_set_param: cutoff must be positive");
+        h->box = 1.0f;
+        h->rc2 = (float)c2;
+    }
+    return PAT_OK;
+}
+
+int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_variant: null handle");
+    PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC, PAT_ERR_UNSUPPORTED,
+                "pat_ir_set_variant: unknown variant %d", variant);
+    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
+                    lanes_per_row == 32 || lanes_per_row == 64,
+                PAT_ERR_UNSUPPORTED, "pat_ir_set_variant: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
+    h->st.variant = variant;
+    h->st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->st.N, h->st.E);
+    return PAT_OK;
+}
+
+int pat_ir_reset_node_data(pat_ir *h, const void *node_data, int node_elm_size)
+{
+    PAT_REQUIRE(h && node_data, PAT_ERR_INVALID, "pat_ir_reset_node_data: null argument");
+    PAT_TRY(h->st.upload3(h->st.pos[0], node_data, node_elm_size));
+    h->have_nodes = true;
+    h->have_result = false;
+    return PAT_OK;
+}
+
+int pat_ir_start(pat_ir *h)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_start: null handle");
+    PAT_REQUIRE(h->have_nodes, PAT_ERR_STATE, "pat_ir_start: no node data (call pat_ir_reset_node_data)");
+    PAT_TRY(h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    h->have_result = true;
+    return PAT_OK;
+}
+
+int pat_ir_get_result(pat_ir *h, float *out)
+{
+    PAT_REQUIRE(h && out, PAT_ERR_INVALID, "pat_ir_get_result: null argument");
+    PAT_REQUIRE(h->have_result, PAT_ERR_STATE, "pat_ir_get_result: no result (call pat_ir_start)");
+    return h->st.download3(out, h->st.force);
+}
+
+int pat_ir_destroy(pat_ir *h)
+{
+    if (!h) return PAT_OK;
+    h->st.release();
+    delete h;
+    return PAT_OK;
+}
+
+// ------------------------------------------------------------- MD loop
+
+int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t num_edges, const int32_t *edges,
+                  double box, int pbc, double cutoff, double dt, pat_md **out)
+{
+    PAT_REQUIRE(out && pos, PAT_ERR_INVALID, "pat_md_create: null argument");
+    PAT_REQUIRE(cutoff > 0 && dt > 0 && (!pbc || box > 2 * cutoff), PAT_ERR_INVALID,
+                "pat_md_create: need cutoff > 0, dt > 0 and box > 2*cutoff");
+    PAT_TRY(check_sizes(num_edges, num_nodes, edges));
+    auto *h = new pat_md();
+    h->box = (float)box;
+    h->pbc = pbc ? 1 : 0;
+    h->rc2 = (float)cutoff * (float)cutoff;
+    h->dt = (float)dt;
+    int rc = h->st.alloc(num_nodes, num_edges, edges, true);
+    if (rc == PAT_OK) rc = h->st.upload3(h->st.pos[0], pos, 12);
+    if (rc == PAT_OK) {
+        std::vector<float> zero(3 * (size_t)num_nodes, 0.0f);
+        rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
+    }
+    if (rc == PAT_OK) rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
+    if (rc == PAT_OK && hipStreamSynchronize(h->st.stream) != hipSuccess) {
+        pat::set_error("pat_md_create: initial force evaluation failed");
+        rc = PAT_ERR_HIP;
+    }
+    if (rc != PAT_OK) {
+        h->st.release();
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PAT_OK;
+}
+
+int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_variant: null handle");
+    PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC, PAT_ERR_UNSUPPORTED,
+                "pat_md_set_variant: unknown variant %d", variant);
+    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
+                    lanes_per_row == 32 || lanes_per_row == 64,
+                PAT_ERR_UNSUPPORTED, "pat_md_set_variant: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
+    h->st.variant = variant;
+    h->st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->st.N, h->st.E);
+    h->drop_graphs();
+    return PAT_OK;
+}
+
+int pat_md_run(pat_md *h, int nsteps)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run: null handle");
+    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run: negative step count");
+    if (nsteps == 0) return PAT_OK;
+    // the atomic path updates positions in place; the CSR path ping-pongs
+    const int start = h->cur;
+    auto key = std::make_pair(nsteps, start);
+    auto it = h->graphs.find(key);
+    if (it == h->graphs.end()) {
+        hipGraph_t g;
+        PAT_HIP(hipStreamBeginCapture(h->st.stream, hipStreamCaptureModeThreadLocal));
+        int rc = h->enqueue(nsteps, start);
+        hipError_t ec = hipStreamEndCapture(h->st.stream, &g);
+        if (rc != PAT_OK) return rc;
+        PAT_HIP(ec);
+        hipGraphExec_t ge;
+        hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PAT_HIP(ei);
+        it = h->graphs.emplace(key, ge).first;
+    }
+    PAT_HIP(hipGraphLaunch(it->second, h->st.stream));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    return PAT_OK;
+}
+
+int pat_md_run_async(pat_md *h, int nsteps, void *stream)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run_async: null handle");
+    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run_async: negative step count");
+    if (nsteps == 0) return PAT_OK;
+    hipStream_t saved = h->st.stream;
+    h->st.stream = stream ? (hipStream_t)stream : saved;
+    int rc = h->enqueue(nsteps, h->cur);
+    h->st.stream = saved;
+    if (rc != PAT_OK) return rc;
+    if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    return PAT_OK;
+}
+
+int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
+{
+    PAT_REQUIRE(h && total_ms, PAT_ERR_INVALID, "pat_md_run_timed: null argument");
+    PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_md_run_timed: need at least one step");
+    hipStream_t s = h->st.stream;
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 2);
+    for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
+    auto cleanup = [&]() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    };
+    PAT_HIP(hipEventRecord(ev[0], s));
+    int c = h->cur;
+    int rc = PAT_OK;
+    if (h->st.variant == PAT_IR_ATOMIC) {
+        for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
+            rc = pat::vv_kick_drift_dev((float4 *)h->st.pos[c], (float4 *)h->st.vel, (const float4 *)h->st.force,
+                                        h->st.N, h->box, h->pbc, h->dt, s);
+            if (rc == PAT_OK) rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
+            if (rc == PAT_OK)
+                rc = pat::lj_force_atomic((const float4 *)h->st.pos[c], h->st.edges, h->st.E, h->st.N, h->box, h->pbc,
+                                          h->rc2, (float4 *)h->st.force, s);
+            if (rc == PAT_OK) rc = hipEventRecord(ev[2 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
+            if (rc == PAT_OK) rc = pat::vv_kick_dev((float4 *)h->st.vel, (const float4 *)h->st.force, h->st.N, h->dt, s);
+        }
+    } else {
+        rc = pat::vv_kick_drift_dev((float4 *)h->st.pos[c], (float4 *)h->st.vel, (const float4 *)h->st.force, h->st.N,
+                                    h->box, h->pbc, h->dt, s);
+        for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
+            const bool last = k == nsteps - 1;
+            rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
+            if (rc == PAT_OK)
+                rc = pat::lj_force_csr((const float4 *)h->st.pos[c], h->st.rowptr, h->st.cols, h->st.N, h->box, h->pbc,
+                                       h->rc2, h->dt, h->st.lpr, last ? kKick : kKickDrift, (float4 *)h->st.force,
+                                       (float4 *)h->st.vel, (float4 *)h->st.pos[1 - c], s);
+            if (rc == PAT_OK) rc = hipEventRecord(ev[2 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
+            if (!last) c = 1 - c;
+        }
+    }
+    if (rc == PAT_OK && hipEventRecord(ev[2 * (size_t)nsteps + 1], s) != hipSuccess) rc = PAT_ERR_HIP;
+    if (rc == PAT_OK && hipStreamSynchronize(s) != hipSuccess) rc = PAT_ERR_HIP;
+    if (rc == PAT_OK) {
+        hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps + 1]);
+        if (kernel_ms)
+            for (int k = 0; k < nsteps; k++) hipEventElapsedTime(&kernel_ms[k], ev[1 + 2 * k], ev[2 + 2 * k]);
+        h->cur = c;
+    } else if (rc == PAT_ERR_HIP) {
+        pat::set_error("pat_md_run_timed: HIP call failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+    cleanup();
+    return rc;
+}
+
+int pat_md_get(pat_md *h, float *pos, float *vel, float *force)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_get: null handle");
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    if (pos) PAT_TRY(h->st.download3(pos, h->st.pos[h->cur]));
+    if (vel) PAT_TRY(h->st.download3(vel, h->st.vel));
+    if (force) PAT_TRY(h->st.download3(force, h->st.force));
+    return PAT_OK;
+}
+
+int pat_md_energy(pat_md *h, double *epot, double *ekin)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_energy: null handle");
+    if (epot) {
+        PAT_TRY(pat::lj_energy_csr((const float4 *)h->st.pos[h->cur], h->st.rowptr, h->st.cols, h->st.N, h->box,
+                                   h->pbc, h->rc2, h->st.node_e, h->st.stream));
+        std::vector<double> e((size_t)h->st.N);
+        PAT_HIP(hipMemcpyAsync(e.data(), h->st.node_e, sizeof(double) * h->st.N, hipMemcpyDeviceToHost, h->st.stream));
+        PAT_HIP(hipStreamSynchronize(h->st.stream));
+        double s = 0;
+        for (double v : e) s += v;
+        *epot = s;
+    }
+    if (ekin) {
+        std::vector<float> v(3 * (size_t)h->st.N);
+        PAT_TRY(h->st.download3(v.data(), h->st.vel));
+        double s = 0;
+        for (float x : v) s += (double)x * x;
+        *ekin = 0.5 * s;
+    }
+    return PAT_OK;
+}
+
+int pat_md_device_buffers(pat_md *h, void **pos4, void **vel4, void **force4)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_device_buffers: null handle");
+    if (pos4) *pos4 = h->st.pos[h->cur];
+    if (vel4) *vel4 = h->st.vel;
+    if (force4) *force4 = h->st.force;
+    return PAT_OK;
+}
+
+int pat_md_destroy(pat_md *h)
+{
+    if (!h) return PAT_OK;
+    h->drop_graphs();
+    h->st.release();
+    delete h;
+    return PAT_OK;
+}
+
+}  // extern "C"

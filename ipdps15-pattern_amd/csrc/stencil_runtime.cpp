@@ -192,6 +192,39 @@ int pat_stencil_run(pat_stencil *s, int nsteps)
     return PAT_OK;
 }
 
+int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *kernel_ms)
+{
+    PAT_REQUIRE(s && total_ms, PAT_ERR_INVALID, "pat_stencil_run_timed: null argument");
+    PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run_timed: no grid uploaded");
+    PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_stencil_run_timed: need at least one step");
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
+    int tbx, tby, tr, tzc;
+    s->resolve_tile(&tbx, &tby, &tr, &tzc);
+    const int v = s->variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
+    int rc = PAT_OK;
+    int c = s->cur;
+    for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
+        (void)hipEventRecord(ev[2 * k], s->stream);
+        rc = pat::heat3d_sweep(s->buf[c], s->buf[1 - c], s->n[0], s->n[1], s->n[2], s->alpha(), v, tbx, tby, tr, tzc,
+                               s->stream);
+        (void)hipEventRecord(ev[2 * k + 1], s->stream);
+        c = 1 - c;
+    }
+    if (rc == PAT_OK && hipStreamSynchronize(s->stream) != hipSuccess) {
+        pat::set_error("pat_stencil_run_timed: stream synchronisation failed");
+        rc = PAT_ERR_HIP;
+    }
+    if (rc == PAT_OK) {
+        hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps - 1]);
+        if (kernel_ms)
+            for (int k = 0; k < nsteps; k++) hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
+        s->cur = c;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
 int pat_stencil_download(pat_stencil *s, void *host)
 {
     PAT_REQUIRE(s && host, PAT_ERR_INVALID, "pat_stencil_download: null argument");

@@ -17,6 +17,10 @@ PAT_STENCIL_HEAT = 0
 PAT_STENCIL_AUTO = 0
 PAT_STENCIL_LDS25D = 1
 PAT_STENCIL_GENERIC = 2
+PAT_IR_LJ_MINIMAGE = 0
+PAT_IR_LJ_OPEN = 1
+PAT_IR_CSR = 0
+PAT_IR_ATOMIC = 1
 
 _lib = None
 
@@ -65,9 +69,32 @@ SIGNATURES = {
     "pat_stencil_set_variant": (c_int, [c_vp, c_int, c_int, c_int, c_int]),
     "pat_stencil_upload": (c_int, [c_vp, c_vp]),
     "pat_stencil_run": (c_int, [c_vp, c_int]),
+    "pat_stencil_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
     "pat_stencil_download": (c_int, [c_vp, c_vp]),
     "pat_stencil_device_ptr": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "pat_stencil_destroy": (c_int, [c_vp]),
+    "pat_ir_csr_workspace": (c_int, [c_i64, c_i64, c_i64p]),
+    "pat_ir_csr_build_dev": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "pat_lj_force_csr_dev": (c_int, [c_vp, c_vp, c_vp, c_i64, c_float, c_int, c_float, c_int, c_vp, c_vp]),
+    "pat_lj_force_atomic_dev": (c_int, [c_vp, c_vp, c_i64, c_i64, c_float, c_int, c_float, c_vp, c_vp]),
+    "pat_ir_create": (c_int, [c_i64, c_i64, c_vp, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
+    "pat_ir_set_map_idx": (c_int, [c_vp, c_int]),
+    "pat_ir_set_param": (c_int, [c_vp, c_vp, c_i64]),
+    "pat_ir_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pat_ir_reset_node_data": (c_int, [c_vp, c_vp, c_int]),
+    "pat_ir_start": (c_int, [c_vp]),
+    "pat_ir_get_result": (c_int, [c_vp, c_vp]),
+    "pat_ir_destroy": (c_int, [c_vp]),
+    "pat_md_create": (c_int, [c_i64, c_vp, c_vp, c_i64, c_vp, c_double, c_int, c_double, c_double,
+                              ctypes.POINTER(c_vp)]),
+    "pat_md_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pat_md_run": (c_int, [c_vp, c_int]),
+    "pat_md_run_async": (c_int, [c_vp, c_int, c_vp]),
+    "pat_md_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
+    "pat_md_get": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pat_md_energy": (c_int, [c_vp, c_dp, c_dp]),
+    "pat_md_device_buffers": (c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "pat_md_destroy": (c_int, [c_vp]),
 }
 
 
@@ -296,6 +323,13 @@ class Stencil:
     def run(self, nsteps):
         call("pat_stencil_run", self.h, nsteps)
 
+    def run_timed(self, nsteps):
+        """Returns (total_ms, per-sweep kernel ms array)."""
+        tot = c_float(0)
+        k = np.zeros(nsteps, dtype=np.float32)
+        call("pat_stencil_run_timed", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
+        return tot.value, k
+
     def download(self):
         out = np.empty(self.shape, dtype=np.float32)
         call("pat_stencil_download", self.h, ptr(out))
@@ -311,3 +345,128 @@ class Stencil:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- irregular
+
+def csr_build_dev(edges_dev, num_edges, num_nodes, stream=None):
+    """Device CSR by destination; returns (rowptr, cols) DeviceArrays."""
+    wb = c_i64(0)
+    call("pat_ir_csr_workspace", num_nodes, num_edges, ctypes.byref(wb))
+    work = DeviceArray((max(wb.value, 1),), np.uint8)
+    rowptr = DeviceArray((num_nodes + 1,), np.int32)
+    cols = DeviceArray((max(2 * num_edges, 1),), np.int32)
+    call("pat_ir_csr_build_dev", ptr(edges_dev), num_edges, num_nodes, rowptr.ptr, cols.ptr, work.ptr, wb.value,
+         stream_ptr(stream))
+    sync(stream)
+    work.free()
+    return rowptr, cols
+
+
+def to_float4(a3):
+    a4 = np.zeros((a3.shape[0], 4), dtype=np.float32)
+    a4[:, :3] = a3
+    return a4
+
+
+class IrregularRuntime:
+    """Handle mirror of IrregularRuntime (kmeans/irregular/irregular_runtime.h)."""
+
+    def __init__(self, edges, num_nodes, map_idx=PAT_IR_LJ_MINIMAGE, param=None):
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        self.n = num_nodes
+        self._param = np.ascontiguousarray(param)
+        h = c_vp()
+        call("pat_ir_create", e.shape[0], num_nodes, ptr(e), map_idx, ptr(self._param), self._param.nbytes,
+             ctypes.byref(h))
+        self.h = h
+
+    def set_variant(self, variant, lanes_per_row=0):
+        call("pat_ir_set_variant", self.h, variant, lanes_per_row)
+
+    def reset_node_data(self, pos):
+        p = np.ascontiguousarray(pos)
+        call("pat_ir_reset_node_data", self.h, ptr(p), p.dtype.itemsize * 3)
+
+    def start(self):
+        call("pat_ir_start", self.h)
+
+    def get_reduction_result(self):
+        out = np.empty((self.n, 3), dtype=np.float32)
+        call("pat_ir_get_result", self.h, ptr(out))
+        return out
+
+    def close(self):
+        if self.h:
+            call("pat_ir_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MD:
+    """Device-resident velocity-Verlet LJ dynamics (pat_md_*)."""
+
+    def __init__(self, pos, vel, edges, box, cutoff, dt, pbc=True, variant=PAT_IR_CSR, lanes_per_row=0):
+        p = np.ascontiguousarray(pos, dtype=np.float32)
+        v = np.ascontiguousarray(vel, dtype=np.float32) if vel is not None else None
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        self.n = p.shape[0]
+        self.num_edges = e.shape[0]
+        h = c_vp()
+        call("pat_md_create", self.n, ptr(p), ptr(v), e.shape[0], ptr(e), float(box), int(pbc), float(cutoff),
+             float(dt), ctypes.byref(h))
+        self.h = h
+        if variant != PAT_IR_CSR or lanes_per_row:
+            self.set_variant(variant, lanes_per_row)
+
+    def set_variant(self, variant, lanes_per_row=0):
+        call("pat_md_set_variant", self.h, variant, lanes_per_row)
+
+    def run(self, nsteps):
+        call("pat_md_run", self.h, nsteps)
+
+    def run_timed(self, nsteps):
+        """Returns (total_ms, per-step force-kernel ms array)."""
+        tot = c_float(0)
+        k = np.zeros(nsteps, dtype=np.float32)
+        call("pat_md_run_timed", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
+        return tot.value, k
+
+    def run_async(self, nsteps, stream=None):
+        call("pat_md_run_async", self.h, nsteps, stream_ptr(stream))
+
+    def get(self):
+        p = np.empty((self.n, 3), dtype=np.float32)
+        v = np.empty((self.n, 3), dtype=np.float32)
+        f = np.empty((self.n, 3), dtype=np.float32)
+        call("pat_md_get", self.h, ptr(p), ptr(v), ptr(f))
+        return p, v, f
+
+    def energy(self):
+        ep, ek = c_double(0), c_double(0)
+        call("pat_md_energy", self.h, ctypes.byref(ep), ctypes.byref(ek))
+        return ep.value, ek.value
+
+    def close(self):
+        if self.h:
+            call("pat_md_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def moldyn_system(ncells, rho=0.8442, jitter=0.05, cutoff=2.5, seed=42, shuffle=True):
+    """FCC lattice of ncells^3 unit cells, jittered, with its half pair list
+    within the cutoff (minimum image) in seeded random order."""
+    pos, box = gen_fcc(ncells, ncells, ncells, rho, jitter, seed)
+    edges = gen_pairs(pos, box[0], cutoff, shuffle=shuffle, seed=seed + 1)
+    return pos, float(box[0]), edges

@@ -1,0 +1,158 @@
+"""GPU parity of the irregular engine (through libpattern's C ABI) vs the oracle.
+
+Bar: forces and energies within 1e-5 relative L-infinity (max |a-b| / max |b|)
+of the FP32 oracle's serial scatter; the CSR by destination must equal a
+NumPy construction exactly; per-pair arithmetic is identical, so only the
+summation order differs.  Errors against the FP64 oracle are printed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def small(pa):
+    return pa.moldyn_system(6, seed=21)  # 864 atoms
+
+
+def test_csr_build_exact(pa, gpu, small):
+    pos, box, edges = small
+    n = len(pos)
+    ed = pa.DeviceArray.from_numpy(edges)
+    rowptr, cols = pa.csr_build_dev(ed, len(edges), n)
+    rp, cl = rowptr.numpy(), cols.numpy()
+    both = np.concatenate([edges, edges[:, ::-1]])
+    order = np.lexsort((both[:, 1], both[:, 0]))
+    both = both[order]
+    expect_rp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(expect_rp, both[:, 0] + 1, 1)
+    assert np.array_equal(rp, np.cumsum(expect_rp))
+    assert np.array_equal(cl, both[:, 1])
+
+
+def test_csr_build_edge_cases(pa, gpu):
+    # isolated nodes, a hub of high degree (> 64, multi-chunk rows), duplicates
+    n = 300
+    hub = [(0, j) for j in range(1, 200)]
+    edges = np.array(hub + [(5, 7), (5, 7), (299, 3)], dtype=np.int32)
+    rowptr, cols = pa.csr_build_dev(pa.DeviceArray.from_numpy(edges), len(edges), n)
+    rp, cl = rowptr.numpy(), cols.numpy()
+    assert rp[0] == 0 and rp[-1] == 2 * len(edges)
+    assert np.array_equal(cl[rp[0]:rp[1]], np.arange(1, 200))
+    assert np.array_equal(cl[rp[5]:rp[6]], [7, 7])
+    assert rp[251] - rp[250] == 0
+
+
+@pytest.mark.parametrize("lpr", [4, 8, 16, 32, 64])
+def test_force_csr_vs_oracle(pa, orc, gpu, small, lpr):
+    pos, box, edges = small
+    n = len(pos)
+    rowptr, cols = pa.csr_build_dev(pa.DeviceArray.from_numpy(edges), len(edges), n)
+    p4 = pa.DeviceArray.from_numpy(pa.to_float4(pos))
+    f4 = pa.DeviceArray((n, 4), np.float32)
+    pa.call("pat_lj_force_csr_dev", p4.ptr, rowptr.ptr, cols.ptr, n, box, 1, 2.5, lpr, f4.ptr, None)
+    pa.sync()
+    got = f4.numpy()[:, :3]
+    ref, _ = orc.lj_force(pos, edges, box, 2.5)
+    assert rel(got, ref) < TOL
+
+
+def test_force_atomic_vs_oracle(pa, orc, gpu, small):
+    pos, box, edges = small
+    n = len(pos)
+    p4 = pa.DeviceArray.from_numpy(pa.to_float4(pos))
+    f4 = pa.DeviceArray((n, 4), np.float32)
+    ed = pa.DeviceArray.from_numpy(edges)
+    pa.call("pat_lj_force_atomic_dev", p4.ptr, ed.ptr, len(edges), n, box, 1, 2.5, f4.ptr, None)
+    pa.sync()
+    ref, _ = orc.lj_force(pos, edges, box, 2.5)
+    assert rel(f4.numpy()[:, :3], ref) < TOL
+
+
+def test_two_particle_and_cutoff(pa, gpu):
+    pos = np.array([[5.0, 5, 5], [6.1, 5, 5], [5.0, 7.6, 5]], dtype=np.float32)
+    edges = np.array([[0, 1], [0, 2]], dtype=np.int32)
+    ir = pa.IrregularRuntime(edges, 3, pa.PAT_IR_LJ_OPEN, np.array([2.5 * 2.5]))
+    ir.reset_node_data(pos)
+    ir.start()
+    f = ir.get_reduction_result()
+    r = np.float32(1.1)
+    F = 48 * (r ** -13 - 0.5 * r ** -7)
+    assert abs(f[0, 0] + F) < 2e-6 * abs(F) and abs(f[1, 0] - F) < 2e-6 * abs(F)
+    assert f[2].tolist() == [0, 0, 0]  # 2.6 apart: beyond the cutoff
+    ir.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_irregular_runtime_handle(pa, orc, gpu, small, variant):
+    pos, box, edges = small
+    para = np.array([box, 2.5], dtype=np.float64)  # struct para {side, cutoffRadius}
+    ir = pa.IrregularRuntime(edges, len(pos), pa.PAT_IR_LJ_MINIMAGE, para)
+    ir.set_variant(variant)
+    ir.reset_node_data(pos.astype(np.float64))  # moldyn's double3 point_data
+    ir.start()
+    got = ir.get_reduction_result()
+    ref, _ = orc.lj_force(pos, edges, box, 2.5)
+    assert rel(got, ref) < TOL
+    ir.close()
+
+
+def test_edge_order_invariance_bitexact(pa, gpu, small):
+    pos, box, edges = small
+    rng = np.random.default_rng(1)
+    e2 = edges[rng.permutation(len(edges))][:, ::-1].copy()
+    outs = []
+    for e in (edges, e2):
+        md = pa.MD(pos, None, e, box, 2.5, 0.005)
+        md.run(3)
+        outs.append(md.get())
+        md.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_md_small_vs_oracle(pa, orc, gpu, small, variant):
+    pos, box, edges = small
+    vel = pa.gen_velocities(len(pos), 1.44, seed=5)
+    md = pa.MD(pos, vel, edges, box, 2.5, 0.005, variant=variant)
+    md.run(10)
+    md.run(10)
+    p, v, f = md.get()
+    rp, rv, rf, rep = orc.md_run(pos, vel, edges, box, 2.5, 0.005, 20)
+    assert rel(f, rf) < TOL and rel(v, rv) < TOL
+    d = p - rp
+    d -= box * np.rint(d / box)
+    assert np.max(np.abs(d)) / box < TOL
+    if variant == 0:
+        ep, ek = md.energy()
+        assert abs(ep - rep) / abs(rep) < TOL
+    md.close()
+
+
+def test_moldyn_workload(pa, orc, gpu):
+    """moldyn: 40^3 FCC cells (256,000 molecules), density 0.8442, jitter
+    +-0.05, half list within 2.5 (random order), 50 VV steps, dt 0.005."""
+    pos, box, edges = pa.moldyn_system(40, seed=42)
+    assert 6.5e6 < len(edges) < 7.5e6
+    md = pa.MD(pos, None, edges, box, 2.5, 0.005)
+    md.run(50)
+    p, v, f = md.get()
+    ep, ek = md.energy()
+    md.close()
+    rp, rv, rf, rep = orc.md_run(pos, np.zeros_like(pos), edges, box, 2.5, 0.005, 50, threads=16)
+    err_f = rel(f, rf)
+    err_v = rel(v, rv)
+    print(f"moldyn 256k x50: pairs={len(edges)} rel err force {err_f:.2e} vel {err_v:.2e} epot {ep:.6e} "
+          f"(oracle {rep:.6e})")
+    assert err_f < TOL and err_v < TOL
+    assert abs(ep - rep) / abs(rep) < TOL
+    # size-independent properties: momentum stays ~0 (forces sum to ~0)
+    assert np.max(np.abs(f.astype(np.float64).sum(axis=0))) / np.max(np.abs(f)) < 1e-3

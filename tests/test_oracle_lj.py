@@ -1,0 +1,164 @@
+"""Pins the LJ oracle (oracle/oracle.c lj_*, md_*, pairs_brute) and the
+workload generators without trusting the oracle.
+
+Checks: the two-particle closed form F(r) = 48(r^-13 - r^-7/2) and
+phi(r) = 4(r^-12 - r^-6) (and F = -dphi/dr by finite differences), the
+minimum image across the box boundary, zero net force on every atom of a
+perfect FCC lattice, Newton's third law, invariance under edge order and node
+renumbering, energy conservation of the velocity-Verlet loop, and the pair
+generator against O(N^2) brute force.
+"""
+import numpy as np
+import pytest
+
+
+def two_atoms(r, axis=0, box=20.0):
+    pos = np.zeros((2, 3))
+    pos[0] = 5.0
+    pos[1] = 5.0
+    pos[0, axis] += r
+    return pos, np.array([[0, 1]], dtype=np.int32)
+
+
+@pytest.mark.parametrize("r", [0.95, 1.0, 2 ** (1 / 6), 1.5, 2.49])
+def test_two_particle_closed_form(orc, r):
+    pos, e = two_atoms(r)
+    f, ep = orc.lj_force(pos, e, 20.0, 2.5, pbc=True, dtype=np.float64)
+    F = 48 * (r ** -13 - 0.5 * r ** -7)
+    assert abs(f[0, 0] - F) <= 1e-12 * max(1, abs(F))
+    assert abs(f[1, 0] + F) <= 1e-12 * max(1, abs(F))
+    assert f[0, 1] == 0 and f[0, 2] == 0
+    assert abs(ep - 4 * (r ** -12 - r ** -6)) < 1e-12
+    p32 = pos.astype(np.float32)
+    f32, _ = orc.lj_force(p32, e, 20.0, 2.5, pbc=True)
+    r32 = float(p32[0, 0]) - float(p32[1, 0])  # the distance the FP32 inputs encode
+    F32 = 48 * (r32 ** -13 - 0.5 * r32 ** -7)
+    assert abs(f32[0, 0] - F32) <= 2e-6 * max(1, abs(F32))
+
+
+def test_force_is_minus_gradient(orc):
+    r, h = 1.3, 1e-6
+    _, ep_p = orc.lj_force(*two_atoms(r + h), 20.0, 2.5, dtype=np.float64)
+    _, ep_m = orc.lj_force(*two_atoms(r - h), 20.0, 2.5, dtype=np.float64)
+    f, _ = orc.lj_force(*two_atoms(r), 20.0, 2.5, dtype=np.float64)
+    assert abs(f[0, 0] - (-(ep_p - ep_m) / (2 * h))) < 1e-6
+
+
+def test_cutoff_excludes(orc):
+    f, ep = orc.lj_force(*two_atoms(2.5), 20.0, 2.5, dtype=np.float64)
+    assert not f.any() and ep == 0
+
+
+def test_minimum_image(orc):
+    L = 10.0
+    pos = np.array([[0.5, 3, 3], [L - 0.6, 3, 3]])
+    e = np.array([[0, 1]], dtype=np.int32)
+    f, _ = orc.lj_force(pos, e, L, 2.5, pbc=True, dtype=np.float64)
+    r = 1.1
+    F = 48 * (r ** -13 - 0.5 * r ** -7)
+    # atom 0 sees atom 1 at x = -0.6 through the boundary: d = +1.1
+    assert abs(f[0, 0] - F) < 1e-12 and abs(f[1, 0] + F) < 1e-12
+    fo, _ = orc.lj_force(pos, e, L, 2.5, pbc=False, dtype=np.float64)
+    assert not fo.any()  # 8.9 apart without images
+
+
+def test_pairs_generator_matches_brute_force(pa, orc):
+    pos, box = pa.gen_fcc(3, 3, 3, rho=0.8442, jitter=0.05, seed=4)
+    e = pa.gen_pairs(pos, box[0], 2.5)
+    b = orc.pairs_brute(pos, box[0], 2.5)
+    assert np.array_equal(e, b)  # both in (i, j) order
+    es = pa.gen_pairs(pos, box[0], 2.5, shuffle=True, seed=9)
+    assert not np.array_equal(es, e)
+    assert np.array_equal(es[np.lexsort((es[:, 1], es[:, 0]))], e)
+
+
+def test_pairs_generator_cell_list_path(pa, orc):
+    # 4^3 cells: box 6.7 > 3*cutoff(2.2), so the cell-list path is used
+    pos, box = pa.gen_fcc(4, 4, 4, rho=0.8442, jitter=0.05, seed=5)
+    e = pa.gen_pairs(pos, box[0], 2.2)
+    assert np.array_equal(e, orc.pairs_brute(pos, box[0], 2.2))
+
+
+def test_fcc_lattice_structure(pa):
+    pos, box = pa.gen_fcc(2, 3, 4, rho=0.8442, jitter=0.0)
+    a = (4 / 0.8442) ** (1 / 3)
+    assert pos.shape == (96, 3)
+    assert np.allclose(box, [2 * a, 3 * a, 4 * a])
+    # every coordinate is a multiple of a/2 with an even index sum
+    idx = np.rint(pos / (a / 2)).astype(int)
+    assert np.allclose(idx * (a / 2), pos, atol=1e-5)
+    assert np.all(idx.sum(axis=1) % 2 == 0)
+    assert len({tuple(r) for r in idx}) == 96
+    # miniMD visit order: z outermost, then y, then x
+    assert np.all(np.diff(idx[:, 2]) >= 0)
+
+
+def test_perfect_fcc_zero_net_force(pa, orc):
+    pos, box = pa.gen_fcc(4, 4, 4, rho=0.8442, jitter=0.0)
+    e = pa.gen_pairs(pos, box[0], 2.5)
+    # exact FP64 lattice sites (the generator's FP32 sites are off by ~1e-7)
+    h = (4 / 0.8442) ** (1 / 3) / 2
+    p64 = np.rint(pos.astype(np.float64) / h) * h
+    f, _ = orc.lj_force(p64, e, 8 * h, 2.5, dtype=np.float64)
+    assert np.max(np.abs(f)) < 1e-9
+    # while a single displaced atom feels a restoring force
+    p64[5, 0] += 0.01
+    f, _ = orc.lj_force(p64, e, 8 * h, 2.5, dtype=np.float64)
+    assert f[5, 0] < -0.1
+
+
+def test_newton_third_law_and_invariances(pa, orc):
+    pos, box = pa.gen_fcc(4, 4, 4, rho=0.8442, jitter=0.05, seed=6)
+    e = pa.gen_pairs(pos, box[0], 2.5)
+    p64 = pos.astype(np.float64)
+    f, ep = orc.lj_force(p64, e, box[0], 2.5, dtype=np.float64)
+    assert np.max(np.abs(f)) > 1.0
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-9
+    rng = np.random.default_rng(0)
+    # edge order and endpoint order
+    e2 = e[rng.permutation(len(e))][:, ::-1].copy()
+    f2, ep2 = orc.lj_force(p64, e2, box[0], 2.5, dtype=np.float64)
+    assert np.max(np.abs(f2 - f)) < 1e-10 and abs(ep2 - ep) < 1e-9
+    # node renumbering
+    perm = rng.permutation(len(pos))
+    inv = np.argsort(perm)
+    f3, _ = orc.lj_force(p64[perm], inv[e].astype(np.int32), box[0], 2.5, dtype=np.float64)
+    assert np.max(np.abs(f3 - f[perm])) < 1e-10
+
+
+def test_f32_close_to_f64(pa, orc):
+    pos, box = pa.gen_fcc(4, 4, 4, jitter=0.05, seed=7)
+    e = pa.gen_pairs(pos, box[0], 2.5)
+    f32, _ = orc.lj_force(pos, e, box[0], 2.5)
+    f64, _ = orc.lj_force(pos.astype(np.float64), e, box[0], 2.5, dtype=np.float64)
+    assert np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)) < 1e-5
+    ft = orc.lj_force_threads(pos, e, box[0], 2.5, threads=4)
+    assert np.max(np.abs(ft - f64)) / np.max(np.abs(f64)) < 1e-5
+
+
+def test_velocity_verlet_conserves_energy(pa, orc):
+    pos, box = pa.gen_fcc(4, 4, 4, jitter=0.05, seed=8)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=3)
+    e = pa.gen_pairs(pos, box[0], 2.8)
+    p0 = pos.astype(np.float64)
+    v0 = vel.astype(np.float64)
+    _, _, _, ep0 = orc.md_run(p0, v0, e, box[0], 2.5, 0.005, 0, dtype=np.float64)
+    e0 = ep0 + 0.5 * np.sum(v0 ** 2)
+    p1, v1, _, ep1 = orc.md_run(p0, v0, e, box[0], 2.5, 0.005, 40, dtype=np.float64)
+    e1 = ep1 + 0.5 * np.sum(v1 ** 2)
+    # truncated (unshifted) LJ: small drift from pairs crossing the cutoff
+    assert abs(e1 - e0) / abs(e0) < 5e-3
+    # total momentum is conserved (it starts at ~1e-7 from FP32 rounding)
+    assert np.max(np.abs(v1.sum(axis=0) - v0.sum(axis=0))) < 1e-9
+    # time reversibility: run back with negated velocities
+    p2, v2, _, _ = orc.md_run(p1, -v1, e, box[0], 2.5, 0.005, 40, dtype=np.float64)
+    d = p2 - p0
+    d -= box[0] * np.rint(d / box[0])
+    assert np.max(np.abs(d)) < 1e-8
+
+
+def test_velocities_generator(pa):
+    v = pa.gen_velocities(4000, 1.44, seed=3)
+    assert np.max(np.abs(v.astype(np.float64).sum(axis=0))) < 1e-3
+    t = np.sum(v.astype(np.float64) ** 2) / (3 * (4000 - 1))
+    assert abs(t - 1.44) < 1e-4
