@@ -230,6 +230,50 @@ int pat_md_energy(pat_md *h, double *epot, double *ekin);
 int pat_md_device_buffers(pat_md *h, void **pos4, void **vel4, void **force4);
 int pat_md_destroy(pat_md *h);
 
+/* ------------------------------------------------------------------------
+ * Generalized-reduction engine, kmeans (replaces the regular compute_gpu,
+ * kmeans/regular/gpu_kernel.cu:39-183, with the LDS/device hash reduction
+ * objects ros.cu/rog.cu, running map1/reduce1, kmeans/regular.h:75-136,
+ * driven by RegularRuntime, kmeans/regular/regular_runtime.h:11-106)
+ * ---------------------------------------------------------------------- */
+
+#define PAT_KM_PLAIN 0 /* FP32 difference form over every centre (map1's arithmetic) */
+#define PAT_KM_MFMA 1  /* BF16 MFMA distances + exact FP32 rescoring inside the error margin */
+
+/* Fixed-point shift s used for the order-independent sums of n values of
+ * magnitude below maxabs (n * maxabs * 2^s < 2^62). */
+int pat_kmeans_fixed_shift(int64_t n, float maxabs, int *shift);
+
+typedef struct pat_kmeans pat_kmeans;
+/* RegularRuntime(num_procs, input, input_size, offsets, n, params, ...)
+ * + RegularInit + set_map_idx(1)/set_reduce_idx(1) (regular_runtime.cu:
+ * 12-126, km.cu:76-80) for n points of dimension d (row-major FP32, the
+ * implicit stride replaces the Offset array) and k centres, initialised to
+ * the first k points.  x is a host pointer (copied) or, with x_on_device, a
+ * device pointer the handle reads in place. */
+int pat_kmeans_create(const float *x, int64_t n, int d, int k, int mode, int x_on_device, pat_kmeans **out);
+int pat_kmeans_set_mode(pat_kmeans *h, int mode);
+/* Replaces the centres (k x d host array). */
+int pat_kmeans_set_centroids(pat_kmeans *h, const float *c);
+/* iters x (RegularStart: assign + reduce; then the Lloyd update
+ * c_k = sum_k / count_k, empty clusters keep their centre).  Replays a
+ * captured hipGraph.  Blocking. */
+int pat_kmeans_run(pat_kmeans *h, int iters);
+/* Same, launched one by one with HIP events: total_ms = whole run,
+ * assign_ms[i] = iteration i's assignment kernel (may be NULL). */
+int pat_kmeans_run_timed(pat_kmeans *h, int iters, float *total_ms, float *assign_ms);
+/* Current centres (k x d), last assignment (n) and its min_dist (n); any
+ * pointer may be NULL. */
+int pat_kmeans_get(pat_kmeans *h, float *centroids, int32_t *assign, float *mindist);
+/* RegularGetOutput (regular_runtime.cu:288-291): the last pass's reduction
+ * object -- per key the fixed-point coordinate sums (k x d), the point
+ * counts (k) and the fixed-point min_dist sums (k), with their shifts
+ * (value = sum * 2^-shift). */
+int pat_kmeans_get_object(pat_kmeans *h, int64_t *sums, int64_t *counts, int64_t *dsums, int *shift, int *dshift);
+/* Points rescored in FP32 so far (MFMA mode) and max |x| of the data. */
+int pat_kmeans_stats(pat_kmeans *h, int64_t *rescored_total, float *maxabs);
+int pat_kmeans_destroy(pat_kmeans *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -497,9 +497,9 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
     if (rc == PAT_OK && hipEventRecord(ev[2 * (size_t)nsteps + 1], s) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc == PAT_OK && hipStreamSynchronize(s) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc == PAT_OK) {
-        hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps + 1]);
+        (void)hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps + 1]);
         if (kernel_ms)
-            for (int k = 0; k < nsteps; k++) hipEventElapsedTime(&kernel_ms[k], ev[1 + 2 * k], ev[2 + 2 * k]);
+            for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[1 + 2 * k], ev[2 + 2 * k]);
         h->cur = c;
     } else if (rc == PAT_ERR_HIP) {
         pat::set_error("pat_md_run_timed: HIP call failed: %s", hipGetErrorString(hipGetLastError()));

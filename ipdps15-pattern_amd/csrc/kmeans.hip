@@ -1,0 +1,622 @@
+// Generalized-reduction engine for gfx950: kmeans assignment + centroid
+// accumulation (the reference's map1/reduce1 functors, kmeans/regular.h:
+// 75-136, run by the regular compute_gpu kernel, kmeans/regular/
+// gpu_kernel.cu:39-183, into an LDS hash reduction object, ros.cu:15-170).
+//
+// Reduction object.  map1 emits {x_0..x_{D-1}, 1, min_dist} under key =
+// nearest centre and reduce1 adds values per key.  The reference adds FP32
+// values in lock-acquisition order, which is not reproducible.  Here each
+// coordinate (and min_dist) is rounded once to an integer multiple of 2^-s
+// (s chosen so the largest possible sum fits in 62 bits) and summed as int64:
+// integer addition is associative, so every summation order -- LDS atomics
+// within a workgroup, global atomics across workgroups, or the CPU oracle's
+// loop -- gives identical bits, and so identical centroids every iteration.
+// The dense per-workgroup object (K x D sums, K counts, K distance sums)
+// lives in LDS and is flushed once per workgroup with global atomics.
+//
+// Assignment, plain path: the functor's arithmetic exactly -- for each centre
+// in order, dist = sum_d (c_d - x_d)^2 in FP32 in dimension order, sqrt,
+// strict < against min_dist starting at 65536*65 (lowest index wins ties).
+// Compiled with -ffp-contract=off so no multiply-add is fused.
+//
+// Assignment, MFMA path: a wavefront takes 32 points and computes
+// ||c||^2 - 2 x.c for every centre with v_mfma_f32_32x32x16_bf16 (x and -2c
+// rounded to BF16, ||c||^2 split into three BF16 terms on an extra K step,
+// FP32 accumulation).  A per-point bound E on the error of that value
+// (BF16 rounding of both operands, the norm split, FP32 accumulation) plus
+// the FP32 rounding of the reference's own difference form gives a margin:
+// every centre whose approximate value is within the margin of the smallest
+// is rescored with the plain path's exact FP32 arithmetic, and the exact
+// results decide (same tie rule).  A centre outside the margin provably has a
+// larger reference distance than the best one, so the assignment equals the
+// plain path's bit for bit.
+#include <cmath>
+
+#include "pat_common.hpp"
+#include "pat_device.hpp"
+#include "kmeans_dev.hpp"
+
+namespace {
+
+using pat::kWave;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr float kInitMin = 65536 * 65;  // map1's initial min_dist (kmeans/regular.h:86)
+
+__device__ __forceinline__ long long to_fixed(float v, double scale)
+{
+    return (long long)__builtin_rint((double)v * scale);
+}
+
+// ------------------------------------------------------------ shared parts
+
+struct KmArgs {
+    const float *x;         // N x D points
+    const float *c;         // K x D centres (FP32)
+    int64_t N;
+    int D, K;
+    double scale, dscale;   // fixed-point scales 2^s, 2^ds
+    int32_t *assign;        // N
+    float *mindist;         // N (may be null)
+    unsigned long long *gsum;   // K x D
+    unsigned long long *gcnt;   // K
+    unsigned long long *gdsum;  // K
+    int use_lds_acc;        // 1: LDS reduction object, 0: global atomics directly
+};
+
+// Adds the wavefront's points [base, base+np) with keys key_of(j) into the
+// reduction object.  Lanes cover (point, dimension) pairs so consecutive
+// lanes touch consecutive 8-byte words.
+template <typename KeyFn, typename DistFn>
+__device__ __forceinline__ void accumulate_points(const KmArgs &a, int64_t base, int np, KeyFn key_of, DistFn dist_of,
+                                                  unsigned long long *acc, unsigned long long *cnt,
+                                                  unsigned long long *dsum)
+{
+    // Trip counts are uniform over the wavefront so that every lane is active
+    // at each cross-lane read of the keys.
+    const int lane = threadIdx.x % kWave;
+    const int D = a.D;
+    const int total = np * D;
+    for (int t0 = 0; t0 < total; t0 += kWave) {
+        const int t = t0 + lane;
+        const int tt = t < total ? t : total - 1;
+        const int j = tt / D, d = tt - j * D;
+        const int k = key_of(j);
+        if (t < total) {
+            const float v = a.x[(base + j) * D + d];
+            atomicAdd(&acc[(int64_t)k * D + d], (unsigned long long)to_fixed(v, a.scale));
+        }
+    }
+    const int jj = lane < np ? lane : np - 1;
+    const int k = key_of(jj);
+    const float dist = dist_of(jj);
+    if (lane < np) {
+        atomicAdd(&cnt[k], 1ULL);
+        atomicAdd(&dsum[k], (unsigned long long)to_fixed(dist, a.dscale));
+    }
+}
+
+__device__ void flush_lds_object(const KmArgs &a, unsigned long long *acc, unsigned long long *cnt,
+                                 unsigned long long *dsum)
+{
+    const int KD = a.K * a.D;
+    for (int i = threadIdx.x; i < KD; i += blockDim.x)
+        if (acc[i]) atomicAdd(&a.gsum[i], acc[i]);
+    for (int i = threadIdx.x; i < a.K; i += blockDim.x) {
+        if (cnt[i]) atomicAdd(&a.gcnt[i], cnt[i]);
+        if (dsum[i]) atomicAdd(&a.gdsum[i], dsum[i]);
+    }
+}
+
+__device__ void zero_lds_object(const KmArgs &a, unsigned long long *acc, unsigned long long *cnt,
+                                unsigned long long *dsum)
+{
+    for (int i = threadIdx.x; i < a.K * a.D; i += blockDim.x) acc[i] = 0;
+    for (int i = threadIdx.x; i < a.K; i += blockDim.x) {
+        cnt[i] = 0;
+        dsum[i] = 0;
+    }
+}
+
+// Words of the LDS reduction object, rounded up to a 16-byte boundary.
+__host__ __device__ __forceinline__ int64_t object_words(int K, int D) { return ((int64_t)K * D + 2 * K + 1) & ~1LL; }
+
+// LDS layout of the reduction object (when it fits).
+__device__ __forceinline__ void lds_object(const KmArgs &a, unsigned long long *base, unsigned long long *&acc,
+                                           unsigned long long *&cnt, unsigned long long *&dsum)
+{
+    if (a.use_lds_acc) {
+        acc = base;
+        cnt = base + (int64_t)a.K * a.D;
+        dsum = cnt + a.K;
+    } else {
+        acc = a.gsum;
+        cnt = a.gcnt;
+        dsum = a.gdsum;
+    }
+}
+
+// ------------------------------------------------------------ plain path
+
+// One thread per point; centres read from LDS when they fit (cs != null).
+template <int DMAX>
+__global__ __launch_bounds__(512) void km_plain(KmArgs a, int centres_in_lds)
+{
+    extern __shared__ unsigned long long smem[];
+    unsigned long long *acc, *cnt, *dsum;
+    lds_object(a, smem, acc, cnt, dsum);
+    float *cs = (float *)(smem + (a.use_lds_acc ? object_words(a.K, a.D) : 0));
+    const float *C = centres_in_lds ? cs : a.c;
+    if (a.use_lds_acc) zero_lds_object(a, acc, cnt, dsum);
+    if (centres_in_lds)
+        for (int i = threadIdx.x; i < a.K * a.D; i += blockDim.x) cs[i] = a.c[i];
+    __syncthreads();
+
+    const int D = a.D, K = a.K;
+    const int lane = threadIdx.x % kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int64_t base = w0 * kWave; base < a.N; base += nwaves * kWave) {
+        const int64_t p = base + lane;
+        const bool live = p < a.N;
+        float x[DMAX];
+#pragma unroll
+        for (int d = 0; d < DMAX; d++) x[d] = (live && d < D) ? a.x[p * D + d] : 0.0f;
+        unsigned int key = 0;
+        float min_dist = kInitMin;
+        int i = 0;
+        for (; i + 4 <= K; i += 4) {
+            // four independent chains for ILP; compared in index order
+            float dv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float *ci = C + (int64_t)(i + q) * D;
+                float dist = 0;
+#pragma unroll
+                for (int d = 0; d < DMAX; d++) {
+                    if (d < D) {
+                        const float t = ci[d] - x[d];
+                        dist = dist + t * t;
+                    }
+                }
+                dv[q] = sqrtf(dist);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (dv[q] < min_dist) {
+                    min_dist = dv[q];
+                    key = i + q;
+                }
+        }
+        for (; i < K; i++) {
+            const float *ci = C + (int64_t)i * D;
+            float dist = 0;
+#pragma unroll
+            for (int d = 0; d < DMAX; d++) {
+                if (d < D) {
+                    const float t = ci[d] - x[d];
+                    dist = dist + t * t;
+                }
+            }
+            dist = sqrtf(dist);
+            if (dist < min_dist) {
+                min_dist = dist;
+                key = i;
+            }
+        }
+        if (live) {
+            a.assign[p] = (int32_t)key;
+            if (a.mindist) a.mindist[p] = min_dist;
+        }
+        const int np = (int)min<int64_t>(kWave, a.N - base);
+        accumulate_points(
+            a, base, np, [&](int j) { return (int)__shfl((int)key, j, kWave); },
+            [&](int j) { return __shfl(min_dist, j, kWave); }, acc, cnt, dsum);
+    }
+    if (a.use_lds_acc) {
+        __syncthreads();
+        flush_lds_object(a, acc, cnt, dsum);
+    }
+}
+
+// ------------------------------------------------------------ MFMA path
+
+// Per-point bound inputs: u = 2^-8 (BF16 unit roundoff), g = FP32
+// accumulation over the MFMA's terms with a factor 2 of slack for any
+// intermediate rounding mode.
+struct MfmaBound {
+    float c_dot;   // multiplies ||x|| * max||c||
+    float c_norm;  // multiplies max||c||^2
+    float c_ref;   // relative error of the reference's FP32 difference form
+};
+
+// Exact plain-path distance (sqrt of the FP32 difference form) of point p
+// to centre k.
+__device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const float *__restrict__ ck, int D)
+{
+    float dist = 0;
+    for (int d = 0; d < D; d++) {
+        const float t = ck[d] - xp[d];
+        dist = dist + t * t;
+    }
+    return sqrtf(dist);
+}
+
+// KS: K-steps of 16 dimensions holding the data (D <= 16*KS); one more
+// K-step holds the norm terms.  KB: 32-centre blocks per chunk.
+// cb: K_pad x (16*(KS+1)) BF16 centre rows [-2c, 0 pad | n1 n2 n3 0...],
+// padded rows have n1 = +inf.  cnorm: ||c_k|| (K entries).
+template <int KS, int KB>
+__global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
+                                               int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
+{
+    constexpr int KW = 16 * (KS + 1);  // BF16 row width of cb
+    extern __shared__ unsigned long long smem[];
+    unsigned long long *acc_o, *cnt_o, *dsum_o;
+    lds_object(a, smem, acc_o, cnt_o, dsum_o);
+    __bf16 *cbs = (__bf16 *)(smem + (a.use_lds_acc ? object_words(a.K, a.D) : 0));
+    __shared__ float red[16];
+    if (a.use_lds_acc) zero_lds_object(a, acc_o, cnt_o, dsum_o);
+    // stage the BF16 centre matrix in LDS (16-byte copies)
+    {
+        const int n16 = K_pad * KW / 8;
+        const bf16x8 *src = (const bf16x8 *)cb;
+        bf16x8 *dst = (bf16x8 *)cbs;
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+    }
+    // max ||c|| over the centres (for the bound)
+    float cm = 0.0f;
+    for (int i = threadIdx.x; i < a.K; i += blockDim.x) cm = fmaxf(cm, cnorm[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cm = fmaxf(cm, __shfl_xor(cm, o, kWave));
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = cm;
+    __syncthreads();
+    float cmax = 0.0f;
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++) cmax = fmaxf(cmax, red[w]);
+    const float cc_max = cmax * cmax;
+
+    const int D = a.D;
+    const int lane = threadIdx.x % kWave;
+    const int col = lane % 32;   // point within the tile
+    const int half = lane / 32;  // which 8 of each 16 K elements this lane holds
+    const int nchunks = (K_pad / 32 + KB - 1) / KB;
+    unsigned long long my_rescored = 0;
+
+    const int64_t ntiles = (a.N + 31) / 32;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; tile < ntiles;
+         tile += nw) {
+        const int64_t base = tile * 32;
+        const int64_t p = base + col;
+        const bool live = p < a.N;
+        const float *xp = a.x + (live ? p : 0) * D;
+
+        // B operand: point p, K elements [16s + 8*half, +8) of [x, 1, 1, 1, 0...]
+        bf16x8 bx[KS + 1];
+        float xx = 0.0f;
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int d = 16 * s + 8 * half + e;
+                const float v = (live && d < D) ? xp[d] : 0.0f;
+                xx = xx + v * v;
+                bx[s][e] = (__bf16)v;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) bx[KS][e] = (__bf16)((half == 0 && e < 3) ? 1.0f : 0.0f);
+        xx = xx + __shfl_xor(xx, 32, kWave);  // ||x||^2 of point p (both halves)
+        const float xn = sqrtf(xx) * 1.0001f;
+        const float E = bd.c_dot * xn * cmax + bd.c_norm * (cc_max + xx);
+
+        float best_d = kInitMin;
+        int best_k = 0x7fffffff;
+        float m = INFINITY;  // running approximate minimum for the point
+        int ncand = 0;
+        for (int ch = 0; ch < nchunks; ch++) {
+            f32x16 acc[KB];
+#pragma unroll
+            for (int b = 0; b < KB; b++) {
+                acc[b] = f32x16{};
+                const int row = (ch * KB + b) * 32 + col;  // A operand row: centre
+#pragma unroll
+                for (int s = 0; s <= KS; s++) {
+                    bf16x8 av = row < K_pad ? *(const bf16x8 *)(cbs + (int64_t)row * KW + 16 * s + 8 * half)
+                                            : bf16x8{};
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[s], acc[b], 0, 0, 0);
+                }
+            }
+            // chunk minimum for this lane's point (rows of both half-waves)
+            float cmn = INFINITY;
+#pragma unroll
+            for (int b = 0; b < KB; b++)
+#pragma unroll
+                for (int i = 0; i < 16; i++) cmn = fminf(cmn, acc[b][i]);
+            cmn = fminf(cmn, __shfl_xor(cmn, 32, kWave));
+            m = fminf(m, cmn);
+            // candidates: approximate value within the margin of the minimum
+            const float dhi = (xx + m + 4.0f * E) * 1.001f + 1e-30f;
+            const float thr = m + 2.0f * E + 2.0f * bd.c_ref * dhi;
+            uint64_t mask = 0;
+#pragma unroll
+            for (int b = 0; b < KB; b++)
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    if (acc[b][i] <= thr) mask |= 1ULL << (b * 16 + i);
+            while (mask) {
+                const int bit = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int b = bit / 16, i = bit % 16;
+                // output layout of the 32x32 MFMA: value i of this lane is
+                // row 8*(i/4) + 4*half + i%4, column = lane % 32
+                const int k = (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4);
+                if (k < a.K && live) {
+                    const float dk = exact_dist(xp, a.c + (int64_t)k * D, D);
+                    ncand++;
+                    if (dk < best_d || (dk == best_d && k < best_k)) {
+                        best_d = dk;
+                        best_k = k;
+                    }
+                }
+            }
+        }
+        // merge the two half-waves' exact winners (same tie rule)
+        const float od = __shfl_xor(best_d, 32, kWave);
+        const int ok = __shfl_xor(best_k, 32, kWave);
+        const int on = __shfl_xor(ncand, 32, kWave);
+        if (od < best_d || (od == best_d && ok < best_k)) {
+            best_d = od;
+            best_k = ok;
+        }
+        if (best_k == 0x7fffffff) best_k = 0;  // unreachable for live points
+        if (live && half == 0) {
+            a.assign[p] = best_k;
+            if (a.mindist) a.mindist[p] = best_d;
+            if (ncand + on > 1) my_rescored++;
+        }
+        const int np = (int)min<int64_t>(32, a.N - base);
+        accumulate_points(
+            a, base, np, [&](int j) { return __shfl(best_k, j, kWave); },
+            [&](int j) { return __shfl(best_d, j, kWave); }, acc_o, cnt_o, dsum_o);
+    }
+    // rescored-point counter
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) my_rescored += __shfl_xor(my_rescored, o, kWave);
+    if (lane == 0 && my_rescored) atomicAdd(rescored, my_rescored);
+    if (a.use_lds_acc) {
+        __syncthreads();
+        flush_lds_object(a, acc_o, cnt_o, dsum_o);
+    }
+}
+
+// ------------------------------------------------------------ update
+
+// One wavefront per centre: applies the Lloyd update from the reduction
+// object (c = (sum * 2^-s) / count in FP64, rounded to FP32; empty clusters
+// keep their centre), writes the BF16 MFMA row and the norm, and clears the
+// object for the next pass.  With update == 0 only the rows are prepared.
+template <int KS>
+__global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K, int K_pad, int D,
+                                                   unsigned long long *__restrict__ gsum,
+                                                   unsigned long long *__restrict__ gcnt,
+                                                   unsigned long long *__restrict__ gdsum, double inv_scale,
+                                                   int update, __bf16 *__restrict__ cb, float *__restrict__ cnorm,
+                                                   unsigned long long *__restrict__ csnap)
+{
+    constexpr int KW = 16 * (KS + 1);
+    const int k = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (k >= K_pad) return;
+    if (k >= K) {  // padding row: never the minimum
+        for (int e = lane; e < KW; e += kWave)
+            cb[(int64_t)k * KW + e] = (__bf16)((e == 16 * KS) ? INFINITY : 0.0f);
+        return;
+    }
+    const long long n = (long long)gcnt[k];
+    double ss = 0.0;
+    for (int d = lane; d < D; d += kWave) {
+        float v = c[(int64_t)k * D + d];
+        if (update && n > 0) v = (float)(((double)(long long)gsum[(int64_t)k * D + d] * inv_scale) / (double)n);
+        if (update) c[(int64_t)k * D + d] = v;
+        ss += (double)v * (double)v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, kWave);
+    for (int d = lane; d < 16 * KS; d += kWave) {
+        const float v = d < D ? c[(int64_t)k * D + d] : 0.0f;
+        cb[(int64_t)k * KW + d] = (__bf16)(-2.0f * v);
+    }
+    if (lane < 16) {
+        const __bf16 n1 = (__bf16)(float)ss;
+        const double r1 = ss - (double)(float)n1;
+        const __bf16 n2 = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)n2;
+        const __bf16 n3 = (__bf16)(float)r2;
+        __bf16 v = (__bf16)0.0f;
+        if (lane == 0) v = n1;
+        if (lane == 1) v = n2;
+        if (lane == 2) v = n3;
+        cb[(int64_t)k * KW + 16 * KS + lane] = v;
+    }
+    if (lane == 0) {
+        cnorm[k] = (float)sqrt(ss) * 1.0001f;
+        if (csnap) csnap[k] = (unsigned long long)n;
+    }
+    // clear the reduction object for the next pass (a fill, not a memset, so
+    // the sequence can be captured in a graph)
+    for (int d = lane; d < D; d += kWave) gsum[(int64_t)k * D + d] = 0;
+    if (lane == 0) {
+        gcnt[k] = 0;
+        gdsum[k] = 0;
+    }
+}
+
+__global__ void km_maxabs(const float *__restrict__ x, int64_t n, unsigned int *__restrict__ out)
+{
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+    if (threadIdx.x % kWave == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void km_zero_u64(unsigned long long *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+namespace pat {
+
+// Fixed-point shift for n values of magnitude < maxabs: the largest possible
+// sum n * maxabs * 2^s stays below 2^62.  Same formula as the oracle's.
+int kmeans_fixed_shift(int64_t n, float maxabs)
+{
+    int eN = 0, eM = 0;
+    while ((1LL << eN) < n) eN++;
+    double m = maxabs > 0 ? maxabs : 1.0f;
+    frexp(m, &eM);
+    int s = 62 - eN - eM;
+    if (s > 60) s = 60;
+    return s;
+}
+
+int kmeans_ks_for(int D)
+{
+    if (D <= 16) return 1;
+    if (D <= 32) return 2;
+    if (D <= 64) return 4;
+    if (D <= 128) return 8;
+    return 0;
+}
+
+int kmeans_kb_for(int ks) { return ks <= 2 ? 4 : (ks == 4 ? 2 : 1); }
+
+// LDS bytes of the reduction object and the staged centres.
+static int64_t object_bytes(int K, int D) { return 8LL * object_words(K, D); }
+
+int kmeans_maxabs(const float *x, int64_t n, unsigned int *out_bits, hipStream_t st)
+{
+    hipLaunchKernelGGL(km_zero_u64, dim3(1), dim3(1), 0, st, (unsigned long long *)out_bits, 1);
+    PAT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(km_maxabs, dim3(1024), dim3(256), 0, st, x, n, out_bits);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+
+static KmArgs make_args(const KmDev &d)
+{
+    KmArgs a;
+    a.x = d.x;
+    a.c = d.c;
+    a.N = d.N;
+    a.D = d.D;
+    a.K = d.K;
+    a.scale = ldexp(1.0, d.shift);
+    a.dscale = ldexp(1.0, d.dshift);
+    a.assign = d.assign;
+    a.mindist = d.mindist;
+    a.gsum = d.gsum;
+    a.gcnt = d.gcnt;
+    a.gdsum = d.gdsum;
+    a.use_lds_acc = object_bytes(d.K, d.D) <= 96 * 1024;
+    return a;
+}
+
+int kmeans_finalize(const KmDev &d, int update, hipStream_t st)
+{
+    const double inv = ldexp(1.0, -d.shift);
+    const dim3 g((unsigned)((d.K_pad + 3) / 4)), b(256);
+    switch (d.KS) {
+    case 1: hipLaunchKernelGGL(km_finalize<1>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
+    case 2: hipLaunchKernelGGL(km_finalize<2>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
+    case 4: hipLaunchKernelGGL(km_finalize<4>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
+    case 8: hipLaunchKernelGGL(km_finalize<8>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
+    default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
+    }
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int kmeans_assign_plain(const KmDev &d, hipStream_t st)
+{
+    KmArgs a = make_args(d);
+    const int64_t cbytes = 4LL * d.K * d.D;
+    int64_t lds = a.use_lds_acc ? object_bytes(d.K, d.D) : 0;
+    int in_lds = lds + cbytes <= 120 * 1024;
+    if (in_lds) lds += cbytes;
+    const int64_t want = (d.N + 511) / 512;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
+#define PAT_KP(DM)                                                                                                     \
+    hipLaunchKernelGGL(km_plain<DM>, dim3(grid), dim3(512), (size_t)lds, st, a, in_lds)
+    if (d.D <= 8) PAT_KP(8);
+    else if (d.D <= 16) PAT_KP(16);
+    else if (d.D <= 32) PAT_KP(32);
+    else if (d.D <= 64) PAT_KP(64);
+    else if (d.D <= 128) PAT_KP(128);
+    else {
+        pat::set_error("kmeans: D = %d above 128", d.D);
+        return PAT_ERR_UNSUPPORTED;
+    }
+#undef PAT_KP
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// Whether the MFMA path can run this shape (BF16 rows staged in LDS).
+bool kmeans_mfma_fits(int K, int D)
+{
+    const int ks = kmeans_ks_for(D);
+    if (!ks) return false;
+    const int K_pad = (K + 31) / 32 * 32;
+    const int64_t cbb = 2LL * K_pad * 16 * (ks + 1);
+    const int64_t obj = object_bytes(K, D) <= 96 * 1024 ? object_bytes(K, D) : 0;
+    return cbb + obj <= 150 * 1024;
+}
+
+int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
+{
+    KmArgs a = make_args(d);
+    PAT_REQUIRE(kmeans_mfma_fits(d.K, d.D), PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K,
+                d.D);
+    const int64_t cbb = 2LL * d.K_pad * 16 * (d.KS + 1);
+    const int64_t lds = (a.use_lds_acc ? object_bytes(d.K, d.D) : 0) + cbb;
+    // MFMA error-bound constants (see the file header)
+    const double u = ldexp(1.0, -8), g = 2.0 * (16.0 * (d.KS + 1)) * ldexp(1.0, -24);
+    MfmaBound bd;
+    bd.c_dot = (float)((4 * u + 2 * u * u + 4 * g * (1 + u) * (1 + u)) * 1.01);
+    bd.c_norm = (float)((u * u * u + 2 * g * (1 + 2 * u)) * 1.01);
+    bd.c_ref = (float)((3.0 + d.D + 16.0) * ldexp(1.0, -24) * 1.01);
+    const int64_t want = (d.N + 31) / 32 / 8;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
+#define PAT_KM(KS_, KB_)                                                                                               \
+    hipLaunchKernelGGL((km_mfma<KS_, KB_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd,      \
+                       d.rescored)
+    switch (d.KS) {
+    case 1: PAT_KM(1, 4); break;
+    case 2: PAT_KM(2, 4); break;
+    case 4: PAT_KM(4, 2); break;
+    case 8: PAT_KM(8, 1); break;
+    default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
+    }
+#undef PAT_KM
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int kmeans_zero(unsigned long long *p, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return PAT_OK;
+    hipLaunchKernelGGL(km_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+}  // namespace pat

@@ -216,9 +216,9 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
         rc = PAT_ERR_HIP;
     }
     if (rc == PAT_OK) {
-        hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps - 1]);
+        (void)hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps - 1]);
         if (kernel_ms)
-            for (int k = 0; k < nsteps; k++) hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
+            for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
         s->cur = c;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

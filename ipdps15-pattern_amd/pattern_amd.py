@@ -21,6 +21,8 @@ PAT_IR_LJ_MINIMAGE = 0
 PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0
 PAT_IR_ATOMIC = 1
+PAT_KM_PLAIN = 0
+PAT_KM_MFMA = 1
 
 _lib = None
 
@@ -95,6 +97,16 @@ SIGNATURES = {
     "pat_md_energy": (c_int, [c_vp, c_dp, c_dp]),
     "pat_md_device_buffers": (c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     "pat_md_destroy": (c_int, [c_vp]),
+    "pat_kmeans_fixed_shift": (c_int, [c_i64, c_float, c_intp]),
+    "pat_kmeans_create": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, ctypes.POINTER(c_vp)]),
+    "pat_kmeans_set_mode": (c_int, [c_vp, c_int]),
+    "pat_kmeans_set_centroids": (c_int, [c_vp, c_vp]),
+    "pat_kmeans_run": (c_int, [c_vp, c_int]),
+    "pat_kmeans_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
+    "pat_kmeans_get": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pat_kmeans_get_object": (c_int, [c_vp, c_vp, c_vp, c_vp, c_intp, c_intp]),
+    "pat_kmeans_stats": (c_int, [c_vp, c_i64p, c_fp]),
+    "pat_kmeans_destroy": (c_int, [c_vp]),
 }
 
 
@@ -470,3 +482,91 @@ def moldyn_system(ncells, rho=0.8442, jitter=0.05, cutoff=2.5, seed=42, shuffle=
     pos, box = gen_fcc(ncells, ncells, ncells, rho, jitter, seed)
     edges = gen_pairs(pos, box[0], cutoff, shuffle=shuffle, seed=seed + 1)
     return pos, float(box[0]), edges
+
+
+# ---------------------------------------------------------------- kmeans
+
+def kmeans_fixed_shift(n, maxabs):
+    v = c_int(0)
+    call("pat_kmeans_fixed_shift", n, float(maxabs), ctypes.byref(v))
+    return v.value
+
+
+class KMeans:
+    """Handle mirror of RegularRuntime running kmeans map1/reduce1 plus the
+    Lloyd update (pat_kmeans_*).  x: host (N, D) float32 array or a
+    DeviceArray (read in place)."""
+
+    def __init__(self, x, k, mode=PAT_KM_MFMA):
+        if isinstance(x, DeviceArray):
+            self.n, self.d = x.shape
+            src, on_dev = x.ptr, 1
+            self._keep = x
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            self.n, self.d = x.shape
+            src, on_dev = x.ctypes.data, 0
+        self.k = k
+        h = c_vp()
+        call("pat_kmeans_create", src, self.n, self.d, k, mode, on_dev, ctypes.byref(h))
+        self.h = h
+
+    def set_mode(self, mode):
+        call("pat_kmeans_set_mode", self.h, mode)
+
+    def set_centroids(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float32)
+        call("pat_kmeans_set_centroids", self.h, c.ctypes.data)
+
+    def run(self, iters):
+        call("pat_kmeans_run", self.h, iters)
+
+    def run_timed(self, iters):
+        tot = c_float(0)
+        k = np.zeros(iters, dtype=np.float32)
+        call("pat_kmeans_run_timed", self.h, iters, ctypes.byref(tot), k.ctypes.data_as(c_fp))
+        return tot.value, k
+
+    def get(self):
+        c = np.empty((self.k, self.d), dtype=np.float32)
+        a = np.empty(self.n, dtype=np.int32)
+        md = np.empty(self.n, dtype=np.float32)
+        call("pat_kmeans_get", self.h, c.ctypes.data, a.ctypes.data, md.ctypes.data)
+        return c, a, md
+
+    def centroids(self):
+        c = np.empty((self.k, self.d), dtype=np.float32)
+        call("pat_kmeans_get", self.h, c.ctypes.data, None, None)
+        return c
+
+    def get_object(self):
+        sums = np.empty((self.k, self.d), dtype=np.int64)
+        counts = np.empty(self.k, dtype=np.int64)
+        dsums = np.empty(self.k, dtype=np.int64)
+        s, ds = c_int(0), c_int(0)
+        call("pat_kmeans_get_object", self.h, sums.ctypes.data, counts.ctypes.data, dsums.ctypes.data,
+             ctypes.byref(s), ctypes.byref(ds))
+        return sums, counts, dsums, s.value, ds.value
+
+    def stats(self):
+        r = c_i64(0)
+        m = c_float(0)
+        call("pat_kmeans_stats", self.h, ctypes.byref(r), ctypes.byref(m))
+        return r.value, m.value
+
+    def close(self):
+        if self.h:
+            call("pat_kmeans_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kmeans_shifts(n, d, maxabs):
+    """(coordinate shift, distance shift) the handle derives from max |x|."""
+    return (kmeans_fixed_shift(n, maxabs),
+            kmeans_fixed_shift(n, float(np.float32(2.0 * float(maxabs) * np.sqrt(float(d))))))

@@ -1,0 +1,124 @@
+"""GPU parity of the generalized-reduction engine (kmeans) vs the oracle.
+
+Bar: assignments, min_dist values, the reduction object (fixed-point sums,
+counts, distance sums) and the updated centres are bit-exact against the
+oracle, for the plain FP32 kernel and for the BF16-MFMA kernel with FP32
+rescoring.  The error of the FP32 centres against an FP64 Lloyd run is
+printed for the mid-size case.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_iters(orc, pa, x, k, iters):
+    maxabs = float(np.max(np.abs(x)))
+    s, ds = pa.kmeans_shifts(len(x), x.shape[1], maxabs)
+    c = x[:k].copy()
+    for _ in range(iters):
+        a, md = orc.kmeans_assign(x, c)
+        sums, counts, dsum = orc.kmeans_accumulate(x, a, k, s, mindist=md, dshift=ds)
+        c_next = orc.kmeans_update(sums, counts, s, c)
+        yield c, a, md, sums, counts, dsum
+        c = c_next
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n,d,k", [(1, 3, 1), (31, 3, 5), (1000, 3, 40), (4099, 8, 17), (20000, 16, 64),
+                                   (5003, 32, 256), (3000, 33, 70), (2000, 64, 100), (1500, 128, 33),
+                                   (3000, 32, 300)])
+def test_single_pass_bitexact(pa, orc, gpu, mode, n, d, k):
+    x = pa.gen_gmm(n, d, max(1, k // 2), sigma=1.0, seed=n + d + k)
+    km = pa.KMeans(x, k, mode=mode)
+    km.run(1)
+    c1, a, md = km.get()
+    sums, counts, dsums, s, ds = km.get_object()
+    km.close()
+    c0, ra, rmd, rsums, rcounts, rdsum = next(oracle_iters(orc, pa, x, k, 1))
+    assert np.array_equal(a, ra)
+    assert np.array_equal(md, rmd)
+    assert np.array_equal(counts, rcounts) and np.array_equal(sums, rsums) and np.array_equal(dsums, rdsum)
+    assert np.array_equal(c1, orc.kmeans_update(rsums, rcounts, s, c0))
+
+
+def adversarial(kind, rng):
+    if kind == "duplicates":  # repeated centres -> exact ties
+        x = rng.normal(size=(4000, 32)).astype(np.float32)
+        x[:64] = x[0]
+        return x, 64
+    if kind == "lattice":  # integer coordinates: many equal distances
+        return rng.integers(-3, 4, size=(6000, 32)).astype(np.float32), 128
+    if kind == "large":  # big norms make the BF16 error bound wide
+        return (rng.normal(size=(5000, 32)) * 0.5 + 3000).astype(np.float32), 96
+    if kind == "close":  # many centres within one tight cluster
+        return (rng.normal(size=(8000, 32)) * 1e-3).astype(np.float32), 256
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["duplicates", "lattice", "large", "close"])
+def test_mfma_adversarial_bitexact(pa, orc, gpu, kind):
+    x, k = adversarial(kind, np.random.default_rng(7))
+    out = []
+    for mode in (0, 1):
+        km = pa.KMeans(x, k, mode=mode)
+        km.run(3)
+        out.append(km.get() + km.get_object()[:3])
+        km.close()
+    for p, m in zip(*out):
+        assert np.array_equal(p, m)
+    ref = list(oracle_iters(orc, pa, x, k, 3))[-1]
+    assert np.array_equal(out[1][1], ref[1])
+
+
+def test_lloyd_20_iterations_midsize(pa, orc, gpu):
+    """N = 200,000, D = 32, K = 256 from a 256-component mixture, 20 iterations:
+    plain and MFMA GPU runs and the oracle agree bit for bit."""
+    x = pa.gen_gmm(200000, 32, 256, sigma=1.0, lo=-10, hi=10, seed=11)
+    res = []
+    for mode in (0, 1):
+        km = pa.KMeans(x, 256, mode=mode)
+        km.run(20)
+        res.append(km.get())
+        resc = km.stats()[0]
+        km.close()
+    assert all(np.array_equal(p, m) for p, m in zip(*res))
+    *_, last = oracle_iters(orc, pa, x, 256, 20)
+    c19, a20, md20 = last[0], last[1], last[2]
+    assert np.array_equal(res[1][1], a20) and np.array_equal(res[1][2], md20)
+    maxabs = float(np.max(np.abs(x)))
+    s, _ = pa.kmeans_shifts(len(x), 32, maxabs)
+    sums, counts, _ = orc.kmeans_accumulate(x, a20, 256, s)
+    assert np.array_equal(res[1][0], orc.kmeans_update(sums, counts, s, c19))
+    c64, _ = orc.kmeans_lloyd_f64(x, 256, 20)
+    err = np.max(np.abs(res[1][0] - c64)) / np.max(np.abs(c64))
+    print(f"kmeans 200k x32 K256 x20: rescored {resc} point-passes ({resc / (20 * len(x)):.2%}); "
+          f"centre rel err vs FP64 Lloyd {err:.2e}")
+
+
+def test_kmeans_workload_full_size(pa, orc, gpu):
+    """kmeans workload: N = 2^24, D = 32, K = 256 (256-component mixture,
+    sigma 1, centres in [-10, 10]^32), 20 iterations.  MFMA and plain agree
+    bit for bit; the last iteration's assignment (on a 65,536-point sample)
+    and centre update are checked against the oracle."""
+    n, d, k = 1 << 24, 32, 256
+    x = pa.gen_gmm(n, d, 256, sigma=1.0, lo=-10, hi=10, seed=2024)
+    xd = pa.DeviceArray.from_numpy(x)
+    res = []
+    for mode in (1, 0):
+        km = pa.KMeans(xd, k, mode=mode)
+        km.run(19)
+        c19 = km.centroids()
+        km.run(1)
+        res.append((c19,) + km.get())
+        km.close()
+    for p, m in zip(*res):
+        assert np.array_equal(p, m)
+    c19, c20, a20, md20 = res[0]
+    idx = np.random.default_rng(0).choice(n, 65536, replace=False)
+    ra, rmd = orc.kmeans_assign(x[idx], c19)
+    assert np.array_equal(a20[idx], ra) and np.array_equal(md20[idx], rmd)
+    s, _ = pa.kmeans_shifts(n, d, float(np.max(np.abs(x))))
+    sums, counts, _ = orc.kmeans_accumulate(x, a20, k, s)
+    assert counts.sum() == n
+    assert np.array_equal(c20, orc.kmeans_update(sums, counts, s, c19))
