@@ -1,0 +1,119 @@
+// IrregularRuntime (kmeans/irregular/irregular_runtime.h:12-112): the
+// irregular-reduction runtime.  Same constructor and calls; the edge functor
+// selected by set_map_idx runs on the GPU through libpattern and results come
+// back in the caller's node order (the caller's arrays are never permuted).
+//
+// Each application compiles its own functor table (the reference builds one
+// binary per app directory).  An application selects its table by defining
+// PAT_IR_MAP_TABLE before including this header; the default is moldyn's
+// (kmeans/irregular.h:116, {map}: minimum-image LJ with
+// parameter struct para {double side; double cutoffRadius;}).  miniMD's table
+// (miniMD/irregular.h:172, {map1}: LJ without images, parameter double
+// cutforcesq) is {PAT_IR_LJ_OPEN}.
+#ifndef IRREGULAR_IRREGULAR_RUNTIME_H_
+#define IRREGULAR_IRREGULAR_RUNTIME_H_
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../lib/common.h"
+#include "../pat_check.h"
+#include "data_type.h"
+
+#ifndef PAT_IR_MAP_TABLE
+#define PAT_IR_MAP_TABLE {PAT_IR_LJ_MINIMAGE}
+#endif
+
+class IrregularRuntime {
+  public:
+    IRIndex global_num_edges_;
+    IRIndex global_num_nodes_;
+    EDGE *global_edges_;
+    void *global_edge_data_;
+    void *global_node_data_;
+    int edge_data_elm_size_;
+    int node_data_elm_size_;
+    int reduction_elm_size_;
+    int node_num_dims_;
+    void *node_coordinates_;
+    int coordinate_size_;
+    int map_idx_;
+    int reduce_idx_;
+    int num_procs_;
+    int current_iter_;
+    int num_iters_;
+    int parameter_size_;
+    void *parameter_;
+
+    IrregularRuntime(IRIndex num_edges, IRIndex num_nodes, EDGE *edges, void *edge_data, void *node_data,
+                     int edge_data_elm_size, int node_data_elm_size, int reduction_elm_size, int node_num_dims,
+                     void *node_coordinates, int coordinate_size, void *parameter, int parameter_size, int num_procs,
+                     int num_iters)
+        : global_num_edges_(num_edges), global_num_nodes_(num_nodes), global_edges_(edges),
+          global_edge_data_(edge_data), global_node_data_(node_data), edge_data_elm_size_(edge_data_elm_size),
+          node_data_elm_size_(node_data_elm_size), reduction_elm_size_(reduction_elm_size),
+          node_num_dims_(node_num_dims), node_coordinates_(node_coordinates), coordinate_size_(coordinate_size),
+          map_idx_(0), reduce_idx_(0), num_procs_(num_procs), current_iter_(0), num_iters_(num_iters),
+          parameter_size_(parameter_size), parameter_(parameter), handle_(nullptr)
+    {
+    }
+
+    ~IrregularRuntime()
+    {
+        if (handle_) pat_ir_destroy(handle_);
+    }
+
+    // Uploads the edge list and builds the by-destination CSR on the GPU
+    // (the reference's reorder + partition + halo set-up).
+    void IrregularInit()
+    {
+        const int table[] = PAT_IR_MAP_TABLE;
+        if (map_idx_ < 0 || map_idx_ >= (int)(sizeof(table) / sizeof(table[0]))) fail("map index not in the table");
+        PAT_API_CALL(pat_ir_create(global_num_edges_, global_num_nodes_, (const int32_t *)global_edges_,
+                                   table[map_idx_], parameter_, parameter_size_, &handle_));
+        if (global_node_data_) reset_node_data(global_node_data_);
+    }
+
+    void set_map_idx(int idx)
+    {
+        map_idx_ = idx;
+        if (handle_) {
+            const int table[] = PAT_IR_MAP_TABLE;
+            if (idx < 0 || idx >= (int)(sizeof(table) / sizeof(table[0]))) fail("map index not in the table");
+            PAT_API_CALL(pat_ir_set_map_idx(handle_, table[idx]));
+            PAT_API_CALL(pat_ir_set_param(handle_, parameter_, parameter_size_));
+        }
+    }
+    void set_reduce_idx(int idx) { reduce_idx_ = idx; }
+
+    // One pass of the edge functor over every edge, reducing into nodes.
+    void IrregularStart()
+    {
+        if (!handle_) fail("IrregularStart before IrregularInit");
+        PAT_API_CALL(pat_ir_start(handle_));
+        current_iter_++;
+    }
+
+    void IrregularBarrier() { PAT_API_CALL(pat_device_sync()); }
+
+    // VALUE (float3) per node in the caller's order.
+    void get_reduction_result(void *buffer) { PAT_API_CALL(pat_ir_get_result(handle_, (float *)buffer)); }
+
+    // New node coordinates (node_data_elm_size bytes per node: 12 = float3,
+    // 24 = double3 as moldyn's point_data).
+    void reset_node_data(void *node_data)
+    {
+        PAT_API_CALL(pat_ir_reset_node_data(handle_, node_data, node_data_elm_size_));
+    }
+
+  private:
+    static void fail(const char *msg)
+    {
+        fprintf(stderr, "pattern error: IrregularRuntime: %s\n", msg);
+        exit(EXIT_FAILURE);
+    }
+    pat_ir *handle_;
+};
+
+#endif

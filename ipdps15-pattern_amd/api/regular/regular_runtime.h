@@ -1,0 +1,142 @@
+// RegularRuntime (kmeans/regular/regular_runtime.h:11-106): the
+// generalized-reduction runtime.  Same constructor and calls; the map/reduce
+// functor pair selected by set_map_idx/set_reduce_idx runs on the GPU through
+// libpattern.  Table of the kmeans application (kmeans/regular.h:143-145):
+// index 1 = map1/reduce1 (nearest centre, emit {x, 1, min_dist}).
+// The parameter is the K x D centre array; RegularStart() is one map+reduce
+// pass over every unit; RegularGetOutput() returns, per key present, the
+// reduced value {sum x_0..x_{D-1}, count, sum min_dist} as floats, the
+// layout reduce1 produces (kmeans/regular.h:121-136).
+#ifndef REGULAR_RUNTIME_H_
+#define REGULAR_RUNTIME_H_
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../pat_check.h"
+#include "data_type.h"
+
+class RegularRuntime {
+  public:
+    int num_procs_;
+    size_t num_offsets_;
+    size_t input_size_;
+    int parameter_size_;
+    int map_idx_;
+    int reduce_idx_;
+    void *input_;
+    Offset *offsets_;
+    void *parameter_;
+
+    RegularRuntime(int num_procs, void *input, size_t input_size, Offset *offsets, size_t offset_number,
+                   void *parameters, int parameter_size)
+        : num_procs_(num_procs), num_offsets_(offset_number), input_size_(input_size), parameter_size_(parameter_size),
+          map_idx_(0), reduce_idx_(0), input_(input), offsets_(offsets), parameter_(parameters), handle_(nullptr),
+          dim_(0), k_(0)
+    {
+        out_ = output{};
+    }
+
+    ~RegularRuntime()
+    {
+        if (handle_) pat_kmeans_destroy(handle_);
+    }
+
+    // Validates the units and uploads them.  Units of equal size laid out
+    // back to back are used in place; any other layout is gathered first.
+    void RegularInit()
+    {
+        if (num_offsets_ == 0 || offsets_ == nullptr) fail("RegularInit: no units");
+        const size_t usz = offsets_[0].size;
+        if (usz == 0 || usz % sizeof(float)) fail("RegularInit: unit size must be a whole number of floats");
+        for (size_t i = 0; i < num_offsets_; i++)
+            if (offsets_[i].size != usz || offsets_[i].offset + usz > input_size_)
+                fail("RegularInit: units must have equal sizes inside the input");
+        dim_ = (int)(usz / sizeof(float));
+        bool packed = true;
+        for (size_t i = 0; i < num_offsets_ && packed; i++) packed = offsets_[i].offset == i * usz;
+        if (!packed) {
+            packed_.resize(num_offsets_ * dim_);
+            for (size_t i = 0; i < num_offsets_; i++)
+                memcpy(&packed_[i * dim_], (const char *)input_ + offsets_[i].offset, usz);
+        }
+        src_ = packed ? (const float *)input_ : packed_.data();
+    }
+
+    void set_map_idx(int idx) { map_idx_ = idx; }
+    void set_reduce_idx(int idx) { reduce_idx_ = idx; }
+
+    void RegularStart()
+    {
+        if (map_idx_ != 1 || reduce_idx_ != 1)
+            fail("RegularStart: only the kmeans map1/reduce1 pair (index 1) has a GPU engine");
+        if (parameter_size_ <= 0 || parameter_size_ % (int)(sizeof(float) * dim_))
+            fail("RegularStart: the parameter must be K x D floats");
+        const int k = parameter_size_ / (int)(sizeof(float) * dim_);
+        if (!handle_ || k != k_) {
+            if (handle_) pat_kmeans_destroy(handle_);
+            const char *m = getenv("PAT_KMEANS_MODE");
+            const int mode = m && atoi(m) == 0 ? PAT_KM_PLAIN : PAT_KM_MFMA;
+            PAT_API_CALL(pat_kmeans_create(src_, (int64_t)num_offsets_, dim_, k, mode, 0, &handle_));
+            k_ = k;
+        }
+        PAT_API_CALL(pat_kmeans_set_centroids(handle_, (const float *)parameter_));
+        PAT_API_CALL(pat_kmeans_run(handle_, 1));
+    }
+
+    struct output RegularGetOutput()
+    {
+        if (!handle_) fail("RegularGetOutput: RegularStart has not run");
+        std::vector<int64_t> sums((size_t)k_ * dim_), counts(k_), dsums(k_);
+        int s = 0, ds = 0;
+        PAT_API_CALL(pat_kmeans_get_object(handle_, sums.data(), counts.data(), dsums.data(), &s, &ds));
+        keys_.clear();
+        vals_.clear();
+        for (int key = 0; key < k_; key++) {
+            if (counts[key] == 0) continue;
+            keys_.push_back((unsigned int)key);
+            for (int d = 0; d < dim_; d++) vals_.push_back((float)std::ldexp((double)sums[(size_t)key * dim_ + d], -s));
+            vals_.push_back((float)counts[key]);
+            vals_.push_back((float)std::ldexp((double)dsums[key], -ds));
+        }
+        kidx_.resize(keys_.size());
+        vidx_.resize(keys_.size());
+        for (size_t i = 0; i < keys_.size(); i++) {
+            kidx_[i] = (unsigned int)(i * sizeof(unsigned int));
+            vidx_[i] = (unsigned int)(i * (dim_ + 2) * sizeof(float));
+        }
+        out_.output_keys = (char *)keys_.data();
+        out_.output_vals = (char *)vals_.data();
+        out_.key_index = kidx_.data();
+        out_.val_index = vidx_.data();
+        out_.key_num = (unsigned int)keys_.size();
+        out_.key_size = sizeof(unsigned int);
+        out_.value_size = (unsigned int)((dim_ + 2) * sizeof(float));
+        return out_;
+    }
+
+    // Assignment of every unit from the last pass (not in the reference API;
+    // the kmeans driver uses it to report cluster sizes).
+    void get_keys(int *keys)
+    {
+        if (!handle_) fail("get_keys: RegularStart has not run");
+        PAT_API_CALL(pat_kmeans_get(handle_, nullptr, keys, nullptr));
+    }
+
+  private:
+    static void fail(const char *msg)
+    {
+        fprintf(stderr, "pattern error: %s\n", msg);
+        exit(EXIT_FAILURE);
+    }
+    pat_kmeans *handle_;
+    int dim_, k_;
+    const float *src_ = nullptr;
+    std::vector<float> packed_;
+    std::vector<unsigned int> keys_, kidx_, vidx_;
+    std::vector<float> vals_;
+    struct output out_;
+};
+
+#endif

@@ -1,0 +1,95 @@
+// moldyn driver over the irregular-reduction API: the step loop of
+// kmeans/moldyn.cu:331-403 (host coordinate update, reset_node_data,
+// IrregularStart, get_reduction_result, host velocity update) as
+// velocity-Verlet, on an FCC lattice of M^3 cells (default 16: 16,384
+// molecules) at density 0.8442, jitter 0.05, cutoff 2.5, dt 0.005, 50 steps.
+// Node data is double3 as moldyn's point_data.  The first force pass is
+// checked against the host map functor.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "irregular.h"
+#include "include/runtime.h"
+#include "irregular/irregular_runtime.h"
+
+int main(int argc, char *argv[])
+{
+    const int M = argc > 1 ? atoi(argv[1]) : 16;
+    const int steps = argc > 2 ? atoi(argv[2]) : 50;
+    const double dt = 0.005, hdt = 0.5 * dt;
+    RuntimeInit(argc, argv);
+
+    const long n = 4L * M * M * M;
+    std::vector<float> pos3(3 * n);
+    double box[3];
+    PAT_API_CALL(pat_gen_fcc(M, M, M, 0.8442, 0.05, 42, pos3.data(), box));
+    int64_t ne = 0;
+    PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], 2.5, 1, 43, nullptr, &ne));
+    std::vector<EDGE> edges(ne);
+    PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], 2.5, 1, 43, (int32_t *)edges.data(), &ne));
+
+    std::vector<point_data> x(n);
+    for (long i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) x[i].points[c] = pos3[3 * i + c];
+    std::vector<double> v(3 * n, 0.0);
+    std::vector<VALUE> f(n);
+    struct para pa = {box[0], 2.5};
+
+    IrregularRuntime runtime(ne, n, edges.data(), nullptr, x.data(), 0, sizeof(point_data), sizeof(VALUE), 3, nullptr,
+                             0, &pa, sizeof(pa), 1, steps);
+    runtime.set_map_idx(0);
+    runtime.set_reduce_idx(0);
+    runtime.IrregularInit();
+    runtime.IrregularStart();
+    runtime.get_reduction_result(f.data());
+
+    // host functor check of the first force pass
+    {
+        FCPU::Cobject obj(n);
+        for (long e = 0; e < ne; e++)
+            FCPU::map_function_table[0](&obj, pos3.data(), nullptr, edges.data(), &pa, (unsigned)e, 0,
+                                        FCPU::reduce_function_table[0]);
+        double fmax = 0, err = 0;
+        for (long i = 0; i < n; i++) {
+            const VALUE h = obj.used(i) ? obj.value(i) : VALUE{0, 0, 0};
+            fmax = std::fmax(fmax, std::fabs(h.x));
+            err = std::fmax(err, std::fabs(h.x - f[i].x));
+            err = std::fmax(err, std::fmax(std::fabs(h.y - f[i].y), std::fabs(h.z - f[i].z)));
+        }
+        printf("moldyn: %ld molecules, %ld pairs; first pass max |dF| / max |F| = %.2e\n", n, (long)ne, err / fmax);
+        if (err > 1e-5 * fmax) return 1;
+    }
+
+    const double L = box[0];
+    for (int step = 0; step < steps; step++) {
+        for (long i = 0; i < n; i++) {
+            const float fi[3] = {f[i].x, f[i].y, f[i].z};
+            for (int c = 0; c < 3; c++) {
+                v[3 * i + c] += hdt * fi[c];
+                double p = x[i].points[c] + dt * v[3 * i + c];
+                if (p < 0) p += L;
+                else if (p >= L) p -= L;
+                x[i].points[c] = p;
+            }
+        }
+        runtime.reset_node_data(x.data());
+        runtime.IrregularStart();
+        runtime.get_reduction_result(f.data());
+        for (long i = 0; i < n; i++) {
+            v[3 * i] += hdt * f[i].x;
+            v[3 * i + 1] += hdt * f[i].y;
+            v[3 * i + 2] += hdt * f[i].z;
+        }
+        runtime.IrregularBarrier();
+    }
+    double ekin = 0, px = 0;
+    for (long i = 0; i < n; i++) {
+        ekin += 0.5 * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        px += v[3 * i];
+    }
+    printf("moldyn: %d steps, kinetic energy %.6e, total x momentum %.2e\n", steps, ekin, px);
+    RuntimeFinalize();
+    return 0;
+}

@@ -530,6 +530,27 @@ static KmArgs make_args(const KmDev &d)
     return a;
 }
 
+// Raises every kmeans kernel's dynamic-LDS limit to the 160 KB a gfx950
+// workgroup can address.  Called once before any launch (outside graph
+// capture).
+int kmeans_prepare_kernels()
+{
+    static int done = 0;
+    if (done) return PAT_OK;
+    const void *fns[] = {(const void *)km_plain<8>, (const void *)km_plain<16>, (const void *)km_plain<32>,
+                         (const void *)km_plain<64>, (const void *)km_plain<128>, (const void *)km_mfma<1, 4>,
+                         (const void *)km_mfma<2, 4>, (const void *)km_mfma<4, 2>, (const void *)km_mfma<8, 1>};
+    for (const void *f : fns) PAT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done = 1;
+    return PAT_OK;
+}
+
+static int allow_lds(const void *, int64_t bytes)
+{
+    PAT_REQUIRE(bytes <= 160 * 1024, PAT_ERR_UNSUPPORTED, "kmeans: %lld bytes of LDS requested", (long long)bytes);
+    return PAT_OK;
+}
+
 int kmeans_finalize(const KmDev &d, int update, hipStream_t st)
 {
     const double inv = ldexp(1.0, -d.shift);
@@ -555,7 +576,10 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
     const int64_t want = (d.N + 511) / 512;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
 #define PAT_KP(DM)                                                                                                     \
-    hipLaunchKernelGGL(km_plain<DM>, dim3(grid), dim3(512), (size_t)lds, st, a, in_lds)
+    do {                                                                                                               \
+        PAT_TRY(allow_lds((const void *)km_plain<DM>, lds));                                                           \
+        hipLaunchKernelGGL(km_plain<DM>, dim3(grid), dim3(512), (size_t)lds, st, a, in_lds);                           \
+    } while (0)
     if (d.D <= 8) PAT_KP(8);
     else if (d.D <= 16) PAT_KP(16);
     else if (d.D <= 32) PAT_KP(32);
@@ -597,8 +621,11 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     const int64_t want = (d.N + 31) / 32 / 8;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
 #define PAT_KM(KS_, KB_)                                                                                               \
-    hipLaunchKernelGGL((km_mfma<KS_, KB_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd,      \
-                       d.rescored)
+    do {                                                                                                               \
+        PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_>, lds));                                                      \
+        hipLaunchKernelGGL((km_mfma<KS_, KB_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb,       \
+                           d.cnorm, d.K_pad, bd, d.rescored);                                                          \
+    } while (0)
     switch (d.KS) {
     case 1: PAT_KM(1, 4); break;
     case 2: PAT_KM(2, 4); break;

@@ -30,5 +30,6 @@ int kmeans_finalize(const KmDev &d, int update, hipStream_t st);
 int kmeans_assign_plain(const KmDev &d, hipStream_t st);
 int kmeans_assign_mfma(const KmDev &d, hipStream_t st);
 int kmeans_zero(unsigned long long *p, int64_t n, hipStream_t st);
+int kmeans_prepare_kernels();
 
 }  // namespace pat

@@ -79,6 +79,7 @@ int amalloc(void **p, size_t b)
 
 int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int x_on_device)
 {
+    PAT_TRY(pat::kmeans_prepare_kernels());
     PAT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     h->mode = mode;
     auto &D = h->d;
