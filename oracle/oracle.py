@@ -193,3 +193,35 @@ def kmeans_lloyd_f64(x, k, iters, threads=0):
     a = np.empty(n, dtype=np.int32)
     lib().orc_kmeans_lloyd_f64(n, d, k, _p(x), iters, _p(c), _p(a), threads)
     return c, a
+
+
+# ---------------------------------------------------------------- reference checker
+
+REF_PATH = os.path.join(_HERE, "_ref", "libref_minimd.so")
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref_minimd_system(n, rho=0.8442, nbin=None, cutneigh=2.8, cutforce=2.5):
+    """miniMD's own create_atoms + Neighbor::build + Force::compute (built
+    from the reference sources into oracle/_ref).  Returns (pos, force,
+    pairs) in FP64; the reference's force omits the 48*eps prefactor."""
+    L = ctypes.CDLL(REF_PATH)
+    f = L.ref_minimd_system
+    f.restype = c_int
+    f.argtypes = [c_int, c_double, c_int, c_double, c_double, c_vp, c_vp, ctypes.POINTER(c_int), c_vp,
+                  ctypes.POINTER(ctypes.c_long)]
+    na = 4 * n ** 3
+    pos = np.empty((na, 3))
+    force = np.empty((na, 3))
+    cap = na * 80
+    pairs = np.empty((cap, 2), dtype=np.int32)
+    nat = c_int(na)
+    npairs = ctypes.c_long(cap)
+    rc = f(n, rho, nbin or n, cutneigh, cutforce, _p(pos), _p(force), ctypes.byref(nat), _p(pairs),
+           ctypes.byref(npairs))
+    if rc != 0:
+        raise RuntimeError(f"ref_minimd_system failed ({rc})")
+    return pos[:nat.value], force[:nat.value], pairs[:npairs.value]
