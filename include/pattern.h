@@ -214,7 +214,20 @@ int pat_ir_destroy(pat_ir *h);
 typedef struct pat_md pat_md;
 int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t num_edges, const int32_t *edges,
                   double box, int pbc, double cutoff, double dt, pat_md **out);
+/* The same loop without a caller edge list: the neighbour CSR (all pairs
+ * within cutoff + skin, periodic images) is built on the device from the
+ * positions by binning, and rebuilt before the force of every step whose
+ * index is a multiple of 'every' -- miniMD's Neighbor::setup/build
+ * (miniMD/neighbor.cpp:76-427) with its reneighbouring interval and skin
+ * (miniMD/lj.in:9-10).  Needs box >= 3 * (cutoff + skin). */
+int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
+                         double skin, int every, double dt, pat_md **out);
 int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row);
+/* Current half-pair count, steps run, neighbour rebuilds and their total
+ * host-measured time. */
+int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);
+/* Current CSR by destination (rowptr: N+1, cols: 2 * num_pairs). */
+int pat_md_csr(pat_md *h, int32_t *rowptr, int32_t *cols);
 /* nsteps steps, blocking. */
 int pat_md_run(pat_md *h, int nsteps);
 /* nsteps steps queued on 'stream' (NULL = the handle's stream), not waited for. */

@@ -28,6 +28,9 @@
 // reference folds into its integrator, kmeans/moldyn.cu:396-398), force on
 // i = d*F.  For the reversed pair the displacement is exactly -d, so each
 // row entry is exactly the negation the serial scatter applies.
+#include <algorithm>
+#include <cmath>
+
 #include "pat_common.hpp"
 #include "pat_device.hpp"
 
@@ -114,6 +117,85 @@ __global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowp
         }
         cols[b + rank] = v;
     }
+}
+
+// ------------------------------------------------------------ neighbour build
+//
+// Binned rebuild of the full neighbour CSR from positions, with periodic
+// images: the device form of miniMD's Neighbor::binatoms / build
+// (miniMD/neighbor.cpp:76-250), which bins atoms into cells at least one
+// cutoff wide and searches a stencil of neighbouring bins.  Here every pair
+// within the cutoff lands in both rows (so the force needs no scatter) and
+// rows are sorted, which makes the CSR identical to the one built from a
+// host pair list with the same cutoff.
+
+struct BinGeom {
+    float L, hL, cut2, inv;  // box, half box, cutoff^2, nb / L
+    int nb;                  // bins per axis (>= 3)
+};
+
+__device__ __forceinline__ int bin_of(float x, const BinGeom &g)
+{
+    int c = (int)floorf(x * g.inv);
+    return c < 0 ? 0 : (c >= g.nb ? g.nb - 1 : c);
+}
+
+__global__ void nb_bin_atoms(const float4 *__restrict__ pos, int N, BinGeom g, int *__restrict__ cell,
+                             int *__restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 p = pos[i];
+    const int c = bin_of(p.x, g) + g.nb * (bin_of(p.y, g) + g.nb * bin_of(p.z, g));
+    cell[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+__global__ void nb_scatter_atoms(const int *__restrict__ cell, int N, int *__restrict__ cursor, int *__restrict__ tmp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    tmp[atomicAdd(&cursor[cell[i]], 1)] = i;
+}
+
+// Minimum-image pair test with the pair generator's float expression.
+__device__ __forceinline__ bool nb_within(float4 a, float4 b, const BinGeom &g)
+{
+    float dx = pat::min_image(a.x - b.x, g.L, g.hL);
+    float dy = pat::min_image(a.y - b.y, g.L, g.hL);
+    float dz = pat::min_image(a.z - b.z, g.L, g.hL);
+    float r2 = dx * dx + dy * dy;
+    r2 = r2 + dz * dz;
+    return r2 < g.cut2;
+}
+
+// One thread per atom over the 27 neighbouring bins: with fill == 0 counts
+// the row length into deg, with fill != 0 writes the row into tmp.
+__global__ void nb_rows(const float4 *__restrict__ pos, int N, BinGeom g, const int *__restrict__ cell,
+                        const int *__restrict__ cstart, const int *__restrict__ catoms, int *__restrict__ deg,
+                        const int *__restrict__ rowptr, int *__restrict__ tmp, int fill)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 pi = pos[i];
+    const int c = cell[i];
+    const int cx = c % g.nb, cy = (c / g.nb) % g.nb, cz = c / (g.nb * g.nb);
+    int n = 0;
+    int w = fill ? rowptr[i] : 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int qx = (cx + dx + g.nb) % g.nb, qy = (cy + dy + g.nb) % g.nb, qz = (cz + dz + g.nb) % g.nb;
+                const int q = qx + g.nb * (qy + g.nb * qz);
+                for (int s = cstart[q]; s < cstart[q + 1]; s++) {
+                    const int j = catoms[s];
+                    if (j != i && nb_within(pi, pos[j], g)) {
+                        if (fill) tmp[w++] = j;
+                        n++;
+                    }
+                }
+            }
+    if (!fill) deg[i] = n;
 }
 
 // ------------------------------------------------------------------ force
@@ -468,6 +550,125 @@ int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream
     hipLaunchKernelGGL(vv_kick, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, vel, force, (int)N,
                        make_params(1.f, 0, 0.f, dt));
     PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+}  // namespace pat
+
+namespace pat {
+
+// Device buffers of the binned neighbour build, kept between rebuilds.
+struct NbWork {
+    int64_t N = 0, ncell = 0, cap = 0;  // cap: capacity of cols/coltmp
+    int nb = 0;
+    int *cell = nullptr, *cstart = nullptr, *cursor = nullptr, *atmp = nullptr, *catoms = nullptr, *deg = nullptr,
+        *ws = nullptr, *coltmp = nullptr;
+};
+
+static void nb_free(NbWork &w)
+{
+    for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp})
+        if (p) (void)hipFree(p);
+    w = NbWork{};
+}
+
+NbWork *nb_create() { return new NbWork(); }
+void nb_destroy(NbWork *w)
+{
+    if (w) {
+        nb_free(*w);
+        delete w;
+    }
+}
+
+static int nb_alloc(int **p, int64_t n)
+{
+    if (hipMalloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        pat::set_error("neighbour build: hipMalloc of %lld ints failed", (long long)n);
+        return PAT_ERR_NOMEM;
+    }
+    return PAT_OK;
+}
+
+// Rebuilds the full, row-sorted neighbour CSR of N float4 positions in a
+// periodic cube of side L for pairs closer than cut.  *cols (capacity
+// *cap entries) is reallocated when too small.  Synchronises the stream once
+// (to size the column array).
+int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr, int32_t **cols,
+                   int64_t *cap, int64_t *total, hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1 && L > 0 && cut > 0, PAT_ERR_INVALID, "neighbour build: bad arguments");
+    const int nb = (int)std::floor(L / (cut * 1.0001));
+    PAT_REQUIRE(nb >= 3, PAT_ERR_UNSUPPORTED, "neighbour build: box %.3f holds fewer than 3 bins of %.3f", L, cut);
+    PAT_REQUIRE((int64_t)nb * nb * nb < (1LL << 30), PAT_ERR_INVALID, "neighbour build: too many bins");
+    const int64_t ncell = (int64_t)nb * nb * nb;
+    if (w->N != N || w->ncell != ncell) {
+        const int64_t cap_keep = w->cap;
+        int *coltmp = w->coltmp;
+        w->coltmp = nullptr;
+        nb_free(*w);
+        w->coltmp = coltmp;
+        w->cap = cap_keep;
+        const int64_t m = std::max<int64_t>(N, ncell) + 1;
+        const int64_t scan_ws = 2 * ((m + kScanBlock - 1) / kScanBlock) + 2 * kScanBlock + 4;
+        PAT_TRY(nb_alloc(&w->cell, N));
+        PAT_TRY(nb_alloc(&w->cstart, ncell + 1));
+        PAT_TRY(nb_alloc(&w->cursor, ncell + 1));
+        PAT_TRY(nb_alloc(&w->atmp, N));
+        PAT_TRY(nb_alloc(&w->catoms, N));
+        PAT_TRY(nb_alloc(&w->deg, N + 1));
+        PAT_TRY(nb_alloc(&w->ws, scan_ws));
+        w->N = N;
+        w->ncell = ncell;
+        w->nb = nb;
+    }
+    BinGeom g;
+    g.L = (float)L;
+    g.hL = 0.5f * g.L;
+    g.cut2 = (float)(cut * cut);
+    g.nb = nb;
+    g.inv = (float)nb / g.L;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    // bin the atoms and sort each bin's list by atom index
+    PAT_HIP(hipMemsetAsync(w->cursor, 0, sizeof(int) * (ncell + 1), st));
+    hipLaunchKernelGGL(nb_bin_atoms, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cursor);
+    PAT_LAUNCH_CHECK();
+    PAT_TRY(scan_exclusive(w->cursor, w->cstart, ncell + 1, w->ws, st));
+    PAT_HIP(hipMemcpyAsync(w->cursor, w->cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(nb_scatter_atoms, dim3(gN), dim3(256), 0, st, w->cell, (int)N, w->cursor, w->atmp);
+    PAT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((ncell * kWave + 255) / 256)), dim3(256), 0, st, w->cstart, ncell,
+                       w->atmp, w->catoms);
+    PAT_LAUNCH_CHECK();
+    // row lengths, then row pointers
+    hipLaunchKernelGGL(nb_rows, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cstart, w->catoms, w->deg,
+                       (const int *)nullptr, (int *)nullptr, 0);
+    PAT_LAUNCH_CHECK();
+    PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
+    PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
+    int tot = 0;
+    PAT_HIP(hipMemcpyAsync(&tot, rowptr + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    PAT_REQUIRE(tot >= 0, PAT_ERR_INVALID, "neighbour build: pair count overflows int32");
+    if (tot > *cap || tot > w->cap) {
+        const int64_t want = (int64_t)(tot * 1.25) + 1024;
+        if (*cols) (void)hipFree(*cols);
+        if (w->coltmp) (void)hipFree(w->coltmp);
+        *cols = nullptr;
+        w->coltmp = nullptr;
+        PAT_TRY(nb_alloc(cols, want));
+        PAT_TRY(nb_alloc(&w->coltmp, want));
+        *cap = want;
+        w->cap = want;
+    }
+    hipLaunchKernelGGL(nb_rows, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cstart, w->catoms, w->deg,
+                       (const int *)rowptr, w->coltmp, 1);
+    PAT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((N * kWave + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
+                       N, w->coltmp, *cols);
+    PAT_LAUNCH_CHECK();
+    *total = tot;
     return PAT_OK;
 }
 

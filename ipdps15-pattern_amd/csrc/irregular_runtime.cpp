@@ -9,6 +9,7 @@
 // arrays are never modified: edges are uploaded once and turned into a
 // by-destination CSR on the device, node data stays resident, and results
 // come back in the caller's node order.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -29,6 +30,11 @@ int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t 
 int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
                       hipStream_t st);
 int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
+struct NbWork;
+NbWork *nb_create();
+void nb_destroy(NbWork *w);
+int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr, int32_t **cols,
+                   int64_t *cap, int64_t *total, hipStream_t st);
 }  // namespace pat
 
 namespace {
@@ -189,6 +195,66 @@ struct pat_md {
     float dt = 0.f;
     int cur = 0;  // position buffer holding the current coordinates
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    // binned neighbour rebuild (0 = fixed list)
+    int every = 0;
+    double cutneigh = 0.0;
+    int64_t cap = 0;    // capacity of st.cols
+    int64_t steps = 0;  // steps completed
+    int64_t rebuilds = 0;
+    float rebuild_ms = 0.0f;  // host-measured time of all rebuilds
+    pat::NbWork *nbw = nullptr;
+
+    // Rebuilds the CSR from the positions in buffer c.
+    int rebuild(int c)
+    {
+        int64_t total = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        PAT_TRY(pat::neighbor_build((const float4 *)st.pos[c], st.N, box, cutneigh, nbw, st.rowptr, &st.cols, &cap,
+                                    &total, st.stream));
+        PAT_HIP(hipStreamSynchronize(st.stream));
+        rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st.E = total / 2;
+        rebuilds++;
+        return PAT_OK;
+    }
+
+    // Binned mode: steps launched eagerly, with a rebuild before the force
+    // evaluation of every step whose index is a multiple of 'every'.
+    int run_binned(int nsteps, float *kernel_ms)
+    {
+        hipStream_t s = st.stream;
+        std::vector<hipEvent_t> ev;
+        if (kernel_ms) {
+            ev.resize(2 * (size_t)nsteps);
+            for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
+        }
+        int c = cur;
+        int rc = pat::vv_kick_drift_dev((float4 *)st.pos[c], (float4 *)st.vel, (const float4 *)st.force, st.N, box,
+                                        pbc, dt, s);
+        for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
+            const bool last = k == nsteps - 1;
+            if ((steps + k + 1) % every == 0) rc = rebuild(c);
+            if (rc != PAT_OK) break;
+            if (kernel_ms) (void)hipEventRecord(ev[2 * k], s);
+            rc = pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.lpr,
+                                   last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
+                                   (float4 *)st.pos[1 - c], s);
+            if (kernel_ms) (void)hipEventRecord(ev[2 * k + 1], s);
+            if (!last) c = 1 - c;
+        }
+        if (rc == PAT_OK && hipStreamSynchronize(s) != hipSuccess) {
+            pat::set_error("md: stream synchronisation failed");
+            rc = PAT_ERR_HIP;
+        }
+        if (rc == PAT_OK) {
+            if (kernel_ms)
+                for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
+            cur = c;
+            steps += nsteps;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
 
     void drop_graphs()
     {
@@ -400,11 +466,81 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
     return PAT_OK;
 }
 
+int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
+                         double skin, int every, double dt, pat_md **out)
+{
+    PAT_REQUIRE(out && pos, PAT_ERR_INVALID, "pat_md_create_binned: null argument");
+    PAT_REQUIRE(num_nodes > 1 && num_nodes < (1LL << 31) - 1, PAT_ERR_INVALID, "pat_md_create_binned: bad node count");
+    PAT_REQUIRE(cutoff > 0 && skin >= 0 && dt > 0 && every > 0, PAT_ERR_INVALID,
+                "pat_md_create_binned: need cutoff > 0, skin >= 0, dt > 0, every > 0");
+    PAT_REQUIRE(box >= 3 * (cutoff + skin) * 1.0001, PAT_ERR_UNSUPPORTED,
+                "pat_md_create_binned: box %.3f must hold 3 bins of cutoff+skin %.3f", box, cutoff + skin);
+    auto *h = new pat_md();
+    h->box = (float)box;
+    h->pbc = 1;
+    h->rc2 = (float)cutoff * (float)cutoff;
+    h->dt = (float)dt;
+    h->every = every;
+    h->cutneigh = cutoff + skin;
+    h->nbw = pat::nb_create();
+    int rc = h->st.alloc(num_nodes, 0, nullptr, true);
+    if (rc == PAT_OK) {
+        (void)hipFree(h->st.cols);
+        h->st.cols = nullptr;
+        rc = h->st.upload3(h->st.pos[0], pos, 12);
+    }
+    if (rc == PAT_OK) {
+        std::vector<float> zero(3 * (size_t)num_nodes, 0.0f);
+        rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
+    }
+    if (rc == PAT_OK) rc = h->rebuild(0);
+    if (rc == PAT_OK) {
+        h->st.lpr = default_lpr(h->st.N, h->st.E);
+        rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
+    }
+    if (rc == PAT_OK && hipStreamSynchronize(h->st.stream) != hipSuccess) {
+        pat::set_error("pat_md_create_binned: initial force evaluation failed");
+        rc = PAT_ERR_HIP;
+    }
+    if (rc != PAT_OK) {
+        pat::nb_destroy(h->nbw);
+        h->st.release();
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PAT_OK;
+}
+
+int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_stats: null handle");
+    if (num_pairs) *num_pairs = h->st.E;
+    if (steps) *steps = h->steps;
+    if (rebuilds) *rebuilds = h->rebuilds;
+    if (rebuild_ms) *rebuild_ms = h->rebuild_ms;
+    return PAT_OK;
+}
+
+int pat_md_csr(pat_md *h, int32_t *rowptr, int32_t *cols)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_csr: null handle");
+    if (rowptr)
+        PAT_HIP(hipMemcpyAsync(rowptr, h->st.rowptr, sizeof(int32_t) * (h->st.N + 1), hipMemcpyDeviceToHost,
+                               h->st.stream));
+    if (cols)
+        PAT_HIP(hipMemcpyAsync(cols, h->st.cols, sizeof(int32_t) * 2 * h->st.E, hipMemcpyDeviceToHost, h->st.stream));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    return PAT_OK;
+}
+
 int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_variant: null handle");
     PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC, PAT_ERR_UNSUPPORTED,
                 "pat_md_set_variant: unknown variant %d", variant);
+    PAT_REQUIRE(!(variant == PAT_IR_ATOMIC && h->every > 0), PAT_ERR_UNSUPPORTED,
+                "pat_md_set_variant: the atomic kernel needs a half edge list (binned handles have a CSR only)");
     PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
                     lanes_per_row == 32 || lanes_per_row == 64,
                 PAT_ERR_UNSUPPORTED, "pat_md_set_variant: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
@@ -419,6 +555,7 @@ int pat_md_run(pat_md *h, int nsteps)
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run: null handle");
     PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run: negative step count");
     if (nsteps == 0) return PAT_OK;
+    if (h->every > 0) return h->run_binned(nsteps, nullptr);
     // the atomic path updates positions in place; the CSR path ping-pongs
     const int start = h->cur;
     auto key = std::make_pair(nsteps, start);
@@ -439,12 +576,14 @@ int pat_md_run(pat_md *h, int nsteps)
     PAT_HIP(hipGraphLaunch(it->second, h->st.stream));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
     if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    h->steps += nsteps;
     return PAT_OK;
 }
 
 int pat_md_run_async(pat_md *h, int nsteps, void *stream)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run_async: null handle");
+    PAT_REQUIRE(h->every == 0, PAT_ERR_UNSUPPORTED, "pat_md_run_async: binned rebuilds need blocking runs");
     PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run_async: negative step count");
     if (nsteps == 0) return PAT_OK;
     hipStream_t saved = h->st.stream;
@@ -453,6 +592,7 @@ int pat_md_run_async(pat_md *h, int nsteps, void *stream)
     h->st.stream = saved;
     if (rc != PAT_OK) return rc;
     if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    h->steps += nsteps;
     return PAT_OK;
 }
 
@@ -460,6 +600,12 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
 {
     PAT_REQUIRE(h && total_ms, PAT_ERR_INVALID, "pat_md_run_timed: null argument");
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_md_run_timed: need at least one step");
+    if (h->every > 0) {
+        auto t0 = std::chrono::steady_clock::now();
+        int r = h->run_binned(nsteps, kernel_ms);
+        *total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return r;
+    }
     hipStream_t s = h->st.stream;
     std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 2);
     for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
@@ -501,6 +647,7 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
         if (kernel_ms)
             for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[1 + 2 * k], ev[2 + 2 * k]);
         h->cur = c;
+        h->steps += nsteps;
     } else if (rc == PAT_ERR_HIP) {
         pat::set_error("pat_md_run_timed: HIP call failed: %s", hipGetErrorString(hipGetLastError()));
     }
@@ -553,6 +700,7 @@ int pat_md_device_buffers(pat_md *h, void **pos4, void **vel4, void **force4)
 int pat_md_destroy(pat_md *h)
 {
     if (!h) return PAT_OK;
+    if (h->nbw) pat::nb_destroy(h->nbw);
     h->drop_graphs();
     h->st.release();
     delete h;

@@ -89,6 +89,10 @@ SIGNATURES = {
     "pat_ir_destroy": (c_int, [c_vp]),
     "pat_md_create": (c_int, [c_i64, c_vp, c_vp, c_i64, c_vp, c_double, c_int, c_double, c_double,
                               ctypes.POINTER(c_vp)]),
+    "pat_md_create_binned": (c_int, [c_i64, c_vp, c_vp, c_double, c_double, c_double, c_int, c_double,
+                                     ctypes.POINTER(c_vp)]),
+    "pat_md_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_fp]),
+    "pat_md_csr": (c_int, [c_vp, c_vp, c_vp]),
     "pat_md_set_variant": (c_int, [c_vp, c_int, c_int]),
     "pat_md_run": (c_int, [c_vp, c_int]),
     "pat_md_run_async": (c_int, [c_vp, c_int, c_vp]),
@@ -421,17 +425,23 @@ class IrregularRuntime:
 
 
 class MD:
-    """Device-resident velocity-Verlet LJ dynamics (pat_md_*)."""
+    """Device-resident velocity-Verlet LJ dynamics (pat_md_*).  With
+    edges=None the neighbour CSR is built and rebuilt on the device
+    (cutoff + skin, every 'every' steps)."""
 
-    def __init__(self, pos, vel, edges, box, cutoff, dt, pbc=True, variant=PAT_IR_CSR, lanes_per_row=0):
+    def __init__(self, pos, vel, edges, box, cutoff, dt, pbc=True, variant=PAT_IR_CSR, lanes_per_row=0, skin=0.3,
+                 every=20):
         p = np.ascontiguousarray(pos, dtype=np.float32)
         v = np.ascontiguousarray(vel, dtype=np.float32) if vel is not None else None
-        e = np.ascontiguousarray(edges, dtype=np.int32)
         self.n = p.shape[0]
-        self.num_edges = e.shape[0]
         h = c_vp()
-        call("pat_md_create", self.n, ptr(p), ptr(v), e.shape[0], ptr(e), float(box), int(pbc), float(cutoff),
-             float(dt), ctypes.byref(h))
+        if edges is None:
+            call("pat_md_create_binned", self.n, ptr(p), ptr(v), float(box), float(cutoff), float(skin), int(every),
+                 float(dt), ctypes.byref(h))
+        else:
+            e = np.ascontiguousarray(edges, dtype=np.int32)
+            call("pat_md_create", self.n, ptr(p), ptr(v), e.shape[0], ptr(e), float(box), int(pbc), float(cutoff),
+                 float(dt), ctypes.byref(h))
         self.h = h
         if variant != PAT_IR_CSR or lanes_per_row:
             self.set_variant(variant, lanes_per_row)
@@ -458,6 +468,19 @@ class MD:
         f = np.empty((self.n, 3), dtype=np.float32)
         call("pat_md_get", self.h, ptr(p), ptr(v), ptr(f))
         return p, v, f
+
+    def stats(self):
+        """(half pairs, steps, rebuilds, rebuild ms)."""
+        a, b, c, d = c_i64(0), c_i64(0), c_i64(0), c_float(0)
+        call("pat_md_stats", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def csr(self):
+        npairs = self.stats()[0]
+        rp = np.empty(self.n + 1, dtype=np.int32)
+        cl = np.empty(max(2 * npairs, 1), dtype=np.int32)
+        call("pat_md_csr", self.h, ptr(rp), ptr(cl))
+        return rp, cl[:2 * npairs]
 
     def energy(self):
         ep, ek = c_double(0), c_double(0)

@@ -1,0 +1,87 @@
+"""GPU parity of the device binned neighbour rebuild and the miniMD workload.
+
+The device-built CSR (all pairs within cutoff + skin, periodic images, rows
+sorted) must equal the CSR of the host pair generator exactly (the pair
+generator is itself pinned to miniMD's Neighbor::build and to brute force).
+Dynamics with rebuilds every 20 steps are compared with the FP32 oracle
+rebuilding its pair list at the same steps: 1e-5 relative L-infinity.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def host_csr(n, edges):
+    both = np.concatenate([edges, edges[:, ::-1]])
+    both = both[np.lexsort((both[:, 1], both[:, 0]))]
+    rp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(rp, both[:, 0] + 1, 1)
+    return np.cumsum(rp), both[:, 1]
+
+
+@pytest.mark.parametrize("cells,seed", [(6, 1), (9, 2), (13, 3)])
+def test_binned_csr_equals_host_pairs(pa, gpu, cells, seed):
+    pos, box = pa.gen_fcc(cells, cells, cells, rho=0.8442, jitter=0.2, seed=seed)
+    md = pa.MD(pos, None, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    rp, cl = md.csr()
+    md.close()
+    erp, ecl = host_csr(len(pos), pa.gen_pairs(pos, box[0], 2.8))
+    assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
+
+
+def oracle_rebuilding(pa, orc, pos, vel, box, steps, every, threads=1):
+    p, v = pos.copy(), vel.copy()
+    done = 0
+    f = None
+    while done < steps:
+        k = min(every - done % every, steps - done)
+        e = pa.gen_pairs(p.astype(np.float32), box, 2.8)
+        p, v, f, ep = orc.md_run(p, v, e, box, 2.5, 0.005, k, threads=threads)
+        done += k
+    return p, v, f, ep
+
+
+def test_binned_md_vs_oracle(pa, orc, gpu):
+    pos, box = pa.gen_fcc(8, 8, 8, rho=0.8442, jitter=0.0, seed=4)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=5)
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    md.run(25)
+    md.run(15)
+    p, v, f = md.get()
+    npairs, steps, rebuilds, _ = md.stats()
+    md.close()
+    assert steps == 40 and rebuilds == 3  # at creation, step 20, step 40
+    rp, rv, rf, _ = oracle_rebuilding(pa, orc, pos, vel, box[0], 40, 20)
+    assert rel(f, rf) < TOL and rel(v, rv) < TOL
+    d = p - rp
+    d -= box[0] * np.rint(d / box[0])
+    assert np.max(np.abs(d)) / box[0] < TOL
+
+
+def test_minimd_workload(pa, orc, gpu):
+    """miniMD: FCC 64^3 cells (1,048,576 atoms), density 0.8442, Maxwell
+    velocities at T = 1.44, cutoff 2.5, skin 0.3, binned rebuild every 20
+    steps, dt 0.005, 100 steps."""
+    pos, box = pa.gen_fcc(64, 64, 64, rho=0.8442, jitter=0.0, seed=7)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=8)
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    ep0, ek0 = md.energy()
+    md.run(100)
+    p, v, f = md.get()
+    ep1, ek1 = md.energy()
+    npairs, steps, rebuilds, rb_ms = md.stats()
+    md.close()
+    print(f"miniMD 64^3: {len(pos)} atoms, {npairs} half pairs within 2.8, {rebuilds} rebuilds ({rb_ms:.1f} ms), "
+          f"E0 {ep0 + ek0:.6e} E100 {ep1 + ek1:.6e}")
+    assert 4.0e7 < npairs < 4.2e7
+    assert abs((ep1 + ek1) - (ep0 + ek0)) / abs(ep0 + ek0) < 2e-3
+    rp, rv, rf, rep = oracle_rebuilding(pa, orc, pos, vel, box[0], 100, 20, threads=16)
+    assert rel(f, rf) < TOL and rel(v, rv) < TOL
+    assert abs(ep1 - rep) / abs(rep) < TOL
