@@ -230,6 +230,26 @@ int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuild
 int pat_md_csr(pat_md *h, int32_t *rowptr, int32_t *cols);
 /* nsteps steps, blocking. */
 int pat_md_run(pat_md *h, int nsteps);
+
+/* Many independent periodic systems advanced as one: system s owns nodes
+ * [offsets[s], offsets[s+1]) of the concatenated pos/vel arrays (N x 3) and
+ * has cubic box box[s].  Each node carries its box in the float4 w lane, so
+ * one fused force+update launch covers every system whatever its size, and
+ * the steps between neighbour rebuilds replay one captured hipGraph.
+ * Rebuilds (cutoff + skin, every 'every' steps) are per system.  The batched
+ * form of the miniMD driver (miniMD/ljs.cu:55-184) for many small systems. */
+typedef struct pat_mdb pat_mdb;
+int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const float *vel, const double *box,
+                   double cutoff, double skin, int every, double dt, pat_mdb **out);
+/* Lanes per CSR row (0 = default); recomputes the stored force. */
+int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row);
+/* nsteps steps (blocking); total_ms = host wall time, kernel_ms[k] = step
+ * k's force kernel when non-NULL (then launched one by one with events). */
+int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms);
+/* System sys's positions, velocities and forces (n_s x 3 each, may be NULL). */
+int pat_mdb_get(pat_mdb *h, int sys, float *pos, float *vel, float *force);
+int pat_mdb_stats(pat_mdb *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);
+int pat_mdb_destroy(pat_mdb *h);
 /* nsteps steps queued on 'stream' (NULL = the handle's stream), not waited for. */
 int pat_md_run_async(pat_md *h, int nsteps, void *stream);
 /* nsteps steps launched one by one on the handle's stream with HIP events

@@ -277,6 +277,10 @@ __global__ __launch_bounds__(256) void lj_csr(const float4 *__restrict__ pos, co
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (row < N) {
         pi = pos[row];
+        if (p.pbc == 2) {  // batched systems: each node carries its box side in w
+            p.L = pi.w;
+            p.hL = 0.5f * pi.w;
+        }
         const int b = rowptr[row], e = rowptr[row + 1];
         for (int k = b + l; k < e; k += LPR) {
             const float4 pj = pos[cols[k]];
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(256) void lj_csr(const float4 *__restrict__ pos, co
                 x.x = pi.x + p.dt * v.x;
                 x.y = pi.y + p.dt * v.y;
                 x.z = pi.z + p.dt * v.z;
-                x.w = 0.0f;
+                x.w = pi.w;
                 if (p.pbc) {
                     x.x = pat::wrap_coord(x.x, p.L);
                     x.y = pat::wrap_coord(x.y, p.L);
@@ -332,6 +336,10 @@ __global__ __launch_bounds__(256) void lj_csr_energy(const float4 *__restrict__ 
     double acc = 0.0;
     if (row < N) {
         const float4 pi = pos[row];
+        if (p.pbc == 2) {
+            p.L = pi.w;
+            p.hL = 0.5f * pi.w;
+        }
         for (int k = rowptr[row] + l; k < rowptr[row + 1]; k += 64) acc += (double)pair_phi(pi, pos[cols[k]], p);
     }
 #pragma unroll
@@ -360,6 +368,17 @@ __global__ __launch_bounds__(256) void lj_atomic(const float4 *__restrict__ pos,
     }
 }
 
+// Copies one system's CSR into the batched CSR, shifting node and entry
+// indices by the system's offsets.
+__global__ void csr_place(const int *__restrict__ rp, const int *__restrict__ cl, int n, int node_off, int entry_off,
+                          int *__restrict__ grp, int *__restrict__ gcl)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= n) grp[node_off + t] = rp[t] + entry_off;
+    const int ne = rp[n];
+    for (int64_t k = t; k < ne; k += (int64_t)gridDim.x * blockDim.x) gcl[entry_off + k] = cl[k] + node_off;
+}
+
 __global__ void fill_zero4(float4 *__restrict__ a, int N)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -373,6 +392,7 @@ __global__ void vv_kick_drift(float4 *__restrict__ pos, float4 *__restrict__ vel
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float4 v = vel[i], f = force[i], x = pos[i];
+    if (p.pbc == 2) p.L = x.w;
     v.x = v.x + p.hdt * f.x;
     v.y = v.y + p.hdt * f.y;
     v.z = v.z + p.hdt * f.z;
@@ -509,6 +529,16 @@ int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, 
     const int64_t blocks = (N + rpb - 1) / rpb;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, st, pos, (const int *)rowptr, (const int *)cols, (int)N,
                        make_params(box, pbc, rc2, dt), force, vel, pos_out, mode);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
+                  int32_t *grp, int32_t *gcl, hipStream_t st)
+{
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 256) / 256 + 1, 4096);
+    hipLaunchKernelGGL(csr_place, dim3(grid), dim3(256), 0, st, (const int *)rp, (const int *)cl, (int)n, (int)node_off,
+                       (int)entry_off, (int *)grp, (int *)gcl);
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }

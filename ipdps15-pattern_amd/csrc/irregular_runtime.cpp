@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 
 #include "pat_common.hpp"
 
@@ -30,6 +31,8 @@ int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t 
 int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
                       hipStream_t st);
 int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
+int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
+                  int32_t *grp, int32_t *gcl, hipStream_t st);
 struct NbWork;
 NbWork *nb_create();
 void nb_destroy(NbWork *w);
@@ -703,6 +706,286 @@ int pat_md_destroy(pat_md *h)
     if (h->nbw) pat::nb_destroy(h->nbw);
     h->drop_graphs();
     h->st.release();
+    delete h;
+    return PAT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------
+// Batched independent systems: nodes of every system concatenated, each node
+// carrying its system's box side in the float4 w lane, so one fused CSR
+// kernel launch advances all systems (uneven sizes included) by one step,
+// and the steps between neighbour rebuilds replay as one captured graph.
+
+struct pat_mdb {
+    pat_md md;                      // the concatenated system (pbc mode 2)
+    std::vector<int64_t> off;       // node offsets, nsys + 1
+    std::vector<double> box;        // per system
+    int32_t *rp_s = nullptr, *cl_s = nullptr;  // one system's CSR during a rebuild
+    int64_t cap_s = 0;
+
+    int rebuild_all(int c)
+    {
+        auto t0 = std::chrono::steady_clock::now();
+        const int nsys = (int)box.size();
+        std::vector<int64_t> totals(nsys);
+        int64_t entries = 0;
+        // pass 1: build each system's CSR and place it
+        for (int s = 0; s < nsys; s++) {
+            const int64_t n = off[s + 1] - off[s];
+            int64_t tot = 0;
+            PAT_TRY(pat::neighbor_build((const float4 *)md.st.pos[c] + off[s], n, box[s], md.cutneigh, md.nbw, rp_s,
+                                        &cl_s, &cap_s, &tot, md.st.stream));
+            if (entries + tot > md.cap) {
+                // grow the batched column array, keeping what is placed
+                const int64_t want = (int64_t)((entries + tot) * 1.3) + 4096;
+                int32_t *nc = nullptr;
+                if (hipMalloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+                    (void)hipGetLastError();
+                    pat::set_error("batched md: hipMalloc of %lld column entries failed", (long long)want);
+                    return PAT_ERR_NOMEM;
+                }
+                if (entries > 0 && md.st.cols)
+                    PAT_HIP(hipMemcpyAsync(nc, md.st.cols, sizeof(int32_t) * entries, hipMemcpyDeviceToDevice,
+                                           md.st.stream));
+                PAT_HIP(hipStreamSynchronize(md.st.stream));
+                if (md.st.cols) (void)hipFree(md.st.cols);
+                md.st.cols = nc;
+                md.cap = want;
+            }
+            PAT_TRY(pat::csr_place_dev(rp_s, cl_s, n, off[s], entries, md.st.rowptr, md.st.cols, md.st.stream));
+            entries += tot;
+            totals[s] = tot;
+        }
+        PAT_HIP(hipStreamSynchronize(md.st.stream));
+        md.st.E = entries / 2;
+        md.rebuilds++;
+        md.rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return PAT_OK;
+    }
+
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> seg_graphs;
+    int32_t *graph_cols = nullptr;  // column array the cached graphs were captured with
+
+    void drop_seg_graphs()
+    {
+        for (auto &g : seg_graphs) (void)hipGraphExecDestroy(g.second);
+        seg_graphs.clear();
+    }
+
+    // seg fused force steps starting from buffer c; the run's final step
+    // (ends_run) closes with a half kick only.
+    int enqueue_segment(int seg, int c, bool ends_run, hipStream_t s, float *kernel_ms)
+    {
+        for (int j = 0; j < seg; j++) {
+            const bool final_step = ends_run && j == seg - 1;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (kernel_ms) {
+                PAT_HIP(hipEventCreate(&e0));
+                PAT_HIP(hipEventCreate(&e1));
+                PAT_HIP(hipEventRecord(e0, s));
+            }
+            PAT_TRY(pat::lj_force_csr((const float4 *)md.st.pos[c], md.st.rowptr, md.st.cols, md.st.N, 1.0f, 2, md.rc2,
+                                      md.dt, md.st.lpr, final_step ? kKick : kKickDrift, (float4 *)md.st.force,
+                                      (float4 *)md.st.vel, (float4 *)md.st.pos[1 - c], s));
+            if (kernel_ms) {
+                PAT_HIP(hipEventRecord(e1, s));
+                PAT_HIP(hipEventSynchronize(e1));
+                (void)hipEventElapsedTime(&kernel_ms[j], e0, e1);
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            }
+            if (!final_step) c = 1 - c;
+        }
+        return PAT_OK;
+    }
+
+    // Step k (1-based over the handle's life) rebuilds before its force when
+    // k % every == 0.  The force steps between rebuilds replay a captured
+    // graph (eager launches with events when kernel times are requested).
+    int run(int nsteps, float *kernel_ms)
+    {
+        hipStream_t s = md.st.stream;
+        int c = md.cur;
+        PAT_TRY(pat::vv_kick_drift_dev((float4 *)md.st.pos[c], (float4 *)md.st.vel, (const float4 *)md.st.force,
+                                       md.st.N, 1.0f, 2, md.dt, s));
+        int k = 0;
+        while (k < nsteps) {
+            if ((md.steps + k + 1) % md.every == 0) PAT_TRY(rebuild_all(c));
+            int k_end = k + 1;
+            while (k_end < nsteps && (md.steps + k_end + 1) % md.every != 0) k_end++;
+            const int seg = k_end - k;
+            const bool ends_run = k_end == nsteps;
+            if (kernel_ms) {
+                PAT_TRY(enqueue_segment(seg, c, ends_run, s, kernel_ms + k));
+            } else {
+                if (graph_cols != md.st.cols) {
+                    drop_seg_graphs();
+                    graph_cols = md.st.cols;
+                }
+                auto key = std::make_tuple(seg, c, (int)ends_run);
+                auto it = seg_graphs.find(key);
+                if (it == seg_graphs.end()) {
+                    hipGraph_t g;
+                    PAT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    int rc = enqueue_segment(seg, c, ends_run, s, nullptr);
+                    hipError_t ec = hipStreamEndCapture(s, &g);
+                    if (rc != PAT_OK) return rc;
+                    PAT_HIP(ec);
+                    hipGraphExec_t ge;
+                    hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    PAT_HIP(ei);
+                    it = seg_graphs.emplace(key, ge).first;
+                }
+                PAT_HIP(hipGraphLaunch(it->second, s));
+            }
+            if ((seg - (ends_run ? 1 : 0)) % 2) c = 1 - c;
+            k = k_end;
+        }
+        PAT_HIP(hipStreamSynchronize(s));
+        md.cur = c;
+        md.steps += nsteps;
+        return PAT_OK;
+    }
+};
+
+extern "C" {
+
+int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const float *vel, const double *box,
+                   double cutoff, double skin, int every, double dt, pat_mdb **out)
+{
+    PAT_REQUIRE(out && offsets && pos && box && nsys > 0, PAT_ERR_INVALID, "pat_mdb_create: null argument");
+    PAT_REQUIRE(cutoff > 0 && skin >= 0 && dt > 0 && every > 0, PAT_ERR_INVALID, "pat_mdb_create: bad parameters");
+    PAT_REQUIRE(offsets[0] == 0, PAT_ERR_INVALID, "pat_mdb_create: offsets must start at 0");
+    for (int s = 0; s < nsys; s++) {
+        PAT_REQUIRE(offsets[s + 1] - offsets[s] > 1, PAT_ERR_INVALID, "pat_mdb_create: system %d has < 2 nodes", s);
+        PAT_REQUIRE(box[s] >= 3 * (cutoff + skin) * 1.0001, PAT_ERR_UNSUPPORTED,
+                    "pat_mdb_create: system %d box %.3f must hold 3 bins of %.3f", s, box[s], cutoff + skin);
+    }
+    const int64_t N = offsets[nsys];
+    PAT_REQUIRE(N < (1LL << 31) - 1, PAT_ERR_INVALID, "pat_mdb_create: too many nodes");
+    auto *h = new pat_mdb();
+    h->off.assign(offsets, offsets + nsys + 1);
+    h->box.assign(box, box + nsys);
+    pat_md &m = h->md;
+    m.box = 1.0f;
+    m.pbc = 2;
+    m.rc2 = (float)cutoff * (float)cutoff;
+    m.dt = (float)dt;
+    m.every = every;
+    m.cutneigh = cutoff + skin;
+    m.nbw = pat::nb_create();
+    int rc = m.st.alloc(N, 0, nullptr, true);
+    if (rc == PAT_OK) {
+        (void)hipFree(m.st.cols);
+        m.st.cols = nullptr;
+        // positions with each node's box side in w
+        std::vector<float> h4(4 * (size_t)N);
+        for (int s = 0; s < nsys; s++)
+            for (int64_t i = offsets[s]; i < offsets[s + 1]; i++) {
+                h4[4 * i] = pos[3 * i];
+                h4[4 * i + 1] = pos[3 * i + 1];
+                h4[4 * i + 2] = pos[3 * i + 2];
+                h4[4 * i + 3] = (float)box[s];
+            }
+        if (hipMemcpy(m.st.pos[0], h4.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice) != hipSuccess) {
+            pat::set_error("pat_mdb_create: position upload failed");
+            rc = PAT_ERR_HIP;
+        }
+    }
+    if (rc == PAT_OK) {
+        std::vector<float> zero(3 * (size_t)N, 0.0f);
+        rc = m.st.upload3(m.st.vel, vel ? vel : zero.data(), 12);
+    }
+    if (rc == PAT_OK) {
+        if (hipMalloc((void **)&h->rp_s, sizeof(int32_t) * (size_t)(N + 1)) != hipSuccess) rc = PAT_ERR_NOMEM;
+    }
+    if (rc == PAT_OK) rc = h->rebuild_all(0);
+    if (rc == PAT_OK) {
+        m.st.lpr = default_lpr(m.st.N, m.st.E);
+        rc = pat::lj_force_csr((const float4 *)m.st.pos[0], m.st.rowptr, m.st.cols, N, 1.0f, 2, m.rc2, 0.f, m.st.lpr,
+                               kForceOnly, (float4 *)m.st.force, nullptr, nullptr, m.st.stream);
+    }
+    if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
+    if (rc != PAT_OK) {
+        if (h->rp_s) (void)hipFree(h->rp_s);
+        if (h->cl_s) (void)hipFree(h->cl_s);
+        pat::nb_destroy(m.nbw);
+        m.nbw = nullptr;
+        m.st.release();
+        m.st = IrState{};
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PAT_OK;
+}
+
+int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_run: null handle");
+    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_mdb_run: negative step count");
+    if (nsteps == 0) return PAT_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = h->run(nsteps, kernel_ms);
+    if (total_ms) *total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_set_lanes: null handle");
+    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
+                    lanes_per_row == 32 || lanes_per_row == 64,
+                PAT_ERR_UNSUPPORTED, "pat_mdb_set_lanes: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
+    h->md.st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->md.st.N, h->md.st.E);
+    h->drop_seg_graphs();
+    // the stored force must come from the same summation order
+    PAT_TRY(pat::lj_force_csr((const float4 *)h->md.st.pos[h->md.cur], h->md.st.rowptr, h->md.st.cols, h->md.st.N,
+                              1.0f, 2, h->md.rc2, 0.f, h->md.st.lpr, kForceOnly, (float4 *)h->md.st.force, nullptr,
+                              nullptr, h->md.st.stream));
+    PAT_HIP(hipStreamSynchronize(h->md.st.stream));
+    return PAT_OK;
+}
+
+int pat_mdb_get(pat_mdb *h, int sys, float *pos, float *vel, float *force)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_get: null handle");
+    PAT_REQUIRE(sys >= 0 && sys < (int)h->box.size(), PAT_ERR_INVALID, "pat_mdb_get: system %d out of range", sys);
+    const int64_t o = h->off[sys], n = h->off[sys + 1] - o;
+    std::vector<float> buf(4 * (size_t)n);
+    auto get = [&](const float *src4, float *dst) -> int {
+        PAT_HIP(hipMemcpy(buf.data(), src4 + 4 * o, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) dst[3 * i + c] = buf[4 * i + c];
+        return PAT_OK;
+    };
+    PAT_HIP(hipStreamSynchronize(h->md.st.stream));
+    if (pos) PAT_TRY(get(h->md.st.pos[h->md.cur], pos));
+    if (vel) PAT_TRY(get(h->md.st.vel, vel));
+    if (force) PAT_TRY(get(h->md.st.force, force));
+    return PAT_OK;
+}
+
+int pat_mdb_stats(pat_mdb *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_stats: null handle");
+    return pat_md_stats(&h->md, num_pairs, steps, rebuilds, rebuild_ms);
+}
+
+int pat_mdb_destroy(pat_mdb *h)
+{
+    if (!h) return PAT_OK;
+    if (h->rp_s) (void)hipFree(h->rp_s);
+    if (h->cl_s) (void)hipFree(h->cl_s);
+    if (h->md.nbw) pat::nb_destroy(h->md.nbw);
+    h->md.nbw = nullptr;
+    h->drop_seg_graphs();
+    h->md.drop_graphs();
+    h->md.st.release();
+    h->md.st = IrState{};
     delete h;
     return PAT_OK;
 }

@@ -94,6 +94,13 @@ SIGNATURES = {
     "pat_md_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_fp]),
     "pat_md_csr": (c_int, [c_vp, c_vp, c_vp]),
     "pat_md_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pat_mdb_create": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_double, c_int, c_double,
+                               ctypes.POINTER(c_vp)]),
+    "pat_mdb_set_lanes": (c_int, [c_vp, c_int]),
+    "pat_mdb_run": (c_int, [c_vp, c_int, c_fp, c_fp]),
+    "pat_mdb_get": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
+    "pat_mdb_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_fp]),
+    "pat_mdb_destroy": (c_int, [c_vp]),
     "pat_md_run": (c_int, [c_vp, c_int]),
     "pat_md_run_async": (c_int, [c_vp, c_int, c_vp]),
     "pat_md_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
@@ -497,6 +504,72 @@ class MD:
             self.close()
         except Exception:
             pass
+
+
+class MDBatch:
+    """Independent periodic LJ systems advanced as one (pat_mdb_*).
+    systems: list of (pos (n, 3), vel (n, 3) or None, box)."""
+
+    def __init__(self, systems, cutoff=2.5, skin=0.3, every=20, dt=0.005, lanes_per_row=0):
+        self.sizes = [len(p) for p, _, _ in systems]
+        off = np.zeros(len(systems) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(self.sizes)
+        pos = np.ascontiguousarray(np.concatenate([p for p, _, _ in systems]), dtype=np.float32)
+        vel = np.ascontiguousarray(np.concatenate([v if v is not None else np.zeros_like(p)
+                                                   for p, v, _ in systems]), dtype=np.float32)
+        box = np.array([b for _, _, b in systems], dtype=np.float64)
+        h = c_vp()
+        call("pat_mdb_create", len(systems), ptr(off), ptr(pos), ptr(vel), ptr(box), float(cutoff), float(skin),
+             int(every), float(dt), ctypes.byref(h))
+        self.h = h
+        if lanes_per_row:
+            call("pat_mdb_set_lanes", self.h, lanes_per_row)
+
+    def run(self, nsteps):
+        tot = c_float(0)
+        call("pat_mdb_run", self.h, nsteps, ctypes.byref(tot), None)
+        return tot.value
+
+    def run_timed(self, nsteps):
+        tot = c_float(0)
+        k = np.zeros(nsteps, dtype=np.float32)
+        call("pat_mdb_run", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
+        return tot.value, k
+
+    def get(self, sys):
+        n = self.sizes[sys]
+        p, v, f = (np.empty((n, 3), dtype=np.float32) for _ in range(3))
+        call("pat_mdb_get", self.h, sys, ptr(p), ptr(v), ptr(f))
+        return p, v, f
+
+    def stats(self):
+        a, b, c, d = c_i64(0), c_i64(0), c_i64(0), c_float(0)
+        call("pat_mdb_stats", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def close(self):
+        if self.h:
+            call("pat_mdb_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_systems(nsys=64, lo=8, hi=32, seed=99, temperature=1.44):
+    """nsys FCC systems with cells per side log-uniform in [lo, hi] (seeded),
+    density 0.8442, Maxwell velocities at the given temperature."""
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=nsys))).astype(int)
+    out = []
+    for i, m in enumerate(sides):
+        pos, box = gen_fcc(int(m), int(m), int(m), 0.8442, 0.0, seed + i)
+        vel = gen_velocities(len(pos), temperature, seed + 1000 + i)
+        out.append((pos, vel, float(box[0])))
+    return out
 
 
 def moldyn_system(ncells, rho=0.8442, jitter=0.05, cutoff=2.5, seed=42, shuffle=True):
