@@ -85,11 +85,11 @@ def bench_lj(steps, warmup, do_cpu):
         nt = cpu_threads()
         vel0 = np.zeros_like(pos)
         c0 = time.perf_counter()
-        orc.md_run(pos, vel0, edges, box, 2.5, 0.005, 2, threads=nt)
+        orc.md_run(pos, vel0, edges, box, 2.5, 0.005, 2, threads=nt, refcpu=True)
         t2 = time.perf_counter() - c0
         sample = int(max(2, min(50, 15.0 / max(t2 / 2, 1e-6))))
         c0 = time.perf_counter()
-        orc.md_run(pos, vel0, edges, box, 2.5, 0.005, sample, threads=nt)
+        orc.md_run(pos, vel0, edges, box, 2.5, 0.005, sample, threads=nt, refcpu=True)
         ct = time.perf_counter() - c0
         cpu = {"value": E * sample / ct, "unit": "pairs/s", "cores": nt, "kind": "port",
                "sample": f"moldyn 40^3 system, {sample} of its VV steps (threaded oracle, chunked edges into "

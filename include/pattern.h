@@ -162,20 +162,24 @@ int pat_stencil_destroy(pat_stencil *s);
 #define PAT_IR_LJ_OPEN 1     /* miniMD map1, miniMD/irregular.h:109-163: no images;
                                 param = double cutforcesq */
 /* Kernel variants. */
-#define PAT_IR_CSR 0    /* by-destination CSR, wavefront-segmented row sums (default) */
-#define PAT_IR_ATOMIC 1 /* edge-parallel, float atomics into both endpoints */
+#define PAT_IR_CSR 0      /* by-destination CSR, rows in edge order, summed in order:
+                             bit-identical to a serial scatter over the edge list (default) */
+#define PAT_IR_ATOMIC 1   /* edge-parallel, float atomics into both endpoints */
+#define PAT_IR_CSR_TREE 2 /* CSR with butterfly (tree-order) row sums: FP32-tolerance parity */
 
 /* Bytes of device workspace pat_ir_csr_build_dev needs. */
 int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes);
-/* Reorders a half edge list (int32 pairs i, j) once into a CSR by
- * destination: row r lists every neighbour of r (both directions), sorted.
+/* Reorders a half edge list (int32 pairs i, j, i != j) once into a CSR by
+ * destination: row r lists every neighbour of r (both directions) in the
+ * order of the edges in the list.
  * rowptr: num_nodes+1, cols: 2*num_edges.  Replaces the host Partitioner /
  * Reorder (kmeans/irregular/partitioner.cpp:20-499, reorder.cpp:13-281). */
 int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int32_t *rowptr, int32_t *cols,
                          void *work, int64_t work_bytes, void *stream);
 /* LJ forces from float4 positions (x, y, z, unused) into float4 forces over
- * the CSR; lanes_per_row in {4, 8, 16, 32, 64}; pbc selects the minimum
- * image in a cubic box of side 'box'. */
+ * the CSR; pbc selects the minimum image in a cubic box of side 'box'.
+ * lanes_per_row 1, 4, 8, 16, 32: rows summed in CSR order (bit-identical for
+ * every choice); -4 ... -64: tree-order sums over that many lanes. */
 int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
                          int pbc, float cutoff, int lanes_per_row, float *force4, void *stream);
 /* The same forces from the half edge list with float atomics (zeroes force4
@@ -194,6 +198,7 @@ int pat_ir_create(int64_t num_edges, int64_t num_nodes, const int32_t *edges, in
 int pat_ir_set_map_idx(pat_ir *h, int idx);
 /* The functor parameter (the runtime's 'parameter' argument). */
 int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size);
+/* Kernel variant and lanes per row (0 = tuned for PAT_IR_CSR). */
 int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row);
 /* reset_node_data (irregular_runtime.cu:427-439): node coordinates,
  * node_elm_size 12 (float3) or 24 (double3, moldyn's point_data). */

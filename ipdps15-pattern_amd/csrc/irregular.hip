@@ -88,14 +88,36 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(int *__restrict__ out, in
     if (g < n) out[g] += offs[blockIdx.x];
 }
 
-__global__ void csr_scatter(const int2 *__restrict__ edges, int64_t E, int *__restrict__ cursor, int *__restrict__ tmp)
+__global__ void csr_scatter(const int2 *__restrict__ edges, int64_t E, int *__restrict__ cursor, int *__restrict__ tmp,
+                            int *__restrict__ tmpe)
 {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
         int2 ij = edges[e];
         int p = atomicAdd(&cursor[ij.x], 1);
         tmp[p] = ij.y;
+        tmpe[p] = (int)e;
         int q = atomicAdd(&cursor[ij.y], 1);
         tmp[q] = ij.x;
+        tmpe[q] = (int)e;
+    }
+}
+
+// One wavefront per row: places each entry at the rank of its edge id, so a
+// row lists its neighbours in the order their edges appear in the input list
+// -- the order in which a serial scatter over that list adds them.
+__global__ __launch_bounds__(256) void csr_row_sort_by_edge(const int *__restrict__ rowptr, int64_t N,
+                                                            const int *__restrict__ tmp, const int *__restrict__ tmpe,
+                                                            int *__restrict__ cols)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= N) return;
+    const int b = rowptr[row], e = rowptr[row + 1];
+    for (int k = b + lane; k < e; k += kWave) {
+        const int v = tmpe[k];
+        int rank = 0;
+        for (int m = b; m < e; m++) rank += tmpe[m] < v;
+        cols[b + rank] = tmp[k];
     }
 }
 
@@ -262,10 +284,13 @@ enum : int {
     kKickDrift = 2,   // f; v += hdt*f; v += hdt*f; x_out = wrap(x + dt*v)
 };
 
-// Rows are handled by groups of LPR lanes; a 256-thread block holds
-// 256/LPR rows.  Rows are dealt to blocks XCD-contiguously.
+// Tree-order variant: rows handled by groups of LPR lanes that stride the
+// row and combine partial sums with butterflies.  Its sums round in a
+// different order from the serial scatter, so its results agree with the
+// oracle to FP32 tolerance rather than bit for bit.  A 256-thread block holds
+// 256/LPR rows; rows are dealt to blocks XCD-contiguously.
 template <int LPR>
-__global__ __launch_bounds__(256) void lj_csr(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+__global__ __launch_bounds__(256) void lj_csr_tree(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
                                               const int *__restrict__ cols, int N, LJParams p, float4 *__restrict__ force,
                                               float4 *__restrict__ vel, float4 *__restrict__ pos_out, int mode)
 {
@@ -324,6 +349,117 @@ __global__ __launch_bounds__(256) void lj_csr(const float4 *__restrict__ pos, co
             vel[row] = v;
         }
     }
+}
+
+// Shared epilogue: force write and the fused velocity-Verlet update.
+__device__ __forceinline__ void lj_epilogue(int row, float4 pi, float fx, float fy, float fz, const LJParams &p,
+                                            float4 *__restrict__ force, float4 *__restrict__ vel,
+                                            float4 *__restrict__ pos_out, int mode)
+{
+    force[row] = make_float4(fx, fy, fz, 0.0f);
+    if (mode == kForceOnly) return;
+    float4 v = vel[row];
+    v.x = v.x + p.hdt * fx;
+    v.y = v.y + p.hdt * fy;
+    v.z = v.z + p.hdt * fz;
+    if (mode == kKickDrift) {
+        v.x = v.x + p.hdt * fx;
+        v.y = v.y + p.hdt * fy;
+        v.z = v.z + p.hdt * fz;
+        float4 x;
+        x.x = pi.x + p.dt * v.x;
+        x.y = pi.y + p.dt * v.y;
+        x.z = pi.z + p.dt * v.z;
+        x.w = pi.w;
+        if (p.pbc) {
+            x.x = pat::wrap_coord(x.x, p.L);
+            x.y = pat::wrap_coord(x.y, p.L);
+            x.z = pat::wrap_coord(x.z, p.L);
+        }
+        pos_out[row] = x;
+    }
+    vel[row] = v;
+}
+
+// In-order row sums, one lane per row: the lane walks its row in CSR order
+// (edge order), adding each pair force as the serial scatter does, with the
+// next four neighbours' loads issued ahead.  Bit-identical to the oracle.
+__global__ __launch_bounds__(256) void lj_csr_seq(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ cols, int N, LJParams p,
+                                                  float4 *__restrict__ force, float4 *__restrict__ vel,
+                                                  float4 *__restrict__ pos_out, int mode)
+{
+    const int blk = pat::xcd_swizzle(blockIdx.x, gridDim.x);
+    const int row = blk * 256 + threadIdx.x;
+    if (row >= N) return;
+    const float4 pi = pos[row];
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int b = rowptr[row], e = rowptr[row + 1];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    int k = b;
+    for (; k + 4 <= e; k += 4) {
+        const int j0 = cols[k], j1 = cols[k + 1], j2 = cols[k + 2], j3 = cols[k + 3];
+        const float4 q0 = pos[j0], q1 = pos[j1], q2 = pos[j2], q3 = pos[j3];
+        float ax, ay, az;
+        pair_force(pi, q0, p, ax, ay, az);
+        fx = fx + ax; fy = fy + ay; fz = fz + az;
+        pair_force(pi, q1, p, ax, ay, az);
+        fx = fx + ax; fy = fy + ay; fz = fz + az;
+        pair_force(pi, q2, p, ax, ay, az);
+        fx = fx + ax; fy = fy + ay; fz = fz + az;
+        pair_force(pi, q3, p, ax, ay, az);
+        fx = fx + ax; fy = fy + ay; fz = fz + az;
+    }
+    for (; k < e; k++) {
+        float ax, ay, az;
+        pair_force(pi, pos[cols[k]], p, ax, ay, az);
+        fx = fx + ax; fy = fy + ay; fz = fz + az;
+    }
+    lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
+// In-order row sums with LPR lanes per row: each chunk of LPR consecutive
+// entries is evaluated in parallel (one pair per lane, coalesced column
+// reads), then every lane of the group folds the chunk's values into the
+// running sum in entry order through cross-lane reads.  Same sums, bit for
+// bit, as lj_csr_seq.
+template <int LPR>
+__global__ __launch_bounds__(256) void lj_csr_ord(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ cols, int N, LJParams p,
+                                                  float4 *__restrict__ force, float4 *__restrict__ vel,
+                                                  float4 *__restrict__ pos_out, int mode)
+{
+    constexpr int RPB = 256 / LPR;
+    const int blk = pat::xcd_swizzle(blockIdx.x, gridDim.x);
+    const int row = blk * RPB + threadIdx.x / LPR;
+    const int l = threadIdx.x % LPR;
+    if (row >= N) return;  // whole groups leave together
+    const float4 pi = pos[row];
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int b = rowptr[row], e = rowptr[row + 1];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int k0 = b; k0 < e; k0 += LPR) {
+        const int k = k0 + l;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (k < e) pair_force(pi, pos[cols[k]], p, ax, ay, az);
+        const int n = min(LPR, e - k0);
+#pragma unroll
+        for (int t = 0; t < LPR; t++) {
+            const float bx = __shfl(ax, t, LPR), by = __shfl(ay, t, LPR), bz = __shfl(az, t, LPR);
+            if (t < n) {
+                fx = fx + bx;
+                fy = fy + by;
+                fz = fz + bz;
+            }
+        }
+    }
+    if (l == 0) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
 }
 
 // Per-node half potential 0.5*sum_j phi(i, j) over the full CSR row.
@@ -446,8 +582,8 @@ int64_t csr_workspace_bytes(int64_t N, int64_t E)
 {
     const int64_t nb = (N + 1 + kScanBlock - 1) / kScanBlock;
     const int64_t nb2 = (nb + kScanBlock - 1) / kScanBlock;
-    // deg/cursor (N+1), block sums (nb), their scan (nb), second level (nb2 x 2), tmp cols (2E)
-    return (int64_t)sizeof(int) * ((N + 1) + 2 * nb + 2 * nb2 + 2 + 2 * E) + 256;
+    // deg/cursor (N+1), block sums (nb), their scan (nb), second level (nb2 x 2), tmp cols and edge ids (2 x 2E)
+    return (int64_t)sizeof(int) * ((N + 1) + 2 * nb + 2 * nb2 + 2 + 4 * E) + 256;
 }
 
 // Exclusive scan of n ints (n <= kScanBlock^3) using ws for block sums.
@@ -496,10 +632,11 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32
     // cursor = rowptr (reuse deg)
     PAT_HIP(hipMemcpyAsync(deg, rowptr, sizeof(int) * (N + 1), hipMemcpyDeviceToDevice, st));
     if (E > 0) {
-        hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp);
+        int *tmpe = tmp + 2 * E;
+        hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp, tmpe);
         PAT_LAUNCH_CHECK();
         const unsigned g2 = (unsigned)((N * kWave + 255) / 256);
-        hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols);
+        hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, tmpe, cols);
         PAT_LAUNCH_CHECK();
     }
     return PAT_OK;
@@ -507,14 +644,26 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32
 
 using CsrKernel = void (*)(const float4 *, const int *, const int *, int, LJParams, float4 *, float4 *, float4 *, int);
 
-static CsrKernel csr_kernel_for(int lpr)
+// lpr: 1 = one lane per row (in order); 4..32 = lanes per row (in order);
+// tree != 0 selects the butterfly variant (4..64 lanes).
+static CsrKernel csr_kernel_for(int lpr, int tree)
 {
+    if (tree) {
+        switch (lpr) {
+        case 4: return lj_csr_tree<4>;
+        case 8: return lj_csr_tree<8>;
+        case 16: return lj_csr_tree<16>;
+        case 32: return lj_csr_tree<32>;
+        case 64: return lj_csr_tree<64>;
+        default: return nullptr;
+        }
+    }
     switch (lpr) {
-    case 4: return lj_csr<4>;
-    case 8: return lj_csr<8>;
-    case 16: return lj_csr<16>;
-    case 32: return lj_csr<32>;
-    case 64: return lj_csr<64>;
+    case 1: return lj_csr_seq;
+    case 4: return lj_csr_ord<4>;
+    case 8: return lj_csr_ord<8>;
+    case 16: return lj_csr_ord<16>;
+    case 32: return lj_csr_ord<32>;
     default: return nullptr;
     }
 }
@@ -522,8 +671,12 @@ static CsrKernel csr_kernel_for(int lpr)
 int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
                  float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out, hipStream_t st)
 {
-    CsrKernel k = csr_kernel_for(lpr);
-    PAT_REQUIRE(k, PAT_ERR_UNSUPPORTED, "lj_force_csr: lanes per row %d not in {4,8,16,32,64}", lpr);
+    // negative lpr selects the tree-order variant
+    const int tree = lpr < 0;
+    if (tree) lpr = -lpr;
+    CsrKernel k = csr_kernel_for(lpr, tree);
+    PAT_REQUIRE(k, PAT_ERR_UNSUPPORTED, "lj_force_csr: lanes per row %d not available (%s)", lpr,
+                tree ? "tree: 4..64" : "in order: 1, 4, 8, 16, 32");
     PAT_REQUIRE(N > 0 && N < (1LL << 31), PAT_ERR_INVALID, "lj_force_csr: bad node count");
     const int rpb = 256 / lpr;
     const int64_t blocks = (N + rpb - 1) / rpb;

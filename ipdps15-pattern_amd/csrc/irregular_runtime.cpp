@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "pat_common.hpp"
@@ -44,8 +45,8 @@ namespace {
 
 constexpr int kForceOnly = 0, kKick = 1, kKickDrift = 2;
 
-// Default lanes per CSR row: the smallest power of two covering about half
-// the mean degree, within [4, 32].
+// Default lanes per CSR row before tuning: the smallest power of two
+// covering about a quarter of the mean degree, within [4, 32].
 int default_lpr(int64_t N, int64_t E)
 {
     double mean = N > 0 ? 2.0 * (double)E / (double)N : 0.0;
@@ -53,6 +54,12 @@ int default_lpr(int64_t N, int64_t E)
     while (l < 32 && l < mean / 4.0) l *= 2;
     return l;
 }
+
+// Tuned lanes per row, cached per (node count, mean degree) class for the
+// process: the in-order variants give bit-identical sums, so the choice only
+// changes speed.
+std::map<std::pair<int64_t, int>, int> g_lpr_cache;
+std::mutex g_lpr_mutex;
 
 // Device state shared by the IrregularRuntime handle and the MD handle.
 struct IrState {
@@ -101,7 +108,7 @@ struct IrState {
         (void)hipFree(work);
         if (rc != PAT_OK) return rc;
         PAT_HIP(se);
-        lpr = default_lpr(N, E);
+        lpr = 1;
         return PAT_OK;
     }
 
@@ -155,14 +162,81 @@ struct IrState {
         return PAT_OK;
     }
 
+    // lanes-per-row argument of the force kernels (negative = tree order)
+    int klpr() const { return variant == PAT_IR_CSR_TREE ? -lpr : lpr; }
+
     int force_eval(const float *p4, float box, int pbc, float rc2)
     {
         if (variant == PAT_IR_ATOMIC)
             return pat::lj_force_atomic((const float4 *)p4, edges, E, N, box, pbc, rc2, (float4 *)force, stream);
-        return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, lpr, kForceOnly,
+        return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, klpr(), kForceOnly,
                                  (float4 *)force, nullptr, nullptr, stream);
     }
+
+    // Times the in-order force kernel for each lanes-per-row candidate on
+    // this CSR and keeps the fastest (cached by size class).  Overwrites the
+    // force buffer: callers recompute the force afterwards.
+    int tune(const float *p4, float box, int pbc, float rc2)
+    {
+        const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
+        const auto key = std::make_pair((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg);
+        if (getenv("PAT_NO_TUNE")) {
+            lpr = 1;
+            return PAT_OK;
+        }
+        {
+            std::lock_guard<std::mutex> g(g_lpr_mutex);
+            auto it = g_lpr_cache.find(key);
+            if (it != g_lpr_cache.end()) {
+                lpr = it->second;
+                return PAT_OK;
+            }
+        }
+        hipEvent_t a, b;
+        PAT_HIP(hipEventCreate(&a));
+        PAT_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        int best_l = 1;
+        for (int l : {1, 4, 8, 16, 32}) {
+            for (int rep = 0; rep < 4; rep++) {
+                PAT_HIP(hipEventRecord(a, stream));
+                PAT_TRY(pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, l, kForceOnly,
+                                          (float4 *)force, nullptr, nullptr, stream));
+                PAT_HIP(hipEventRecord(b, stream));
+                PAT_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                PAT_HIP(hipEventElapsedTime(&ms, a, b));
+                if (rep > 0 && ms < best) {  // first launch of each is warm-up
+                    best = ms;
+                    best_l = l;
+                }
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        lpr = best_l;
+        std::lock_guard<std::mutex> g(g_lpr_mutex);
+        g_lpr_cache[key] = best_l;
+        return PAT_OK;
+    }
 };
+
+int check_variant(int variant, int lanes_per_row)
+{
+    PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC || variant == PAT_IR_CSR_TREE, PAT_ERR_UNSUPPORTED,
+                "irregular: unknown variant %d", variant);
+    if (variant == PAT_IR_CSR)
+        PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 1 || lanes_per_row == 4 || lanes_per_row == 8 ||
+                        lanes_per_row == 16 || lanes_per_row == 32,
+                    PAT_ERR_UNSUPPORTED, "irregular: in-order lanes per row %d not in {0 (tuned), 1, 4, 8, 16, 32}",
+                    lanes_per_row);
+    if (variant == PAT_IR_CSR_TREE)
+        PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
+                        lanes_per_row == 32 || lanes_per_row == 64,
+                    PAT_ERR_UNSUPPORTED, "irregular: tree lanes per row %d not in {0, 4, 8, 16, 32, 64}",
+                    lanes_per_row);
+    return PAT_OK;
+}
 
 int check_sizes(int64_t num_edges, int64_t num_nodes, const int32_t *edges)
 {
@@ -175,6 +249,9 @@ int check_sizes(int64_t num_edges, int64_t num_nodes, const int32_t *edges)
         PAT_REQUIRE(edges[e] >= 0 && edges[e] < num_nodes, PAT_ERR_INVALID,
                     "irregular: edge %lld references node %d outside [0, %lld)", (long long)(e / 2), edges[e],
                     (long long)num_nodes);
+    for (int64_t e = 0; e < num_edges; e++)
+        PAT_REQUIRE(edges[2 * e] != edges[2 * e + 1], PAT_ERR_INVALID, "irregular: edge %lld joins node %d to itself",
+                    (long long)e, edges[2 * e]);
     return PAT_OK;
 }
 
@@ -239,7 +316,7 @@ struct pat_md {
             if ((steps + k + 1) % every == 0) rc = rebuild(c);
             if (rc != PAT_OK) break;
             if (kernel_ms) (void)hipEventRecord(ev[2 * k], s);
-            rc = pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.lpr,
+            rc = pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.klpr(),
                                    last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
                                    (float4 *)st.pos[1 - c], s);
             if (kernel_ms) (void)hipEventRecord(ev[2 * k + 1], s);
@@ -285,7 +362,7 @@ struct pat_md {
         int c = start;
         for (int k = 0; k < nsteps; k++) {
             const bool last = k == nsteps - 1;
-            PAT_TRY(pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.lpr,
+            PAT_TRY(pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.klpr(),
                                       last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
                                       (float4 *)st.pos[1 - c], s));
             if (!last) c = 1 - c;
@@ -318,7 +395,7 @@ int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t
 {
     PAT_REQUIRE(pos4 && rowptr && cols && force4, PAT_ERR_INVALID, "pat_lj_force_csr_dev: null argument");
     PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_csr_dev: box must exceed twice the cutoff");
-    return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc, cutoff * cutoff, 0.f,
+    return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff, 0.f,
                              lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, (hipStream_t)stream);
 }
 
@@ -391,13 +468,10 @@ int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size)
 int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_variant: null handle");
-    PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC, PAT_ERR_UNSUPPORTED,
-                "pat_ir_set_variant: unknown variant %d", variant);
-    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
-                    lanes_per_row == 32 || lanes_per_row == 64,
-                PAT_ERR_UNSUPPORTED, "pat_ir_set_variant: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
+    PAT_TRY(check_variant(variant, lanes_per_row));
     h->st.variant = variant;
-    h->st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->st.N, h->st.E);
+    h->st.lpr = lanes_per_row ? lanes_per_row : (variant == PAT_IR_CSR_TREE ? default_lpr(h->st.N, h->st.E) : 1);
+    h->have_result = false;
     return PAT_OK;
 }
 
@@ -455,6 +529,7 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
         std::vector<float> zero(3 * (size_t)num_nodes, 0.0f);
         rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
     }
+    if (rc == PAT_OK) rc = h->st.tune(h->st.pos[0], h->box, h->pbc, h->rc2);
     if (rc == PAT_OK) rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
     if (rc == PAT_OK && hipStreamSynchronize(h->st.stream) != hipSuccess) {
         pat::set_error("pat_md_create: initial force evaluation failed");
@@ -497,10 +572,8 @@ int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, 
         rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
     }
     if (rc == PAT_OK) rc = h->rebuild(0);
-    if (rc == PAT_OK) {
-        h->st.lpr = default_lpr(h->st.N, h->st.E);
-        rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
-    }
+    if (rc == PAT_OK) rc = h->st.tune(h->st.pos[0], h->box, h->pbc, h->rc2);
+    if (rc == PAT_OK) rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
     if (rc == PAT_OK && hipStreamSynchronize(h->st.stream) != hipSuccess) {
         pat::set_error("pat_md_create_binned: initial force evaluation failed");
         rc = PAT_ERR_HIP;
@@ -540,16 +613,17 @@ int pat_md_csr(pat_md *h, int32_t *rowptr, int32_t *cols)
 int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_variant: null handle");
-    PAT_REQUIRE(variant == PAT_IR_CSR || variant == PAT_IR_ATOMIC, PAT_ERR_UNSUPPORTED,
-                "pat_md_set_variant: unknown variant %d", variant);
+    PAT_TRY(check_variant(variant, lanes_per_row));
     PAT_REQUIRE(!(variant == PAT_IR_ATOMIC && h->every > 0), PAT_ERR_UNSUPPORTED,
                 "pat_md_set_variant: the atomic kernel needs a half edge list (binned handles have a CSR only)");
-    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
-                    lanes_per_row == 32 || lanes_per_row == 64,
-                PAT_ERR_UNSUPPORTED, "pat_md_set_variant: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
     h->st.variant = variant;
-    h->st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->st.N, h->st.E);
+    if (lanes_per_row) h->st.lpr = lanes_per_row;
+    else if (variant == PAT_IR_CSR_TREE) h->st.lpr = default_lpr(h->st.N, h->st.E);
+    else PAT_TRY(h->st.tune(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
     h->drop_graphs();
+    // the stored force seeds the next half kick: recompute it with this variant
+    PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
     return PAT_OK;
 }
 
@@ -637,7 +711,7 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
             rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
             if (rc == PAT_OK)
                 rc = pat::lj_force_csr((const float4 *)h->st.pos[c], h->st.rowptr, h->st.cols, h->st.N, h->box, h->pbc,
-                                       h->rc2, h->dt, h->st.lpr, last ? kKick : kKickDrift, (float4 *)h->st.force,
+                                       h->rc2, h->dt, h->st.klpr(), last ? kKick : kKickDrift, (float4 *)h->st.force,
                                        (float4 *)h->st.vel, (float4 *)h->st.pos[1 - c], s);
             if (rc == PAT_OK) rc = hipEventRecord(ev[2 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
             if (!last) c = 1 - c;
@@ -787,7 +861,7 @@ struct pat_mdb {
                 PAT_HIP(hipEventRecord(e0, s));
             }
             PAT_TRY(pat::lj_force_csr((const float4 *)md.st.pos[c], md.st.rowptr, md.st.cols, md.st.N, 1.0f, 2, md.rc2,
-                                      md.dt, md.st.lpr, final_step ? kKick : kKickDrift, (float4 *)md.st.force,
+                                      md.dt, md.st.klpr(), final_step ? kKick : kKickDrift, (float4 *)md.st.force,
                                       (float4 *)md.st.vel, (float4 *)md.st.pos[1 - c], s));
             if (kernel_ms) {
                 PAT_HIP(hipEventRecord(e1, s));
@@ -903,11 +977,10 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
         if (hipMalloc((void **)&h->rp_s, sizeof(int32_t) * (size_t)(N + 1)) != hipSuccess) rc = PAT_ERR_NOMEM;
     }
     if (rc == PAT_OK) rc = h->rebuild_all(0);
-    if (rc == PAT_OK) {
-        m.st.lpr = default_lpr(m.st.N, m.st.E);
+    if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
+    if (rc == PAT_OK)
         rc = pat::lj_force_csr((const float4 *)m.st.pos[0], m.st.rowptr, m.st.cols, N, 1.0f, 2, m.rc2, 0.f, m.st.lpr,
                                kForceOnly, (float4 *)m.st.force, nullptr, nullptr, m.st.stream);
-    }
     if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc != PAT_OK) {
         if (h->rp_s) (void)hipFree(h->rp_s);
@@ -937,10 +1010,9 @@ int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms)
 int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_set_lanes: null handle");
-    PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
-                    lanes_per_row == 32 || lanes_per_row == 64,
-                PAT_ERR_UNSUPPORTED, "pat_mdb_set_lanes: lanes per row %d not in {4,8,16,32,64}", lanes_per_row);
-    h->md.st.lpr = lanes_per_row ? lanes_per_row : default_lpr(h->md.st.N, h->md.st.E);
+    PAT_TRY(check_variant(PAT_IR_CSR, lanes_per_row));
+    if (lanes_per_row) h->md.st.lpr = lanes_per_row;
+    else PAT_TRY(h->md.st.tune(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2));
     h->drop_seg_graphs();
     // the stored force must come from the same summation order
     PAT_TRY(pat::lj_force_csr((const float4 *)h->md.st.pos[h->md.cur], h->md.st.rowptr, h->md.st.cols, h->md.st.N,

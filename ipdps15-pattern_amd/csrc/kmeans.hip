@@ -321,12 +321,17 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 #pragma unroll
             for (int b = 0; b < KB; b++) {
                 acc[b] = f32x16{};
-                const int row = (ch * KB + b) * 32 + col;  // A operand row: centre
+                if ((ch * KB + b) * 32 < K_pad) {
+                    const int row = (ch * KB + b) * 32 + col;  // A operand row: centre
 #pragma unroll
-                for (int s = 0; s <= KS; s++) {
-                    bf16x8 av = row < K_pad ? *(const bf16x8 *)(cbs + (int64_t)row * KW + 16 * s + 8 * half)
-                                            : bf16x8{};
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[s], acc[b], 0, 0, 0);
+                    for (int s = 0; s <= KS; s++) {
+                        const bf16x8 av = *(const bf16x8 *)(cbs + (int64_t)row * KW + 16 * s + 8 * half);
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[s], acc[b], 0, 0, 0);
+                    }
+                } else {
+                    // a block past the last centre row: never a candidate
+#pragma unroll
+                    for (int i = 0; i < 16; i++) acc[b][i] = INFINITY;
                 }
             }
             // chunk minimum for this lane's point (rows of both half-waves)
@@ -511,6 +516,9 @@ int kmeans_maxabs(const float *x, int64_t n, unsigned int *out_bits, hipStream_t
 }
 
 
+// Largest dynamic LDS a workgroup may request on this device (queried once).
+static int64_t g_lds_limit = 64 * 1024;
+
 static KmArgs make_args(const KmDev &d)
 {
     KmArgs a;
@@ -526,28 +534,37 @@ static KmArgs make_args(const KmDev &d)
     a.gsum = d.gsum;
     a.gcnt = d.gcnt;
     a.gdsum = d.gdsum;
-    a.use_lds_acc = object_bytes(d.K, d.D) <= 96 * 1024;
+    a.use_lds_acc = object_bytes(d.K, d.D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024);
     return a;
 }
 
-// Raises every kmeans kernel's dynamic-LDS limit to the 160 KB a gfx950
-// workgroup can address.  Called once before any launch (outside graph
-// capture).
+
+// Raises every kmeans kernel's dynamic-LDS limit to what the device allows
+// per workgroup (160 KB on gfx950).  Best effort: the per-block limit the
+// device reports is what the launch checks use.  Called once before any
+// launch (outside graph capture).
 int kmeans_prepare_kernels()
 {
     static int done = 0;
     if (done) return PAT_OK;
+    int dev = 0, lim = 0;
+    PAT_HIP(hipGetDevice(&dev));
+    PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    g_lds_limit = lim - 256;  // room for the kernels' static LDS
     const void *fns[] = {(const void *)km_plain<8>, (const void *)km_plain<16>, (const void *)km_plain<32>,
                          (const void *)km_plain<64>, (const void *)km_plain<128>, (const void *)km_mfma<1, 4>,
                          (const void *)km_mfma<2, 4>, (const void *)km_mfma<4, 2>, (const void *)km_mfma<8, 1>};
-    for (const void *f : fns) PAT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    for (const void *f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
+            (void)hipGetLastError();
     done = 1;
     return PAT_OK;
 }
 
 static int allow_lds(const void *, int64_t bytes)
 {
-    PAT_REQUIRE(bytes <= 160 * 1024, PAT_ERR_UNSUPPORTED, "kmeans: %lld bytes of LDS requested", (long long)bytes);
+    PAT_REQUIRE(bytes <= g_lds_limit, PAT_ERR_UNSUPPORTED, "kmeans: %lld bytes of LDS requested, device allows %lld",
+                (long long)bytes, (long long)g_lds_limit);
     return PAT_OK;
 }
 
@@ -571,7 +588,7 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
     KmArgs a = make_args(d);
     const int64_t cbytes = 4LL * d.K * d.D;
     int64_t lds = a.use_lds_acc ? object_bytes(d.K, d.D) : 0;
-    int in_lds = lds + cbytes <= 120 * 1024;
+    int in_lds = lds + cbytes <= g_lds_limit;
     if (in_lds) lds += cbytes;
     const int64_t want = (d.N + 511) / 512;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
@@ -602,7 +619,7 @@ bool kmeans_mfma_fits(int K, int D)
     const int K_pad = (K + 31) / 32 * 32;
     const int64_t cbb = 2LL * K_pad * 16 * (ks + 1);
     const int64_t obj = object_bytes(K, D) <= 96 * 1024 ? object_bytes(K, D) : 0;
-    return cbb + obj <= 150 * 1024;
+    return cbb + obj <= g_lds_limit;
 }
 
 int kmeans_assign_mfma(const KmDev &d, hipStream_t st)

@@ -146,6 +146,7 @@ int pat_kmeans_fixed_shift(int64_t n, float maxabs, int *shift)
 int pat_kmeans_create(const float *x, int64_t n, int d, int k, int mode, int x_on_device, pat_kmeans **out)
 {
     PAT_REQUIRE(out && x, PAT_ERR_INVALID, "pat_kmeans_create: null argument");
+    PAT_TRY(pat::kmeans_prepare_kernels());
     PAT_REQUIRE(n > 0 && n < (1LL << 40), PAT_ERR_INVALID, "pat_kmeans_create: bad point count");
     PAT_REQUIRE(d > 0 && d <= 128, PAT_ERR_UNSUPPORTED, "pat_kmeans_create: D = %d outside 1..128", d);
     PAT_REQUIRE(k > 0 && k <= n && k <= 65536, PAT_ERR_INVALID, "pat_kmeans_create: K = %d outside 1..min(N, 65536)",

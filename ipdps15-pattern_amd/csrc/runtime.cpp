@@ -204,11 +204,13 @@ int pat_device_query(int *num_cus, int64_t *total_bytes, int64_t *free_bytes)
 int pat_info(char *buf, int64_t buf_len)
 {
     PAT_REQUIRE(buf && buf_len > 0, PAT_ERR_INVALID, "pat_info: null buffer");
-    char name[600] = "none";
+    char name[800] = "none";
     int n = 0;
     if (hipGetDeviceCount(&n) == hipSuccess && n > 0) {
         hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, pat::g_device) == hipSuccess) snprintf(name, sizeof(name), "%s (%s)", p.name, p.gcnArchName);
+        if (hipGetDeviceProperties(&p, pat::g_device) == hipSuccess)
+            snprintf(name, sizeof(name), "%s (%s) cus=%d lds_per_block=%zu clock_khz=%d", p.name, p.gcnArchName,
+                     p.multiProcessorCount, (size_t)p.sharedMemPerBlock, p.clockRate);
     }
     snprintf(buf, (size_t)buf_len, "libpattern 0.1 target=gfx950 device=%s", name);
     return PAT_OK;

@@ -21,6 +21,7 @@ PAT_IR_LJ_MINIMAGE = 0
 PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0
 PAT_IR_ATOMIC = 1
+PAT_IR_CSR_TREE = 2
 PAT_KM_PLAIN = 0
 PAT_KM_MFMA = 1
 

@@ -342,6 +342,146 @@ void orc_lj_force_f32_threads(int64_t N, int64_t E, const int32_t *edges, const 
 }
 
 /*
+ * Gather form of the serial scatter: for each node, the contributions of its
+ * incident edges are added in edge order, exactly as orc_lj_force_f32 adds
+ * them, so the result is bit-identical to the serial scatter while nodes are
+ * processed in parallel.  The incidence lists are built once per call by a
+ * stable counting sort of (node, edge).  Used where the serial order must be
+ * kept at full size.
+ */
+void orc_lj_force_f32_gather(int64_t N, int64_t E, const int32_t *edges, const float *pos, float L, int pbc,
+                             float rc2, float *f, int nthreads)
+{
+    const float hL = 0.5f * L;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    int64_t *start = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+    int64_t *inc = (int64_t *)malloc(sizeof(int64_t) * 2 * (size_t)(E > 0 ? E : 1));
+    for (int64_t e = 0; e < E; e++) {
+        start[edges[2 * e] + 1]++;
+        start[edges[2 * e + 1] + 1]++;
+    }
+    for (int64_t i = 0; i < N; i++) start[i + 1] += start[i];
+    {
+        int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N > 0 ? N : 1));
+        memcpy(cur, start, sizeof(int64_t) * N);
+        for (int64_t e = 0; e < E; e++) {  /* edge order within each node */
+            inc[cur[edges[2 * e]]++] = 2 * e;          /* node is the i side */
+            inc[cur[edges[2 * e + 1]]++] = 2 * e + 1;  /* node is the j side */
+        }
+        free(cur);
+    }
+    #pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+    for (int64_t r = 0; r < N; r++) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (int64_t k = start[r]; k < start[r + 1]; k++) {
+            const int64_t e = inc[k] >> 1;
+            const int side = (int)(inc[k] & 1);
+            const int i = edges[2 * e], j = edges[2 * e + 1];
+            float fx, fy, fz;
+            if (pair_force_f32(pos + 3 * (int64_t)i, pos + 3 * (int64_t)j, L, hL, pbc, rc2, &fx, &fy, &fz, 0)) {
+                if (side == 0) {
+                    ax += fx;
+                    ay += fy;
+                    az += fz;
+                } else {
+                    ax -= fx;
+                    ay -= fy;
+                    az -= fz;
+                }
+            }
+        }
+        f[3 * r] = ax;
+        f[3 * r + 1] = ay;
+        f[3 * r + 2] = az;
+    }
+    free(start);
+    free(inc);
+}
+
+/*
+ * Cell-list half pair list: every pair i<j whose minimum-image distance
+ * squared (same float expression as orc_pairs_brute) is below cut2, in
+ * (i, j) lexicographic order.  Cells are at least one cutoff wide (needs
+ * L >= 3 * cutoff).  The binned build of miniMD's Neighbor::build
+ * (miniMD/neighbor.cpp:76-205), with periodic images.  Returns the count;
+ * writes at most cap pairs.
+ */
+static int cmp_i32(const void *a, const void *b)
+{
+    int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+int64_t orc_pairs_cells(int64_t N, const float *pos, float L, float cut, int32_t *edges, int64_t cap)
+{
+    const float hL = 0.5f * L, cut2 = cut * cut;
+    int nb = (int)floor((double)L / ((double)cut * 1.0001));
+    if (nb < 3) return -1;
+    const int64_t ncell = (int64_t)nb * nb * nb;
+    int64_t *head = (int64_t *)calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t *cellof = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int64_t i = 0; i < N; i++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            int q = (int)floor((double)pos[3 * i + d] * nb / L);
+            c[d] = q < 0 ? 0 : (q >= nb ? nb - 1 : q);
+        }
+        cellof[i] = c[0] + (int64_t)nb * (c[1] + (int64_t)nb * c[2]);
+        head[cellof[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) head[c + 1] += head[c];
+    {
+        int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+        memcpy(cur, head, sizeof(int64_t) * ncell);
+        for (int64_t i = 0; i < N; i++) order[cur[cellof[i]]++] = (int32_t)i;
+        free(cur);
+    }
+    int64_t n = 0;
+    int32_t *row = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int64_t i = 0; i < N; i++) {
+        const int64_t c = cellof[i];
+        const int cx = (int)(c % nb), cy = (int)((c / nb) % nb), cz = (int)(c / ((int64_t)nb * nb));
+        int64_t m = 0;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int qx = (cx + dx + nb) % nb, qy = (cy + dy + nb) % nb, qz = (cz + dz + nb) % nb;
+                    const int64_t q = qx + (int64_t)nb * (qy + (int64_t)nb * qz);
+                    for (int64_t s = head[q]; s < head[q + 1]; s++) {
+                        const int32_t j = order[s];
+                        if (j <= i) continue;
+                        float ddx = pos[3 * i] - pos[3 * j];
+                        float ddy = pos[3 * i + 1] - pos[3 * j + 1];
+                        float ddz = pos[3 * i + 2] - pos[3 * j + 2];
+                        if (ddx < -hL) ddx += L;
+                        if (ddy < -hL) ddy += L;
+                        if (ddz < -hL) ddz += L;
+                        if (ddx > hL) ddx -= L;
+                        if (ddy > hL) ddy -= L;
+                        if (ddz > hL) ddz -= L;
+                        float r2 = ddx * ddx + ddy * ddy;
+                        r2 = r2 + ddz * ddz;
+                        if (r2 < cut2) row[m++] = j;
+                    }
+                }
+        qsort(row, (size_t)m, sizeof(int32_t), cmp_i32);
+        for (int64_t t = 0; t < m; t++) {
+            if (edges && n < cap) {
+                edges[2 * n] = (int32_t)i;
+                edges[2 * n + 1] = row[t];
+            }
+            n++;
+        }
+    }
+    free(row);
+    free(head);
+    free(cellof);
+    free(order);
+    return n;
+}
+
+/*
  * Velocity-Verlet with unit mass and periodic wrap into [0, L), the
  * integrator the moldyn and miniMD workloads run around the force
  * reduction (moldyn's own update is a leapfrog with folded constants,
@@ -362,21 +502,70 @@ static inline float wrap_f32(float x, float L)
 double orc_md_run_f32(int64_t N, int64_t E, const int32_t *edges, float *pos, float *vel, float *f, float L,
                       int pbc, float rc2, float dt, int nsteps, int nthreads, float *scratch)
 {
+    /* Forces: serial order (gather form, any thread count) unless scratch is
+     * given, in which case the reference CPU engine's chunked thread-private
+     * accumulation is used (its sums round in another order). */
     const float hdt = 0.5f * dt;
     double epot = 0.0;
-    if (nthreads == 1 || !scratch) epot = orc_lj_force_f32(N, E, edges, pos, L, pbc, rc2, f);
-    else orc_lj_force_f32_threads(N, E, edges, pos, L, pbc, rc2, f, scratch, nthreads);
+#define ORC_FORCE()                                                                                                    \
+    do {                                                                                                               \
+        if (scratch) orc_lj_force_f32_threads(N, E, edges, pos, L, pbc, rc2, f, scratch, nthreads);                    \
+        else if (nthreads > 1) orc_lj_force_f32_gather(N, E, edges, pos, L, pbc, rc2, f, nthreads);                    \
+        else epot = orc_lj_force_f32(N, E, edges, pos, L, pbc, rc2, f);                                                \
+    } while (0)
+    ORC_FORCE();
     for (int s = 0; s < nsteps; s++) {
         for (int64_t m = 0; m < 3 * N; m++) {
             vel[m] = vel[m] + hdt * f[m];
             float x = pos[m] + dt * vel[m];
             pos[m] = pbc ? wrap_f32(x, L) : x;
         }
-        if (nthreads == 1 || !scratch) epot = orc_lj_force_f32(N, E, edges, pos, L, pbc, rc2, f);
-        else orc_lj_force_f32_threads(N, E, edges, pos, L, pbc, rc2, f, scratch, nthreads);
+        ORC_FORCE();
         for (int64_t m = 0; m < 3 * N; m++) vel[m] = vel[m] + hdt * f[m];
     }
+#undef ORC_FORCE
     return epot;
+}
+
+/*
+ * The MD loop with a neighbour list rebuilt from the positions before the
+ * force of step 0 and of every step whose index is a multiple of 'every'
+ * (miniMD's reneighbouring, miniMD/lj.in:9-10): pairs within cutneigh by
+ * orc_pairs_cells, forces within rc by the gather form (serial order).
+ */
+double orc_md_run_rebuild_f32(int64_t N, float *pos, float *vel, float *f, float L, float rc2, float cutneigh,
+                              int every, float dt, int nsteps, int nthreads, int64_t *npairs_out)
+{
+    const float hdt = 0.5f * dt;
+    int64_t cap = 0, np = 0;
+    int32_t *edges = NULL;
+    for (int s = 0; s <= nsteps; s++) {
+        if (s > 0) {
+            for (int64_t m = 0; m < 3 * N; m++) {
+                vel[m] = vel[m] + hdt * f[m];
+                pos[m] = wrap_f32(pos[m] + dt * vel[m], L);
+            }
+        }
+        if (s == 0 || s % every == 0) {
+            np = orc_pairs_cells(N, pos, L, cutneigh, NULL, 0);
+            if (np < 0) {
+                free(edges);
+                return NAN;
+            }
+            if (np > cap) {
+                free(edges);
+                cap = np;
+                edges = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(cap > 0 ? cap : 1));
+            }
+            orc_pairs_cells(N, pos, L, cutneigh, edges, cap);
+        }
+        orc_lj_force_f32_gather(N, np, edges, pos, L, 1, rc2, f, nthreads);
+        if (s > 0)
+            for (int64_t m = 0; m < 3 * N; m++) vel[m] = vel[m] + hdt * f[m];
+    }
+    if (npairs_out) *npairs_out = np;
+    free(edges);
+    return 0.0;
 }
 
 static inline double wrap_f64(double x, double L)

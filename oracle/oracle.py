@@ -30,6 +30,10 @@ _SIG = {
                                   c_int, c_vp]),
     "orc_md_run_f64": (c_double, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_double, c_int, c_double, c_double, c_int]),
     "orc_pairs_brute": (c_i64, [c_i64, c_vp, c_float, c_int, c_float, c_vp, c_i64]),
+    "orc_pairs_cells": (c_i64, [c_i64, c_vp, c_float, c_float, c_vp, c_i64]),
+    "orc_lj_force_f32_gather": (None, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp, c_int]),
+    "orc_md_run_rebuild_f32": (c_double, [c_i64, c_vp, c_vp, c_vp, c_float, c_float, c_float, c_int, c_float, c_int,
+                                          c_int, ctypes.POINTER(c_i64)]),
     "orc_kmeans_assign_f32": (None, [c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int]),
     "orc_kmeans_assign_f64": (None, [c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int]),
     "orc_kmeans_fixed_shift": (c_int, [c_i64, c_float]),
@@ -106,20 +110,60 @@ def lj_force_threads(pos, edges, box, rc, pbc=True, threads=0):
     return f
 
 
-def md_run(pos, vel, edges, box, rc, dt, nsteps, pbc=True, dtype=np.float32, threads=1):
-    """Velocity-Verlet MD; returns (pos, vel, force, epot)."""
+def md_run(pos, vel, edges, box, rc, dt, nsteps, pbc=True, dtype=np.float32, threads=1, refcpu=False):
+    """Velocity-Verlet MD; returns (pos, vel, force, epot).  Forces sum in the
+    serial scatter's order for any thread count, unless refcpu selects the
+    reference CPU engine's chunked thread-private accumulation (baseline
+    timing).  epot is only computed by the single-thread serial path."""
     n = pos.shape[0]
     e = np.ascontiguousarray(edges, dtype=np.int32)
     p = np.ascontiguousarray(pos, dtype=dtype).copy()
     v = np.ascontiguousarray(vel, dtype=dtype).copy()
     f = np.zeros((n, 3), dtype=dtype)
     if dtype == np.float32:
-        scratch = np.empty((threads, n, 3), dtype=np.float32) if threads > 1 else None
+        scratch = np.empty((threads, n, 3), dtype=np.float32) if refcpu else None
         ep = lib().orc_md_run_f32(n, e.shape[0], _p(e), _p(p), _p(v), _p(f), box, int(pbc), np.float32(rc) ** 2,
                                   dt, nsteps, threads, _p(scratch))
     else:
         ep = lib().orc_md_run_f64(n, e.shape[0], _p(e), _p(p), _p(v), _p(f), box, int(pbc), rc * rc, dt, nsteps)
     return p, v, f, ep
+
+
+def lj_force_gather(pos, edges, box, rc, pbc=True, threads=0):
+    n = pos.shape[0]
+    e = np.ascontiguousarray(edges, dtype=np.int32)
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    f = np.zeros((n, 3), dtype=np.float32)
+    lib().orc_lj_force_f32_gather(n, e.shape[0], _p(e), _p(p), box, int(pbc), np.float32(rc) ** 2, _p(f),
+                                  threads or os.cpu_count())
+    return f
+
+
+def pairs_cells(pos, box, cut):
+    n = pos.shape[0]
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    cnt = lib().orc_pairs_cells(n, _p(p), box, cut, None, 0)
+    if cnt < 0:
+        raise ValueError("box too small for the cell list")
+    e = np.empty((max(cnt, 1), 2), dtype=np.int32)
+    lib().orc_pairs_cells(n, _p(p), box, cut, _p(e), cnt)
+    return e[:cnt]
+
+
+def md_run_rebuild(pos, vel, box, rc, cutneigh, every, dt, nsteps, threads=0):
+    """MD with the pair list rebuilt (cell list within cutneigh) before the
+    force of step 0 and of every step index multiple of 'every'; forces in
+    serial order.  Returns (pos, vel, force, last pair count)."""
+    n = pos.shape[0]
+    p = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    v = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    f = np.zeros((n, 3), dtype=np.float32)
+    npairs = c_i64(0)
+    r = lib().orc_md_run_rebuild_f32(n, _p(p), _p(v), _p(f), box, np.float32(rc) ** 2, cutneigh, every, dt, nsteps,
+                                     threads or os.cpu_count(), ctypes.byref(npairs))
+    if r != r:
+        raise ValueError("box too small for the cell list")
+    return p, v, f, npairs.value
 
 
 def pairs_brute(pos, box, cut, pbc=True):

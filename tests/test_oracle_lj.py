@@ -162,3 +162,38 @@ def test_velocities_generator(pa):
     assert np.max(np.abs(v.astype(np.float64).sum(axis=0))) < 1e-3
     t = np.sum(v.astype(np.float64) ** 2) / (3 * (4000 - 1))
     assert abs(t - 1.44) < 1e-4
+
+
+def test_gather_form_equals_serial_scatter(pa, orc):
+    pos, box, edges = pa.moldyn_system(5, seed=3)  # shuffled edge order
+    f_serial, _ = orc.lj_force(pos, edges, box, 2.5)
+    for t in (1, 3, 8):
+        assert np.array_equal(orc.lj_force_gather(pos, edges, box, 2.5, threads=t), f_serial)
+
+
+def test_md_threads_keep_serial_order(pa, orc):
+    pos, box, edges = pa.moldyn_system(4, seed=5)
+    vel = pa.gen_velocities(len(pos), 1.0, seed=6)
+    a = orc.md_run(pos, vel, edges, box, 2.5, 0.005, 6, threads=1)
+    b = orc.md_run(pos, vel, edges, box, 2.5, 0.005, 6, threads=7)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("cells,cut", [(4, 2.2), (6, 2.8), (7, 2.5)])
+def test_cell_list_pairs(pa, orc, cells, cut):
+    pos, box = pa.gen_fcc(cells, cells, cells, jitter=0.2, seed=cells)
+    e = orc.pairs_cells(pos, box[0], cut)
+    assert np.array_equal(e, orc.pairs_brute(pos, box[0], cut))
+    assert np.array_equal(e, pa.gen_pairs(pos, box[0], cut))
+
+
+def test_md_rebuild_matches_fixed_list_between_rebuilds(pa, orc):
+    # no rebuild inside the run: the same trajectory as a fixed list built at t=0
+    pos, box = pa.gen_fcc(6, 6, 6, jitter=0.0, seed=9)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=10)
+    e = orc.pairs_cells(pos, box[0], 2.8)
+    a = orc.md_run(pos, vel, e, box[0], 2.5, 0.005, 15, threads=2)
+    b = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 100, 0.005, 15, threads=2)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
