@@ -144,6 +144,9 @@ int pat_stencil_run(pat_stencil *s, int nsteps);
 int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *kernel_ms);
 /* Copies the current halo-inclusive grid out. */
 int pat_stencil_download(pat_stencil *s, void *host);
+/* Tile the runs use (tuned on the first run when none was set): x width,
+ * tile height in rows, z-chunk in planes. */
+int pat_stencil_tile(pat_stencil *s, int *bx, int *by, int *zc);
 /* Device pointer of the current grid. */
 int pat_stencil_device_ptr(pat_stencil *s, void **dptr);
 /* StencilFinalize + delete. */

@@ -22,6 +22,10 @@
 // halo-inclusive (nx+2)(ny+2)(nz+2), x fastest.
 // Compiled with -ffp-contract=off: each cell's arithmetic rounds exactly as
 // written, so results are bit-identical to a plain C sweep of the same rule.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "pat_common.hpp"
 
 namespace {
@@ -219,6 +223,61 @@ void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, i
     default_tile(nx, ny, nz, bx, by, ry, zc);
 }
 
+// Autotuner: times every instantiated tile shape with z-chunks of 8..64
+// planes on the caller's grid (in -> out; out is scratch until the first
+// real sweep writes it, and no kernel writes the halo), keeps the fastest,
+// and caches it per grid shape for the process.
+static std::map<std::tuple<int64_t, int64_t, int64_t>, std::tuple<int, int, int, int>> g_tile_cache;
+static std::mutex g_tile_mutex;
+
+int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
+                      int *bx, int *by, int *ry, int *zc)
+{
+    const auto key = std::make_tuple(nx, ny, nz);
+    {
+        std::lock_guard<std::mutex> g(g_tile_mutex);
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+            std::tie(*bx, *by, *ry, *zc) = it->second;
+            return PAT_OK;
+        }
+    }
+    default_tile(nx, ny, nz, bx, by, ry, zc);
+    if (getenv("PAT_NO_TUNE")) return PAT_OK;
+    hipEvent_t a, b;
+    PAT_HIP(hipEventCreate(&a));
+    PAT_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    for (const auto &sh : kShapes) {
+        for (int z : {8, 16, 32, 64}) {
+            if (z > nz && z != 8) continue;
+            const int zz = (int)std::min<int64_t>(z, nz);
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                PAT_HIP(hipEventRecord(a, st));
+                PAT_TRY(heat3d_sweep(in, out, nx, ny, nz, alpha, PAT_STENCIL_LDS25D, sh.bx, sh.by, sh.ry, zz, st));
+                PAT_HIP(hipEventRecord(b, st));
+                PAT_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                PAT_HIP(hipEventElapsedTime(&ms, a, b));
+                if (rep > 0) tmin = std::min(tmin, ms);
+            }
+            if (tmin < best) {
+                best = tmin;
+                *bx = sh.bx;
+                *by = sh.by;
+                *ry = sh.ry;
+                *zc = zz;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    std::lock_guard<std::mutex> g(g_tile_mutex);
+    g_tile_cache[key] = std::make_tuple(*bx, *by, *ry, *zc);
+    return PAT_OK;
+}
+
 int heat3d_shapes(const int **triples)
 {
     static int flat[sizeof(kShapes) / sizeof(kShapes[0]) * 3];
@@ -243,7 +302,10 @@ extern "C" int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, in
     PAT_REQUIRE(variant == PAT_STENCIL_AUTO || variant == PAT_STENCIL_LDS25D || variant == PAT_STENCIL_GENERIC,
                 PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: unknown variant %d", variant);
     int dbx, dby, dry, dzc;
-    default_tile(nx, ny, nz, &dbx, &dby, &dry, &dzc);
+    if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0)
+        PAT_TRY(pat::heat3d_tuned_tile(a, b, nx, ny, nz, alpha, (hipStream_t)stream, &dbx, &dby, &dry, &dzc));
+    else
+        default_tile(nx, ny, nz, &dbx, &dby, &dry, &dzc);
     // 'by' at the C ABI is the tile height in rows; split it into thread rows
     // and rows per thread using the instantiated shapes.
     int tbx = bx > 0 ? bx : dbx;

@@ -17,6 +17,8 @@ int heat3d_sweep(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
                  int by, int ry, int zc, hipStream_t stream);
 void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, int *ry, int *zc);
 int heat3d_shapes(const int **triples);
+int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
+                      int *bx, int *by, int *ry, int *zc);
 }  // namespace pat
 
 struct pat_stencil {
@@ -50,10 +52,14 @@ struct pat_stencil {
         return a;
     }
 
-    int resolve_tile(int *tbx, int *tby, int *tr, int *tzc) const
+    int resolve_tile(int *tbx, int *tby, int *tr, int *tzc)
     {
         int dbx, dby, dry, dzc;
-        pat::heat3d_default_tile(n[0], n[1], n[2], &dbx, &dby, &dry, &dzc);
+        if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0 && loaded)
+            PAT_TRY(pat::heat3d_tuned_tile(buf[cur], buf[1 - cur], n[0], n[1], n[2], alpha(), stream, &dbx, &dby, &dry,
+                                           &dzc));
+        else
+            pat::heat3d_default_tile(n[0], n[1], n[2], &dbx, &dby, &dry, &dzc);
         *tbx = bx > 0 ? bx : dbx;
         *tby = by > 0 ? by : dby;
         *tr = ry > 0 ? ry : dry;
@@ -64,7 +70,7 @@ struct pat_stencil {
     int enqueue(int nsteps, int start)
     {
         int tbx, tby, tr, tzc;
-        resolve_tile(&tbx, &tby, &tr, &tzc);
+        PAT_TRY(resolve_tile(&tbx, &tby, &tr, &tzc));
         int v = variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
         int s = start;
         for (int k = 0; k < nsteps; k++) {
@@ -174,6 +180,8 @@ int pat_stencil_run(pat_stencil *s, int nsteps)
     auto key = std::make_pair(nsteps, s->cur);
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
+        int tbx, tby, tr, tzc;
+        PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));  // tunes outside the capture
         hipGraph_t g;
         PAT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = s->enqueue(nsteps, s->cur);
@@ -200,7 +208,7 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
     for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
     int tbx, tby, tr, tzc;
-    s->resolve_tile(&tbx, &tby, &tr, &tzc);
+    PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));
     const int v = s->variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
     int rc = PAT_OK;
     int c = s->cur;
@@ -231,6 +239,17 @@ int pat_stencil_download(pat_stencil *s, void *host)
     PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_download: no grid uploaded");
     PAT_HIP(hipMemcpyAsync(host, s->buf[s->cur], sizeof(float) * (size_t)s->cells(), hipMemcpyDeviceToHost, s->stream));
     PAT_HIP(hipStreamSynchronize(s->stream));
+    return PAT_OK;
+}
+
+int pat_stencil_tile(pat_stencil *s, int *bx, int *by, int *zc)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_tile: null handle");
+    int tbx, tby, tr, tzc;
+    PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));
+    if (bx) *bx = tbx;
+    if (by) *by = tby * tr;
+    if (zc) *zc = tzc;
     return PAT_OK;
 }
 

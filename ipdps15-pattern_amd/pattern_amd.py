@@ -74,6 +74,7 @@ SIGNATURES = {
     "pat_stencil_run": (c_int, [c_vp, c_int]),
     "pat_stencil_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
     "pat_stencil_download": (c_int, [c_vp, c_vp]),
+    "pat_stencil_tile": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_stencil_device_ptr": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "pat_stencil_destroy": (c_int, [c_vp]),
     "pat_ir_csr_workspace": (c_int, [c_i64, c_i64, c_i64p]),
@@ -358,6 +359,11 @@ class Stencil:
         out = np.empty(self.shape, dtype=np.float32)
         call("pat_stencil_download", self.h, ptr(out))
         return out
+
+    def tile(self):
+        a, b, c = c_int(0), c_int(0), c_int(0)
+        call("pat_stencil_tile", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
 
     def close(self):
         if self.h:
