@@ -652,7 +652,7 @@ int pat_md_run(pat_md *h, int nsteps)
     }
     PAT_HIP(hipGraphLaunch(it->second, h->st.stream));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
-    if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    if (h->st.variant != PAT_IR_ATOMIC && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
     h->steps += nsteps;
     return PAT_OK;
 }
@@ -668,7 +668,7 @@ int pat_md_run_async(pat_md *h, int nsteps, void *stream)
     int rc = h->enqueue(nsteps, h->cur);
     h->st.stream = saved;
     if (rc != PAT_OK) return rc;
-    if (h->st.variant == PAT_IR_CSR && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
+    if (h->st.variant != PAT_IR_ATOMIC && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
     h->steps += nsteps;
     return PAT_OK;
 }
@@ -1016,7 +1016,7 @@ int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
     h->drop_seg_graphs();
     // the stored force must come from the same summation order
     PAT_TRY(pat::lj_force_csr((const float4 *)h->md.st.pos[h->md.cur], h->md.st.rowptr, h->md.st.cols, h->md.st.N,
-                              1.0f, 2, h->md.rc2, 0.f, h->md.st.lpr, kForceOnly, (float4 *)h->md.st.force, nullptr,
+                              1.0f, 2, h->md.rc2, 0.f, h->md.st.klpr(), kForceOnly, (float4 *)h->md.st.force, nullptr,
                               nullptr, h->md.st.stream));
     PAT_HIP(hipStreamSynchronize(h->md.st.stream));
     return PAT_OK;
