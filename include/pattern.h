@@ -165,20 +165,30 @@ int pat_stencil_destroy(pat_stencil *s);
 #define PAT_IR_LJ_OPEN 1     /* miniMD map1, miniMD/irregular.h:109-163: no images;
                                 param = double cutforcesq */
 /* Kernel variants. */
-#define PAT_IR_CSR 0      /* by-destination CSR, rows in edge order, summed in order:
-                             bit-identical to a serial scatter over the edge list (default) */
+#define PAT_IR_CSR 0      /* by-destination CSR, each row summed in its stored order: bit-identical
+                             to a serial scatter over the edge list in that order (default) */
 #define PAT_IR_ATOMIC 1   /* edge-parallel, float atomics into both endpoints */
 #define PAT_IR_CSR_TREE 2 /* CSR with butterfly (tree-order) row sums: FP32-tolerance parity */
+/* lanes_per_row value of PAT_IR_CSR selecting the sliced layout: one lane per
+ * row over 64-row slices stored entry-major (SELL-64), so a wavefront's k-th
+ * column read is one contiguous load.  Same sums, bit for bit, as every other
+ * in-order choice; available when padding stays within 1.5x the entries. */
+#define PAT_IR_SLICED 64
+/* CSR row orders.  A row's order is the order its pair forces are added in. */
+#define PAT_IR_ORDER_EDGE 0 /* as the edges appear in the list: the serial scatter over the list as given */
+#define PAT_IR_ORDER_NODE 1 /* by neighbour index: the serial scatter over the list sorted by
+                               (min(i, j), max(i, j)); keeps gathers of neighbouring rows
+                               close in memory (handle default) */
 
 /* Bytes of device workspace pat_ir_csr_build_dev needs. */
 int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes);
 /* Reorders a half edge list (int32 pairs i, j, i != j) once into a CSR by
- * destination: row r lists every neighbour of r (both directions) in the
- * order of the edges in the list.
+ * destination: row r lists every neighbour of r (both directions), in edge
+ * list order (PAT_IR_ORDER_EDGE) or by neighbour index (PAT_IR_ORDER_NODE).
  * rowptr: num_nodes+1, cols: 2*num_edges.  Replaces the host Partitioner /
  * Reorder (kmeans/irregular/partitioner.cpp:20-499, reorder.cpp:13-281). */
-int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int32_t *rowptr, int32_t *cols,
-                         void *work, int64_t work_bytes, void *stream);
+int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int order, int32_t *rowptr,
+                         int32_t *cols, void *work, int64_t work_bytes, void *stream);
 /* LJ forces from float4 positions (x, y, z, unused) into float4 forces over
  * the CSR; pbc selects the minimum image in a cubic box of side 'box'.
  * lanes_per_row 1, 4, 8, 16, 32: rows summed in CSR order (bit-identical for
@@ -203,6 +213,11 @@ int pat_ir_set_map_idx(pat_ir *h, int idx);
 int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size);
 /* Kernel variant and lanes per row (0 = tuned for PAT_IR_CSR). */
 int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row);
+/* Row order of the CSR (rebuilt from the uploaded edges when it changes);
+ * handles start in PAT_IR_ORDER_NODE.  The reference's runtime also sums in
+ * an order of its own, the edge order of its partitions
+ * (kmeans/irregular/reorder.cpp:13-281). */
+int pat_ir_set_row_order(pat_ir *h, int order);
 /* reset_node_data (irregular_runtime.cu:427-439): node coordinates,
  * node_elm_size 12 (float3) or 24 (double3, moldyn's point_data). */
 int pat_ir_reset_node_data(pat_ir *h, const void *node_data, int node_elm_size);
@@ -231,6 +246,12 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
 int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
                          double skin, int every, double dt, pat_md **out);
 int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row);
+/* Row order (see pat_ir_set_row_order); recomputes the stored force.  Binned
+ * handles only have PAT_IR_ORDER_NODE. */
+int pat_md_set_row_order(pat_md *h, int order);
+/* The kernel in use: variant, lanes per row (after tuning; PAT_IR_SLICED =
+ * the sliced layout) and row order. */
+int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order);
 /* Current half-pair count, steps run, neighbour rebuilds and their total
  * host-measured time. */
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);

@@ -30,6 +30,7 @@
 // row entry is exactly the negation the serial scatter applies.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "pat_common.hpp"
 #include "pat_device.hpp"
@@ -462,6 +463,93 @@ __global__ __launch_bounds__(256) void lj_csr_ord(const float4 *__restrict__ pos
     if (l == 0) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
 }
 
+// ------------------------------------------------------------ sliced layout
+//
+// SELL-64 (sliced ELLPACK with 64-row slices, one row per wavefront lane):
+// slice s covers rows 64s .. 64s+63 and is width_s = max degree in the slice
+// entries wide; entry k of row 64s+l sits at sptr[s] + 64k + l, so the k-th
+// column read of a wavefront is one contiguous 256-byte load.  Padding
+// entries repeat the row's own index (rows past N repeat node N-1) and are
+// masked out of the sum.
+
+__global__ void sell_width(const int *__restrict__ rowptr, int N, int nsl, int *__restrict__ width)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsl) return;
+    const int r0 = s * kWave, r1 = min(r0 + kWave, N);
+    int w = 0;
+    for (int r = r0; r < r1; r++) w = max(w, rowptr[r + 1] - rowptr[r]);
+    width[s] = w;
+}
+
+__global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr, const int *__restrict__ cols, int N,
+                                                 int nsl, const int *__restrict__ sptr, int *__restrict__ scol)
+{
+    const int s = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nsl) return;
+    const int row = s * kWave + lane;
+    const int b = row < N ? rowptr[row] : 0, e = row < N ? rowptr[row + 1] : 0;
+    const int self = min(row, N - 1);
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? cols[b + k] : self;
+}
+
+// In-order row sums over the sliced layout, one wavefront per slice and one
+// lane per row: the lane adds its row's pair forces in CSR order (padding
+// masked), U entries' column reads and position gathers issued ahead of
+// their arithmetic.  Bit-identical to lj_csr_seq.
+template <int U>
+__global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                               const int *__restrict__ sptr, const int *__restrict__ scol, int N,
+                                               LJParams p, float4 *__restrict__ force, float4 *__restrict__ vel,
+                                               float4 *__restrict__ pos_out, int mode)
+{
+    const int nsl = (N + kWave - 1) / kWave;
+    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
+    if (s >= nsl) return;  // whole wavefronts
+    const int lane = threadIdx.x % kWave;
+    const int row = s * kWave + lane;
+    const float4 pi = pos[min(row, N - 1)];
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int deg = row < N ? rowptr[row + 1] - rowptr[row] : 0;
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    const int *__restrict__ c = scol + base + lane;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        int j[U];
+        float4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) j[u] = c[(k + u) * kWave];
+#pragma unroll
+        for (int u = 0; u < U; u++) q[u] = pos[j[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            float ax, ay, az;
+            pair_force(pi, q[u], p, ax, ay, az);
+            if (k + u < deg) {
+                fx = fx + ax;
+                fy = fy + ay;
+                fz = fz + az;
+            }
+        }
+    }
+    for (; k < w; k++) {
+        float ax, ay, az;
+        pair_force(pi, pos[c[k * kWave]], p, ax, ay, az);
+        if (k < deg) {
+            fx = fx + ax;
+            fy = fy + ay;
+            fz = fz + az;
+        }
+    }
+    if (row < N) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
 // Per-node half potential 0.5*sum_j phi(i, j) over the full CSR row.
 __global__ __launch_bounds__(256) void lj_csr_energy(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
                                                      const int *__restrict__ cols, int N, LJParams p,
@@ -612,9 +700,11 @@ static int scan_exclusive(const int *in, int *out, int64_t n, int *ws, hipStream
     return PAT_OK;
 }
 
-int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *cols, void *work,
+// order 0: rows in edge-list order; 1: rows sorted by neighbour index.
+int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st)
 {
+    PAT_REQUIRE(order == 0 || order == 1, PAT_ERR_INVALID, "csr_build: row order %d not in {0, 1}", order);
     PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1 && E >= 0 && 2 * E < (1LL << 31), PAT_ERR_INVALID,
                 "csr_build: sizes out of range (N=%lld, E=%lld)", (long long)N, (long long)E);
     PAT_REQUIRE(work_bytes >= csr_workspace_bytes(N, E), PAT_ERR_INVALID, "csr_build: workspace too small");
@@ -636,7 +726,10 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32
         hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp, tmpe);
         PAT_LAUNCH_CHECK();
         const unsigned g2 = (unsigned)((N * kWave + 255) / 256);
-        hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, tmpe, cols);
+        if (order == 0)
+            hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, tmpe, cols);
+        else
+            hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols);
         PAT_LAUNCH_CHECK();
     }
     return PAT_OK;
@@ -682,6 +775,68 @@ int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, 
     const int64_t blocks = (N + rpb - 1) / rpb;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, st, pos, (const int *)rowptr, (const int *)cols, (int)N,
                        make_params(box, pbc, rc2, dt), force, vel, pos_out, mode);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// Sliced layout of a CSR: widths on the device, slice offsets scanned on the
+// host in 64-bit (the padded total must stay below 2^31), then the fill.
+// width_h / sptr_h: host scratch of nsl and nsl+1 ints; *scol grows to the
+// padded total (capacity *cap).  Refuses (PAT_ERR_UNSUPPORTED, nothing
+// changed) when padding would exceed max_pad times the entry count plus one
+// slice of 64 entries per slice.
+int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
+               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, hipStream_t st)
+{
+    const int64_t nsl = (N + kWave - 1) / kWave;
+    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr, (int)N,
+                       (int)nsl, (int *)width_d);
+    PAT_LAUNCH_CHECK();
+    std::vector<int32_t> w((size_t)nsl);
+    std::vector<int32_t> sp((size_t)nsl + 1);
+    int32_t nnz = 0;
+    PAT_HIP(hipMemcpyAsync(w.data(), width_d, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipMemcpyAsync(&nnz, rowptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    int64_t acc = 0;
+    for (int64_t s = 0; s < nsl; s++) {
+        sp[s] = (int32_t)acc;
+        acc += (int64_t)kWave * w[s];
+        PAT_REQUIRE(acc < (1LL << 31), PAT_ERR_UNSUPPORTED, "sell_build: padded layout exceeds 2^31 entries");
+    }
+    sp[nsl] = (int32_t)acc;
+    PAT_REQUIRE((double)acc <= max_pad * (double)nnz + (double)kWave * nsl, PAT_ERR_UNSUPPORTED,
+                "sell_build: padding %lld for %d entries exceeds the limit", (long long)acc, nnz);
+    if (acc > *cap) {
+        const int64_t want = std::min<int64_t>((int64_t)(acc * 1.25) + 4096, (1LL << 31) - 1);
+        int32_t *nc = nullptr;
+        if (hipMalloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+            (void)hipGetLastError();
+            pat::set_error("sell_build: hipMalloc of %lld entries failed", (long long)want);
+            return PAT_ERR_NOMEM;
+        }
+        if (*scol) (void)hipFree(*scol);
+        *scol = nc;
+        *cap = want;
+    }
+    PAT_HIP(hipMemcpyAsync(sptr, sp.data(), sizeof(int32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(sell_fill, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, (const int *)rowptr,
+                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)*scol);
+    PAT_LAUNCH_CHECK();
+    PAT_HIP(hipStreamSynchronize(st));  // sp leaves scope
+    *total = acc;
+    return PAT_OK;
+}
+
+int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
+                  float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
+                  hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - kWave, PAT_ERR_INVALID, "lj_force_sell: bad node count");
+    const int64_t nsl = (N + kWave - 1) / kWave;
+    hipLaunchKernelGGL(lj_sell<4>, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, pos, (const int *)rowptr,
+                       (const int *)sptr, (const int *)scol, (int)N, make_params(box, pbc, rc2, dt), force, vel,
+                       pos_out, mode);
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }

@@ -20,8 +20,13 @@
 
 namespace pat {
 int64_t csr_workspace_bytes(int64_t N, int64_t E);
-int csr_build(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *cols, void *work,
+int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st);
+int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
+               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, hipStream_t st);
+int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
+                  float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
+                  hipStream_t st);
 int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
                  float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out,
                  hipStream_t st);
@@ -55,10 +60,13 @@ int default_lpr(int64_t N, int64_t E)
     return l;
 }
 
-// Tuned lanes per row, cached per (node count, mean degree) class for the
-// process: the in-order variants give bit-identical sums, so the choice only
-// changes speed.
-std::map<std::pair<int64_t, int>, int> g_lpr_cache;
+// Padded entries the sliced layout may hold, as a multiple of the CSR's.
+constexpr double kSellMaxPad = 1.5;
+
+// Tuned lanes per row, cached per (node count, mean degree, row order) class
+// for the process: the in-order variants give bit-identical sums, so the
+// choice only changes speed.
+std::map<std::tuple<int64_t, int, int>, int> g_lpr_cache;
 std::mutex g_lpr_mutex;
 
 // Device state shared by the IrregularRuntime handle and the MD handle.
@@ -74,8 +82,18 @@ struct IrState {
     hipStream_t stream = nullptr;
     int variant = PAT_IR_CSR;
     int lpr = 0;
+    int order = PAT_IR_ORDER_NODE;  // row order of a CSR built from the edge list
+    // sliced copy of the CSR (see sell_build); sell_ok when it exists
+    int32_t *sptr = nullptr, *scol = nullptr, *swidth = nullptr;
+    int64_t scap = 0, stotal = 0;
+    bool sell_ok = false;
+    // bumped whenever a buffer or kernel choice the force launches use
+    // changes, so captured graphs can tell they are stale
+    int64_t layout_gen = 0;
 
-    int alloc(int64_t n, int64_t e, const int32_t *host_edges, bool md)
+    // csr: build the CSR from host_edges now (false for handles whose CSR
+    // comes from a neighbour build)
+    int alloc(int64_t n, int64_t e, const int32_t *host_edges, bool md, bool csr = true)
     {
         N = n;
         E = e;
@@ -99,16 +117,48 @@ struct IrState {
             PAT_TRY(am((void **)&pos[1], sizeof(float) * 4 * (size_t)N));
             PAT_TRY(am((void **)&vel, sizeof(float) * 4 * (size_t)N));
         }
+        const int64_t nsl = (N + 63) / 64;
+        PAT_TRY(am((void **)&sptr, sizeof(int32_t) * (size_t)(nsl + 1)));
+        PAT_TRY(am((void **)&swidth, sizeof(int32_t) * (size_t)nsl));
         if (E > 0) PAT_HIP(hipMemcpyAsync(edges, host_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, stream));
+        lpr = 1;
+        return csr ? build_csr(order) : PAT_OK;
+    }
+
+    // (Re)builds the CSR from the uploaded edge list with rows in order o,
+    // then its sliced copy.
+    int build_csr(int o)
+    {
         const int64_t wb = pat::csr_workspace_bytes(N, E);
         void *work = nullptr;
-        PAT_TRY(am(&work, (size_t)wb));
-        int rc = pat::csr_build(edges, E, N, rowptr, cols, work, wb, stream);
+        if (hipMalloc(&work, (size_t)wb) != hipSuccess) {
+            (void)hipGetLastError();
+            pat::set_error("irregular: hipMalloc of %lld workspace bytes failed", (long long)wb);
+            return PAT_ERR_NOMEM;
+        }
+        int rc = pat::csr_build(edges, E, N, o, rowptr, cols, work, wb, stream);
         hipError_t se = hipStreamSynchronize(stream);
         (void)hipFree(work);
         if (rc != PAT_OK) return rc;
         PAT_HIP(se);
-        lpr = 1;
+        order = o;
+        return build_sell();
+    }
+
+    // Sliced copy of the current CSR; left unavailable (in-order kernels on
+    // the plain CSR only) when its padding would exceed kSellMaxPad.
+    int build_sell()
+    {
+        layout_gen++;
+        int rc = pat::sell_build(rowptr, cols, N, kSellMaxPad, swidth, sptr, &scol, &scap, &stotal, stream);
+        if (rc == PAT_ERR_UNSUPPORTED) {
+            pat::clear_error();
+            sell_ok = false;
+            if (lpr == PAT_IR_SLICED && variant == PAT_IR_CSR) lpr = 1;  // same sums, bit for bit
+            return PAT_OK;
+        }
+        if (rc != PAT_OK) return rc;
+        sell_ok = true;
         return PAT_OK;
     }
 
@@ -116,7 +166,7 @@ struct IrState {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)edges, (void *)rowptr, (void *)cols, (void *)pos[0], (void *)pos[1], (void *)vel,
-                        (void *)force, (void *)node_e})
+                        (void *)force, (void *)node_e, (void *)sptr, (void *)scol, (void *)swidth})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -162,15 +212,34 @@ struct IrState {
         return PAT_OK;
     }
 
-    // lanes-per-row argument of the force kernels (negative = tree order)
+    // lanes-per-row argument of the force kernels (negative = tree order,
+    // PAT_IR_SLICED = the sliced layout)
     int klpr() const { return variant == PAT_IR_CSR_TREE ? -lpr : lpr; }
+
+    // One CSR force launch with kernel choice l (klpr() form) reading
+    // positions p4; mode kKick/kKickDrift also update vel (and write pos_out).
+    int csr_force_l(int l, const float *p4, float box, int pbc, float rc2, float dt, int mode, float *pos_out,
+                    hipStream_t s)
+    {
+        if (l == PAT_IR_SLICED) {
+            PAT_REQUIRE(sell_ok, PAT_ERR_STATE, "irregular: the sliced layout is not built");
+            return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, N, box, pbc, rc2, dt, mode,
+                                      (float4 *)force, (float4 *)vel, (float4 *)pos_out, s);
+        }
+        return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, dt, l, mode, (float4 *)force,
+                                 (float4 *)vel, (float4 *)pos_out, s);
+    }
+
+    int csr_force(const float *p4, float box, int pbc, float rc2, float dt, int mode, float *pos_out, hipStream_t s)
+    {
+        return csr_force_l(klpr(), p4, box, pbc, rc2, dt, mode, pos_out, s);
+    }
 
     int force_eval(const float *p4, float box, int pbc, float rc2)
     {
         if (variant == PAT_IR_ATOMIC)
             return pat::lj_force_atomic((const float4 *)p4, edges, E, N, box, pbc, rc2, (float4 *)force, stream);
-        return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, klpr(), kForceOnly,
-                                 (float4 *)force, nullptr, nullptr, stream);
+        return csr_force(p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream);
     }
 
     // Times the in-order force kernel for each lanes-per-row candidate on
@@ -179,15 +248,16 @@ struct IrState {
     int tune(const float *p4, float box, int pbc, float rc2)
     {
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
-        const auto key = std::make_pair((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg);
+        const auto key = std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg, order);
+        layout_gen++;
         if (getenv("PAT_NO_TUNE")) {
-            lpr = 1;
+            lpr = sell_ok ? PAT_IR_SLICED : 1;
             return PAT_OK;
         }
         {
             std::lock_guard<std::mutex> g(g_lpr_mutex);
             auto it = g_lpr_cache.find(key);
-            if (it != g_lpr_cache.end()) {
+            if (it != g_lpr_cache.end() && (it->second != PAT_IR_SLICED || sell_ok)) {
                 lpr = it->second;
                 return PAT_OK;
             }
@@ -197,11 +267,11 @@ struct IrState {
         PAT_HIP(hipEventCreate(&b));
         float best = 1e30f;
         int best_l = 1;
-        for (int l : {1, 4, 8, 16, 32}) {
+        for (int l : {1, 4, 8, 16, 32, PAT_IR_SLICED}) {
+            if (l == PAT_IR_SLICED && !sell_ok) continue;
             for (int rep = 0; rep < 4; rep++) {
                 PAT_HIP(hipEventRecord(a, stream));
-                PAT_TRY(pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, 0.f, l, kForceOnly,
-                                          (float4 *)force, nullptr, nullptr, stream));
+                PAT_TRY(csr_force_l(l, p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream));
                 PAT_HIP(hipEventRecord(b, stream));
                 PAT_HIP(hipEventSynchronize(b));
                 float ms = 0;
@@ -227,14 +297,22 @@ int check_variant(int variant, int lanes_per_row)
                 "irregular: unknown variant %d", variant);
     if (variant == PAT_IR_CSR)
         PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 1 || lanes_per_row == 4 || lanes_per_row == 8 ||
-                        lanes_per_row == 16 || lanes_per_row == 32,
-                    PAT_ERR_UNSUPPORTED, "irregular: in-order lanes per row %d not in {0 (tuned), 1, 4, 8, 16, 32}",
-                    lanes_per_row);
+                        lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == PAT_IR_SLICED,
+                    PAT_ERR_UNSUPPORTED,
+                    "irregular: in-order lanes per row %d not in {0 (tuned), 1, 4, 8, 16, 32, %d (sliced)}",
+                    lanes_per_row, PAT_IR_SLICED);
     if (variant == PAT_IR_CSR_TREE)
         PAT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
                         lanes_per_row == 32 || lanes_per_row == 64,
                     PAT_ERR_UNSUPPORTED, "irregular: tree lanes per row %d not in {0, 4, 8, 16, 32, 64}",
                     lanes_per_row);
+    return PAT_OK;
+}
+
+int check_sliced(const IrState &st, int variant, int lanes_per_row)
+{
+    PAT_REQUIRE(!(variant == PAT_IR_CSR && lanes_per_row == PAT_IR_SLICED && !st.sell_ok), PAT_ERR_UNSUPPORTED,
+                "irregular: the sliced layout is not available for this CSR (padding above %.1fx)", kSellMaxPad);
     return PAT_OK;
 }
 
@@ -292,8 +370,9 @@ struct pat_md {
         PAT_TRY(pat::neighbor_build((const float4 *)st.pos[c], st.N, box, cutneigh, nbw, st.rowptr, &st.cols, &cap,
                                     &total, st.stream));
         PAT_HIP(hipStreamSynchronize(st.stream));
-        rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         st.E = total / 2;
+        PAT_TRY(st.build_sell());
+        rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         rebuilds++;
         return PAT_OK;
     }
@@ -316,9 +395,7 @@ struct pat_md {
             if ((steps + k + 1) % every == 0) rc = rebuild(c);
             if (rc != PAT_OK) break;
             if (kernel_ms) (void)hipEventRecord(ev[2 * k], s);
-            rc = pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.klpr(),
-                                   last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
-                                   (float4 *)st.pos[1 - c], s);
+            rc = st.csr_force(st.pos[c], box, pbc, rc2, dt, last ? kKick : kKickDrift, st.pos[1 - c], s);
             if (kernel_ms) (void)hipEventRecord(ev[2 * k + 1], s);
             if (!last) c = 1 - c;
         }
@@ -362,9 +439,7 @@ struct pat_md {
         int c = start;
         for (int k = 0; k < nsteps; k++) {
             const bool last = k == nsteps - 1;
-            PAT_TRY(pat::lj_force_csr((const float4 *)st.pos[c], st.rowptr, st.cols, st.N, box, pbc, rc2, dt, st.klpr(),
-                                      last ? kKick : kKickDrift, (float4 *)st.force, (float4 *)st.vel,
-                                      (float4 *)st.pos[1 - c], s));
+            PAT_TRY(st.csr_force(st.pos[c], box, pbc, rc2, dt, last ? kKick : kKickDrift, st.pos[1 - c], s));
             if (!last) c = 1 - c;
         }
         return PAT_OK;
@@ -382,12 +457,14 @@ int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes)
     return PAT_OK;
 }
 
-int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int32_t *rowptr, int32_t *cols,
-                         void *work, int64_t work_bytes, void *stream)
+int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int order, int32_t *rowptr,
+                         int32_t *cols, void *work, int64_t work_bytes, void *stream)
 {
     PAT_REQUIRE(rowptr && (num_edges == 0 || (edges && cols)) && work, PAT_ERR_INVALID,
                 "pat_ir_csr_build_dev: null argument");
-    return pat::csr_build(edges, num_edges, num_nodes, rowptr, cols, work, work_bytes, (hipStream_t)stream);
+    PAT_REQUIRE(order == PAT_IR_ORDER_EDGE || order == PAT_IR_ORDER_NODE, PAT_ERR_INVALID,
+                "pat_ir_csr_build_dev: row order %d not in {PAT_IR_ORDER_EDGE, PAT_IR_ORDER_NODE}", order);
+    return pat::csr_build(edges, num_edges, num_nodes, order, rowptr, cols, work, work_bytes, (hipStream_t)stream);
 }
 
 int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
@@ -469,8 +546,20 @@ int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_variant: null handle");
     PAT_TRY(check_variant(variant, lanes_per_row));
+    PAT_TRY(check_sliced(h->st, variant, lanes_per_row));
     h->st.variant = variant;
     h->st.lpr = lanes_per_row ? lanes_per_row : (variant == PAT_IR_CSR_TREE ? default_lpr(h->st.N, h->st.E) : 1);
+    h->st.layout_gen++;
+    h->have_result = false;
+    return PAT_OK;
+}
+
+int pat_ir_set_row_order(pat_ir *h, int order)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_row_order: null handle");
+    PAT_REQUIRE(order == PAT_IR_ORDER_EDGE || order == PAT_IR_ORDER_NODE, PAT_ERR_INVALID,
+                "pat_ir_set_row_order: row order %d not in {PAT_IR_ORDER_EDGE, PAT_IR_ORDER_NODE}", order);
+    if (order != h->st.order) PAT_TRY(h->st.build_csr(order));
     h->have_result = false;
     return PAT_OK;
 }
@@ -561,7 +650,7 @@ int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, 
     h->every = every;
     h->cutneigh = cutoff + skin;
     h->nbw = pat::nb_create();
-    int rc = h->st.alloc(num_nodes, 0, nullptr, true);
+    int rc = h->st.alloc(num_nodes, 0, nullptr, true, false);
     if (rc == PAT_OK) {
         (void)hipFree(h->st.cols);
         h->st.cols = nullptr;
@@ -616,12 +705,43 @@ int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
     PAT_TRY(check_variant(variant, lanes_per_row));
     PAT_REQUIRE(!(variant == PAT_IR_ATOMIC && h->every > 0), PAT_ERR_UNSUPPORTED,
                 "pat_md_set_variant: the atomic kernel needs a half edge list (binned handles have a CSR only)");
+    PAT_TRY(check_sliced(h->st, variant, lanes_per_row));
     h->st.variant = variant;
+    h->st.layout_gen++;
     if (lanes_per_row) h->st.lpr = lanes_per_row;
     else if (variant == PAT_IR_CSR_TREE) h->st.lpr = default_lpr(h->st.N, h->st.E);
     else PAT_TRY(h->st.tune(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
     h->drop_graphs();
     // the stored force seeds the next half kick: recompute it with this variant
+    PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    return PAT_OK;
+}
+
+int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_get_variant: null handle");
+    if (variant) *variant = h->st.variant;
+    if (lanes_per_row) *lanes_per_row = h->st.lpr;
+    if (order) *order = h->st.order;
+    return PAT_OK;
+}
+
+int pat_md_set_row_order(pat_md *h, int order)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_row_order: null handle");
+    PAT_REQUIRE(order == PAT_IR_ORDER_EDGE || order == PAT_IR_ORDER_NODE, PAT_ERR_INVALID,
+                "pat_md_set_row_order: row order %d not in {PAT_IR_ORDER_EDGE, PAT_IR_ORDER_NODE}", order);
+    if (h->every > 0) {
+        PAT_REQUIRE(order == PAT_IR_ORDER_NODE, PAT_ERR_UNSUPPORTED,
+                    "pat_md_set_row_order: binned handles have no edge list; their rows are in node order");
+        return PAT_OK;
+    }
+    if (order == h->st.order) return PAT_OK;
+    PAT_TRY(h->st.build_csr(order));
+    if (h->st.variant == PAT_IR_CSR) PAT_TRY(h->st.tune(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    h->drop_graphs();
+    // the stored force seeds the next half kick: recompute it in the new order
     PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
     return PAT_OK;
@@ -710,9 +830,8 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
             const bool last = k == nsteps - 1;
             rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
             if (rc == PAT_OK)
-                rc = pat::lj_force_csr((const float4 *)h->st.pos[c], h->st.rowptr, h->st.cols, h->st.N, h->box, h->pbc,
-                                       h->rc2, h->dt, h->st.klpr(), last ? kKick : kKickDrift, (float4 *)h->st.force,
-                                       (float4 *)h->st.vel, (float4 *)h->st.pos[1 - c], s);
+                rc = h->st.csr_force(h->st.pos[c], h->box, h->pbc, h->rc2, h->dt, last ? kKick : kKickDrift,
+                                     h->st.pos[1 - c], s);
             if (rc == PAT_OK) rc = hipEventRecord(ev[2 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
             if (!last) c = 1 - c;
         }
@@ -834,13 +953,14 @@ struct pat_mdb {
         }
         PAT_HIP(hipStreamSynchronize(md.st.stream));
         md.st.E = entries / 2;
+        PAT_TRY(md.st.build_sell());
         md.rebuilds++;
         md.rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return PAT_OK;
     }
 
     std::map<std::tuple<int, int, int>, hipGraphExec_t> seg_graphs;
-    int32_t *graph_cols = nullptr;  // column array the cached graphs were captured with
+    int64_t graph_gen = -1;  // layout generation the cached graphs were captured with
 
     void drop_seg_graphs()
     {
@@ -860,9 +980,8 @@ struct pat_mdb {
                 PAT_HIP(hipEventCreate(&e1));
                 PAT_HIP(hipEventRecord(e0, s));
             }
-            PAT_TRY(pat::lj_force_csr((const float4 *)md.st.pos[c], md.st.rowptr, md.st.cols, md.st.N, 1.0f, 2, md.rc2,
-                                      md.dt, md.st.klpr(), final_step ? kKick : kKickDrift, (float4 *)md.st.force,
-                                      (float4 *)md.st.vel, (float4 *)md.st.pos[1 - c], s));
+            PAT_TRY(md.st.csr_force(md.st.pos[c], 1.0f, 2, md.rc2, md.dt, final_step ? kKick : kKickDrift,
+                                    md.st.pos[1 - c], s));
             if (kernel_ms) {
                 PAT_HIP(hipEventRecord(e1, s));
                 PAT_HIP(hipEventSynchronize(e1));
@@ -894,9 +1013,9 @@ struct pat_mdb {
             if (kernel_ms) {
                 PAT_TRY(enqueue_segment(seg, c, ends_run, s, kernel_ms + k));
             } else {
-                if (graph_cols != md.st.cols) {
+                if (graph_gen != md.st.layout_gen) {
                     drop_seg_graphs();
-                    graph_cols = md.st.cols;
+                    graph_gen = md.st.layout_gen;
                 }
                 auto key = std::make_tuple(seg, c, (int)ends_run);
                 auto it = seg_graphs.find(key);
@@ -951,7 +1070,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     m.every = every;
     m.cutneigh = cutoff + skin;
     m.nbw = pat::nb_create();
-    int rc = m.st.alloc(N, 0, nullptr, true);
+    int rc = m.st.alloc(N, 0, nullptr, true, false);
     if (rc == PAT_OK) {
         (void)hipFree(m.st.cols);
         m.st.cols = nullptr;
@@ -979,8 +1098,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     if (rc == PAT_OK) rc = h->rebuild_all(0);
     if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK)
-        rc = pat::lj_force_csr((const float4 *)m.st.pos[0], m.st.rowptr, m.st.cols, N, 1.0f, 2, m.rc2, 0.f, m.st.lpr,
-                               kForceOnly, (float4 *)m.st.force, nullptr, nullptr, m.st.stream);
+        rc = m.st.csr_force(m.st.pos[0], 1.0f, 2, m.rc2, 0.f, kForceOnly, nullptr, m.st.stream);
     if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc != PAT_OK) {
         if (h->rp_s) (void)hipFree(h->rp_s);
@@ -1011,13 +1129,14 @@ int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_set_lanes: null handle");
     PAT_TRY(check_variant(PAT_IR_CSR, lanes_per_row));
+    PAT_TRY(check_sliced(h->md.st, PAT_IR_CSR, lanes_per_row));
     if (lanes_per_row) h->md.st.lpr = lanes_per_row;
     else PAT_TRY(h->md.st.tune(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2));
+    h->md.st.layout_gen++;
     h->drop_seg_graphs();
     // the stored force must come from the same summation order
-    PAT_TRY(pat::lj_force_csr((const float4 *)h->md.st.pos[h->md.cur], h->md.st.rowptr, h->md.st.cols, h->md.st.N,
-                              1.0f, 2, h->md.rc2, 0.f, h->md.st.klpr(), kForceOnly, (float4 *)h->md.st.force, nullptr,
-                              nullptr, h->md.st.stream));
+    PAT_TRY(h->md.st.csr_force(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2, 0.f, kForceOnly, nullptr,
+                               h->md.st.stream));
     PAT_HIP(hipStreamSynchronize(h->md.st.stream));
     return PAT_OK;
 }

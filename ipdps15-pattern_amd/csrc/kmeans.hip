@@ -20,11 +20,14 @@
 // Compiled with -ffp-contract=off so no multiply-add is fused.
 //
 // Assignment, MFMA path: a wavefront takes 32 points and computes
-// ||c||^2 - 2 x.c for every centre with v_mfma_f32_32x32x16_bf16 (x and -2c
-// rounded to BF16, ||c||^2 split into three BF16 terms on an extra K step,
-// FP32 accumulation).  A per-point bound E on the error of that value
-// (BF16 rounding of both operands, the norm split, FP32 accumulation) plus
-// the FP32 rounding of the reference's own difference form gives a margin:
+// ||c||^2 - 2 x.c for every centre with v_mfma_f32_32x32x16_bf16 (FP32
+// accumulation, ||c||^2 split into three BF16 terms on an extra K step).
+// With NT = 3 terms, x and -2c are each split into BF16 high and low parts
+// and the product is xh.ch + xh.cl + xl.ch (the dropped terms are O(2^-16)
+// relative); with NT = 1 both are rounded to BF16 once (O(2^-8)).  A
+// per-point bound E on the error of that value (operand rounding, the norm
+// split, FP32 accumulation) plus the FP32 rounding of the reference's own
+// difference form gives a margin:
 // every centre whose approximate value is within the margin of the smallest
 // is rescored with the plain path's exact FP32 arithmetic, and the exact
 // results decide (same tie rule).  A centre outside the margin provably has a
@@ -244,15 +247,18 @@ __device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const 
     return sqrtf(dist);
 }
 
-// KS: K-steps of 16 dimensions holding the data (D <= 16*KS); one more
-// K-step holds the norm terms.  KB: 32-centre blocks per chunk.
-// cb: K_pad x (16*(KS+1)) BF16 centre rows [-2c, 0 pad | n1 n2 n3 0...],
-// padded rows have n1 = +inf.  cnorm: ||c_k|| (K entries).
-template <int KS, int KB>
+// KS: K-steps of 16 dimensions holding the data (D <= 16*KS); NT: terms
+// (1 or 3), each KS steps; one more K-step holds the norm terms.  KB:
+// 32-centre blocks per chunk.  cb: K_pad x (16*(NT*KS+1)) BF16 centre rows
+// [-2c | n1 n2 n3 0...] (NT = 1) or [hi(-2c) | lo(-2c) | hi(-2c) | n1 n2 n3
+// 0...] (NT = 3), data zero-padded to 16*KS; padded rows have n1 = +inf.
+// cnorm: ||c_k|| (K entries).
+template <int KS, int KB, int NT>
 __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
 {
-    constexpr int KW = 16 * (KS + 1);  // BF16 row width of cb
+    constexpr int NS = NT * KS + 1;  // K-steps per centre row
+    constexpr int KW = 16 * NS;      // BF16 row width of cb
     extern __shared__ unsigned long long smem[];
     unsigned long long *acc_o, *cnt_o, *dsum_o;
     lds_object(a, smem, acc_o, cnt_o, dsum_o);
@@ -278,6 +284,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     const float cc_max = cmax * cmax;
 
     const int D = a.D;
+    const bool vec_x = (D % 4) == 0 && ((uintptr_t)a.x % 16) == 0;  // rows start 16-byte aligned
     const int lane = threadIdx.x % kWave;
     const int col = lane % 32;   // point within the tile
     const int half = lane / 32;  // which 8 of each 16 K elements this lane holds
@@ -293,38 +300,53 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         const bool live = p < a.N;
         const float *xp = a.x + (live ? p : 0) * D;
 
-        // B operand: point p, K elements [16s + 8*half, +8) of [x, 1, 1, 1, 0...]
-        bf16x8 bx[KS + 1];
+        // B operand: point p, K elements [16s + 8*half, +8) of
+        // [x, 1, 1, 1, 0...] or [xh, xh, xl, 1, 1, 1, 0...]
+        bf16x8 bx[NS];
         float xx = 0.0f;
 #pragma unroll
         for (int s = 0; s < KS; s++) {
+            const int d0 = 16 * s + 8 * half;
+            float v[8];
+            if (vec_x && live && d0 + 8 <= D) {
+                const float4 lo = *(const float4 *)(xp + d0), hi = *(const float4 *)(xp + d0 + 4);
+                v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+                v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[e] = (live && d0 + e < D) ? xp[d0 + e] : 0.0f;
+            }
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const int d = 16 * s + 8 * half + e;
-                const float v = (live && d < D) ? xp[d] : 0.0f;
-                xx = xx + v * v;
-                bx[s][e] = (__bf16)v;
+                xx = xx + v[e] * v[e];
+                const __bf16 h = (__bf16)v[e];
+                bx[s][e] = h;
+                if (NT == 3) {
+                    bx[KS + s][e] = h;
+                    bx[2 * KS + s][e] = (__bf16)(v[e] - (float)h);
+                }
             }
         }
 #pragma unroll
-        for (int e = 0; e < 8; e++) bx[KS][e] = (__bf16)((half == 0 && e < 3) ? 1.0f : 0.0f);
+        for (int e = 0; e < 8; e++) bx[NT * KS][e] = (__bf16)((half == 0 && e < 3) ? 1.0f : 0.0f);
         xx = xx + __shfl_xor(xx, 32, kWave);  // ||x||^2 of point p (both halves)
         const float xn = sqrtf(xx) * 1.0001f;
         const float E = bd.c_dot * xn * cmax + bd.c_norm * (cc_max + xx);
 
         float best_d = kInitMin;
         int best_k = 0x7fffffff;
-        float m = INFINITY;  // running approximate minimum for the point
         int ncand = 0;
-        for (int ch = 0; ch < nchunks; ch++) {
-            f32x16 acc[KB];
+        // approximate values of chunk ch (KB blocks of 32 centres) for this
+        // lane's point: value i of block b is centre row 8*(i/4) + 4*half +
+        // i%4 of the block (the 32x32 MFMA output layout), column = point
+        auto chunk_values = [&](int ch, f32x16 *acc) {
 #pragma unroll
             for (int b = 0; b < KB; b++) {
                 acc[b] = f32x16{};
                 if ((ch * KB + b) * 32 < K_pad) {
                     const int row = (ch * KB + b) * 32 + col;  // A operand row: centre
 #pragma unroll
-                    for (int s = 0; s <= KS; s++) {
+                    for (int s = 0; s < NS; s++) {
                         const bf16x8 av = *(const bf16x8 *)(cbs + (int64_t)row * KW + 16 * s + 8 * half);
                         acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[s], acc[b], 0, 0, 0);
                     }
@@ -334,36 +356,57 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
                     for (int i = 0; i < 16; i++) acc[b][i] = INFINITY;
                 }
             }
-            // chunk minimum for this lane's point (rows of both half-waves)
+        };
+        auto chunk_min = [&](const f32x16 *acc) {
             float cmn = INFINITY;
 #pragma unroll
             for (int b = 0; b < KB; b++)
 #pragma unroll
                 for (int i = 0; i < 16; i++) cmn = fminf(cmn, acc[b][i]);
-            cmn = fminf(cmn, __shfl_xor(cmn, 32, kWave));
-            m = fminf(m, cmn);
+            return fminf(cmn, __shfl_xor(cmn, 32, kWave));  // rows of both half-waves
+        };
+        // The candidate threshold needs the minimum over every centre: with
+        // one chunk it comes with the values, with several a first pass
+        // finds it (the values are recomputed rather than held).
+        float m = INFINITY;
+        if (nchunks > 1) {
+            for (int ch = 0; ch < nchunks; ch++) {
+                f32x16 acc[KB];
+                chunk_values(ch, acc);
+                m = fminf(m, chunk_min(acc));
+            }
+        }
+        for (int ch = 0; ch < nchunks; ch++) {
+            f32x16 acc[KB];
+            chunk_values(ch, acc);
+            if (nchunks == 1) m = chunk_min(acc);
             // candidates: approximate value within the margin of the minimum
             const float dhi = (xx + m + 4.0f * E) * 1.001f + 1e-30f;
             const float thr = m + 2.0f * E + 2.0f * bd.c_ref * dhi;
-            uint64_t mask = 0;
+            constexpr int NM = (KB * 16 + 63) / 64;  // 64-bit candidate masks (4 blocks each)
+            uint64_t mask[NM];
+#pragma unroll
+            for (int w = 0; w < NM; w++) mask[w] = 0;
 #pragma unroll
             for (int b = 0; b < KB; b++)
 #pragma unroll
                 for (int i = 0; i < 16; i++)
-                    if (acc[b][i] <= thr) mask |= 1ULL << (b * 16 + i);
-            while (mask) {
-                const int bit = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int b = bit / 16, i = bit % 16;
-                // output layout of the 32x32 MFMA: value i of this lane is
-                // row 8*(i/4) + 4*half + i%4, column = lane % 32
-                const int k = (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4);
-                if (k < a.K && live) {
-                    const float dk = exact_dist(xp, a.c + (int64_t)k * D, D);
-                    ncand++;
-                    if (dk < best_d || (dk == best_d && k < best_k)) {
-                        best_d = dk;
-                        best_k = k;
+                    if (acc[b][i] <= thr) mask[b / 4] |= 1ULL << ((b % 4) * 16 + i);
+#pragma unroll
+            for (int w = 0; w < NM; w++) {
+                uint64_t mw = mask[w];
+                while (mw) {
+                    const int bit = __builtin_ctzll(mw);
+                    mw &= mw - 1;
+                    const int b = 4 * w + bit / 16, i = bit % 16;
+                    const int k = (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4);
+                    if (k < a.K && live) {
+                        const float dk = exact_dist(xp, a.c + (int64_t)k * D, D);
+                        ncand++;
+                        if (dk < best_d || (dk == best_d && k < best_k)) {
+                            best_d = dk;
+                            best_k = k;
+                        }
                     }
                 }
             }
@@ -403,7 +446,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 // object (c = (sum * 2^-s) / count in FP64, rounded to FP32; empty clusters
 // keep their centre), writes the BF16 MFMA row and the norm, and clears the
 // object for the next pass.  With update == 0 only the rows are prepared.
-template <int KS>
+template <int KS, int NT>
 __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K, int K_pad, int D,
                                                    unsigned long long *__restrict__ gsum,
                                                    unsigned long long *__restrict__ gcnt,
@@ -411,13 +454,14 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
                                                    int update, __bf16 *__restrict__ cb, float *__restrict__ cnorm,
                                                    unsigned long long *__restrict__ csnap)
 {
-    constexpr int KW = 16 * (KS + 1);
+    constexpr int KW = 16 * (NT * KS + 1);
+    constexpr int NB = 16 * NT * KS;  // first norm column
     const int k = blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (k >= K_pad) return;
     if (k >= K) {  // padding row: never the minimum
         for (int e = lane; e < KW; e += kWave)
-            cb[(int64_t)k * KW + e] = (__bf16)((e == 16 * KS) ? INFINITY : 0.0f);
+            cb[(int64_t)k * KW + e] = (__bf16)((e == NB) ? INFINITY : 0.0f);
         return;
     }
     const long long n = (long long)gcnt[k];
@@ -431,8 +475,13 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, kWave);
     for (int d = lane; d < 16 * KS; d += kWave) {
-        const float v = d < D ? c[(int64_t)k * D + d] : 0.0f;
-        cb[(int64_t)k * KW + d] = (__bf16)(-2.0f * v);
+        const float v = d < D ? -2.0f * c[(int64_t)k * D + d] : 0.0f;  // exact
+        const __bf16 h = (__bf16)v;
+        cb[(int64_t)k * KW + d] = h;
+        if (NT == 3) {
+            cb[(int64_t)k * KW + 16 * KS + d] = (__bf16)(v - (float)h);
+            cb[(int64_t)k * KW + 32 * KS + d] = h;
+        }
     }
     if (lane < 16) {
         const __bf16 n1 = (__bf16)(float)ss;
@@ -444,7 +493,7 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
         if (lane == 0) v = n1;
         if (lane == 1) v = n2;
         if (lane == 2) v = n3;
-        cb[(int64_t)k * KW + 16 * KS + lane] = v;
+        cb[(int64_t)k * KW + NB + lane] = v;
     }
     if (lane == 0) {
         cnorm[k] = (float)sqrt(ss) * 1.0001f;
@@ -501,7 +550,8 @@ int kmeans_ks_for(int D)
     return 0;
 }
 
-int kmeans_kb_for(int ks) { return ks <= 2 ? 4 : (ks == 4 ? 2 : 1); }
+// 32-centre blocks per chunk (accumulator registers: 16 per block).
+int kmeans_kb_for(int ks) { return ks <= 2 ? 8 : (ks == 4 ? 4 : 2); }
 
 // LDS bytes of the reduction object and the staged centres.
 static int64_t object_bytes(int K, int D) { return 8LL * object_words(K, D); }
@@ -551,9 +601,11 @@ int kmeans_prepare_kernels()
     PAT_HIP(hipGetDevice(&dev));
     PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     g_lds_limit = lim - 256;  // room for the kernels' static LDS
-    const void *fns[] = {(const void *)km_plain<8>, (const void *)km_plain<16>, (const void *)km_plain<32>,
-                         (const void *)km_plain<64>, (const void *)km_plain<128>, (const void *)km_mfma<1, 4>,
-                         (const void *)km_mfma<2, 4>, (const void *)km_mfma<4, 2>, (const void *)km_mfma<8, 1>};
+    const void *fns[] = {(const void *)km_plain<8>,       (const void *)km_plain<16>,      (const void *)km_plain<32>,
+                         (const void *)km_plain<64>,      (const void *)km_plain<128>,     (const void *)km_mfma<1, 8, 1>,
+                         (const void *)km_mfma<2, 8, 1>,  (const void *)km_mfma<4, 4, 1>,  (const void *)km_mfma<8, 2, 1>,
+                         (const void *)km_mfma<1, 8, 3>,  (const void *)km_mfma<2, 8, 3>,  (const void *)km_mfma<4, 4, 3>,
+                         (const void *)km_mfma<8, 2, 3>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
@@ -572,13 +624,22 @@ int kmeans_finalize(const KmDev &d, int update, hipStream_t st)
 {
     const double inv = ldexp(1.0, -d.shift);
     const dim3 g((unsigned)((d.K_pad + 3) / 4)), b(256);
-    switch (d.KS) {
-    case 1: hipLaunchKernelGGL(km_finalize<1>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
-    case 2: hipLaunchKernelGGL(km_finalize<2>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
-    case 4: hipLaunchKernelGGL(km_finalize<4>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
-    case 8: hipLaunchKernelGGL(km_finalize<8>, g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv, update, (__bf16 *)d.cb, d.cnorm, d.csnap); break;
-    default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
+#define PAT_KF(KS_, NT_)                                                                                               \
+    hipLaunchKernelGGL((km_finalize<KS_, NT_>), g, b, 0, st, d.c, d.K, d.K_pad, d.D, d.gsum, d.gcnt, d.gdsum, inv,    \
+                       update, (__bf16 *)d.cb, d.cnorm, d.csnap)
+    const int code = d.KS * 4 + d.NT;
+    switch (code) {
+    case 1 * 4 + 1: PAT_KF(1, 1); break;
+    case 2 * 4 + 1: PAT_KF(2, 1); break;
+    case 4 * 4 + 1: PAT_KF(4, 1); break;
+    case 8 * 4 + 1: PAT_KF(8, 1); break;
+    case 1 * 4 + 3: PAT_KF(1, 3); break;
+    case 2 * 4 + 3: PAT_KF(2, 3); break;
+    case 4 * 4 + 3: PAT_KF(4, 3); break;
+    case 8 * 4 + 3: PAT_KF(8, 3); break;
+    default: pat::set_error("kmeans: D = %d or terms %d not supported", d.D, d.NT); return PAT_ERR_UNSUPPORTED;
     }
+#undef PAT_KF
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }
@@ -611,43 +672,71 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
     return PAT_OK;
 }
 
-// Whether the MFMA path can run this shape (BF16 rows staged in LDS).
-bool kmeans_mfma_fits(int K, int D)
+// LDS bytes of the MFMA kernel with nt terms (staged BF16 rows plus the
+// reduction object when it is kept in LDS).
+static int64_t mfma_lds_bytes(int K, int D, int nt)
 {
     const int ks = kmeans_ks_for(D);
-    if (!ks) return false;
     const int K_pad = (K + 31) / 32 * 32;
-    const int64_t cbb = 2LL * K_pad * 16 * (ks + 1);
-    const int64_t obj = object_bytes(K, D) <= 96 * 1024 ? object_bytes(K, D) : 0;
-    return cbb + obj <= g_lds_limit;
+    const int64_t cbb = 2LL * K_pad * 16 * (nt * ks + 1);
+    const int64_t obj = object_bytes(K, D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024)
+                            ? object_bytes(K, D) : 0;
+    return cbb + obj;
 }
+
+// Terms of the MFMA path for this shape: 3 (split BF16) when its rows fit in
+// LDS, else 1, 0 when neither fits.
+int kmeans_nt_for(int K, int D)
+{
+    if (!kmeans_ks_for(D)) return 0;
+    if (mfma_lds_bytes(K, D, 3) <= g_lds_limit) return 3;
+    if (mfma_lds_bytes(K, D, 1) <= g_lds_limit) return 1;
+    return 0;
+}
+
+bool kmeans_mfma_fits(int K, int D) { return kmeans_nt_for(K, D) != 0; }
 
 int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
 {
     KmArgs a = make_args(d);
-    PAT_REQUIRE(kmeans_mfma_fits(d.K, d.D), PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K,
+    PAT_REQUIRE(d.NT == 1 || d.NT == 3, PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K,
                 d.D);
-    const int64_t cbb = 2LL * d.K_pad * 16 * (d.KS + 1);
-    const int64_t lds = (a.use_lds_acc ? object_bytes(d.K, d.D) : 0) + cbb;
-    // MFMA error-bound constants (see the file header)
-    const double u = ldexp(1.0, -8), g = 2.0 * (16.0 * (d.KS + 1)) * ldexp(1.0, -24);
+    const int64_t lds = mfma_lds_bytes(d.K, d.D, d.NT);
+    PAT_REQUIRE(lds <= g_lds_limit, PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K, d.D);
+    // Error-bound constants.  u = 2^-8 (BF16 unit roundoff); g = FP32
+    // accumulation over every term of the MFMA chain, with a factor 2 of
+    // slack for any internal rounding mode.
+    //  NT = 1: |x.c - bf(x).bf(c)| <= (2u + u^2) sum |x_d c_d|.
+    //  NT = 3: x = xh + xl + rx with |rx| <= u^2 |x|, |xl| <= u(1+u)|x| (same
+    //          for c); the dropped xl.cl + (terms with rx, rc) are bounded by
+    //          (3u^2 + 4u^3)(1+u)^2 sum |x_d c_d|.
+    //  Both scaled by 2 for the -2 factor; the norm split leaves u^3 ||c||^2.
+    const double u = ldexp(1.0, -8), g = 2.0 * (16.0 * (d.NT * d.KS + 1)) * ldexp(1.0, -24);
     MfmaBound bd;
-    bd.c_dot = (float)((4 * u + 2 * u * u + 4 * g * (1 + u) * (1 + u)) * 1.01);
+    if (d.NT == 3)
+        bd.c_dot = (float)((2 * (3 * u * u + 4 * u * u * u) * (1 + u) * (1 + u) +
+                            2 * g * (1 + u) * (1 + u) * (1 + 2 * u)) * 1.01);
+    else
+        bd.c_dot = (float)((4 * u + 2 * u * u + 4 * g * (1 + u) * (1 + u)) * 1.01);
     bd.c_norm = (float)((u * u * u + 2 * g * (1 + 2 * u)) * 1.01);
     bd.c_ref = (float)((3.0 + d.D + 16.0) * ldexp(1.0, -24) * 1.01);
     const int64_t want = (d.N + 31) / 32 / 8;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
-#define PAT_KM(KS_, KB_)                                                                                               \
+#define PAT_KM(KS_, KB_, NT_)                                                                                          \
     do {                                                                                                               \
-        PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_>, lds));                                                      \
-        hipLaunchKernelGGL((km_mfma<KS_, KB_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb,       \
+        PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_>, lds));                                                 \
+        hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb,  \
                            d.cnorm, d.K_pad, bd, d.rescored);                                                          \
     } while (0)
-    switch (d.KS) {
-    case 1: PAT_KM(1, 4); break;
-    case 2: PAT_KM(2, 4); break;
-    case 4: PAT_KM(4, 2); break;
-    case 8: PAT_KM(8, 1); break;
+    switch (d.KS * 4 + d.NT) {
+    case 1 * 4 + 1: PAT_KM(1, 8, 1); break;
+    case 2 * 4 + 1: PAT_KM(2, 8, 1); break;
+    case 4 * 4 + 1: PAT_KM(4, 4, 1); break;
+    case 8 * 4 + 1: PAT_KM(8, 2, 1); break;
+    case 1 * 4 + 3: PAT_KM(1, 8, 3); break;
+    case 2 * 4 + 3: PAT_KM(2, 8, 3); break;
+    case 4 * 4 + 3: PAT_KM(4, 4, 3); break;
+    case 8 * 4 + 3: PAT_KM(8, 2, 3); break;
     default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
     }
 #undef PAT_KM

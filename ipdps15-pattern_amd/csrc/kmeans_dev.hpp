@@ -12,11 +12,12 @@ struct KmDev {
     float *c;        // K x D centres (FP32)
     int64_t N;
     int D, K, K_pad, KS;  // K_pad: K rounded up to 32; KS: 16-wide K-steps of the data
+    int NT;               // MFMA data terms: 3 (split BF16), 1 (BF16), 0 (no MFMA path)
     int shift, dshift;    // fixed-point scales 2^shift (coordinates), 2^dshift (distances)
     int32_t *assign;      // N
     float *mindist;       // N
     unsigned long long *gsum, *gcnt, *gdsum, *csnap;  // reduction object (K x D, K, K), count snapshot
-    void *cb;             // BF16 MFMA rows, K_pad x 16*(KS+1)
+    void *cb;             // BF16 MFMA rows, K_pad x 16*(NT*KS+1)
     float *cnorm;         // ||c_k|| (rounded up)
     unsigned long long *rescored;  // points with more than one candidate
     int cus;              // compute units
@@ -25,6 +26,7 @@ struct KmDev {
 int kmeans_fixed_shift(int64_t n, float maxabs);
 int kmeans_ks_for(int D);
 bool kmeans_mfma_fits(int K, int D);
+int kmeans_nt_for(int K, int D);
 int kmeans_maxabs(const float *x, int64_t n, unsigned int *out_bits, hipStream_t st);
 int kmeans_finalize(const KmDev &d, int update, hipStream_t st);
 int kmeans_assign_plain(const KmDev &d, hipStream_t st);

@@ -10,6 +10,7 @@
 // the centres start as the first K points, and each iteration is one
 // assignment+accumulation kernel and one update kernel on the handle's
 // stream.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -88,6 +89,7 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     D.K = k;
     D.K_pad = (k + 31) / 32 * 32;
     D.KS = pat::kmeans_ks_for(d);
+    D.NT = std::max(1, pat::kmeans_nt_for(k, d));  // rows are written even when only the plain path fits
     int dev = 0;
     PAT_HIP(hipGetDevice(&dev));
     PAT_HIP(hipDeviceGetAttribute(&D.cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -106,7 +108,7 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     PAT_TRY(amalloc((void **)&D.gcnt, 8 * (size_t)k));
     PAT_TRY(amalloc((void **)&D.gdsum, 8 * (size_t)k));
     PAT_TRY(amalloc((void **)&D.csnap, 8 * (size_t)k));
-    PAT_TRY(amalloc(&h->rowbuf, 2 * (size_t)D.K_pad * 16 * (D.KS + 1)));
+    PAT_TRY(amalloc(&h->rowbuf, 2 * (size_t)D.K_pad * 16 * (D.NT * D.KS + 1)));
     D.cb = h->rowbuf;
     PAT_TRY(amalloc((void **)&D.cnorm, sizeof(float) * (size_t)k));
     PAT_TRY(amalloc((void **)&D.rescored, 8));

@@ -22,6 +22,9 @@ PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0
 PAT_IR_ATOMIC = 1
 PAT_IR_CSR_TREE = 2
+PAT_IR_SLICED = 64
+PAT_IR_ORDER_EDGE = 0
+PAT_IR_ORDER_NODE = 1
 PAT_KM_PLAIN = 0
 PAT_KM_MFMA = 1
 
@@ -78,13 +81,14 @@ SIGNATURES = {
     "pat_stencil_device_ptr": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "pat_stencil_destroy": (c_int, [c_vp]),
     "pat_ir_csr_workspace": (c_int, [c_i64, c_i64, c_i64p]),
-    "pat_ir_csr_build_dev": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "pat_ir_csr_build_dev": (c_int, [c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pat_lj_force_csr_dev": (c_int, [c_vp, c_vp, c_vp, c_i64, c_float, c_int, c_float, c_int, c_vp, c_vp]),
     "pat_lj_force_atomic_dev": (c_int, [c_vp, c_vp, c_i64, c_i64, c_float, c_int, c_float, c_vp, c_vp]),
     "pat_ir_create": (c_int, [c_i64, c_i64, c_vp, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "pat_ir_set_map_idx": (c_int, [c_vp, c_int]),
     "pat_ir_set_param": (c_int, [c_vp, c_vp, c_i64]),
     "pat_ir_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pat_ir_set_row_order": (c_int, [c_vp, c_int]),
     "pat_ir_reset_node_data": (c_int, [c_vp, c_vp, c_int]),
     "pat_ir_start": (c_int, [c_vp]),
     "pat_ir_get_result": (c_int, [c_vp, c_vp]),
@@ -96,6 +100,8 @@ SIGNATURES = {
     "pat_md_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_fp]),
     "pat_md_csr": (c_int, [c_vp, c_vp, c_vp]),
     "pat_md_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pat_md_set_row_order": (c_int, [c_vp, c_int]),
+    "pat_md_get_variant": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_mdb_create": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_double, c_int, c_double,
                                ctypes.POINTER(c_vp)]),
     "pat_mdb_set_lanes": (c_int, [c_vp, c_int]),
@@ -379,15 +385,15 @@ class Stencil:
 
 # ---------------------------------------------------------------- irregular
 
-def csr_build_dev(edges_dev, num_edges, num_nodes, stream=None):
+def csr_build_dev(edges_dev, num_edges, num_nodes, stream=None, order=PAT_IR_ORDER_EDGE):
     """Device CSR by destination; returns (rowptr, cols) DeviceArrays."""
     wb = c_i64(0)
     call("pat_ir_csr_workspace", num_nodes, num_edges, ctypes.byref(wb))
     work = DeviceArray((max(wb.value, 1),), np.uint8)
     rowptr = DeviceArray((num_nodes + 1,), np.int32)
     cols = DeviceArray((max(2 * num_edges, 1),), np.int32)
-    call("pat_ir_csr_build_dev", ptr(edges_dev), num_edges, num_nodes, rowptr.ptr, cols.ptr, work.ptr, wb.value,
-         stream_ptr(stream))
+    call("pat_ir_csr_build_dev", ptr(edges_dev), num_edges, num_nodes, order, rowptr.ptr, cols.ptr, work.ptr,
+         wb.value, stream_ptr(stream))
     sync(stream)
     work.free()
     return rowptr, cols
@@ -413,6 +419,9 @@ class IrregularRuntime:
 
     def set_variant(self, variant, lanes_per_row=0):
         call("pat_ir_set_variant", self.h, variant, lanes_per_row)
+
+    def set_row_order(self, order):
+        call("pat_ir_set_row_order", self.h, order)
 
     def reset_node_data(self, pos):
         p = np.ascontiguousarray(pos)
@@ -444,7 +453,7 @@ class MD:
     (cutoff + skin, every 'every' steps)."""
 
     def __init__(self, pos, vel, edges, box, cutoff, dt, pbc=True, variant=PAT_IR_CSR, lanes_per_row=0, skin=0.3,
-                 every=20):
+                 every=20, order=PAT_IR_ORDER_NODE):
         p = np.ascontiguousarray(pos, dtype=np.float32)
         v = np.ascontiguousarray(vel, dtype=np.float32) if vel is not None else None
         self.n = p.shape[0]
@@ -457,11 +466,19 @@ class MD:
             call("pat_md_create", self.n, ptr(p), ptr(v), e.shape[0], ptr(e), float(box), int(pbc), float(cutoff),
                  float(dt), ctypes.byref(h))
         self.h = h
+        if order != PAT_IR_ORDER_NODE:
+            call("pat_md_set_row_order", self.h, order)
         if variant != PAT_IR_CSR or lanes_per_row:
             self.set_variant(variant, lanes_per_row)
 
     def set_variant(self, variant, lanes_per_row=0):
         call("pat_md_set_variant", self.h, variant, lanes_per_row)
+
+    def variant(self):
+        """(variant, lanes per row in use, row order)."""
+        a, b, c = c_int(0), c_int(0), c_int(0)
+        call("pat_md_get_variant", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
 
     def run(self, nsteps):
         call("pat_md_run", self.h, nsteps)

@@ -1,12 +1,15 @@
 """GPU parity of the irregular engine (through libpattern's C ABI) vs the oracle.
 
-Bar: the in-order CSR kernels (the default) are bit-identical to the FP32
-oracle's serial scatter -- forces, and whole velocity-Verlet trajectories --
-because each row lists a node's edges in edge order and sums them in that
-order with the same per-pair arithmetic.  The atomic and tree-order kernels
-sum in another order and are held to 1e-5 relative L-infinity
-(max |a-b| / max |b|) per force evaluation.  The CSR must equal a NumPy
-construction exactly.  Errors against the FP64 oracle are printed.
+Bar: the in-order CSR kernels (the default, on the plain or the sliced
+layout) are bit-identical to the FP32 oracle's serial scatter -- forces, and
+whole velocity-Verlet trajectories -- because each row lists a node's edges
+in the row order and sums them in that order with the same per-pair
+arithmetic.  Rows in edge order match the scatter over the list as given;
+rows in node order (the handle default) match the scatter over the list
+sorted by (min(i, j), max(i, j)).  The atomic and tree-order kernels sum in
+another order and are held to 1e-5 relative L-infinity (max |a-b| / max |b|)
+per force evaluation.  The CSR must equal a NumPy construction exactly.
+Errors against the FP64 oracle are printed.
 """
 import numpy as np
 import pytest
@@ -18,6 +21,17 @@ TOL = 1e-5
 
 def rel(a, b):
     return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def node_sorted(edges):
+    """The edge list in the order whose serial scatter adds each row's
+    neighbours by increasing index: sorted by (min(i, j), max(i, j))."""
+    lo, hi = np.minimum(edges[:, 0], edges[:, 1]), np.maximum(edges[:, 0], edges[:, 1])
+    return edges[np.lexsort((hi, lo))]
+
+
+def oracle_edges(pa, edges, order):
+    return node_sorted(edges) if order == pa.PAT_IR_ORDER_NODE else edges
 
 
 @pytest.fixture(scope="module")
@@ -40,6 +54,21 @@ def test_csr_build_exact(pa, gpu, small):
     np.add.at(expect_rp, both[:, 0] + 1, 1)
     assert np.array_equal(rp, np.cumsum(expect_rp))
     assert np.array_equal(cl, both[:, 1])
+
+
+def test_csr_build_node_order(pa, gpu, small):
+    pos, box, edges = small
+    n = len(pos)
+    rowptr, cols = pa.csr_build_dev(pa.DeviceArray.from_numpy(edges), len(edges), n, order=pa.PAT_IR_ORDER_NODE)
+    both = np.concatenate([edges, edges[:, ::-1]])
+    both = both[np.lexsort((both[:, 1], both[:, 0]))]
+    assert np.array_equal(cols.numpy(), both[:, 1])
+    # the sorted list's scatter visits each row in exactly this order
+    s = node_sorted(edges)
+    eid = np.concatenate([np.arange(len(s)), np.arange(len(s))])
+    b2 = np.concatenate([s, s[:, ::-1]])
+    b2 = b2[np.lexsort((eid, b2[:, 0]))]
+    assert np.array_equal(cols.numpy(), b2[:, 1])
 
 
 def test_csr_build_edge_cases(pa, gpu):
@@ -98,16 +127,18 @@ def test_two_particle_and_cutoff(pa, gpu):
     ir.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
-def test_irregular_runtime_handle(pa, orc, gpu, small, variant):
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("variant,lanes", [(0, 0), (0, 1), (0, 64), (1, 0), (2, 0)])
+def test_irregular_runtime_handle(pa, orc, gpu, small, variant, lanes, order):
     pos, box, edges = small
     para = np.array([box, 2.5], dtype=np.float64)  # struct para {side, cutoffRadius}
     ir = pa.IrregularRuntime(edges, len(pos), pa.PAT_IR_LJ_MINIMAGE, para)
-    ir.set_variant(variant)
+    ir.set_row_order(order)
+    ir.set_variant(variant, lanes)
     ir.reset_node_data(pos.astype(np.float64))  # moldyn's double3 point_data
     ir.start()
     got = ir.get_reduction_result()
-    ref, _ = orc.lj_force(pos, edges, box, 2.5)
+    ref, _ = orc.lj_force(pos, oracle_edges(pa, edges, order), box, 2.5)
     if variant == 0:
         assert np.array_equal(got, ref)
     else:
@@ -115,12 +146,27 @@ def test_irregular_runtime_handle(pa, orc, gpu, small, variant):
     ir.close()
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32])
-def test_lanes_per_row_do_not_change_results(pa, gpu, small, lanes):
+def test_sliced_layout_refused_when_padding_is_large(pa, gpu):
+    # a hub of degree 199 pads its whole 64-row slice to 199 entries
+    n = 300
+    edges = np.array([(0, j) for j in range(1, 200)] + [(250, 251)], dtype=np.int32)
+    pos = np.random.default_rng(3).uniform(0, 20, (n, 3)).astype(np.float32)
+    ir = pa.IrregularRuntime(edges, n, pa.PAT_IR_LJ_OPEN, np.array([6.25]))
+    with pytest.raises(RuntimeError, match="sliced layout"):
+        ir.set_variant(pa.PAT_IR_CSR, pa.PAT_IR_SLICED)
+    ir.set_variant(pa.PAT_IR_CSR, 1)
+    ir.reset_node_data(pos)
+    ir.start()
+    ir.close()
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32, 64])
+def test_lanes_per_row_do_not_change_results(pa, gpu, small, lanes, order):
     pos, box, edges = small
     outs = []
     for l in (1, lanes):
-        md = pa.MD(pos, None, edges, box, 2.5, 0.005, lanes_per_row=l)
+        md = pa.MD(pos, None, edges, box, 2.5, 0.005, lanes_per_row=l, order=order)
         md.run(5)
         outs.append(md.get())
         md.close()
@@ -128,15 +174,15 @@ def test_lanes_per_row_do_not_change_results(pa, gpu, small, lanes):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
-def test_md_small_vs_oracle(pa, orc, gpu, small, variant):
+@pytest.mark.parametrize("variant,order", [(0, 1), (0, 0), (1, 1), (2, 1), (2, 0)])
+def test_md_small_vs_oracle(pa, orc, gpu, small, variant, order):
     pos, box, edges = small
     vel = pa.gen_velocities(len(pos), 1.44, seed=5)
-    md = pa.MD(pos, vel, edges, box, 2.5, 0.005, variant=variant)
+    md = pa.MD(pos, vel, edges, box, 2.5, 0.005, variant=variant, order=order)
     md.run(10)
     md.run(10)
     p, v, f = md.get()
-    rp, rv, rf, rep = orc.md_run(pos, vel, edges, box, 2.5, 0.005, 20)
+    rp, rv, rf, rep = orc.md_run(pos, vel, oracle_edges(pa, edges, order), box, 2.5, 0.005, 20)
     if variant == 0:
         assert np.array_equal(f, rf) and np.array_equal(v, rv) and np.array_equal(p, rp)
         ep, ek = md.energy()
@@ -149,19 +195,21 @@ def test_md_small_vs_oracle(pa, orc, gpu, small, variant):
     md.close()
 
 
-def test_moldyn_workload(pa, orc, gpu):
+@pytest.mark.parametrize("order", [1, 0])
+def test_moldyn_workload(pa, orc, gpu, order):
     """moldyn: 40^3 FCC cells (256,000 molecules), density 0.8442, jitter
     +-0.05, half list within 2.5 (random order), 50 VV steps, dt 0.005."""
     pos, box, edges = pa.moldyn_system(40, seed=42)
     assert 6.5e6 < len(edges) < 7.5e6
-    md = pa.MD(pos, None, edges, box, 2.5, 0.005)
+    md = pa.MD(pos, None, edges, box, 2.5, 0.005, order=order)
     md.run(50)
     p, v, f = md.get()
     ep, ek = md.energy()
     md.close()
-    rp, rv, rf, _ = orc.md_run(pos, np.zeros_like(pos), edges, box, 2.5, 0.005, 50, threads=16)
+    oe = oracle_edges(pa, edges, order)
+    rp, rv, rf, _ = orc.md_run(pos, np.zeros_like(pos), oe, box, 2.5, 0.005, 50, threads=16)
     assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
-    _, rep = orc.lj_force(rp, edges, box, 2.5)
+    _, rep = orc.lj_force(rp, oe, box, 2.5)
     p64, v64, f64, _ = orc.md_run(pos.astype(np.float64), np.zeros((len(pos), 3)), edges, box, 2.5, 0.005, 50,
                                   dtype=np.float64)
     print(f"moldyn 256k x50: pairs={len(edges)} bit-exact vs FP32 oracle; rel err vs FP64: force "

@@ -36,12 +36,14 @@ def test_binned_csr_equals_host_pairs(pa, gpu, cells, seed):
     assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
 
 
-def test_binned_md_vs_oracle(pa, orc, gpu):
+@pytest.mark.parametrize("lanes", [0, 1, 64])
+def test_binned_md_vs_oracle(pa, orc, gpu, lanes):
     """Rebuilds before the force of steps 20 and 40: bit-identical to the
-    oracle rebuilding its cell-list pair list at the same steps."""
+    oracle rebuilding its cell-list pair list at the same steps (tuned,
+    one lane per row, sliced layout)."""
     pos, box = pa.gen_fcc(8, 8, 8, rho=0.8442, jitter=0.0, seed=4)
     vel = pa.gen_velocities(len(pos), 1.44, seed=5)
-    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20, lanes_per_row=lanes)
     md.run(25)
     md.run(15)
     p, v, f = md.get()
