@@ -101,6 +101,15 @@ int pat_gen_pairs(int64_t n, const float *pos, double box, double cut, int shuff
  * point picks a component uniformly and adds sigma * N(0, 1) per axis.
  * x: n x d row-major. */
 int pat_gen_gmm(int64_t n, int d, int ncomp, double sigma, double lo, double hi, uint64_t seed, float *x);
+/* Device forms of the generators, writing device memory on 'stream' (not
+ * waited for): the heat3D grid (halo-inclusive, as pat_gen_heat3d) and the
+ * FCC lattice (float4 x, y, z, 0 per atom) are bit-identical to the host
+ * generators; the mixture points use the device's FP64 log/cos/sqrt. */
+int pat_gen_heat3d_dev(int64_t nx, int64_t ny, int64_t nz, uint64_t seed, float *grid, void *stream);
+int pat_gen_fcc_dev(int64_t ncx, int64_t ncy, int64_t ncz, double rho, double jitter, uint64_t seed, float *pos4,
+                    double *box_out, void *stream);
+int pat_gen_gmm_dev(int64_t n, int d, int ncomp, double sigma, double lo, double hi, uint64_t seed, float *x,
+                    void *stream);
 
 /* ------------------------------------------------------------------------
  * Stencil engine (replaces compute_cuda_internal, kmeans/lib/compute_cuda.cu:
@@ -245,6 +254,9 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
  * (miniMD/lj.in:9-10).  Needs box >= 3 * (cutoff + skin). */
 int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
                          double skin, int every, double dt, pat_md **out);
+/* The same from device float4 arrays (vel4 may be NULL = 0), copied. */
+int pat_md_create_binned_dev(int64_t num_nodes, const float *pos4, const float *vel4, double box, double cutoff,
+                             double skin, int every, double dt, pat_md **out);
 int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row);
 /* Row order (see pat_ir_set_row_order); recomputes the stored force.  Binned
  * handles only have PAT_IR_ORDER_NODE. */
@@ -335,6 +347,52 @@ int pat_kmeans_get_object(pat_kmeans *h, int64_t *sums, int64_t *counts, int64_t
 /* Points rescored in FP32 so far (MFMA mode) and max |x| of the data. */
 int pat_kmeans_stats(pat_kmeans *h, int64_t *rescored_total, float *maxabs);
 int pat_kmeans_destroy(pat_kmeans *h);
+
+/* ------------------------------------------------------------------------
+ * Serving loop.  No counterpart in the reference, whose mains run one
+ * application per process (RuntimeInit .. RuntimeFinalize,
+ * kmeans/include/runtime.cpp:4-13): here heat3D, LJ and kmeans jobs of any
+ * size arrive over time, are admitted while their device-memory estimate
+ * fits the budget and a slot is free, and run concurrently (one HIP stream
+ * per slot), their inputs generated on the device from the job's seed.
+ * ---------------------------------------------------------------------- */
+#define PAT_JOB_HEAT3D 0 /* size = cube side; steps of u' = u + 0.1 (sum6 - 6u); interior U[0,1) */
+#define PAT_JOB_LJ 1     /* size = FCC cells per side (rho 0.8442, jitter 0.05); steps of velocity
+                            Verlet, cutoff 2.5, skin 0.3, rebuild every 20, dt 0.005 */
+#define PAT_JOB_KMEANS 2 /* size = points, dim, k (also the mixture's components, sigma 1,
+                            centres in [-10, 10]^dim); steps = Lloyd iterations */
+typedef struct pat_job {
+    int kind;
+    int64_t size;
+    int dim, k;
+    int steps;
+    uint64_t seed;
+    float *out;      /* optional host buffer for the full result, or NULL: heat3D the size^3
+                        interior; LJ 4*size^3 x 3 positions; kmeans k x dim centres */
+    int64_t out_len; /* floats available at out */
+} pat_job;
+typedef struct pat_job_result {
+    int status;    /* PAT_OK or the job's error code */
+    int kernel;    /* engine choice: kmeans PAT_KM_*, LJ lanes per row */
+    double checksum;                          /* FP64 sum of the result (for a heat3D grid, of every cell) */
+    double t_submit_ms, t_start_ms, t_end_ms; /* since pat_serve_start; end = result on the host */
+    int64_t device_bytes;                     /* admission estimate held while it ran */
+} pat_job_result;
+typedef struct pat_serve pat_serve;
+#define PAT_SERVE_PLAIN 1 /* flag: kmeans on the plain FP32 kernel */
+/* Starts 'slots' workers over an arena of hbm_budget device bytes. */
+int pat_serve_start(int64_t hbm_budget, int slots, int flags, pat_serve **out);
+/* Queues a job (non-blocking); *id identifies it. */
+int pat_serve_submit(pat_serve *s, const pat_job *job, int64_t *id);
+int pat_serve_poll(pat_serve *s, int64_t id, int *done, pat_job_result *res);
+/* Blocks until job id has finished. */
+int pat_serve_wait(pat_serve *s, int64_t id, pat_job_result *res);
+/* Blocks until every submitted job has finished. */
+int pat_serve_drain(pat_serve *s);
+int pat_serve_stats(pat_serve *s, int64_t *submitted, int64_t *finished, int64_t *queued, int64_t *reserved_bytes,
+                    int64_t *arena_in_use);
+/* Runs what is queued, then stops the workers and frees the arena. */
+int pat_serve_stop(pat_serve *s);
 
 #ifdef __cplusplus
 }

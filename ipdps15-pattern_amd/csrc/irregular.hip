@@ -810,12 +810,12 @@ int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max
     if (acc > *cap) {
         const int64_t want = std::min<int64_t>((int64_t)(acc * 1.25) + 4096, (1LL << 31) - 1);
         int32_t *nc = nullptr;
-        if (hipMalloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+        if (pat::dev_malloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
             (void)hipGetLastError();
             pat::set_error("sell_build: hipMalloc of %lld entries failed", (long long)want);
             return PAT_ERR_NOMEM;
         }
-        if (*scol) (void)hipFree(*scol);
+        if (*scol) (void)pat::dev_free(*scol);
         *scol = nc;
         *cap = want;
     }
@@ -906,7 +906,7 @@ struct NbWork {
 static void nb_free(NbWork &w)
 {
     for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp})
-        if (p) (void)hipFree(p);
+        if (p) (void)pat::dev_free(p);
     w = NbWork{};
 }
 
@@ -921,7 +921,7 @@ void nb_destroy(NbWork *w)
 
 static int nb_alloc(int **p, int64_t n)
 {
-    if (hipMalloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)) != hipSuccess) {
+    if (pat::dev_malloc((void **)p, sizeof(int) * (size_t)(n > 0 ? n : 1)) != hipSuccess) {
         (void)hipGetLastError();
         pat::set_error("neighbour build: hipMalloc of %lld ints failed", (long long)n);
         return PAT_ERR_NOMEM;
@@ -991,8 +991,8 @@ int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w
     PAT_REQUIRE(tot >= 0, PAT_ERR_INVALID, "neighbour build: pair count overflows int32");
     if (tot > *cap || tot > w->cap) {
         const int64_t want = (int64_t)(tot * 1.25) + 1024;
-        if (*cols) (void)hipFree(*cols);
-        if (w->coltmp) (void)hipFree(w->coltmp);
+        if (*cols) (void)pat::dev_free(*cols);
+        if (w->coltmp) (void)pat::dev_free(w->coltmp);
         *cols = nullptr;
         w->coltmp = nullptr;
         PAT_TRY(nb_alloc(cols, want));

@@ -100,7 +100,7 @@ struct IrState {
         PAT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         auto am = [](void **p, size_t b) -> int {
             if (b == 0) b = 16;
-            if (hipMalloc(p, b) != hipSuccess) {
+            if (pat::dev_malloc(p, b) != hipSuccess) {
                 (void)hipGetLastError();
                 pat::set_error("irregular: hipMalloc of %zu bytes failed", b);
                 return PAT_ERR_NOMEM;
@@ -131,14 +131,14 @@ struct IrState {
     {
         const int64_t wb = pat::csr_workspace_bytes(N, E);
         void *work = nullptr;
-        if (hipMalloc(&work, (size_t)wb) != hipSuccess) {
+        if (pat::dev_malloc(&work, (size_t)wb) != hipSuccess) {
             (void)hipGetLastError();
             pat::set_error("irregular: hipMalloc of %lld workspace bytes failed", (long long)wb);
             return PAT_ERR_NOMEM;
         }
         int rc = pat::csr_build(edges, E, N, o, rowptr, cols, work, wb, stream);
         hipError_t se = hipStreamSynchronize(stream);
-        (void)hipFree(work);
+        (void)pat::dev_free(work);
         if (rc != PAT_OK) return rc;
         PAT_HIP(se);
         order = o;
@@ -167,7 +167,7 @@ struct IrState {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)edges, (void *)rowptr, (void *)cols, (void *)pos[0], (void *)pos[1], (void *)vel,
                         (void *)force, (void *)node_e, (void *)sptr, (void *)scol, (void *)swidth})
-            if (p) (void)hipFree(p);
+            if (p) (void)pat::dev_free(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -250,7 +250,7 @@ struct IrState {
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
         const auto key = std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg, order);
         layout_gen++;
-        if (getenv("PAT_NO_TUNE")) {
+        if (!pat::tuning_enabled()) {
             lpr = sell_ok ? PAT_IR_SLICED : 1;
             return PAT_OK;
         }
@@ -633,8 +633,12 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
     return PAT_OK;
 }
 
-int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
-                         double skin, int every, double dt, pat_md **out)
+namespace {
+
+// Binned MD handle whose positions/velocities come from host float3 arrays
+// (dev = false) or device float4 arrays (dev = true, vel may be null).
+int md_create_binned(int64_t num_nodes, const float *pos, const float *vel, bool dev, double box, double cutoff,
+                     double skin, int every, double dt, pat_md **out)
 {
     PAT_REQUIRE(out && pos, PAT_ERR_INVALID, "pat_md_create_binned: null argument");
     PAT_REQUIRE(num_nodes > 1 && num_nodes < (1LL << 31) - 1, PAT_ERR_INVALID, "pat_md_create_binned: bad node count");
@@ -652,11 +656,22 @@ int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, 
     h->nbw = pat::nb_create();
     int rc = h->st.alloc(num_nodes, 0, nullptr, true, false);
     if (rc == PAT_OK) {
-        (void)hipFree(h->st.cols);
+        (void)pat::dev_free(h->st.cols);
         h->st.cols = nullptr;
-        rc = h->st.upload3(h->st.pos[0], pos, 12);
+        if (dev) {
+            const size_t b4 = sizeof(float) * 4 * (size_t)num_nodes;
+            if (hipMemcpyAsync(h->st.pos[0], pos, b4, hipMemcpyDeviceToDevice, h->st.stream) != hipSuccess ||
+                (vel ? hipMemcpyAsync(h->st.vel, vel, b4, hipMemcpyDeviceToDevice, h->st.stream)
+                     : hipMemsetAsync(h->st.vel, 0, b4, h->st.stream)) != hipSuccess ||
+                hipStreamSynchronize(h->st.stream) != hipSuccess) {
+                pat::set_error("pat_md_create_binned_dev: copying the device arrays failed");
+                rc = PAT_ERR_HIP;
+            }
+        } else {
+            rc = h->st.upload3(h->st.pos[0], pos, 12);
+        }
     }
-    if (rc == PAT_OK) {
+    if (rc == PAT_OK && !dev) {
         std::vector<float> zero(3 * (size_t)num_nodes, 0.0f);
         rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
     }
@@ -675,6 +690,20 @@ int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, 
     }
     *out = h;
     return PAT_OK;
+}
+
+}  // namespace
+
+int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
+                         double skin, int every, double dt, pat_md **out)
+{
+    return md_create_binned(num_nodes, pos, vel, false, box, cutoff, skin, every, dt, out);
+}
+
+int pat_md_create_binned_dev(int64_t num_nodes, const float *pos4, const float *vel4, double box, double cutoff,
+                             double skin, int every, double dt, pat_md **out)
+{
+    return md_create_binned(num_nodes, pos4, vel4, true, box, cutoff, skin, every, dt, out);
 }
 
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms)
@@ -934,7 +963,7 @@ struct pat_mdb {
                 // grow the batched column array, keeping what is placed
                 const int64_t want = (int64_t)((entries + tot) * 1.3) + 4096;
                 int32_t *nc = nullptr;
-                if (hipMalloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+                if (pat::dev_malloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
                     (void)hipGetLastError();
                     pat::set_error("batched md: hipMalloc of %lld column entries failed", (long long)want);
                     return PAT_ERR_NOMEM;
@@ -943,7 +972,7 @@ struct pat_mdb {
                     PAT_HIP(hipMemcpyAsync(nc, md.st.cols, sizeof(int32_t) * entries, hipMemcpyDeviceToDevice,
                                            md.st.stream));
                 PAT_HIP(hipStreamSynchronize(md.st.stream));
-                if (md.st.cols) (void)hipFree(md.st.cols);
+                if (md.st.cols) (void)pat::dev_free(md.st.cols);
                 md.st.cols = nc;
                 md.cap = want;
             }
@@ -1072,7 +1101,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     m.nbw = pat::nb_create();
     int rc = m.st.alloc(N, 0, nullptr, true, false);
     if (rc == PAT_OK) {
-        (void)hipFree(m.st.cols);
+        (void)pat::dev_free(m.st.cols);
         m.st.cols = nullptr;
         // positions with each node's box side in w
         std::vector<float> h4(4 * (size_t)N);
@@ -1093,7 +1122,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
         rc = m.st.upload3(m.st.vel, vel ? vel : zero.data(), 12);
     }
     if (rc == PAT_OK) {
-        if (hipMalloc((void **)&h->rp_s, sizeof(int32_t) * (size_t)(N + 1)) != hipSuccess) rc = PAT_ERR_NOMEM;
+        if (pat::dev_malloc((void **)&h->rp_s, sizeof(int32_t) * (size_t)(N + 1)) != hipSuccess) rc = PAT_ERR_NOMEM;
     }
     if (rc == PAT_OK) rc = h->rebuild_all(0);
     if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
@@ -1101,8 +1130,8 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
         rc = m.st.csr_force(m.st.pos[0], 1.0f, 2, m.rc2, 0.f, kForceOnly, nullptr, m.st.stream);
     if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc != PAT_OK) {
-        if (h->rp_s) (void)hipFree(h->rp_s);
-        if (h->cl_s) (void)hipFree(h->cl_s);
+        if (h->rp_s) (void)pat::dev_free(h->rp_s);
+        if (h->cl_s) (void)pat::dev_free(h->cl_s);
         pat::nb_destroy(m.nbw);
         m.nbw = nullptr;
         m.st.release();
@@ -1169,8 +1198,8 @@ int pat_mdb_stats(pat_mdb *h, int64_t *num_pairs, int64_t *steps, int64_t *rebui
 int pat_mdb_destroy(pat_mdb *h)
 {
     if (!h) return PAT_OK;
-    if (h->rp_s) (void)hipFree(h->rp_s);
-    if (h->cl_s) (void)hipFree(h->cl_s);
+    if (h->rp_s) (void)pat::dev_free(h->rp_s);
+    if (h->cl_s) (void)pat::dev_free(h->cl_s);
     if (h->md.nbw) pat::nb_destroy(h->md.nbw);
     h->md.nbw = nullptr;
     h->drop_seg_graphs();

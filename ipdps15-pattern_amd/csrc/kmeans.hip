@@ -48,9 +48,30 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr float kInitMin = 65536 * 65;  // map1's initial min_dist (kmeans/regular.h:86)
 
-__device__ __forceinline__ long long to_fixed(float v, double scale)
+// Position of element e of centre row k in the BF16 MFMA operand array,
+// stored in fragment order: for 32-row block b and K-step s, lane
+// l = (k % 32) + 32 * ((e % 16) / 8) holds elements 8*(l/32) .. +8 of row
+// b*32 + l%32 at 16-byte slot (b*NS + s)*64 + l, so a wavefront's operand
+// read is 1 KB of consecutive LDS bytes (no bank conflicts).
+__host__ __device__ __forceinline__ int64_t frag_index(int k, int e, int ns)
 {
-    return (long long)__builtin_rint((double)v * scale);
+    const int b = k / 32, col = k % 32, s = e / 16, h = (e % 16) / 8, j = e % 8;
+    return ((((int64_t)b * ns + s) * 2 + h) * 32 + col) * 8 + j;
+}
+
+// rint(v * 2^s) as a 64-bit two's-complement integer, with scale = 2^s an
+// FP32 power of two: v * scale is exact (no overflow: |v| 2^s < 2^62 by the
+// choice of s), rounding it to an integer is exact in FP32, and its
+// magnitude splits exactly into 32-bit halves.  The same integer as
+// rint((double)v * 2^s) in FP64.
+__device__ __forceinline__ unsigned long long to_fixed(float v, float scale)
+{
+    const float f = __builtin_rintf(v * scale);
+    const float m = fabsf(f);
+    const float hi = floorf(m * 0x1p-32f);
+    const float lo = m - hi * 0x1p32f;  // exact, in [0, 2^32)
+    const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) + (unsigned long long)(unsigned)lo;
+    return f < 0.0f ? 0ULL - u : u;
 }
 
 // ------------------------------------------------------------ shared parts
@@ -60,7 +81,7 @@ struct KmArgs {
     const float *c;         // K x D centres (FP32)
     int64_t N;
     int D, K;
-    double scale, dscale;   // fixed-point scales 2^s, 2^ds
+    float scale, dscale;    // fixed-point scales 2^s, 2^ds (FP32 powers of two)
     int32_t *assign;        // N
     float *mindist;         // N (may be null)
     unsigned long long *gsum;   // K x D
@@ -82,14 +103,22 @@ __device__ __forceinline__ void accumulate_points(const KmArgs &a, int64_t base,
     const int lane = threadIdx.x % kWave;
     const int D = a.D;
     const int total = np * D;
+    // lane covers flat index t = j*D + d, advanced by kWave per trip without
+    // a division inside the loop
+    const int step_j = kWave / D, step_d = kWave % D;
+    int j = lane / D, d = lane % D;
     for (int t0 = 0; t0 < total; t0 += kWave) {
-        const int t = t0 + lane;
-        const int tt = t < total ? t : total - 1;
-        const int j = tt / D, d = tt - j * D;
-        const int k = key_of(j);
-        if (t < total) {
+        const bool in = t0 + lane < total;
+        const int k = key_of(in ? j : np - 1);
+        if (in) {
             const float v = a.x[(base + j) * D + d];
-            atomicAdd(&acc[(int64_t)k * D + d], (unsigned long long)to_fixed(v, a.scale));
+            atomicAdd(&acc[(int64_t)k * D + d], to_fixed(v, a.scale));
+        }
+        j += step_j;
+        d += step_d;
+        if (d >= D) {
+            d -= D;
+            j++;
         }
     }
     const int jj = lane < np ? lane : np - 1;
@@ -97,7 +126,7 @@ __device__ __forceinline__ void accumulate_points(const KmArgs &a, int64_t base,
     const float dist = dist_of(jj);
     if (lane < np) {
         atomicAdd(&cnt[k], 1ULL);
-        atomicAdd(&dsum[k], (unsigned long long)to_fixed(dist, a.dscale));
+        atomicAdd(&dsum[k], to_fixed(dist, a.dscale));
     }
 }
 
@@ -144,8 +173,10 @@ __device__ __forceinline__ void lds_object(const KmArgs &a, unsigned long long *
 // ------------------------------------------------------------ plain path
 
 // One thread per point; centres read from LDS when they fit (cs != null).
-template <int DMAX>
-__global__ __launch_bounds__(512) void km_plain(KmArgs a, int centres_in_lds)
+// BS threads per workgroup: the LDS object allows one workgroup per CU, so
+// BS sets the waves per SIMD that hide LDS latency (4 at 1024 threads).
+template <int DMAX, int BS>
+__global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
 {
     extern __shared__ unsigned long long smem[];
     unsigned long long *acc, *cnt, *dsum;
@@ -235,14 +266,62 @@ struct MfmaBound {
     float c_ref;   // relative error of the reference's FP32 difference form
 };
 
-// Exact plain-path distance (sqrt of the FP32 difference form) of point p
-// to centre k.
-__device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const float *__restrict__ ck, int D)
+// Exact plain-path distance (sqrt of the FP32 difference form, dimensions in
+// order) of a point held in registers (xr, zero past D) to centre row ck.
+template <int DP>
+__device__ __forceinline__ float exact_dist(const float (&xr)[DP], const float *__restrict__ ck, int D, bool vec)
 {
     float dist = 0;
-    for (int d = 0; d < D; d++) {
-        const float t = ck[d] - xp[d];
-        dist = dist + t * t;
+    if (vec) {  // D % 4 == 0 and 16-byte aligned rows
+#pragma unroll
+        for (int q = 0; q < DP / 4; q++) {
+            if (4 * q < D) {
+                const float4 cv = *(const float4 *)(ck + 4 * q);
+                float t = cv.x - xr[4 * q];
+                dist = dist + t * t;
+                t = cv.y - xr[4 * q + 1];
+                dist = dist + t * t;
+                t = cv.z - xr[4 * q + 2];
+                dist = dist + t * t;
+                t = cv.w - xr[4 * q + 3];
+                dist = dist + t * t;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DP; d++) {
+            if (d < D) {
+                const float t = ck[d] - xr[d];
+                dist = dist + t * t;
+            }
+        }
+    }
+    return sqrtf(dist);
+}
+
+// The same distance with the point read from memory (rows too long to keep
+// in registers).
+__device__ __forceinline__ float exact_dist_mem(const float *__restrict__ xp, const float *__restrict__ ck, int D,
+                                               bool vec)
+{
+    float dist = 0;
+    if (vec) {
+        for (int d = 0; d < D; d += 4) {
+            const float4 cv = *(const float4 *)(ck + d), xv = *(const float4 *)(xp + d);
+            float t = cv.x - xv.x;
+            dist = dist + t * t;
+            t = cv.y - xv.y;
+            dist = dist + t * t;
+            t = cv.z - xv.z;
+            dist = dist + t * t;
+            t = cv.w - xv.w;
+            dist = dist + t * t;
+        }
+    } else {
+        for (int d = 0; d < D; d++) {
+            const float t = ck[d] - xp[d];
+            dist = dist + t * t;
+        }
     }
     return sqrtf(dist);
 }
@@ -251,14 +330,24 @@ __device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const 
 // (1 or 3), each KS steps; one more K-step holds the norm terms.  KB:
 // 32-centre blocks per chunk.  cb: K_pad x (16*(NT*KS+1)) BF16 centre rows
 // [-2c | n1 n2 n3 0...] (NT = 1) or [hi(-2c) | lo(-2c) | hi(-2c) | n1 n2 n3
-// 0...] (NT = 3), data zero-padded to 16*KS; padded rows have n1 = +inf.
-// cnorm: ||c_k|| (K entries).
+// 0...] (NT = 3), data zero-padded to 16*KS, stored in fragment order
+// (frag_index); padded rows have n1 = +inf.  cnorm: ||c_k|| (K entries).
+//
+// Per 32-point tile a wavefront: loads each point's row into registers
+// (lanes l and l+32 both hold point l), computes the approximate values of
+// every centre with MFMAs, takes their minimum m and the threshold, counts
+// the centres at or below it and, when the point has exactly one (the usual
+// case), rescores only that centre; otherwise every candidate is rescored.
+// The exact FP32 results decide with map1's rule, and the point is added to
+// the LDS reduction object from the registers.
 template <int KS, int KB, int NT>
 __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
 {
     constexpr int NS = NT * KS + 1;  // K-steps per centre row
     constexpr int KW = 16 * NS;      // BF16 row width of cb
+    constexpr bool kRow = KS <= 2;   // whole row in registers for the rescoring
+    constexpr int DP = kRow ? 16 * KS : 4;  // padded dimensions held per lane
     extern __shared__ unsigned long long smem[];
     unsigned long long *acc_o, *cnt_o, *dsum_o;
     lds_object(a, smem, acc_o, cnt_o, dsum_o);
@@ -285,6 +374,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 
     const int D = a.D;
     const bool vec_x = (D % 4) == 0 && ((uintptr_t)a.x % 16) == 0;  // rows start 16-byte aligned
+    const bool vec_c = (D % 4) == 0 && ((uintptr_t)a.c % 16) == 0;
     const int lane = threadIdx.x % kWave;
     const int col = lane % 32;   // point within the tile
     const int half = lane / 32;  // which 8 of each 16 K elements this lane holds
@@ -295,35 +385,62 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; tile < ntiles;
          tile += nw) {
-        const int64_t base = tile * 32;
-        const int64_t p = base + col;
+        const int64_t p = tile * 32 + col;
         const bool live = p < a.N;
         const float *xp = a.x + (live ? p : 0) * D;
 
+        // the point's row, zero past D (rows up to 32 dimensions)
+        float xr[DP];
+        if (!kRow) {
+#pragma unroll
+            for (int d = 0; d < DP; d++) xr[d] = 0.0f;
+        } else if (vec_x) {
+#pragma unroll
+            for (int q = 0; q < DP / 4; q++) {
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (live && 4 * q < D) v = *(const float4 *)(xp + 4 * q);
+                xr[4 * q] = v.x;
+                xr[4 * q + 1] = v.y;
+                xr[4 * q + 2] = v.z;
+                xr[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < DP; d++) xr[d] = (live && d < D) ? xp[d] : 0.0f;
+        }
+        // this lane's half of the row (dimensions 16s + 8*half + e), loaded
+        // by address: selecting it from xr with the runtime half index would
+        // turn xr into a scratch array
+        float xo[8 * KS];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int d0 = 16 * s + 8 * half;
+            if (vec_x) {
+                float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+                if (live && d0 < D) lo = *(const float4 *)(xp + d0);
+                if (live && d0 + 4 < D) hi = *(const float4 *)(xp + d0 + 4);
+                xo[8 * s] = lo.x; xo[8 * s + 1] = lo.y; xo[8 * s + 2] = lo.z; xo[8 * s + 3] = lo.w;
+                xo[8 * s + 4] = hi.x; xo[8 * s + 5] = hi.y; xo[8 * s + 6] = hi.z; xo[8 * s + 7] = hi.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) xo[8 * s + e] = (live && d0 + e < D) ? xp[d0 + e] : 0.0f;
+            }
+        }
         // B operand: point p, K elements [16s + 8*half, +8) of
         // [x, 1, 1, 1, 0...] or [xh, xh, xl, 1, 1, 1, 0...]
         bf16x8 bx[NS];
         float xx = 0.0f;
 #pragma unroll
         for (int s = 0; s < KS; s++) {
-            const int d0 = 16 * s + 8 * half;
-            float v[8];
-            if (vec_x && live && d0 + 8 <= D) {
-                const float4 lo = *(const float4 *)(xp + d0), hi = *(const float4 *)(xp + d0 + 4);
-                v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
-                v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; e++) v[e] = (live && d0 + e < D) ? xp[d0 + e] : 0.0f;
-            }
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                xx = xx + v[e] * v[e];
-                const __bf16 h = (__bf16)v[e];
+                const float v = xo[8 * s + e];
+                xx = xx + v * v;
+                const __bf16 h = (__bf16)v;
                 bx[s][e] = h;
                 if (NT == 3) {
                     bx[KS + s][e] = h;
-                    bx[2 * KS + s][e] = (__bf16)(v[e] - (float)h);
+                    bx[2 * KS + s][e] = (__bf16)(v - (float)h);
                 }
             }
         }
@@ -333,9 +450,6 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         const float xn = sqrtf(xx) * 1.0001f;
         const float E = bd.c_dot * xn * cmax + bd.c_norm * (cc_max + xx);
 
-        float best_d = kInitMin;
-        int best_k = 0x7fffffff;
-        int ncand = 0;
         // approximate values of chunk ch (KB blocks of 32 centres) for this
         // lane's point: value i of block b is centre row 8*(i/4) + 4*half +
         // i%4 of the block (the 32x32 MFMA output layout), column = point
@@ -344,12 +458,11 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
             for (int b = 0; b < KB; b++) {
                 acc[b] = f32x16{};
                 if ((ch * KB + b) * 32 < K_pad) {
-                    const int row = (ch * KB + b) * 32 + col;  // A operand row: centre
+                    // A operand: centre rows of block ch*KB + b, fragment order
+                    const bf16x8 *ab = (const bf16x8 *)cbs + (int64_t)(ch * KB + b) * NS * kWave + lane;
 #pragma unroll
-                    for (int s = 0; s < NS; s++) {
-                        const bf16x8 av = *(const bf16x8 *)(cbs + (int64_t)row * KW + 16 * s + 8 * half);
-                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[s], acc[b], 0, 0, 0);
-                    }
+                    for (int s = 0; s < NS; s++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ab[s * kWave], bx[s], acc[b], 0, 0, 0);
                 } else {
                     // a block past the last centre row: never a candidate
 #pragma unroll
@@ -357,55 +470,77 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
                 }
             }
         };
-        auto chunk_min = [&](const f32x16 *acc) {
-            float cmn = INFINITY;
+        auto centre_of = [&](int ch, int b, int i) { return (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4); };
+
+        // minimum over every centre (both half-waves' rows)
+        f32x16 acc[KB];
+        float m = INFINITY;
+        for (int ch = 0; ch < nchunks; ch++) {
+            chunk_values(ch, acc);
 #pragma unroll
             for (int b = 0; b < KB; b++)
 #pragma unroll
-                for (int i = 0; i < 16; i++) cmn = fminf(cmn, acc[b][i]);
-            return fminf(cmn, __shfl_xor(cmn, 32, kWave));  // rows of both half-waves
-        };
-        // The candidate threshold needs the minimum over every centre: with
-        // one chunk it comes with the values, with several a first pass
-        // finds it (the values are recomputed rather than held).
-        float m = INFINITY;
-        if (nchunks > 1) {
-            for (int ch = 0; ch < nchunks; ch++) {
-                f32x16 acc[KB];
-                chunk_values(ch, acc);
-                m = fminf(m, chunk_min(acc));
-            }
+                for (int i = 0; i < 16; i++) m = fminf(m, acc[b][i]);
         }
+        m = fminf(m, __shfl_xor(m, 32, kWave));
+        // candidates: approximate value within the margin of the minimum
+        const float dhi = (xx + m + 4.0f * E) * 1.001f + 1e-30f;
+        const float thr = m + 2.0f * E + 2.0f * bd.c_ref * dhi;
+        int cnt = 0, kc = 0;
         for (int ch = 0; ch < nchunks; ch++) {
-            f32x16 acc[KB];
-            chunk_values(ch, acc);
-            if (nchunks == 1) m = chunk_min(acc);
-            // candidates: approximate value within the margin of the minimum
-            const float dhi = (xx + m + 4.0f * E) * 1.001f + 1e-30f;
-            const float thr = m + 2.0f * E + 2.0f * bd.c_ref * dhi;
-            constexpr int NM = (KB * 16 + 63) / 64;  // 64-bit candidate masks (4 blocks each)
-            uint64_t mask[NM];
-#pragma unroll
-            for (int w = 0; w < NM; w++) mask[w] = 0;
+            if (nchunks > 1) chunk_values(ch, acc);
 #pragma unroll
             for (int b = 0; b < KB; b++)
 #pragma unroll
                 for (int i = 0; i < 16; i++)
-                    if (acc[b][i] <= thr) mask[b / 4] |= 1ULL << ((b % 4) * 16 + i);
+                    if (acc[b][i] <= thr) {
+                        cnt++;
+                        kc = centre_of(ch, b, i);
+                    }
+        }
+        const int cnt_all = cnt + __shfl_xor(cnt, 32, kWave);
+
+        float best_d = kInitMin;
+        int best_k = 0x7fffffff;
+        const bool multi = cnt_all != 1;
+        if (!multi) {
+            // one candidate: the half-wave holding it rescores it
+            if (cnt == 1 && kc < a.K && live) {
+                best_d = kRow ? exact_dist(xr, a.c + (int64_t)kc * D, D, vec_c)
+                              : exact_dist_mem(xp, a.c + (int64_t)kc * D, D, vec_c && vec_x);
+                best_k = kc;
+            }
+        }
+        // Several candidates: mask them (4 blocks per 64-bit word) and rescore
+        // each set bit.  The branch is taken by the whole wavefront (a vote),
+        // because recomputing a chunk's values needs every lane's operands.
+        if (__any(multi)) {
+            static_assert(KB <= 8, "two 64-bit candidate words cover at most 8 blocks");
+            for (int ch = 0; ch < nchunks; ch++) {
+                if (nchunks > 1) chunk_values(ch, acc);
+                if (!multi) continue;
+                uint64_t m0 = 0, m1 = 0;
 #pragma unroll
-            for (int w = 0; w < NM; w++) {
-                uint64_t mw = mask[w];
-                while (mw) {
-                    const int bit = __builtin_ctzll(mw);
-                    mw &= mw - 1;
-                    const int b = 4 * w + bit / 16, i = bit % 16;
-                    const int k = (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4);
-                    if (k < a.K && live) {
-                        const float dk = exact_dist(xp, a.c + (int64_t)k * D, D);
-                        ncand++;
-                        if (dk < best_d || (dk == best_d && k < best_k)) {
-                            best_d = dk;
-                            best_k = k;
+                for (int b = 0; b < KB; b++)
+#pragma unroll
+                    for (int i = 0; i < 16; i++)
+                        if (acc[b][i] <= thr) {
+                            if (b < 4) m0 |= 1ULL << (b * 16 + i);
+                            else m1 |= 1ULL << ((b - 4) * 16 + i);
+                        }
+                for (int w = 0; w < 2; w++) {
+                    uint64_t mw = w ? m1 : m0;
+                    while (mw) {
+                        const int bit = __builtin_ctzll(mw);
+                        mw &= mw - 1;
+                        const int k = centre_of(ch, 4 * w + bit / 16, bit % 16);
+                        if (k < a.K && live) {
+                            const float dk = kRow ? exact_dist(xr, a.c + (int64_t)k * D, D, vec_c)
+                                                  : exact_dist_mem(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
+                            if (dk < best_d || (dk == best_d && k < best_k)) {
+                                best_d = dk;
+                                best_k = k;
+                            }
                         }
                     }
                 }
@@ -414,21 +549,29 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         // merge the two half-waves' exact winners (same tie rule)
         const float od = __shfl_xor(best_d, 32, kWave);
         const int ok = __shfl_xor(best_k, 32, kWave);
-        const int on = __shfl_xor(ncand, 32, kWave);
         if (od < best_d || (od == best_d && ok < best_k)) {
             best_d = od;
             best_k = ok;
         }
         if (best_k == 0x7fffffff) best_k = 0;  // unreachable for live points
-        if (live && half == 0) {
-            a.assign[p] = best_k;
-            if (a.mindist) a.mindist[p] = best_d;
-            if (ncand + on > 1) my_rescored++;
+        if (live) {
+            if (half == 0) {
+                a.assign[p] = best_k;
+                if (a.mindist) a.mindist[p] = best_d;
+                if (cnt_all > 1) my_rescored++;
+                atomicAdd(&cnt_o[best_k], 1ULL);
+                atomicAdd(&dsum_o[best_k], to_fixed(best_d, a.dscale));
+            }
+            // this half-wave's dimensions into the reduction object
+            unsigned long long *row = acc_o + (int64_t)best_k * D;
+#pragma unroll
+            for (int s = 0; s < KS; s++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int d = 16 * s + 8 * half + e;
+                    if (d < D) atomicAdd(&row[d], to_fixed(xo[8 * s + e], a.scale));
+                }
         }
-        const int np = (int)min<int64_t>(32, a.N - base);
-        accumulate_points(
-            a, base, np, [&](int j) { return __shfl(best_k, j, kWave); },
-            [&](int j) { return __shfl(best_d, j, kWave); }, acc_o, cnt_o, dsum_o);
     }
     // rescored-point counter
 #pragma unroll
@@ -461,7 +604,7 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
     if (k >= K_pad) return;
     if (k >= K) {  // padding row: never the minimum
         for (int e = lane; e < KW; e += kWave)
-            cb[(int64_t)k * KW + e] = (__bf16)((e == NB) ? INFINITY : 0.0f);
+            cb[frag_index(k, e, KW / 16)] = (__bf16)((e == NB) ? INFINITY : 0.0f);
         return;
     }
     const long long n = (long long)gcnt[k];
@@ -477,10 +620,10 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
     for (int d = lane; d < 16 * KS; d += kWave) {
         const float v = d < D ? -2.0f * c[(int64_t)k * D + d] : 0.0f;  // exact
         const __bf16 h = (__bf16)v;
-        cb[(int64_t)k * KW + d] = h;
+        cb[frag_index(k, d, KW / 16)] = h;
         if (NT == 3) {
-            cb[(int64_t)k * KW + 16 * KS + d] = (__bf16)(v - (float)h);
-            cb[(int64_t)k * KW + 32 * KS + d] = h;
+            cb[frag_index(k, 16 * KS + d, KW / 16)] = (__bf16)(v - (float)h);
+            cb[frag_index(k, 32 * KS + d, KW / 16)] = h;
         }
     }
     if (lane < 16) {
@@ -493,7 +636,7 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
         if (lane == 0) v = n1;
         if (lane == 1) v = n2;
         if (lane == 2) v = n3;
-        cb[(int64_t)k * KW + NB + lane] = v;
+        cb[frag_index(k, NB + lane, KW / 16)] = v;
     }
     if (lane == 0) {
         cnorm[k] = (float)sqrt(ss) * 1.0001f;
@@ -577,8 +720,8 @@ static KmArgs make_args(const KmDev &d)
     a.N = d.N;
     a.D = d.D;
     a.K = d.K;
-    a.scale = ldexp(1.0, d.shift);
-    a.dscale = ldexp(1.0, d.dshift);
+    a.scale = ldexpf(1.0f, d.shift);
+    a.dscale = ldexpf(1.0f, d.dshift);
     a.assign = d.assign;
     a.mindist = d.mindist;
     a.gsum = d.gsum;
@@ -601,8 +744,8 @@ int kmeans_prepare_kernels()
     PAT_HIP(hipGetDevice(&dev));
     PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     g_lds_limit = lim - 256;  // room for the kernels' static LDS
-    const void *fns[] = {(const void *)km_plain<8>,       (const void *)km_plain<16>,      (const void *)km_plain<32>,
-                         (const void *)km_plain<64>,      (const void *)km_plain<128>,     (const void *)km_mfma<1, 8, 1>,
+    const void *fns[] = {(const void *)km_plain<8, 1024>, (const void *)km_plain<16, 1024>, (const void *)km_plain<32, 1024>,
+                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1>,
                          (const void *)km_mfma<2, 8, 1>,  (const void *)km_mfma<4, 4, 1>,  (const void *)km_mfma<8, 2, 1>,
                          (const void *)km_mfma<1, 8, 3>,  (const void *)km_mfma<2, 8, 3>,  (const void *)km_mfma<4, 4, 3>,
                          (const void *)km_mfma<8, 2, 3>};
@@ -651,12 +794,13 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
     int64_t lds = a.use_lds_acc ? object_bytes(d.K, d.D) : 0;
     int in_lds = lds + cbytes <= g_lds_limit;
     if (in_lds) lds += cbytes;
-    const int64_t want = (d.N + 511) / 512;
+    const int64_t want = (d.N + 1023) / 1024;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
 #define PAT_KP(DM)                                                                                                     \
     do {                                                                                                               \
-        PAT_TRY(allow_lds((const void *)km_plain<DM>, lds));                                                           \
-        hipLaunchKernelGGL(km_plain<DM>, dim3(grid), dim3(512), (size_t)lds, st, a, in_lds);                           \
+        constexpr int bs = DM <= 64 ? 1024 : 512;                                                                      \
+        PAT_TRY(allow_lds((const void *)km_plain<DM, bs>, lds));                                                       \
+        hipLaunchKernelGGL((km_plain<DM, bs>), dim3(grid), dim3(bs), (size_t)lds, st, a, in_lds);                      \
     } while (0)
     if (d.D <= 8) PAT_KP(8);
     else if (d.D <= 16) PAT_KP(16);

@@ -60,7 +60,7 @@ struct pat_kmeans {
         for (void *p : {(void *)xbuf, (void *)d.c, (void *)d.assign, (void *)d.mindist, (void *)d.gsum, (void *)d.gcnt,
                         (void *)d.gdsum, (void *)d.csnap, rowbuf, (void *)d.cnorm, (void *)d.rescored,
                         (void *)maxbits, (void *)snap})
-            if (p) (void)hipFree(p);
+            if (p) (void)pat::dev_free(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -70,7 +70,7 @@ namespace {
 int amalloc(void **p, size_t b)
 {
     if (b == 0) b = 16;
-    if (hipMalloc(p, b) != hipSuccess) {
+    if (pat::dev_malloc(p, b) != hipSuccess) {
         (void)hipGetLastError();
         pat::set_error("kmeans: hipMalloc of %zu bytes failed", b);
         return PAT_ERR_NOMEM;
@@ -125,6 +125,10 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     memcpy(&h->maxabs, &bits, 4);
     D.shift = pat::kmeans_fixed_shift(n, h->maxabs);
     D.dshift = pat::kmeans_fixed_shift(n, (float)(2.0 * (double)h->maxabs * std::sqrt((double)d)));
+    // the kernels scale by 2^shift in FP32
+    PAT_REQUIRE(std::isfinite(h->maxabs) && D.shift >= -120 && D.dshift >= -120, PAT_ERR_INVALID,
+                "kmeans: point coordinates out of range for the fixed-point reduction object (max |x| = %g)",
+                (double)h->maxabs);
     PAT_TRY(pat::kmeans_zero(D.gsum, (int64_t)k * d, h->stream));
     PAT_TRY(pat::kmeans_zero(D.gcnt, k, h->stream));
     PAT_TRY(pat::kmeans_zero(D.gdsum, k, h->stream));

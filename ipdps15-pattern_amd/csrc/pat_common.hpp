@@ -30,6 +30,31 @@ void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &body);
 // Library's current device (set by pat_init, default 0).
 int current_device();
 
+// Device memory for every buffer the library owns.  By default these are
+// hipMalloc/hipFree; on a thread that has an arena bound (the serving loop's
+// workers) allocations are carved from that arena instead, so jobs neither
+// allocate from nor free to the runtime (hipFree waits for the whole
+// device).  dev_free finds the owner by address, so it may run on any thread.
+struct Arena;
+hipError_t dev_malloc(void **p, size_t bytes);
+template <typename T>
+inline hipError_t dev_malloc(T **p, size_t bytes)
+{
+    return dev_malloc((void **)p, bytes);
+}
+hipError_t dev_free(void *p);
+Arena *arena_create(size_t bytes);  // one hipMalloc of 'bytes'; nullptr on failure (error set)
+void arena_destroy(Arena *a);       // the caller guarantees nothing in it is still in use
+void arena_bind(Arena *a);          // this thread's allocations come from a (nullptr: the runtime)
+size_t arena_in_use(const Arena *a);
+size_t arena_largest_free(const Arena *a);
+
+// Whether kernel autotuning may run on this thread (default yes; off with
+// PAT_NO_TUNE in the environment, or per thread, as the serving loop's
+// workers do: timings taken beside other jobs would be noise).
+bool tuning_enabled();
+void set_thread_tuning(bool on);
+
 }  // namespace pat
 
 #define PAT_HIP(call)                                                                                      \

@@ -56,6 +56,12 @@ void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &body)
 
 int current_device() { return g_device; }
 
+static thread_local bool t_tuning = true;
+
+bool tuning_enabled() { return t_tuning && !getenv("PAT_NO_TUNE"); }
+
+void set_thread_tuning(bool on) { t_tuning = on; }
+
 }  // namespace pat
 
 extern "C" {

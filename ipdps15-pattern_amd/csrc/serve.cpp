@@ -1,0 +1,433 @@
+// Serving loop: heat3D, LJ and kmeans jobs of different sizes arriving over
+// time, admitted by device-memory budget and run concurrently.
+//
+// The reference's runtimes run one application per process (its mains call
+// RuntimeInit, one runtime, RuntimeFinalize).  Here one process keeps the
+// GPU busy with many jobs: a pool of worker threads ("slots"), each with its
+// own HIP stream, takes jobs from an arrival queue.  A job is admitted when
+// a slot is free and its device-memory estimate fits in what the budget has
+// left, so jobs that arrive while others run start as soon as room frees up
+// and their kernels overlap on the device.  Every job allocates from one
+// arena reserved at start (no hipMalloc/hipFree per job: hipFree waits for
+// the whole device), generates its input on the device from its seed, runs
+// the engine, and hands back a checksum (and, if asked, the full result).
+//
+// Admission is first-fit over the queue in arrival order; after 64
+// consecutive admissions that passed over the oldest job, only that job may
+// start, so a large job is never starved.  A job whose allocation fails
+// while others hold memory is queued again and then runs alone.
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "kmeans_dev.hpp"
+#include "pat_common.hpp"
+
+namespace pat {
+int sum_blocks();
+int sum_f32_dev(const float *p, int64_t n, double *part, double *out, hipStream_t st);
+}  // namespace pat
+
+namespace {
+
+constexpr int kMaxBypass = 64;
+constexpr float kHeatAlpha = 0.1f;  // heat3D update u' = u + 0.1 (sum6 - 6u)
+constexpr double kRho = 0.8442, kJitter = 0.05, kCut = 2.5, kSkin = 0.3, kDt = 0.005;
+constexpr int kEvery = 20;
+
+struct Job {
+    int64_t id;
+    pat_job d;
+    int64_t est;
+    double t_submit;
+    int attempts = 0;
+};
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct pat_serve {
+    pat::Arena *arena = nullptr;
+    int64_t budget = 0;
+    int slots = 0;
+    int flags = 0;
+    int device = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job> pending;
+    int64_t reserved = 0;
+    int running = 0;
+    int64_t next_id = 1;
+    int bypass = 0;
+    bool stopping = false;
+    std::map<int64_t, pat_job_result> done;
+    int64_t submitted = 0, finished = 0;
+    std::vector<std::thread> workers;
+
+    // Device bytes a job will hold (its buffers, the handle's work space).
+    static int64_t estimate(const pat_job &j)
+    {
+        const int64_t MB = 1 << 20;
+        switch (j.kind) {
+        case PAT_JOB_HEAT3D: {
+            const int64_t s = j.size + 2;
+            return 2 * s * s * s * 4 + MB;
+        }
+        case PAT_JOB_LJ: {
+            const int64_t n = 4 * j.size * j.size * j.size;
+            return 1600 * n + 2 * MB;
+        }
+        case PAT_JOB_KMEANS: {
+            const int64_t kd = (int64_t)j.k * j.dim;
+            return j.size * j.dim * 4 + 8 * j.size + 40 * kd + 64 * (int64_t)j.k + MB;
+        }
+        default: return 0;
+        }
+    }
+
+    // Oldest eligible job that fits, under mu.
+    bool pick(Job *out)
+    {
+        for (size_t i = 0; i < pending.size(); i++) {
+            const Job &j = pending[i];
+            const bool fits = j.attempts > 0 ? running == 0 : (reserved + j.est <= budget || running == 0);
+            if (fits) {
+                *out = j;
+                pending.erase(pending.begin() + (long)i);
+                bypass = i > 0 ? bypass + 1 : 0;
+                return true;
+            }
+            if (bypass >= kMaxBypass) return false;  // the oldest job waits for room
+        }
+        return false;
+    }
+
+    int run_heat(const pat_job &j, hipStream_t st, double *part, pat_job_result *r)
+    {
+        const int64_t n = j.size, cells = (n + 2) * (n + 2) * (n + 2);
+        float *a = nullptr, *b = nullptr;
+        int rc = PAT_OK;
+        if (pat::dev_malloc(&a, sizeof(float) * cells) != hipSuccess ||
+            pat::dev_malloc(&b, sizeof(float) * cells) != hipSuccess) {
+            pat::set_error("serve: heat3D job %lld: no device memory for 2 x %lld cells", (long long)j.size,
+                           (long long)cells);
+            rc = PAT_ERR_NOMEM;
+        }
+        int in_b = 0;
+        if (rc == PAT_OK) rc = pat_gen_heat3d_dev(n, n, n, j.seed, a, st);
+        if (rc == PAT_OK && hipMemsetAsync(b, 0, sizeof(float) * cells, st) != hipSuccess) rc = PAT_ERR_HIP;
+        if (rc == PAT_OK)
+            rc = pat_heat3d_run_dev(a, b, n, n, n, j.steps, kHeatAlpha, PAT_STENCIL_AUTO, 0, 0, 0, &in_b, st);
+        const float *res = in_b ? b : a;
+        if (rc == PAT_OK) rc = pat::sum_f32_dev(res, cells, part, &r->checksum, st);
+        if (rc == PAT_OK && j.out) {
+            if (j.out_len < n * n * n) {
+                pat::set_error("serve: heat3D job needs %lld output floats", (long long)(n * n * n));
+                rc = PAT_ERR_INVALID;
+            } else {
+                std::vector<float> h((size_t)cells);
+                if (hipMemcpyAsync(h.data(), res, sizeof(float) * cells, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess) {
+                    rc = PAT_ERR_HIP;
+                } else {
+                    const int64_t s = n + 2;
+                    for (int64_t z = 0; z < n; z++)
+                        for (int64_t y = 0; y < n; y++)
+                            memcpy(j.out + (z * n + y) * n, h.data() + ((z + 1) * s + (y + 1)) * s + 1,
+                                   sizeof(float) * n);
+                }
+            }
+        }
+        (void)hipStreamSynchronize(st);
+        (void)pat::dev_free(a);
+        (void)pat::dev_free(b);
+        return rc;
+    }
+
+    int run_lj(const pat_job &j, hipStream_t st, double *part, pat_job_result *r)
+    {
+        const int64_t c = j.size, n = 4 * c * c * c;
+        double box[3];
+        float *pos4 = nullptr;
+        int rc = PAT_OK;
+        if (pat::dev_malloc(&pos4, sizeof(float) * 4 * n) != hipSuccess) {
+            pat::set_error("serve: LJ job: no device memory for %lld atoms", (long long)n);
+            return PAT_ERR_NOMEM;
+        }
+        rc = pat_gen_fcc_dev(c, c, c, kRho, kJitter, j.seed, pos4, box, st);
+        if (rc == PAT_OK && hipStreamSynchronize(st) != hipSuccess) rc = PAT_ERR_HIP;
+        pat_md *md = nullptr;
+        if (rc == PAT_OK) {
+            if (box[0] >= 3 * (kCut + kSkin) * 1.0001) {
+                // binned neighbour CSR rebuilt on the device every kEvery steps
+                rc = pat_md_create_binned_dev(n, pos4, nullptr, box[0], kCut, kSkin, kEvery, kDt, &md);
+            } else {
+                // box too small for three bins: a fixed list within the cutoff
+                std::vector<float> p4((size_t)n * 4), p3((size_t)n * 3);
+                if (hipMemcpyAsync(p4.data(), pos4, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)
+                    rc = PAT_ERR_HIP;
+                for (int64_t i = 0; i < n && rc == PAT_OK; i++)
+                    for (int q = 0; q < 3; q++) p3[3 * i + q] = p4[4 * i + q];
+                int64_t ne = 0;
+                if (rc == PAT_OK) rc = pat_gen_pairs(n, p3.data(), box[0], kCut, 0, 0, nullptr, &ne);
+                std::vector<int32_t> e((size_t)std::max<int64_t>(2 * ne, 2));
+                if (rc == PAT_OK) rc = pat_gen_pairs(n, p3.data(), box[0], kCut, 0, 0, e.data(), &ne);
+                if (rc == PAT_OK) rc = pat_md_create(n, p3.data(), nullptr, ne, e.data(), box[0], 1, kCut, kDt, &md);
+            }
+        }
+        if (rc == PAT_OK) {
+            int v = 0, l = 0, o = 0;
+            if (pat_md_get_variant(md, &v, &l, &o) == PAT_OK) r->kernel = l;
+        }
+        if (rc == PAT_OK) rc = pat_md_run(md, j.steps);
+        if (rc == PAT_OK) {
+            void *p = nullptr;
+            rc = pat_md_device_buffers(md, &p, nullptr, nullptr);
+            if (rc == PAT_OK) rc = pat::sum_f32_dev((const float *)p, 4 * n, part, &r->checksum, st);
+        }
+        if (rc == PAT_OK && j.out) {
+            if (j.out_len < 3 * n) {
+                pat::set_error("serve: LJ job needs %lld output floats", (long long)(3 * n));
+                rc = PAT_ERR_INVALID;
+            } else {
+                rc = pat_md_get(md, j.out, nullptr, nullptr);
+            }
+        }
+        if (md) (void)pat_md_destroy(md);
+        (void)pat::dev_free(pos4);
+        return rc;
+    }
+
+    int run_kmeans(const pat_job &j, hipStream_t st, pat_job_result *r)
+    {
+        float *x = nullptr;
+        if (pat::dev_malloc(&x, sizeof(float) * j.size * j.dim) != hipSuccess) {
+            pat::set_error("serve: kmeans job: no device memory for %lld x %d points", (long long)j.size, j.dim);
+            return PAT_ERR_NOMEM;
+        }
+        int rc = pat_gen_gmm_dev(j.size, j.dim, j.k, 1.0, -10.0, 10.0, j.seed, x, st);
+        if (rc == PAT_OK && hipStreamSynchronize(st) != hipSuccess) rc = PAT_ERR_HIP;
+        const int mode = (!(flags & PAT_SERVE_PLAIN) && pat::kmeans_mfma_fits(j.k, j.dim)) ? PAT_KM_MFMA : PAT_KM_PLAIN;
+        r->kernel = mode;
+        pat_kmeans *km = nullptr;
+        if (rc == PAT_OK) rc = pat_kmeans_create(x, j.size, j.dim, j.k, mode, 1, &km);
+        float tot = 0.f;
+        if (rc == PAT_OK) rc = pat_kmeans_run_timed(km, j.steps, &tot, nullptr);
+        std::vector<float> c((size_t)j.k * j.dim);
+        if (rc == PAT_OK) rc = pat_kmeans_get(km, c.data(), nullptr, nullptr);
+        if (rc == PAT_OK) {
+            double s = 0.0;
+            for (float v : c) s += (double)v;
+            r->checksum = s;
+            if (j.out) {
+                if (j.out_len < (int64_t)c.size()) {
+                    pat::set_error("serve: kmeans job needs %lld output floats", (long long)c.size());
+                    rc = PAT_ERR_INVALID;
+                } else {
+                    memcpy(j.out, c.data(), sizeof(float) * c.size());
+                }
+            }
+        }
+        if (km) (void)pat_kmeans_destroy(km);
+        (void)pat::dev_free(x);
+        return rc;
+    }
+
+    void worker()
+    {
+        (void)hipSetDevice(device);
+        pat::arena_bind(arena);
+        pat::set_thread_tuning(false);
+        hipStream_t st = nullptr;
+        double *part = nullptr;
+        const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                        pat::dev_malloc(&part, sizeof(double) * pat::sum_blocks()) == hipSuccess;
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv_work.wait(g, [&] { return pick_ready() || (stopping && pending.empty()); });
+                if (!pick(&j)) break;  // stopping with nothing left
+                reserved += j.est;
+                running++;
+            }
+            pat_job_result r;
+            memset(&r, 0, sizeof(r));
+            r.t_submit_ms = j.t_submit;
+            r.t_start_ms = ms_since(t0);
+            r.device_bytes = j.est;
+            int rc = PAT_ERR_STATE;
+            if (ok) {
+                switch (j.d.kind) {
+                case PAT_JOB_HEAT3D: rc = run_heat(j.d, st, part, &r); break;
+                case PAT_JOB_LJ: rc = run_lj(j.d, st, part, &r); break;
+                case PAT_JOB_KMEANS: rc = run_kmeans(j.d, st, &r); break;
+                default: rc = PAT_ERR_INVALID;
+                }
+            }
+            r.t_end_ms = ms_since(t0);
+            r.status = rc;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                reserved -= j.est;
+                running--;
+                if (rc == PAT_ERR_NOMEM && j.attempts == 0 && running > 0) {
+                    j.attempts = 1;  // run again once it can have the arena alone
+                    pending.push_front(j);
+                } else {
+                    done[j.id] = r;
+                    finished++;
+                    cv_done.notify_all();
+                }
+            }
+            cv_work.notify_all();
+        }
+        if (part) (void)pat::dev_free(part);
+        if (st) (void)hipStreamDestroy(st);
+        pat::arena_bind(nullptr);
+    }
+
+    // Whether pick() would succeed now (for the condition variable), under mu.
+    bool pick_ready()
+    {
+        for (size_t i = 0; i < pending.size(); i++) {
+            const Job &j = pending[i];
+            if (j.attempts > 0 ? running == 0 : (reserved + j.est <= budget || running == 0)) return true;
+            if (bypass >= kMaxBypass) return false;
+        }
+        return false;
+    }
+};
+
+extern "C" {
+
+int pat_serve_start(int64_t hbm_budget, int slots, int flags, pat_serve **out)
+{
+    PAT_REQUIRE(out && hbm_budget > 0 && slots > 0 && slots <= 64, PAT_ERR_INVALID,
+                "pat_serve_start: need a positive budget and 1..64 slots");
+    auto *s = new pat_serve();
+    s->budget = hbm_budget;
+    s->slots = slots;
+    s->flags = flags;
+    s->device = pat::current_device();
+    PAT_HIP(hipSetDevice(s->device));
+    PAT_TRY(pat::kmeans_prepare_kernels());  // attribute calls stay off the workers' path
+    s->arena = pat::arena_create((size_t)hbm_budget);
+    if (!s->arena) {
+        delete s;
+        return PAT_ERR_NOMEM;
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < slots; i++) s->workers.emplace_back([s] { s->worker(); });
+    *out = s;
+    return PAT_OK;
+}
+
+int pat_serve_submit(pat_serve *s, const pat_job *job, int64_t *id)
+{
+    PAT_REQUIRE(s && job, PAT_ERR_INVALID, "pat_serve_submit: null argument");
+    const pat_job &j = *job;
+    switch (j.kind) {
+    case PAT_JOB_HEAT3D:
+        PAT_REQUIRE(j.size >= 1 && j.size <= 2048 && j.steps >= 0, PAT_ERR_INVALID,
+                    "pat_serve_submit: heat3D side %lld / steps %d out of range", (long long)j.size, j.steps);
+        break;
+    case PAT_JOB_LJ:
+        PAT_REQUIRE(j.size >= 2 && j.size <= 400 && j.steps >= 0, PAT_ERR_INVALID,
+                    "pat_serve_submit: LJ cells %lld / steps %d out of range", (long long)j.size, j.steps);
+        break;
+    case PAT_JOB_KMEANS:
+        PAT_REQUIRE(j.dim >= 1 && j.dim <= 128 && j.k >= 1 && j.k <= j.size && j.steps >= 1, PAT_ERR_INVALID,
+                    "pat_serve_submit: kmeans N=%lld D=%d K=%d iters=%d out of range", (long long)j.size, j.dim, j.k,
+                    j.steps);
+        break;
+    default: PAT_REQUIRE(false, PAT_ERR_INVALID, "pat_serve_submit: unknown job kind %d", j.kind);
+    }
+    Job q;
+    q.d = j;
+    q.est = pat_serve::estimate(j);
+    PAT_REQUIRE(q.est <= s->budget, PAT_ERR_NOMEM, "pat_serve_submit: job needs about %lld bytes, budget %lld",
+                (long long)q.est, (long long)s->budget);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        PAT_REQUIRE(!s->stopping, PAT_ERR_STATE, "pat_serve_submit: the loop is stopping");
+        q.id = s->next_id++;
+        q.t_submit = ms_since(s->t0);
+        s->pending.push_back(q);
+        s->submitted++;
+    }
+    s->cv_work.notify_one();
+    if (id) *id = q.id;
+    return PAT_OK;
+}
+
+int pat_serve_poll(pat_serve *s, int64_t id, int *done, pat_job_result *res)
+{
+    PAT_REQUIRE(s && done, PAT_ERR_INVALID, "pat_serve_poll: null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    PAT_REQUIRE(id > 0 && id < s->next_id, PAT_ERR_INVALID, "pat_serve_poll: unknown job %lld", (long long)id);
+    auto it = s->done.find(id);
+    *done = it != s->done.end();
+    if (*done && res) *res = it->second;
+    return PAT_OK;
+}
+
+int pat_serve_wait(pat_serve *s, int64_t id, pat_job_result *res)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_serve_wait: null handle");
+    std::unique_lock<std::mutex> g(s->mu);
+    PAT_REQUIRE(id > 0 && id < s->next_id, PAT_ERR_INVALID, "pat_serve_wait: unknown job %lld", (long long)id);
+    s->cv_done.wait(g, [&] { return s->done.count(id) > 0; });
+    if (res) *res = s->done[id];
+    return PAT_OK;
+}
+
+int pat_serve_drain(pat_serve *s)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_serve_drain: null handle");
+    std::unique_lock<std::mutex> g(s->mu);
+    s->cv_done.wait(g, [&] { return s->finished == s->submitted; });
+    return PAT_OK;
+}
+
+int pat_serve_stats(pat_serve *s, int64_t *submitted, int64_t *finished, int64_t *queued, int64_t *reserved_bytes,
+                    int64_t *arena_in_use)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_serve_stats: null handle");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (submitted) *submitted = s->submitted;
+    if (finished) *finished = s->finished;
+    if (queued) *queued = (int64_t)s->pending.size();
+    if (reserved_bytes) *reserved_bytes = s->reserved;
+    if (arena_in_use) *arena_in_use = (int64_t)pat::arena_in_use(s->arena);
+    return PAT_OK;
+}
+
+int pat_serve_stop(pat_serve *s)
+{
+    if (!s) return PAT_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    (void)hipDeviceSynchronize();
+    pat::arena_destroy(s->arena);
+    delete s;
+    return PAT_OK;
+}
+
+}  // extern "C"

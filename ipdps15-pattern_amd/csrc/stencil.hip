@@ -243,7 +243,7 @@ int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64
         }
     }
     default_tile(nx, ny, nz, bx, by, ry, zc);
-    if (getenv("PAT_NO_TUNE")) return PAT_OK;
+    if (!pat::tuning_enabled()) return PAT_OK;
     hipEvent_t a, b;
     PAT_HIP(hipEventCreate(&a));
     PAT_HIP(hipEventCreate(&b));

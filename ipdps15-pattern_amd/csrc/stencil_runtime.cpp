@@ -102,16 +102,16 @@ int pat_stencil_create(int ndims, const int64_t *interior, int width, int unit_s
     if (param_size > 0) s->param.assign((const unsigned char *)param, (const unsigned char *)param + param_size);
     const size_t bytes = sizeof(float) * (size_t)s->cells();
     for (int b = 0; b < 2; b++) {
-        if (hipMalloc(&s->buf[b], bytes) != hipSuccess) {
-            if (s->buf[0]) (void)hipFree(s->buf[0]);
+        if (pat::dev_malloc(&s->buf[b], bytes) != hipSuccess) {
+            if (s->buf[0]) (void)pat::dev_free(s->buf[0]);
             delete s;
             pat::set_error("pat_stencil_create: hipMalloc of %zu bytes failed", bytes);
             return PAT_ERR_NOMEM;
         }
     }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipFree(s->buf[0]);
-        (void)hipFree(s->buf[1]);
+        (void)pat::dev_free(s->buf[0]);
+        (void)pat::dev_free(s->buf[1]);
         delete s;
         pat::set_error("pat_stencil_create: stream creation failed");
         return PAT_ERR_HIP;
@@ -265,8 +265,8 @@ int pat_stencil_destroy(pat_stencil *s)
     if (!s) return PAT_OK;
     (void)hipStreamSynchronize(s->stream);
     s->drop_graphs();
-    (void)hipFree(s->buf[0]);
-    (void)hipFree(s->buf[1]);
+    (void)pat::dev_free(s->buf[0]);
+    (void)pat::dev_free(s->buf[1]);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return PAT_OK;

@@ -27,6 +27,10 @@ PAT_IR_ORDER_EDGE = 0
 PAT_IR_ORDER_NODE = 1
 PAT_KM_PLAIN = 0
 PAT_KM_MFMA = 1
+PAT_JOB_HEAT3D = 0
+PAT_JOB_LJ = 1
+PAT_JOB_KMEANS = 2
+PAT_SERVE_PLAIN = 1
 
 _lib = None
 
@@ -41,6 +45,21 @@ c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_fp = ctypes.POINTER(ctypes.c_float)
 c_dp = ctypes.POINTER(ctypes.c_double)
 c_intp = ctypes.POINTER(ctypes.c_int)
+
+
+class Job(ctypes.Structure):
+    """struct pat_job."""
+    _fields_ = [("kind", ctypes.c_int), ("size", ctypes.c_int64), ("dim", ctypes.c_int), ("k", ctypes.c_int),
+                ("steps", ctypes.c_int), ("seed", ctypes.c_uint64), ("out", ctypes.c_void_p),
+                ("out_len", ctypes.c_int64)]
+
+
+class JobResult(ctypes.Structure):
+    """struct pat_job_result."""
+    _fields_ = [("status", ctypes.c_int), ("kernel", ctypes.c_int), ("checksum", ctypes.c_double),
+                ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double), ("t_end_ms", ctypes.c_double),
+                ("device_bytes", ctypes.c_int64)]
+
 
 # name -> (restype, argtypes) for every symbol include/pattern.h declares
 SIGNATURES = {
@@ -68,6 +87,16 @@ SIGNATURES = {
     "pat_gen_velocities": (c_int, [c_i64, c_double, c_u64, c_vp]),
     "pat_gen_pairs": (c_int, [c_i64, c_vp, c_double, c_double, c_int, c_u64, c_vp, c_i64p]),
     "pat_gen_gmm": (c_int, [c_i64, c_int, c_int, c_double, c_double, c_double, c_u64, c_vp]),
+    "pat_gen_heat3d_dev": (c_int, [c_i64, c_i64, c_i64, c_u64, c_vp, c_vp]),
+    "pat_gen_fcc_dev": (c_int, [c_i64, c_i64, c_i64, c_double, c_double, c_u64, c_vp, c_dp, c_vp]),
+    "pat_gen_gmm_dev": (c_int, [c_i64, c_int, c_int, c_double, c_double, c_double, c_u64, c_vp, c_vp]),
+    "pat_serve_start": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_vp)]),
+    "pat_serve_submit": (c_int, [c_vp, ctypes.POINTER(Job), c_i64p]),
+    "pat_serve_poll": (c_int, [c_vp, c_i64, c_intp, ctypes.POINTER(JobResult)]),
+    "pat_serve_wait": (c_int, [c_vp, c_i64, ctypes.POINTER(JobResult)]),
+    "pat_serve_drain": (c_int, [c_vp]),
+    "pat_serve_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "pat_serve_stop": (c_int, [c_vp]),
     "pat_heat3d_run_dev": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_int, c_float, c_int, c_int, c_int, c_int,
                                    c_intp, c_vp]),
     "pat_stencil_create": (c_int, [c_int, c_i64p, c_int, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
@@ -97,6 +126,8 @@ SIGNATURES = {
                               ctypes.POINTER(c_vp)]),
     "pat_md_create_binned": (c_int, [c_i64, c_vp, c_vp, c_double, c_double, c_double, c_int, c_double,
                                      ctypes.POINTER(c_vp)]),
+    "pat_md_create_binned_dev": (c_int, [c_i64, c_vp, c_vp, c_double, c_double, c_double, c_int, c_double,
+                                         c_vp]),
     "pat_md_stats": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_fp]),
     "pat_md_csr": (c_int, [c_vp, c_vp, c_vp]),
     "pat_md_set_variant": (c_int, [c_vp, c_int, c_int]),
@@ -690,3 +721,98 @@ def kmeans_shifts(n, d, maxabs):
     """(coordinate shift, distance shift) the handle derives from max |x|."""
     return (kmeans_fixed_shift(n, maxabs),
             kmeans_fixed_shift(n, float(np.float32(2.0 * float(maxabs) * np.sqrt(float(d))))))
+
+
+# ---------------------------------------------------------------- device generators
+
+
+def gen_heat3d_dev(nx, ny, nz, seed=42):
+    """Halo-inclusive grid generated on the device (DeviceArray)."""
+    g = DeviceArray((nz + 2, ny + 2, nx + 2), np.float32)
+    call("pat_gen_heat3d_dev", nx, ny, nz, seed, g.ptr, None)
+    sync()
+    return g
+
+
+def gen_fcc_dev(ncx, ncy, ncz, rho=0.8442, jitter=0.0, seed=1):
+    """(float4 positions DeviceArray, box) generated on the device."""
+    n = 4 * ncx * ncy * ncz
+    p = DeviceArray((n, 4), np.float32)
+    box = np.zeros(3)
+    call("pat_gen_fcc_dev", ncx, ncy, ncz, rho, jitter, seed, p.ptr, box.ctypes.data_as(c_dp), None)
+    sync()
+    return p, box
+
+
+def gen_gmm_dev(n, d, ncomp, sigma=1.0, lo=-10.0, hi=10.0, seed=1):
+    x = DeviceArray((n, d), np.float32)
+    call("pat_gen_gmm_dev", n, d, ncomp, sigma, lo, hi, seed, x.ptr, None)
+    sync()
+    return x
+
+
+# ---------------------------------------------------------------- serving
+
+
+def job(kind, size, steps, seed, dim=0, k=0, out=None):
+    """A pat_job; out (float32 numpy array) receives the full result."""
+    j = Job(kind=kind, size=size, dim=dim, k=k, steps=steps, seed=seed)
+    if out is not None:
+        j.out = out.ctypes.data
+        j.out_len = out.size
+    return j
+
+
+def result_dict(r):
+    return {"status": r.status, "kernel": r.kernel, "checksum": r.checksum, "submit_ms": r.t_submit_ms,
+            "start_ms": r.t_start_ms, "end_ms": r.t_end_ms, "latency_ms": r.t_end_ms - r.t_submit_ms,
+            "device_bytes": r.device_bytes}
+
+
+class Serve:
+    """The serving loop (pat_serve_*): jobs admitted by device-memory budget
+    and run concurrently on 'slots' streams."""
+
+    def __init__(self, budget_bytes, slots=8, plain=False):
+        h = c_vp()
+        call("pat_serve_start", int(budget_bytes), slots, PAT_SERVE_PLAIN if plain else 0, ctypes.byref(h))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, j, keep=None):
+        i = c_i64(0)
+        call("pat_serve_submit", self.h, ctypes.byref(j), ctypes.byref(i))
+        if keep is not None:
+            self._keep[i.value] = keep  # output buffer stays alive until the job is done
+        return i.value
+
+    def wait(self, job_id):
+        r = JobResult()
+        call("pat_serve_wait", self.h, job_id, ctypes.byref(r))
+        self._keep.pop(job_id, None)
+        return result_dict(r)
+
+    def poll(self, job_id):
+        d, r = c_int(0), JobResult()
+        call("pat_serve_poll", self.h, job_id, ctypes.byref(d), ctypes.byref(r))
+        return result_dict(r) if d.value else None
+
+    def drain(self):
+        call("pat_serve_drain", self.h)
+
+    def stats(self):
+        v = [c_i64(0) for _ in range(5)]
+        call("pat_serve_stats", self.h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("submitted", "finished", "queued", "reserved", "arena_in_use"), [x.value for x in v]))
+
+    def close(self):
+        if self.h:
+            call("pat_serve_stop", self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

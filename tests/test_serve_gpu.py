@@ -1,0 +1,132 @@
+"""The serving loop and the device generators it uses.
+
+Bar: the device generators equal the host ones bit for bit (heat3D grid,
+FCC lattice; the mixture points, which use the device's FP64 log/cos, within
+1e-6 relative); every job's result is bit-identical alone and in a mix of
+concurrent jobs, and equal to the standalone handle path on the same input;
+admission never lets the jobs in flight exceed the memory budget.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MB = 1 << 20
+
+
+def test_device_generators_match_host(pa, gpu):
+    g = pa.gen_heat3d_dev(40, 33, 21, seed=7).numpy()
+    assert np.array_equal(g, pa.gen_heat3d(40, 33, 21, seed=7))
+    p4, box = pa.gen_fcc_dev(6, 5, 7, rho=0.8442, jitter=0.05, seed=3)
+    pos, hbox = pa.gen_fcc(6, 5, 7, rho=0.8442, jitter=0.05, seed=3)
+    assert np.array_equal(box, hbox)
+    p4 = p4.numpy()
+    assert np.array_equal(p4[:, :3], pos) and not p4[:, 3].any()
+    x = pa.gen_gmm_dev(5000, 24, 17, sigma=1.0, lo=-10, hi=10, seed=9).numpy()
+    hx = pa.gen_gmm(5000, 24, 17, sigma=1.0, lo=-10, hi=10, seed=9)
+    same = np.mean(x == hx)
+    print(f"gmm device vs host: {same:.4%} of values bit-identical")
+    assert np.max(np.abs(x - hx) / np.maximum(np.abs(hx), 1.0)) < 1e-6
+
+
+def mixed_jobs(pa):
+    """Small jobs of every kind, each with its own output buffer."""
+    specs = [(pa.PAT_JOB_HEAT3D, 20, 7, 0, 0), (pa.PAT_JOB_HEAT3D, 37, 12, 0, 0), (pa.PAT_JOB_HEAT3D, 64, 5, 0, 0),
+             (pa.PAT_JOB_LJ, 4, 10, 0, 0), (pa.PAT_JOB_LJ, 6, 25, 0, 0), (pa.PAT_JOB_LJ, 9, 21, 0, 0),
+             (pa.PAT_JOB_KMEANS, 3000, 4, 16, 20), (pa.PAT_JOB_KMEANS, 12000, 3, 32, 256),
+             (pa.PAT_JOB_KMEANS, 5000, 3, 128, 40), (pa.PAT_JOB_KMEANS, 4000, 2, 64, 300),
+             (pa.PAT_JOB_HEAT3D, 48, 9, 0, 0), (pa.PAT_JOB_LJ, 7, 3, 0, 0)]
+    out = []
+    for i, (kind, size, steps, dim, k) in enumerate(specs):
+        n = {pa.PAT_JOB_HEAT3D: size ** 3, pa.PAT_JOB_LJ: 4 * size ** 3 * 3,
+             pa.PAT_JOB_KMEANS: dim * k}[kind]
+        out.append((kind, size, steps, dim, k, 1000 + i, n))
+    return out
+
+
+def run_jobs(pa, specs, slots, together, plain=False):
+    s = pa.Serve(2 << 30, slots=slots, plain=plain)
+    bufs, ids, res = [], [], []
+    for kind, size, steps, dim, k, seed, n in specs:
+        b = np.zeros(n, dtype=np.float32)
+        bufs.append(b)
+        i = s.submit(pa.job(kind, size, steps, seed, dim=dim, k=k, out=b), keep=b)
+        if not together:
+            res.append(s.wait(i))
+        ids.append(i)
+    if together:
+        res = [s.wait(i) for i in ids]
+    s.close()
+    return bufs, res
+
+
+def test_serve_alone_equals_mix(pa, gpu):
+    specs = mixed_jobs(pa)
+    alone, ra = run_jobs(pa, specs, slots=1, together=False)
+    mix, rm = run_jobs(pa, specs, slots=8, together=True)
+    for spec, a, m, x, y in zip(specs, alone, mix, ra, rm):
+        assert x["status"] == 0 and y["status"] == 0, (spec, x, y)
+        assert np.array_equal(a, m), spec
+        assert x["checksum"] == y["checksum"] and x["kernel"] == y["kernel"]
+        assert np.all(np.isfinite(a)) and np.any(a != 0)
+    # the mix really overlapped on the device
+    starts = np.array([r["start_ms"] for r in rm])
+    ends = np.array([r["end_ms"] for r in rm])
+    overlap = max(np.sum((starts <= t) & (ends > t)) for t in starts)
+    print(f"mix: up to {overlap} jobs in flight; alone total {sum(r['end_ms'] - r['start_ms'] for r in ra):.1f} ms, "
+          f"mix span {ends.max() - starts.min():.1f} ms")
+    assert overlap > 1
+
+
+def test_serve_equals_handles(pa, gpu):
+    """Each job's result equals the standalone handle path on the same input."""
+    s = pa.Serve(1 << 30, slots=4)
+    # heat3D: 30^3, 9 steps
+    h = np.zeros(30 ** 3, dtype=np.float32)
+    ih = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 30, 9, 77, out=h), keep=h)
+    # LJ: 7^3 cells, 23 steps (a rebuild at step 20)
+    p = np.zeros(4 * 7 ** 3 * 3, dtype=np.float32)
+    il = s.submit(pa.job(pa.PAT_JOB_LJ, 7, 23, 78, out=p), keep=p)
+    # kmeans: 20000 x 32, K = 64, 5 iterations
+    c = np.zeros(64 * 32, dtype=np.float32)
+    ik = s.submit(pa.job(pa.PAT_JOB_KMEANS, 20000, 5, 79, dim=32, k=64, out=c), keep=c)
+    rh, rl, rk = s.wait(ih), s.wait(il), s.wait(ik)
+    s.close()
+    assert rh["status"] == rl["status"] == rk["status"] == 0
+
+    st = pa.Stencil(30, 30, 30, alpha=0.1)
+    st.upload(pa.gen_heat3d(30, 30, 30, seed=77))
+    st.run(9)
+    assert np.array_equal(h.reshape(30, 30, 30), st.download()[1:-1, 1:-1, 1:-1])
+    st.close()
+
+    pos, box = pa.gen_fcc(7, 7, 7, rho=0.8442, jitter=0.05, seed=78)
+    md = pa.MD(pos, None, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    md.run(23)
+    assert np.array_equal(p.reshape(-1, 3), md.get()[0])
+    md.close()
+
+    x = pa.gen_gmm_dev(20000, 32, 64, seed=79)
+    km = pa.KMeans(x, 64, mode=pa.PAT_KM_MFMA)
+    km.run(5)
+    assert np.array_equal(c.reshape(64, 32), km.centroids())
+    km.close()
+
+
+def test_serve_admission_respects_budget(pa, gpu):
+    # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
+    s = pa.Serve(100 * MB, slots=4)
+    ids = [s.submit(pa.job(pa.PAT_JOB_HEAT3D, 200, 3, 500 + i)) for i in range(5)]
+    small = [s.submit(pa.job(pa.PAT_JOB_HEAT3D, 16, 3, 600 + i)) for i in range(6)]
+    res = [s.wait(i) for i in ids + small]
+    with pytest.raises(RuntimeError, match="budget"):
+        s.submit(pa.job(pa.PAT_JOB_HEAT3D, 300, 1, 1))
+    s.close()
+    assert all(r["status"] == 0 for r in res)
+    ev = sorted([(r["start_ms"], r["device_bytes"]) for r in res] + [(r["end_ms"], -r["device_bytes"]) for r in res],
+                key=lambda t: (t[0], t[1]))
+    held = peak = 0
+    for _, b in ev:
+        held += b
+        peak = max(peak, held)
+    assert peak <= 100 * MB
