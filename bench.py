@@ -173,14 +173,102 @@ def bench_kmeans(iters=20):
     return out
 
 
+def serving_mix(n, seed):
+    """The serving mix: one third each of kmeans (N log-uniform 10k-4M,
+    D in {8, 16, 32, 64, 128}, K log-uniform 16-1024, 10 iterations),
+    heat3D (cube side uniform 32-384, 50 steps) and LJ (FCC cells per side
+    uniform 4-48, 20 steps), in a seeded random order."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.permutation(np.arange(n) % 3)
+    jobs = []
+    for i, kind in enumerate(kinds):
+        if kind == 0:
+            npts = int(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
+            d = int(rng.choice([8, 16, 32, 64, 128]))
+            k = int(np.exp(rng.uniform(np.log(16), np.log(1024))))
+            jobs.append((pa.PAT_JOB_KMEANS, npts, 10, d, k))
+        elif kind == 1:
+            jobs.append((pa.PAT_JOB_HEAT3D, int(rng.integers(32, 385)), 50, 0, 0))
+        else:
+            jobs.append((pa.PAT_JOB_LJ, int(rng.integers(4, 49)), 20, 0, 0))
+    return jobs
+
+
+def run_serving(jobs, slots, plain, rate=None, seed=0, budget=200 << 30):
+    """Submits the jobs (all at once, or as a Poisson stream at 'rate' jobs/s)
+    and waits for them; returns latencies (ms, arrival to result on the host)
+    and the wall time."""
+    s = pa.Serve(budget, slots=slots, plain=plain)
+    arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, len(jobs))) if rate else None
+    ids = []
+    t0 = time.perf_counter()
+    for i, (kind, size, steps, d, k) in enumerate(jobs):
+        if arrivals is not None:
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+        ids.append(s.submit(pa.job(kind, size, steps, 7000 + i, dim=d, k=k)))
+    res = [s.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    s.close()
+    bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} serving jobs failed, first status {bad[0]['status']}")
+    return np.array([r["latency_ms"] for r in res]), wall
+
+
+def bench_serving(njobs, slots=8):
+    jobs = serving_mix(njobs, seed=5)
+    _, sat_wall = run_serving(jobs, slots, False)  # saturated: capacity of the loop
+    capacity = njobs / sat_wall
+    rate = 0.7 * capacity
+    lat, wall = run_serving(jobs, slots, False, rate=rate, seed=6)
+    # one job at a time on the plain kernels, same arrivals
+    plat, pwall = run_serving(jobs, 1, True, rate=rate, seed=6)
+    _, seq_wall = run_serving(jobs, 1, True)
+    out = {"jobs": njobs, "slots": slots, "mix": "1/3 kmeans (N 10k-4M log, D 8-128, K 16-1024 log, 10 it), "
+                                                 "1/3 heat3D (side 32-384, 50 steps), 1/3 LJ (4^3-48^3 cells, 20 steps)",
+           "capacity_jobs_per_s": capacity, "arrival_rate_jobs_per_s": rate,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "jobs_per_s": njobs / wall,
+           "plain_one_at_a_time": {"p50_ms": float(np.percentile(plat, 50)), "p99_ms": float(np.percentile(plat, 99)),
+                                   "jobs_per_s": njobs / pwall, "saturated_jobs_per_s": njobs / seq_wall},
+           "speedup_vs_sequential_plain": capacity / (njobs / seq_wall)}
+    log(f"[serve] {njobs} jobs, {slots} slots: capacity {capacity:.1f} jobs/s; at {rate:.1f} jobs/s p50 "
+        f"{out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms; plain one-at-a-time p50 "
+        f"{out['plain_one_at_a_time']['p50_ms']:.1f} ms p99 {out['plain_one_at_a_time']['p99_ms']:.1f} ms, "
+        f"sequential {njobs / seq_wall:.1f} jobs/s")
+    return out
+
+
+def bench_copy(mib=512, reps=5):
+    """A plain device-to-device copy of known size: the reference point that
+    scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the kernels."""
+    n = mib << 20
+    a = pa.DeviceArray((n,), np.uint8)
+    b = pa.DeviceArray((n,), np.uint8)
+    pa.call("pat_memset_dev", a.ptr, 1, n, None)
+    pa.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pa.call("pat_memcpy_d2d", b.ptr, a.ptr, n, None)
+    pa.sync()
+    dt = (time.perf_counter() - t0) / reps
+    a.free()
+    b.free()
+    log(f"[copy] {mib} MiB device copy: {2 * n / dt / 1e9:.0f} GB/s (read + write)")
+    return {"bytes": n, "gbs": 2 * n / dt / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--engines", default="lj,stencil,kmeans",
-                    help="comma-separated subset of lj,stencil,kmeans (lj carries value)")
+    ap.add_argument("--engines", default="lj,stencil,kmeans,serve",
+                    help="comma-separated subset of lj,stencil,kmeans,serve,copy (lj carries value)")
+    ap.add_argument("--serve-jobs", type=int, default=600, help="jobs in the serving mix (the config's is 10000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every engine here runs its whole domain on a single MI355X, and no "
@@ -190,6 +278,10 @@ def main():
 
     which = set(args.engines.split(","))
     engines = {}
+    if "copy" in which:
+        engines["copy"] = bench_copy()
+    if "serve" in which:
+        engines["serving"] = bench_serving(args.serve_jobs)
     if "kmeans" in which:
         engines["kmeans"] = bench_kmeans()
     if "stencil" in which:

@@ -204,6 +204,11 @@ int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_no
  * every choice); -4 ... -64: tree-order sums over that many lanes. */
 int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
                          int pbc, float cutoff, int lanes_per_row, float *force4, void *stream);
+/* Compares, for every FP32 value in [lo, hi], the IEEE quotient 1.0f / b
+ * with the hardware reciprocal refined by one and by two Newton steps, and
+ * returns how many results differ for each (the check behind the LJ
+ * kernels' reciprocal; no counterpart in the reference). */
+int pat_lj_rcp_selftest(float lo, float hi, int64_t *bad_one_step, int64_t *bad_two_step);
 /* The same forces from the half edge list with float atomics (zeroes force4
  * first). */
 int pat_lj_force_atomic_dev(const float *pos4, const int32_t *edges, int64_t num_edges, int64_t num_nodes, float box,

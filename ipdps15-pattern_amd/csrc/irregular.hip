@@ -230,6 +230,31 @@ struct LJParams {
     int pbc;
 };
 
+// 1.0f / b, correctly rounded.  For b in [2^-126, 2^125] one Newton step on
+// v_rcp_f32 (e = fma(-b, y, 1); y + e*y) gives the IEEE quotient bit for bit
+// -- checked for every FP32 value in that range on gfx950 (the
+// pat_lj_rcp_selftest entry point repeats the check) -- at 3 instructions
+// instead of the 11 of the division sequence; outside it (0, subnormals,
+// above 2^125) the division itself runs.
+__device__ __forceinline__ float exact_rcp(float b)
+{
+    const float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    float r = __builtin_fmaf(e, y, y);
+    if (__builtin_expect(!(b >= 0x1p-126f && b <= 0x1p125f), 0)) r = 1.0f / b;
+    return r;
+}
+
+// Minimum image of one component, the functor's two conditional moves
+// (kmeans/irregular.h:55-60) as one subtraction: for |d| < L at most one of
+// them applies, and d - (-L) is exactly d + L.
+__device__ __forceinline__ float min_image_sel(float d, float L, float hL)
+{
+    float s = d > hL ? L : 0.0f;
+    s = d < -hL ? -L : s;
+    return d - s;
+}
+
 // Force on i from j (both float4 positions); returns 0 beyond the cutoff.
 __device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams &p, float &fx, float &fy, float &fz)
 {
@@ -237,14 +262,14 @@ __device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams 
     float dy = pi.y - pj.y;
     float dz = pi.z - pj.z;
     if (p.pbc) {
-        dx = pat::min_image(dx, p.L, p.hL);
-        dy = pat::min_image(dy, p.L, p.hL);
-        dz = pat::min_image(dz, p.L, p.hL);
+        dx = min_image_sel(dx, p.L, p.hL);
+        dy = min_image_sel(dy, p.L, p.hL);
+        dz = min_image_sel(dz, p.L, p.hL);
     }
     float rd = dx * dx + dy * dy;
     rd = rd + dz * dz;
     if (rd < p.rc2) {
-        const float rrd = 1.0f / rd;
+        const float rrd = exact_rcp(rd);
         const float rrd2 = rrd * rrd;
         const float rrd4 = rrd2 * rrd2;
         const float rrd6 = rrd2 * rrd4;
@@ -263,14 +288,14 @@ __device__ __forceinline__ float pair_phi(float4 pi, float4 pj, const LJParams &
 {
     float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     if (p.pbc) {
-        dx = pat::min_image(dx, p.L, p.hL);
-        dy = pat::min_image(dy, p.L, p.hL);
-        dz = pat::min_image(dz, p.L, p.hL);
+        dx = min_image_sel(dx, p.L, p.hL);
+        dy = min_image_sel(dy, p.L, p.hL);
+        dz = min_image_sel(dz, p.L, p.hL);
     }
     float rd = dx * dx + dy * dy;
     rd = rd + dz * dz;
     if (!(rd < p.rc2)) return 0.0f;
-    const float rrd = 1.0f / rd;
+    const float rrd = exact_rcp(rd);
     const float rrd2 = rrd * rrd;
     const float rrd3 = rrd2 * rrd;
     const float rrd4 = rrd2 * rrd2;
@@ -461,6 +486,43 @@ __global__ __launch_bounds__(256) void lj_csr_ord(const float4 *__restrict__ pos
         }
     }
     if (l == 0) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
+// ------------------------------------------------------------ reciprocal check
+//
+// Exhaustive comparison, over every FP32 value in [lo, hi] (bit patterns),
+// of the IEEE quotient 1.0f / b with v_rcp_f32 refined by one and by two
+// Newton steps (e = fma(-b, y, 1); y = fma(e, y, y)).  A refined form may
+// stand in for the division only where this reports no difference.
+__device__ __forceinline__ float rcp_newton(float b, int steps)
+{
+    float y = __builtin_amdgcn_rcpf(b);
+    for (int i = 0; i < steps; i++) {
+        const float e = __builtin_fmaf(-b, y, 1.0f);
+        y = __builtin_fmaf(e, y, y);
+    }
+    return y;
+}
+
+__global__ __launch_bounds__(256) void rcp_check(uint32_t lo, uint32_t hi, unsigned long long *__restrict__ bad)
+{
+    unsigned long long b1 = 0, b2 = 0;
+    for (uint64_t i = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= hi;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float b = __uint_as_float((uint32_t)i);
+        const uint32_t ref = __float_as_uint(1.0f / b);
+        b1 += __float_as_uint(rcp_newton(b, 1)) != ref;
+        b2 += __float_as_uint(rcp_newton(b, 2)) != ref;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        b1 += __shfl_xor(b1, o, kWave);
+        b2 += __shfl_xor(b2, o, kWave);
+    }
+    if (threadIdx.x % kWave == 0) {
+        if (b1) atomicAdd(&bad[0], b1);
+        if (b2) atomicAdd(&bad[1], b2);
+    }
 }
 
 // ------------------------------------------------------------ sliced layout
@@ -838,6 +900,28 @@ int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr,
                        (const int *)sptr, (const int *)scol, (int)N, make_params(box, pbc, rc2, dt), force, vel,
                        pos_out, mode);
     PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+int rcp_selftest(uint32_t lo, uint32_t hi, int64_t *bad_one, int64_t *bad_two)
+{
+    unsigned long long *d = nullptr;
+    PAT_HIP(hipMalloc((void **)&d, 16));
+    unsigned long long h[2] = {0, 0};
+    int rc = PAT_OK;
+    if (hipMemset(d, 0, 16) != hipSuccess) rc = PAT_ERR_HIP;
+    if (rc == PAT_OK) {
+        hipLaunchKernelGGL(rcp_check, dim3(65536), dim3(256), 0, 0, lo, hi, d);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(h, d, 16, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = PAT_ERR_HIP;
+    }
+    (void)hipFree(d);
+    if (rc != PAT_OK) {
+        pat::set_error("rcp_selftest: HIP call failed");
+        return rc;
+    }
+    *bad_one = (int64_t)h[0];
+    *bad_two = (int64_t)h[1];
     return PAT_OK;
 }
 

@@ -39,6 +39,7 @@ int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, 
 int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
 int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
                   int32_t *grp, int32_t *gcl, hipStream_t st);
+int rcp_selftest(uint32_t lo, uint32_t hi, int64_t *bad_one, int64_t *bad_two);
 struct NbWork;
 NbWork *nb_create();
 void nb_destroy(NbWork *w);
@@ -474,6 +475,16 @@ int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t
     PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_csr_dev: box must exceed twice the cutoff");
     return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff, 0.f,
                              lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pat_lj_rcp_selftest(float lo, float hi, int64_t *bad_one_step, int64_t *bad_two_step)
+{
+    PAT_REQUIRE(bad_one_step && bad_two_step && lo > 0 && hi >= lo && std::isfinite(hi), PAT_ERR_INVALID,
+                "pat_lj_rcp_selftest: need 0 < lo <= hi < inf");
+    uint32_t a, b;
+    memcpy(&a, &lo, 4);
+    memcpy(&b, &hi, 4);
+    return pat::rcp_selftest(a, b, bad_one_step, bad_two_step);
 }
 
 int pat_lj_force_atomic_dev(const float *pos4, const int32_t *edges, int64_t num_edges, int64_t num_nodes, float box,

@@ -90,6 +90,7 @@ SIGNATURES = {
     "pat_gen_heat3d_dev": (c_int, [c_i64, c_i64, c_i64, c_u64, c_vp, c_vp]),
     "pat_gen_fcc_dev": (c_int, [c_i64, c_i64, c_i64, c_double, c_double, c_u64, c_vp, c_dp, c_vp]),
     "pat_gen_gmm_dev": (c_int, [c_i64, c_int, c_int, c_double, c_double, c_double, c_u64, c_vp, c_vp]),
+    "pat_lj_rcp_selftest": (c_int, [c_float, c_float, c_i64p, c_i64p]),
     "pat_serve_start": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_vp)]),
     "pat_serve_submit": (c_int, [c_vp, ctypes.POINTER(Job), c_i64p]),
     "pat_serve_poll": (c_int, [c_vp, c_i64, c_intp, ctypes.POINTER(JobResult)]),

@@ -217,3 +217,13 @@ def test_moldyn_workload(pa, orc, gpu, order):
     assert abs(ep - rep) / abs(rep) < 1e-9
     # size-independent properties: momentum stays ~0 (forces sum to ~0)
     assert np.max(np.abs(f.astype(np.float64).sum(axis=0))) / np.max(np.abs(f)) < 1e-3
+
+
+def test_reciprocal_matches_ieee_division(pa, gpu):
+    """The force kernels' reciprocal (one Newton step on v_rcp_f32) equals the
+    IEEE quotient 1.0f / b for every FP32 value in [2^-126, 2^125]: checked
+    exhaustively (about 2.1e9 values)."""
+    import ctypes
+    one, two = ctypes.c_int64(-1), ctypes.c_int64(-1)
+    pa.call("pat_lj_rcp_selftest", 2.0 ** -126, 2.0 ** 125, ctypes.byref(one), ctypes.byref(two))
+    assert one.value == 0 and two.value == 0
