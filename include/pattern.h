@@ -204,6 +204,19 @@ int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_no
  * every choice); -4 ... -64: tree-order sums over that many lanes. */
 int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, float box,
                          int pbc, float cutoff, int lanes_per_row, float *force4, void *stream);
+/* Sliced copy (SELL-64) of a CSR for pat_lj_force_sell_dev: slice s covers
+ * rows 64s..64s+63 and is as wide as its longest row; entry k of row 64s+l
+ * is scol[sptr[s] + 64k + l] (padding repeats the row's own index).  sptr:
+ * num_nodes/64 + 1 ints (rounded up); scol: scol_capacity ints.
+ * *num_entries = the padded size; PAT_ERR_INVALID when the capacity is
+ * short (call again with that much). */
+int pat_ir_sell_build_dev(const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, int32_t *sptr,
+                          int32_t *scol, int64_t scol_capacity, int64_t *num_entries, void *stream);
+/* LJ forces over the sliced copy, each row summed in CSR order (the same
+ * bits as pat_lj_force_csr_dev); kind 0, 1, 2 = plain, software-pipelined by
+ * 4, by 8 entries. */
+int pat_lj_force_sell_dev(const float *pos4, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol,
+                          int64_t num_nodes, float box, int pbc, float cutoff, int kind, float *force4, void *stream);
 /* Compares, for every FP32 value in [lo, hi], the IEEE quotient 1.0f / b
  * with the hardware reciprocal refined by one and by two Newton steps, and
  * returns how many results differ for each (the check behind the LJ

@@ -247,11 +247,11 @@ __device__ __forceinline__ float exact_rcp(float b)
 
 // Minimum image of one component, the functor's two conditional moves
 // (kmeans/irregular.h:55-60) as one subtraction: for |d| < L at most one of
-// them applies, and d - (-L) is exactly d + L.
+// them applies -- d < -hL or d > hL, i.e. |d| > hL -- and the move is
+// d - copysign(L, d) (d - (-L) is exactly d + L).
 __device__ __forceinline__ float min_image_sel(float d, float L, float hL)
 {
-    float s = d > hL ? L : 0.0f;
-    s = d < -hL ? -L : s;
+    const float s = fabsf(d) > hL ? __builtin_copysignf(L, d) : 0.0f;
     return d - s;
 }
 
@@ -544,24 +544,39 @@ __global__ void sell_width(const int *__restrict__ rowptr, int N, int nsl, int *
     width[s] = w;
 }
 
+// pad < 0: padding repeats the row's own index; otherwise it is 'pad' (a
+// sentinel node placed beyond every cutoff).
 __global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr, const int *__restrict__ cols, int N,
-                                                 int nsl, const int *__restrict__ sptr, int *__restrict__ scol)
+                                                 int nsl, const int *__restrict__ sptr, int *__restrict__ scol, int pad)
 {
     const int s = blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (s >= nsl) return;
     const int row = s * kWave + lane;
     const int b = row < N ? rowptr[row] : 0, e = row < N ? rowptr[row + 1] : 0;
-    const int self = min(row, N - 1);
+    const int fill = pad < 0 ? min(row, N - 1) : pad;
     const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
-    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? cols[b + k] : self;
+    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? cols[b + k] : fill;
+}
+
+// Neighbour position j.  SENT kernels address with a 32-bit byte offset
+// (the handle keeps fewer than 2^28 nodes), the others with 64-bit indices.
+template <bool SENT>
+__device__ __forceinline__ float4 gather_pos(const float4 *__restrict__ pos, int j)
+{
+    if (SENT) return *(const float4 *)((const char *)pos + ((uint32_t)j << 4));
+    return pos[j];
 }
 
 // In-order row sums over the sliced layout, one wavefront per slice and one
-// lane per row: the lane adds its row's pair forces in CSR order (padding
-// masked), U entries' column reads and position gathers issued ahead of
-// their arithmetic.  Bit-identical to lj_csr_seq.
-template <int U>
+// lane per row: the lane adds its row's pair forces in CSR order, U
+// entries' column reads and position gathers issued ahead of their
+// arithmetic.  Padding entries either repeat the row's own index and are
+// masked (SENT = false), or point at a sentinel node N whose position is
+// beyond every cutoff (SENT = true): its pair force is exactly +0, and
+// adding +0 to a sum that started at +0 leaves every bit unchanged, so no
+// mask is needed.  Bit-identical to lj_csr_seq either way.
+template <int U, bool SENT>
 __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
                                                const int *__restrict__ sptr, const int *__restrict__ scol, int N,
                                                LJParams p, float4 *__restrict__ force, float4 *__restrict__ vel,
@@ -577,7 +592,7 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
         p.L = pi.w;
         p.hL = 0.5f * pi.w;
     }
-    const int deg = row < N ? rowptr[row + 1] - rowptr[row] : 0;
+    const int deg = SENT ? 0 : (row < N ? rowptr[row + 1] - rowptr[row] : 0);
     const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
     const int *__restrict__ c = scol + base + lane;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -588,12 +603,12 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
 #pragma unroll
         for (int u = 0; u < U; u++) j[u] = c[(k + u) * kWave];
 #pragma unroll
-        for (int u = 0; u < U; u++) q[u] = pos[j[u]];
+        for (int u = 0; u < U; u++) q[u] = gather_pos<SENT>(pos, j[u]);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             float ax, ay, az;
             pair_force(pi, q[u], p, ax, ay, az);
-            if (k + u < deg) {
+            if (SENT || k + u < deg) {
                 fx = fx + ax;
                 fy = fy + ay;
                 fz = fz + az;
@@ -602,12 +617,75 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
     }
     for (; k < w; k++) {
         float ax, ay, az;
-        pair_force(pi, pos[c[k * kWave]], p, ax, ay, az);
-        if (k < deg) {
+        pair_force(pi, gather_pos<SENT>(pos, c[k * kWave]), p, ax, ay, az);
+        if (SENT || k < deg) {
             fx = fx + ax;
             fy = fy + ay;
             fz = fz + az;
         }
+    }
+    if (row < N) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
+// The same sums, software-pipelined: while a group of U pairs is evaluated,
+// the positions of the next group and the column indices of the one after
+// are already in flight (two register sets used alternately), so a lane
+// always has a group's memory latency to hide behind arithmetic.
+// Bit-identical to lj_sell.
+template <int U, bool SENT>
+__global__ __launch_bounds__(256) void lj_sell_pipe(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ sptr, const int *__restrict__ scol, int N,
+                                                    LJParams p, float4 *__restrict__ force, float4 *__restrict__ vel,
+                                                    float4 *__restrict__ pos_out, int mode)
+{
+    const int nsl = (N + kWave - 1) / kWave;
+    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
+    if (s >= nsl) return;  // whole wavefronts
+    const int lane = threadIdx.x % kWave;
+    const int row = s * kWave + lane;
+    const int self = min(row, N - 1);
+    const float4 pi = pos[self];
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int deg = SENT ? 0 : (row < N ? rowptr[row + 1] - rowptr[row] : 0);
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    const int *__restrict__ c = scol + base + lane;
+    // column of entry k; past the slice width a valid dummy whose force is
+    // masked (or zero, for the sentinel)
+    const int dummy = SENT ? N : self;
+    auto col_at = [&](int k) { return k < w ? c[k * kWave] : dummy; };
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    // evaluates group k from qc, loads the positions of group k+U (from jx)
+    // into qx and the columns of group k+2U into jy
+    auto step = [&](int k, const float4 (&qc)[U], const int (&jx)[U], float4 (&qx)[U], int (&jy)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) qx[u] = gather_pos<SENT>(pos, jx[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) jy[u] = col_at(k + 2 * U + u);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            float ax, ay, az;
+            pair_force(pi, qc[u], p, ax, ay, az);
+            if (SENT || k + u < deg) {
+                fx = fx + ax;
+                fy = fy + ay;
+                fz = fz + az;
+            }
+        }
+    };
+    int jA[U], jB[U];
+    float4 qA[U], qB[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) jA[u] = col_at(u);
+#pragma unroll
+    for (int u = 0; u < U; u++) jB[u] = col_at(U + u);
+#pragma unroll
+    for (int u = 0; u < U; u++) qA[u] = gather_pos<SENT>(pos, jA[u]);
+    for (int k = 0; k < w; k += 2 * U) {
+        step(k, qA, jB, qB, jA);
+        if (k + U < w) step(k + U, qB, jA, qA, jB);
     }
     if (row < N) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
 }
@@ -848,7 +926,7 @@ int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, 
 // changed) when padding would exceed max_pad times the entry count plus one
 // slice of 64 entries per slice.
 int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
-               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, hipStream_t st)
+               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st)
 {
     const int64_t nsl = (N + kWave - 1) / kWave;
     hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr, (int)N,
@@ -883,20 +961,63 @@ int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max
     }
     PAT_HIP(hipMemcpyAsync(sptr, sp.data(), sizeof(int32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(sell_fill, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, (const int *)rowptr,
-                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)*scol);
+                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)*scol, pad);
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipStreamSynchronize(st));  // sp leaves scope
     *total = acc;
     return PAT_OK;
 }
 
+// Sliced layout into caller buffers (sptr: N/64+1 ints, also the width
+// scratch; scol: cap entries); *total = padded entries, or the needed
+// capacity with PAT_ERR_INVALID when cap is short.
+int sell_build_into(const int32_t *rowptr, const int32_t *cols, int64_t N, int32_t *sptr, int32_t *scol, int64_t cap,
+                    int64_t *total, hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - kWave, PAT_ERR_INVALID, "sell_build: bad node count");
+    const int64_t nsl = (N + kWave - 1) / kWave;
+    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr, (int)N,
+                       (int)nsl, (int *)sptr);
+    PAT_LAUNCH_CHECK();
+    std::vector<int32_t> sp((size_t)nsl + 1);
+    PAT_HIP(hipMemcpyAsync(sp.data(), sptr, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    int64_t acc = 0;
+    for (int64_t i = 0; i < nsl; i++) {
+        const int64_t wi = sp[i];
+        sp[i] = (int32_t)acc;
+        acc += (int64_t)kWave * wi;
+        PAT_REQUIRE(acc < (1LL << 31), PAT_ERR_UNSUPPORTED, "sell_build: padded layout exceeds 2^31 entries");
+    }
+    sp[nsl] = (int32_t)acc;
+    *total = acc;
+    PAT_REQUIRE(acc <= cap, PAT_ERR_INVALID, "sell_build: %lld entries needed, capacity %lld", (long long)acc,
+                (long long)cap);
+    PAT_HIP(hipMemcpyAsync(sptr, sp.data(), sizeof(int32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(sell_fill, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, (const int *)rowptr,
+                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)scol, -1);
+    PAT_LAUNCH_CHECK();
+    PAT_HIP(hipStreamSynchronize(st));  // sp leaves scope
+    return PAT_OK;
+}
+
+// kind: 0 = lj_sell<4>, 1 = lj_sell_pipe<4>, 2 = lj_sell_pipe<8> over a
+// layout whose padding repeats the row; 3, 4, 5 = the same over a layout
+// padded with the sentinel node N (pos then holds N+1 entries, the last
+// beyond every cutoff, and N < 2^28).  Same sums for all.
 int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
                   float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
-                  hipStream_t st)
+                  int kind, hipStream_t st)
 {
     PAT_REQUIRE(N > 0 && N < (1LL << 31) - kWave, PAT_ERR_INVALID, "lj_force_sell: bad node count");
+    using K = void (*)(const float4 *, const int *, const int *, const int *, int, LJParams, float4 *, float4 *,
+                       float4 *, int);
+    const K ks[] = {lj_sell<4, false>, lj_sell_pipe<4, false>, lj_sell_pipe<8, false>,
+                    lj_sell<4, true>,  lj_sell_pipe<4, true>,  lj_sell_pipe<8, true>};
+    PAT_REQUIRE(kind >= 0 && kind < 6, PAT_ERR_INVALID, "lj_force_sell: kind %d", kind);
+    PAT_REQUIRE(kind < 3 || N < (1LL << 28) - 1, PAT_ERR_INVALID, "lj_force_sell: sentinel kinds need N < 2^28 - 1");
     const int64_t nsl = (N + kWave - 1) / kWave;
-    hipLaunchKernelGGL(lj_sell<4>, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, pos, (const int *)rowptr,
+    hipLaunchKernelGGL(ks[kind], dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, pos, (const int *)rowptr,
                        (const int *)sptr, (const int *)scol, (int)N, make_params(box, pbc, rc2, dt), force, vel,
                        pos_out, mode);
     PAT_LAUNCH_CHECK();

@@ -23,10 +23,10 @@ int64_t csr_workspace_bytes(int64_t N, int64_t E);
 int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st);
 int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
-               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, hipStream_t st);
+               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st);
 int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
                   float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
-                  hipStream_t st);
+                  int kind, hipStream_t st);
 int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
                  float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out,
                  hipStream_t st);
@@ -40,6 +40,8 @@ int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream
 int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
                   int32_t *grp, int32_t *gcl, hipStream_t st);
 int rcp_selftest(uint32_t lo, uint32_t hi, int64_t *bad_one, int64_t *bad_two);
+int sell_build_into(const int32_t *rowptr, const int32_t *cols, int64_t N, int32_t *sptr, int32_t *scol, int64_t cap,
+                    int64_t *total, hipStream_t st);
 struct NbWork;
 NbWork *nb_create();
 void nb_destroy(NbWork *w);
@@ -67,7 +69,7 @@ constexpr double kSellMaxPad = 1.5;
 // Tuned lanes per row, cached per (node count, mean degree, row order) class
 // for the process: the in-order variants give bit-identical sums, so the
 // choice only changes speed.
-std::map<std::tuple<int64_t, int, int>, int> g_lpr_cache;
+std::map<std::tuple<int64_t, int, int>, std::pair<int, int>> g_lpr_cache;  // -> (lanes, sliced kind)
 std::mutex g_lpr_mutex;
 
 // Device state shared by the IrregularRuntime handle and the MD handle.
@@ -88,6 +90,12 @@ struct IrState {
     int32_t *sptr = nullptr, *scol = nullptr, *swidth = nullptr;
     int64_t scap = 0, stotal = 0;
     bool sell_ok = false;
+    int sell_kind = 4;  // sliced kernel form (lj_force_sell kind), chosen by tune()
+
+    // Sliced kernels padded with a sentinel node: positions hold N+1
+    // entries, the last far beyond any cutoff (set at allocation, never
+    // written by a step), which needs N < 2^28 - 1.
+    bool sentinel() const { return N < (1LL << 28) - 1; }
     // bumped whenever a buffer or kernel choice the force launches use
     // changes, so captured graphs can tell they are stale
     int64_t layout_gen = 0;
@@ -111,13 +119,18 @@ struct IrState {
         PAT_TRY(am((void **)&edges, sizeof(int32_t) * 2 * (size_t)E));
         PAT_TRY(am((void **)&rowptr, sizeof(int32_t) * (size_t)(N + 1)));
         PAT_TRY(am((void **)&cols, sizeof(int32_t) * 2 * (size_t)E));
-        PAT_TRY(am((void **)&pos[0], sizeof(float) * 4 * (size_t)N));
+        PAT_TRY(am((void **)&pos[0], sizeof(float) * 4 * (size_t)(N + 1)));
         PAT_TRY(am((void **)&force, sizeof(float) * 4 * (size_t)N));
         PAT_TRY(am((void **)&node_e, sizeof(double) * (size_t)N));
         if (md) {
-            PAT_TRY(am((void **)&pos[1], sizeof(float) * 4 * (size_t)N));
+            PAT_TRY(am((void **)&pos[1], sizeof(float) * 4 * (size_t)(N + 1)));
             PAT_TRY(am((void **)&vel, sizeof(float) * 4 * (size_t)N));
         }
+        // sentinel node N in both position buffers: beyond every cutoff
+        const float far[4] = {3.0e38f, 3.0e38f, 3.0e38f, 0.0f};
+        for (float *pb : {pos[0], pos[1]})
+            if (pb) PAT_HIP(hipMemcpyAsync(pb + 4 * N, far, sizeof(far), hipMemcpyHostToDevice, stream));
+        PAT_HIP(hipStreamSynchronize(stream));
         const int64_t nsl = (N + 63) / 64;
         PAT_TRY(am((void **)&sptr, sizeof(int32_t) * (size_t)(nsl + 1)));
         PAT_TRY(am((void **)&swidth, sizeof(int32_t) * (size_t)nsl));
@@ -151,7 +164,9 @@ struct IrState {
     int build_sell()
     {
         layout_gen++;
-        int rc = pat::sell_build(rowptr, cols, N, kSellMaxPad, swidth, sptr, &scol, &scap, &stotal, stream);
+        // padded with the sentinel node N when the handle can hold it
+        int rc = pat::sell_build(rowptr, cols, N, kSellMaxPad, swidth, sptr, &scol, &scap, &stotal,
+                                 sentinel() ? (int)N : -1, stream);
         if (rc == PAT_ERR_UNSUPPORTED) {
             pat::clear_error();
             sell_ok = false;
@@ -225,7 +240,7 @@ struct IrState {
         if (l == PAT_IR_SLICED) {
             PAT_REQUIRE(sell_ok, PAT_ERR_STATE, "irregular: the sliced layout is not built");
             return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, N, box, pbc, rc2, dt, mode,
-                                      (float4 *)force, (float4 *)vel, (float4 *)pos_out, s);
+                                      (float4 *)force, (float4 *)vel, (float4 *)pos_out, sell_kind, s);
         }
         return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, dt, l, mode, (float4 *)force,
                                  (float4 *)vel, (float4 *)pos_out, s);
@@ -253,13 +268,15 @@ struct IrState {
         layout_gen++;
         if (!pat::tuning_enabled()) {
             lpr = sell_ok ? PAT_IR_SLICED : 1;
+            sell_kind = sentinel() ? 4 : 1;
             return PAT_OK;
         }
         {
             std::lock_guard<std::mutex> g(g_lpr_mutex);
             auto it = g_lpr_cache.find(key);
-            if (it != g_lpr_cache.end() && (it->second != PAT_IR_SLICED || sell_ok)) {
-                lpr = it->second;
+            if (it != g_lpr_cache.end() && (it->second.first != PAT_IR_SLICED || sell_ok)) {
+                lpr = it->second.first;
+                sell_kind = it->second.second;
                 return PAT_OK;
             }
         }
@@ -267,9 +284,14 @@ struct IrState {
         PAT_HIP(hipEventCreate(&a));
         PAT_HIP(hipEventCreate(&b));
         float best = 1e30f;
-        int best_l = 1;
-        for (int l : {1, 4, 8, 16, 32, PAT_IR_SLICED}) {
+        int best_l = 1, best_kind = sell_kind;
+        const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
+        const std::pair<int, int> cands[] = {{1, 0}, {4, 0}, {8, 0}, {16, 0}, {32, 0},
+                                             {PAT_IR_SLICED, k0}, {PAT_IR_SLICED, k0 + 1}, {PAT_IR_SLICED, k0 + 2}};
+        for (const auto &cand : cands) {
+            const int l = cand.first;
             if (l == PAT_IR_SLICED && !sell_ok) continue;
+            sell_kind = cand.second;
             for (int rep = 0; rep < 4; rep++) {
                 PAT_HIP(hipEventRecord(a, stream));
                 PAT_TRY(csr_force_l(l, p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream));
@@ -280,14 +302,16 @@ struct IrState {
                 if (rep > 0 && ms < best) {  // first launch of each is warm-up
                     best = ms;
                     best_l = l;
+                    best_kind = cand.second;
                 }
             }
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         lpr = best_l;
+        sell_kind = best_kind;
         std::lock_guard<std::mutex> g(g_lpr_mutex);
-        g_lpr_cache[key] = best_l;
+        g_lpr_cache[key] = std::make_pair(best_l, best_kind);
         return PAT_OK;
     }
 };
@@ -475,6 +499,24 @@ int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t
     PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_csr_dev: box must exceed twice the cutoff");
     return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff, 0.f,
                              lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pat_ir_sell_build_dev(const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, int32_t *sptr,
+                          int32_t *scol, int64_t scol_capacity, int64_t *num_entries, void *stream)
+{
+    PAT_REQUIRE(rowptr && sptr && num_entries && (scol || scol_capacity == 0), PAT_ERR_INVALID,
+                "pat_ir_sell_build_dev: null argument");
+    return pat::sell_build_into(rowptr, cols, num_nodes, sptr, scol, scol_capacity, num_entries, (hipStream_t)stream);
+}
+
+int pat_lj_force_sell_dev(const float *pos4, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol,
+                          int64_t num_nodes, float box, int pbc, float cutoff, int kind, float *force4, void *stream)
+{
+    PAT_REQUIRE(pos4 && rowptr && sptr && scol && force4, PAT_ERR_INVALID, "pat_lj_force_sell_dev: null argument");
+    PAT_REQUIRE(kind >= 0 && kind < 3, PAT_ERR_INVALID, "pat_lj_force_sell_dev: kind %d not in 0..2", kind);
+    PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_sell_dev: box must exceed twice the cutoff");
+    return pat::lj_force_sell((const float4 *)pos4, rowptr, sptr, scol, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff,
+                              0.f, kForceOnly, (float4 *)force4, nullptr, nullptr, kind, (hipStream_t)stream);
 }
 
 int pat_lj_rcp_selftest(float lo, float hi, int64_t *bad_one_step, int64_t *bad_two_step)

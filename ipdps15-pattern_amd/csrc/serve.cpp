@@ -358,7 +358,7 @@ int pat_serve_submit(pat_serve *s, const pat_job *job, int64_t *id)
     Job q;
     q.d = j;
     q.est = pat_serve::estimate(j);
-    PAT_REQUIRE(q.est <= s->budget, PAT_ERR_NOMEM, "pat_serve_submit: job needs about %lld bytes, budget %lld",
+    PAT_REQUIRE(q.est <= s->budget, PAT_ERR_BUDGET, "pat_serve_submit: job needs about %lld bytes, budget %lld",
                 (long long)q.est, (long long)s->budget);
     {
         std::lock_guard<std::mutex> g(s->mu);

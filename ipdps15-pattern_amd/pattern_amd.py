@@ -91,6 +91,8 @@ SIGNATURES = {
     "pat_gen_fcc_dev": (c_int, [c_i64, c_i64, c_i64, c_double, c_double, c_u64, c_vp, c_dp, c_vp]),
     "pat_gen_gmm_dev": (c_int, [c_i64, c_int, c_int, c_double, c_double, c_double, c_u64, c_vp, c_vp]),
     "pat_lj_rcp_selftest": (c_int, [c_float, c_float, c_i64p, c_i64p]),
+    "pat_ir_sell_build_dev": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64p, c_vp]),
+    "pat_lj_force_sell_dev": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_float, c_int, c_float, c_int, c_vp, c_vp]),
     "pat_serve_start": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_vp)]),
     "pat_serve_submit": (c_int, [c_vp, ctypes.POINTER(Job), c_i64p]),
     "pat_serve_poll": (c_int, [c_vp, c_i64, c_intp, ctypes.POINTER(JobResult)]),
@@ -429,6 +431,22 @@ def csr_build_dev(edges_dev, num_edges, num_nodes, stream=None, order=PAT_IR_ORD
     sync(stream)
     work.free()
     return rowptr, cols
+
+
+def sell_build_dev(rowptr, cols, num_nodes, stream=None):
+    """Sliced (SELL-64) copy of a device CSR; returns (sptr, scol) DeviceArrays."""
+    nsl = (num_nodes + 63) // 64
+    sptr = DeviceArray((nsl + 1,), np.int32)
+    total = c_i64(0)
+    # sizing pass: capacity 0 returns PAT_ERR_INVALID (-1) with the size needed
+    rc = lib().pat_ir_sell_build_dev(rowptr.ptr, cols.ptr, num_nodes, sptr.ptr, None, 0, ctypes.byref(total),
+                                     stream_ptr(stream))
+    if rc not in (PAT_OK, -1):
+        check("pat_ir_sell_build_dev", rc)
+    scol = DeviceArray((max(total.value, 1),), np.int32)
+    call("pat_ir_sell_build_dev", rowptr.ptr, cols.ptr, num_nodes, sptr.ptr, scol.ptr, total.value,
+         ctypes.byref(total), stream_ptr(stream))
+    return sptr, scol
 
 
 def to_float4(a3):

@@ -227,3 +227,20 @@ def test_reciprocal_matches_ieee_division(pa, gpu):
     one, two = ctypes.c_int64(-1), ctypes.c_int64(-1)
     pa.call("pat_lj_rcp_selftest", 2.0 ** -126, 2.0 ** 125, ctypes.byref(one), ctypes.byref(two))
     assert one.value == 0 and two.value == 0
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_force_sliced_kinds_vs_oracle(pa, orc, gpu, small, kind, order):
+    """Every form of the sliced kernel sums each row in CSR order: bit-identical
+    to the serial scatter over the list in the row order."""
+    pos, box, edges = small
+    n = len(pos)
+    rowptr, cols = pa.csr_build_dev(pa.DeviceArray.from_numpy(edges), len(edges), n, order=order)
+    sptr, scol = pa.sell_build_dev(rowptr, cols, n)
+    p4 = pa.DeviceArray.from_numpy(pa.to_float4(pos))
+    f4 = pa.DeviceArray((n, 4), np.float32)
+    pa.call("pat_lj_force_sell_dev", p4.ptr, rowptr.ptr, sptr.ptr, scol.ptr, n, box, 1, 2.5, kind, f4.ptr, None)
+    pa.sync()
+    ref, _ = orc.lj_force(pos, oracle_edges(pa, edges, order), box, 2.5)
+    assert np.array_equal(f4.numpy()[:, :3], ref)
