@@ -260,6 +260,18 @@ def bench_copy(mib=512, reps=5):
     return {"bytes": n, "gbs": 2 * n / dt / 1e9}
 
 
+def profiled_traffic(key):
+    """HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes over this benchmark (scaled by a plain copy), from the committed
+    profile summary; None when absent."""
+    f = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(f) as fh:
+            return float(json.load(fh)[key])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,7 +328,7 @@ def main():
                    "pairs": lj["pairs"], "nodes": lj["nodes"],
                    "kernel": f"in-order CSR force sums in node order, {head['kernel']}, VV update fused"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None},
+                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic("lj_step_bytes")},
         "cpu_baseline": lj["cpu"],
         "engines": engines,
     }
