@@ -88,6 +88,8 @@ struct KmArgs {
     unsigned long long *gcnt;   // K
     unsigned long long *gdsum;  // K
     int use_lds_acc;        // 1: LDS reduction object, 0: global atomics directly
+    int rs;                 // row stride (8-byte words) of the coordinate sums: D in global memory,
+                            // odd in LDS so that rows of different keys start in different banks
 };
 
 // Adds the wavefront's points [base, base+np) with keys key_of(j) into the
@@ -112,7 +114,7 @@ __device__ __forceinline__ void accumulate_points(const KmArgs &a, int64_t base,
         const int k = key_of(in ? j : np - 1);
         if (in) {
             const float v = a.x[(base + j) * D + d];
-            atomicAdd(&acc[(int64_t)k * D + d], to_fixed(v, a.scale));
+            atomicAdd(&acc[(int64_t)k * a.rs + d], to_fixed(v, a.scale));
         }
         j += step_j;
         d += step_d;
@@ -134,8 +136,11 @@ __device__ void flush_lds_object(const KmArgs &a, unsigned long long *acc, unsig
                                  unsigned long long *dsum)
 {
     const int KD = a.K * a.D;
-    for (int i = threadIdx.x; i < KD; i += blockDim.x)
-        if (acc[i]) atomicAdd(&a.gsum[i], acc[i]);
+    for (int i = threadIdx.x; i < KD; i += blockDim.x) {
+        const int k = i / a.D, d = i - k * a.D;
+        const unsigned long long v = acc[(int64_t)k * a.rs + d];
+        if (v) atomicAdd(&a.gsum[i], v);
+    }
     for (int i = threadIdx.x; i < a.K; i += blockDim.x) {
         if (cnt[i]) atomicAdd(&a.gcnt[i], cnt[i]);
         if (dsum[i]) atomicAdd(&a.gdsum[i], dsum[i]);
@@ -145,15 +150,24 @@ __device__ void flush_lds_object(const KmArgs &a, unsigned long long *acc, unsig
 __device__ void zero_lds_object(const KmArgs &a, unsigned long long *acc, unsigned long long *cnt,
                                 unsigned long long *dsum)
 {
-    for (int i = threadIdx.x; i < a.K * a.D; i += blockDim.x) acc[i] = 0;
+    for (int i = threadIdx.x; i < a.K * a.rs; i += blockDim.x) acc[i] = 0;
     for (int i = threadIdx.x; i < a.K; i += blockDim.x) {
         cnt[i] = 0;
         dsum[i] = 0;
     }
 }
 
+// Row stride of the LDS reduction object: D rounded up to odd, so that the
+// rows of different keys start in different banks (with an even stride every
+// row would start in the same bank, and the wavefront's atomics on one
+// dimension of 32 points would serialise).
+__host__ __device__ __forceinline__ int object_stride(int D) { return D | 1; }
+
 // Words of the LDS reduction object, rounded up to a 16-byte boundary.
-__host__ __device__ __forceinline__ int64_t object_words(int K, int D) { return ((int64_t)K * D + 2 * K + 1) & ~1LL; }
+__host__ __device__ __forceinline__ int64_t object_words(int K, int D)
+{
+    return ((int64_t)K * object_stride(D) + 2 * K + 1) & ~1LL;
+}
 
 // LDS layout of the reduction object (when it fits).
 __device__ __forceinline__ void lds_object(const KmArgs &a, unsigned long long *base, unsigned long long *&acc,
@@ -161,7 +175,7 @@ __device__ __forceinline__ void lds_object(const KmArgs &a, unsigned long long *
 {
     if (a.use_lds_acc) {
         acc = base;
-        cnt = base + (int64_t)a.K * a.D;
+        cnt = base + (int64_t)a.K * a.rs;
         dsum = cnt + a.K;
     } else {
         acc = a.gsum;
@@ -267,45 +281,12 @@ struct MfmaBound {
 };
 
 // Exact plain-path distance (sqrt of the FP32 difference form, dimensions in
-// order) of a point held in registers (xr, zero past D) to centre row ck.
-template <int DP>
-__device__ __forceinline__ float exact_dist(const float (&xr)[DP], const float *__restrict__ ck, int D, bool vec)
+// order) of point row xp to centre row ck, both read from memory (the point
+// row was just loaded by the wavefront, so it is in cache).
+__device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const float *__restrict__ ck, int D, bool vec)
 {
     float dist = 0;
     if (vec) {  // D % 4 == 0 and 16-byte aligned rows
-#pragma unroll
-        for (int q = 0; q < DP / 4; q++) {
-            if (4 * q < D) {
-                const float4 cv = *(const float4 *)(ck + 4 * q);
-                float t = cv.x - xr[4 * q];
-                dist = dist + t * t;
-                t = cv.y - xr[4 * q + 1];
-                dist = dist + t * t;
-                t = cv.z - xr[4 * q + 2];
-                dist = dist + t * t;
-                t = cv.w - xr[4 * q + 3];
-                dist = dist + t * t;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < DP; d++) {
-            if (d < D) {
-                const float t = ck[d] - xr[d];
-                dist = dist + t * t;
-            }
-        }
-    }
-    return sqrtf(dist);
-}
-
-// The same distance with the point read from memory (rows too long to keep
-// in registers).
-__device__ __forceinline__ float exact_dist_mem(const float *__restrict__ xp, const float *__restrict__ ck, int D,
-                                               bool vec)
-{
-    float dist = 0;
-    if (vec) {
         for (int d = 0; d < D; d += 4) {
             const float4 cv = *(const float4 *)(ck + d), xv = *(const float4 *)(xp + d);
             float t = cv.x - xv.x;
@@ -326,28 +307,43 @@ __device__ __forceinline__ float exact_dist_mem(const float *__restrict__ xp, co
     return sqrtf(dist);
 }
 
+// Approximate value v with its position idx (0..127) in the low 7 bits of
+// the significand: a key that orders like v (up to 127 ulps) and carries its
+// index through a plain float minimum.  |key - v| <= 2^-16 |v| + 2^-126.
+__device__ __forceinline__ float pack_key(float v, int idx)
+{
+    return __int_as_float((__float_as_int(v) & ~127) | idx);
+}
+
+// Stand-in for a value past the last centre row: finite (so its key is not
+// a NaN) and above any threshold.
+constexpr float kFar = 3.0e38f;
+
 // KS: K-steps of 16 dimensions holding the data (D <= 16*KS); NT: terms
 // (1 or 3), each KS steps; one more K-step holds the norm terms.  KB:
 // 32-centre blocks per chunk.  cb: K_pad x (16*(NT*KS+1)) BF16 centre rows
 // [-2c | n1 n2 n3 0...] (NT = 1) or [hi(-2c) | lo(-2c) | hi(-2c) | n1 n2 n3
 // 0...] (NT = 3), data zero-padded to 16*KS, stored in fragment order
-// (frag_index); padded rows have n1 = +inf.  cnorm: ||c_k|| (K entries).
+// (frag_index); padded rows have n1 = 1e30 (far, but finite so that their
+// keys are numbers).  cnorm: ||c_k|| (K entries).
 //
-// Per 32-point tile a wavefront: loads each point's row into registers
-// (lanes l and l+32 both hold point l), computes the approximate values of
-// every centre with MFMAs, takes their minimum m and the threshold, counts
-// the centres at or below it and, when the point has exactly one (the usual
-// case), rescores only that centre; otherwise every candidate is rescored.
-// The exact FP32 results decide with map1's rule, and the point is added to
-// the LDS reduction object from the registers.
-template <int KS, int KB, int NT>
+// Per 32-point tile a wavefront (lanes l and l+32 hold halves of point l):
+// computes the approximate values of every centre with MFMAs, packs each
+// with its index (pack_key) and keeps the smallest and second-smallest keys.
+// The threshold is the minimum plus the error bound plus the packing error;
+// when only the smallest key is under it (the usual case) that one centre is
+// rescored, otherwise every centre under it is.  The exact FP32 results
+// decide with map1's rule, and the point is added to the LDS reduction object
+// from the registers holding its half rows.
+// TAIL = false: D == 16*KS and the rows are 16-byte aligned, so every
+// K-step is full and the loads need no per-element guard.
+template <int KS, int KB, int NT, bool TAIL>
 __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
 {
     constexpr int NS = NT * KS + 1;  // K-steps per centre row
     constexpr int KW = 16 * NS;      // BF16 row width of cb
-    constexpr bool kRow = KS <= 2;   // whole row in registers for the rescoring
-    constexpr int DP = kRow ? 16 * KS : 4;  // padded dimensions held per lane
+    static_assert(KB * 16 <= 128, "pack_key indexes at most 128 values per chunk");
     extern __shared__ unsigned long long smem[];
     unsigned long long *acc_o, *cnt_o, *dsum_o;
     lds_object(a, smem, acc_o, cnt_o, dsum_o);
@@ -381,41 +377,56 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     const int nchunks = (K_pad / 32 + KB - 1) / KB;
     unsigned long long my_rescored = 0;
 
+    // approximate values of chunk ch for this lane's point: value i of block
+    // b is centre row 8*(i/4) + 4*half + i%4 of the block (the 32x32 MFMA
+    // output layout).  The A fragments are read a few K-steps ahead of the
+    // MFMA that uses them.
+    // Blocks past the last centre row (when K_pad is not a multiple of
+    // KB*32) are computed from block 0's rows -- the block index is clamped
+    // on the scalar unit -- and skipped by the scans.
+    const int kblocks = K_pad / 32;
+    auto chunk_values = [&](int ch, const bf16x8 (&bx)[NS], f32x16 (&acc)[KB]) {
+        constexpr int T = KB * NS, P = 3;
+        const bf16x8 *ab = (const bf16x8 *)cbs + lane;
+        auto frag = [&](int t) {
+            const int blk = ch * KB + t / NS;
+            return ab[((int64_t)(blk < kblocks ? blk : 0) * NS + t % NS) * kWave];
+        };
+        bf16x8 fr[P];
+#pragma unroll
+        for (int t = 0; t < P; t++) fr[t] = frag(t);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int b = t / NS, st = t % NS;
+            const bf16x8 av = fr[t % P];
+            if (t + P < T) fr[t % P] = frag(t + P);
+            if (st == 0) acc[b] = f32x16{};
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bx[st], acc[b], 0, 0, 0);
+        }
+    };
+    auto centre_of = [&](int ch, int idx) {
+        const int b = idx >> 4, i = idx & 15;
+        return (ch * KB + b) * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+    };
+
     const int64_t ntiles = (a.N + 31) / 32;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; tile < ntiles;
          tile += nw) {
         const int64_t p = tile * 32 + col;
         const bool live = p < a.N;
-        const float *xp = a.x + (live ? p : 0) * D;
+        const float *xp = a.x + (live ? p : a.N - 1) * D;  // dead lanes read a valid row
 
-        // the point's row, zero past D (rows up to 32 dimensions)
-        float xr[DP];
-        if (!kRow) {
-#pragma unroll
-            for (int d = 0; d < DP; d++) xr[d] = 0.0f;
-        } else if (vec_x) {
-#pragma unroll
-            for (int q = 0; q < DP / 4; q++) {
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (live && 4 * q < D) v = *(const float4 *)(xp + 4 * q);
-                xr[4 * q] = v.x;
-                xr[4 * q + 1] = v.y;
-                xr[4 * q + 2] = v.z;
-                xr[4 * q + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int d = 0; d < DP; d++) xr[d] = (live && d < D) ? xp[d] : 0.0f;
-        }
-        // this lane's half of the row (dimensions 16s + 8*half + e), loaded
-        // by address: selecting it from xr with the runtime half index would
-        // turn xr into a scratch array
+        // this lane's half of the row (dimensions 16s + 8*half + e)
         float xo[8 * KS];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int d0 = 16 * s + 8 * half;
-            if (vec_x) {
+            if (!TAIL) {
+                const float4 lo = *(const float4 *)(xp + d0), hi = *(const float4 *)(xp + d0 + 4);
+                xo[8 * s] = lo.x; xo[8 * s + 1] = lo.y; xo[8 * s + 2] = lo.z; xo[8 * s + 3] = lo.w;
+                xo[8 * s + 4] = hi.x; xo[8 * s + 5] = hi.y; xo[8 * s + 6] = hi.z; xo[8 * s + 7] = hi.w;
+            } else if (vec_x) {
                 float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
                 if (live && d0 < D) lo = *(const float4 *)(xp + d0);
                 if (live && d0 + 4 < D) hi = *(const float4 *)(xp + d0 + 4);
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const float v = xo[8 * s + e];
-                xx = xx + v * v;
+                xx = __builtin_fmaf(v, v, xx);  // a bound input only (used with slack)
                 const __bf16 h = (__bf16)v;
                 bx[s][e] = h;
                 if (NT == 3) {
@@ -449,94 +460,79 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         xx = xx + __shfl_xor(xx, 32, kWave);  // ||x||^2 of point p (both halves)
         const float xn = sqrtf(xx) * 1.0001f;
         const float E = bd.c_dot * xn * cmax + bd.c_norm * (cc_max + xx);
+        // packing error bound: |values| <= ||c||^2 + 2 ||x|| ||c||
+        const float dp = 0x1p-16f * (cc_max + 2.0f * xn * cmax) * 1.01f + 0x1p-126f;
 
-        // approximate values of chunk ch (KB blocks of 32 centres) for this
-        // lane's point: value i of block b is centre row 8*(i/4) + 4*half +
-        // i%4 of the block (the 32x32 MFMA output layout), column = point
-        auto chunk_values = [&](int ch, f32x16 *acc) {
+        // smallest and second-smallest keys over every centre of this half
+        f32x16 acc[KB];
+        float k1 = kFar, k2 = kFar;
+        int ch1 = 0;
+        for (int ch = 0; ch < nchunks; ch++) {
+            chunk_values(ch, bx, acc);
+            float c1 = kFar, c2 = kFar;
 #pragma unroll
             for (int b = 0; b < KB; b++) {
-                acc[b] = f32x16{};
-                if ((ch * KB + b) * 32 < K_pad) {
-                    // A operand: centre rows of block ch*KB + b, fragment order
-                    const bf16x8 *ab = (const bf16x8 *)cbs + (int64_t)(ch * KB + b) * NS * kWave + lane;
+                if (ch * KB + b >= kblocks) break;  // uniform
 #pragma unroll
-                    for (int s = 0; s < NS; s++)
-                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ab[s * kWave], bx[s], acc[b], 0, 0, 0);
-                } else {
-                    // a block past the last centre row: never a candidate
-#pragma unroll
-                    for (int i = 0; i < 16; i++) acc[b][i] = INFINITY;
+                for (int i = 0; i < 16; i++) {
+                    const float key = pack_key(acc[b][i], b * 16 + i);
+                    c2 = __builtin_amdgcn_fmed3f(c1, c2, key);
+                    c1 = __builtin_amdgcn_fmed3f(c1, key, -kFar);  // min (keys are finite)
                 }
             }
-        };
-        auto centre_of = [&](int ch, int b, int i) { return (ch * KB + b) * 32 + 8 * (i / 4) + 4 * half + (i % 4); };
-
-        // minimum over every centre (both half-waves' rows)
-        f32x16 acc[KB];
-        float m = INFINITY;
-        for (int ch = 0; ch < nchunks; ch++) {
-            chunk_values(ch, acc);
-#pragma unroll
-            for (int b = 0; b < KB; b++)
-#pragma unroll
-                for (int i = 0; i < 16; i++) m = fminf(m, acc[b][i]);
+            if (c1 < k1) {
+                k2 = fminf(k1, c2);
+                k1 = c1;
+                ch1 = ch;
+            } else {
+                k2 = fminf(k2, c1);
+            }
         }
-        m = fminf(m, __shfl_xor(m, 32, kWave));
-        // candidates: approximate value within the margin of the minimum
-        const float dhi = (xx + m + 4.0f * E) * 1.001f + 1e-30f;
-        const float thr = m + 2.0f * E + 2.0f * bd.c_ref * dhi;
-        int cnt = 0, kc = 0;
-        for (int ch = 0; ch < nchunks; ch++) {
-            if (nchunks > 1) chunk_values(ch, acc);
-#pragma unroll
-            for (int b = 0; b < KB; b++)
-#pragma unroll
-                for (int i = 0; i < 16; i++)
-                    if (acc[b][i] <= thr) {
-                        cnt++;
-                        kc = centre_of(ch, b, i);
-                    }
-        }
-        const int cnt_all = cnt + __shfl_xor(cnt, 32, kWave);
+        // both halves: overall smallest and second smallest
+        const float o1 = __shfl_xor(k1, 32, kWave), o2 = __shfl_xor(k2, 32, kWave);
+        const float m1 = fminf(k1, o1);
+        const float m2 = fminf(fmaxf(k1, o1), fminf(k2, o2));
+        // candidates: keys within the margin of the minimum
+        const float dhi = (xx + m1 + 4.0f * E + 4.0f * dp) * 1.001f + 1e-30f;
+        const float thr = m1 + 2.0f * E + 2.0f * dp + 2.0f * bd.c_ref * dhi;
+        const bool multi = !(m2 > thr);
 
         float best_d = kInitMin;
         int best_k = 0x7fffffff;
-        const bool multi = cnt_all != 1;
-        if (!multi) {
-            // one candidate: the half-wave holding it rescores it
-            if (cnt == 1 && kc < a.K && live) {
-                best_d = kRow ? exact_dist(xr, a.c + (int64_t)kc * D, D, vec_c)
-                              : exact_dist_mem(xp, a.c + (int64_t)kc * D, D, vec_c && vec_x);
-                best_k = kc;
+        if (!multi && k1 < o1 && live) {
+            // one candidate, held by this half: rescore it
+            const int k = centre_of(ch1, __float_as_int(k1) & 127);
+            if (k < a.K) {
+                best_d = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
+                best_k = k;
             }
         }
-        // Several candidates: mask them (4 blocks per 64-bit word) and rescore
-        // each set bit.  The branch is taken by the whole wavefront (a vote),
-        // because recomputing a chunk's values needs every lane's operands.
+        // Several candidates: rescore every key under the threshold.  Taken
+        // by the whole wavefront (a vote), because recomputing a chunk's
+        // values needs every lane's operands.
         if (__any(multi)) {
-            static_assert(KB <= 8, "two 64-bit candidate words cover at most 8 blocks");
             for (int ch = 0; ch < nchunks; ch++) {
-                if (nchunks > 1) chunk_values(ch, acc);
+                if (nchunks > 1) chunk_values(ch, bx, acc);
                 if (!multi) continue;
-                uint64_t m0 = 0, m1 = 0;
+                uint64_t w0 = 0, w1 = 0;
 #pragma unroll
-                for (int b = 0; b < KB; b++)
+                for (int b = 0; b < KB; b++) {
+                    if (ch * KB + b >= kblocks) break;  // uniform
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        if (acc[b][i] <= thr) {
-                            if (b < 4) m0 |= 1ULL << (b * 16 + i);
-                            else m1 |= 1ULL << ((b - 4) * 16 + i);
+                        if (pack_key(acc[b][i], b * 16 + i) <= thr) {
+                            if (b < 4) w0 |= 1ULL << (b * 16 + i);
+                            else w1 |= 1ULL << ((b - 4) * 16 + i);
                         }
+                }
                 for (int w = 0; w < 2; w++) {
-                    uint64_t mw = w ? m1 : m0;
+                    uint64_t mw = w ? w1 : w0;
                     while (mw) {
                         const int bit = __builtin_ctzll(mw);
                         mw &= mw - 1;
-                        const int k = centre_of(ch, 4 * w + bit / 16, bit % 16);
+                        const int k = centre_of(ch, 64 * w + bit);
                         if (k < a.K && live) {
-                            const float dk = kRow ? exact_dist(xr, a.c + (int64_t)k * D, D, vec_c)
-                                                  : exact_dist_mem(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
+                            const float dk = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
                             if (dk < best_d || (dk == best_d && k < best_k)) {
                                 best_d = dk;
                                 best_k = k;
@@ -558,18 +554,18 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
             if (half == 0) {
                 a.assign[p] = best_k;
                 if (a.mindist) a.mindist[p] = best_d;
-                if (cnt_all > 1) my_rescored++;
+                if (multi) my_rescored++;
                 atomicAdd(&cnt_o[best_k], 1ULL);
                 atomicAdd(&dsum_o[best_k], to_fixed(best_d, a.dscale));
             }
             // this half-wave's dimensions into the reduction object
-            unsigned long long *row = acc_o + (int64_t)best_k * D;
+            unsigned long long *row = acc_o + (int64_t)best_k * a.rs;
 #pragma unroll
             for (int s = 0; s < KS; s++)
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     const int d = 16 * s + 8 * half + e;
-                    if (d < D) atomicAdd(&row[d], to_fixed(xo[8 * s + e], a.scale));
+                    if (!TAIL || d < D) atomicAdd(&row[d], to_fixed(xo[8 * s + e], a.scale));
                 }
         }
     }
@@ -604,7 +600,7 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
     if (k >= K_pad) return;
     if (k >= K) {  // padding row: never the minimum
         for (int e = lane; e < KW; e += kWave)
-            cb[frag_index(k, e, KW / 16)] = (__bf16)((e == NB) ? INFINITY : 0.0f);
+            cb[frag_index(k, e, KW / 16)] = (__bf16)((e == NB) ? 1.0e30f : 0.0f);  // far, finite
         return;
     }
     const long long n = (long long)gcnt[k];
@@ -728,6 +724,7 @@ static KmArgs make_args(const KmDev &d)
     a.gcnt = d.gcnt;
     a.gdsum = d.gdsum;
     a.use_lds_acc = object_bytes(d.K, d.D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024);
+    a.rs = a.use_lds_acc ? object_stride(d.D) : d.D;
     return a;
 }
 
@@ -745,10 +742,15 @@ int kmeans_prepare_kernels()
     PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     g_lds_limit = lim - 256;  // room for the kernels' static LDS
     const void *fns[] = {(const void *)km_plain<8, 1024>, (const void *)km_plain<16, 1024>, (const void *)km_plain<32, 1024>,
-                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1>,
-                         (const void *)km_mfma<2, 8, 1>,  (const void *)km_mfma<4, 4, 1>,  (const void *)km_mfma<8, 2, 1>,
-                         (const void *)km_mfma<1, 8, 3>,  (const void *)km_mfma<2, 8, 3>,  (const void *)km_mfma<4, 4, 3>,
-                         (const void *)km_mfma<8, 2, 3>};
+                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1, true>,
+                         (const void *)km_mfma<2, 8, 1, true>,  (const void *)km_mfma<4, 4, 1, true>,
+                         (const void *)km_mfma<8, 2, 1, true>,  (const void *)km_mfma<1, 8, 3, true>,
+                         (const void *)km_mfma<2, 8, 3, true>,  (const void *)km_mfma<4, 4, 3, true>,
+                         (const void *)km_mfma<8, 2, 3, true>,  (const void *)km_mfma<1, 8, 1, false>,
+                         (const void *)km_mfma<2, 8, 1, false>, (const void *)km_mfma<4, 4, 1, false>,
+                         (const void *)km_mfma<8, 2, 1, false>, (const void *)km_mfma<1, 8, 3, false>,
+                         (const void *)km_mfma<2, 8, 3, false>, (const void *)km_mfma<4, 4, 3, false>,
+                         (const void *)km_mfma<8, 2, 3, false>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
@@ -866,11 +868,19 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     bd.c_ref = (float)((3.0 + d.D + 16.0) * ldexp(1.0, -24) * 1.01);
     const int64_t want = (d.N + 31) / 32 / 8;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
+    // no tail: every K-step full and 16-byte aligned rows
+    const bool tail = d.D != 16 * d.KS || ((uintptr_t)d.x % 16) != 0;
 #define PAT_KM(KS_, KB_, NT_)                                                                                          \
     do {                                                                                                               \
-        PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_>, lds));                                                 \
-        hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_>), dim3(grid), dim3(512), (size_t)lds, st, a, (const __bf16 *)d.cb,  \
-                           d.cnorm, d.K_pad, bd, d.rescored);                                                          \
+        if (tail) {                                                                                                    \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, true>, lds));                                       \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, true>), dim3(grid), dim3(512), (size_t)lds, st, a,              \
+                               (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
+        } else {                                                                                                       \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, false>, lds));                                      \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, false>), dim3(grid), dim3(512), (size_t)lds, st, a,             \
+                               (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
+        }                                                                                                              \
     } while (0)
     switch (d.KS * 4 + d.NT) {
     case 1 * 4 + 1: PAT_KM(1, 8, 1); break;
