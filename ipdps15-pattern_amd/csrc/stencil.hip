@@ -27,6 +27,7 @@
 #include <tuple>
 
 #include "pat_common.hpp"
+#include "stencil_dev.hpp"
 
 namespace {
 
@@ -119,6 +120,127 @@ __global__ __launch_bounds__(BX *BY) void heat_lds25d(const float *__restrict__ 
             id[r] += sxy;
             if (more) above[r] = in[id[r] + sxy];
         }
+    }
+}
+
+// Two sweeps fused per pass (temporal blocking).  A workgroup of 64 x 4
+// threads covers a 64 x 4*RY window of the x-y plane and marches a z-chunk;
+// the window's outer two cells on each side are its halo, so it writes a
+// 60 x (4*RY - 4) tile of the second sweep's field.  Per plane it loads the
+// input (level 0) once, computes the first sweep (level 1) on the window
+// minus one cell -- kept in registers and in an LDS plane, never in HBM --
+// and from three level-1 planes the second sweep (level 2) of the plane
+// below.  Level 1 at the grid's halo cells is the fixed zero.  Each cell's
+// arithmetic is the single sweep's, so two passes' worth of output is
+// bit-identical to two sweeps; the field crosses HBM once per two steps.
+template <int RY>
+__global__ __launch_bounds__(256) void heat_tb2(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                                int nz, int zc, float alpha, int tiles_x, int tiles_y, int tiles_z)
+{
+    constexpr int WX = 64, WY = 4 * RY, OX = WX - 4, OY = WY - 4;
+    __shared__ float p0[WY][WX];     // level-0 plane z
+    __shared__ float p1[2][WY][WX];  // level-1 planes (ping-pong)
+
+    const int nblocks = tiles_x * tiles_y * tiles_z;
+    const int t = xcd_swizzle(blockIdx.x, nblocks);
+    const int bx = t % tiles_x;
+    const int by = (t / tiles_x) % tiles_y;
+    const int bz = t / (tiles_x * tiles_y);
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int sx = nx + 2;
+    const int64_t sxy = (int64_t)sx * (ny + 2);
+
+    // column of this thread: global x = bx*OX - 1 + tx (window starts two
+    // cells left of the tile's first output x, which is bx*OX + 1)
+    const int gx = bx * OX - 1 + tx;
+    const int xl = min(max(gx, 0), nx + 1);  // clamped for loads
+    const bool xin1 = tx >= 1 && tx <= WX - 2;   // level-1 columns of the window
+    const bool xin2 = tx >= 2 && tx <= WX - 3 && gx <= nx;  // output columns
+    const bool xhalo = gx <= 0 || gx >= nx + 1;  // level 1 is the fixed zero here
+
+    const int z0 = bz * zc + 1;
+    const int z1 = min(z0 + zc, nz + 1);  // output planes [z0, z1)
+
+    int64_t col[RY];       // offset of (x, y) in a plane
+    bool act1[RY], act2[RY], zero1[RY];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const int row = ty * RY + r;
+        const int gy = by * OY - 1 + row;
+        const int yl = min(max(gy, 0), ny + 1);
+        col[r] = xl + (int64_t)sx * yl;
+        act1[r] = xin1 && row >= 1 && row <= WY - 2;
+        act2[r] = xin2 && row >= 2 && row <= WY - 3 && gy <= ny;
+        zero1[r] = xhalo || gy <= 0 || gy >= ny + 1;
+    }
+    auto load = [&](int r, int z) { return in[col[r] + sxy * (int64_t)min(max(z, 0), nz + 1)]; };
+
+    // level 0 at planes z-1, z, z+1 and level 1 at planes z-1, z (z = the
+    // level-1 plane computed next, starting one below the first output)
+    float a_m[RY], a_c[RY], a_p[RY], b_m[RY], b_c[RY];
+    int z = z0 - 1;
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        a_m[r] = load(r, z - 1);
+        a_c[r] = load(r, z);
+        a_p[r] = load(r, z + 1);
+        b_m[r] = 0.0f;
+        b_c[r] = 0.0f;
+    }
+    int cur = 0;
+    for (; z <= z1; z++) {
+        // level 0 plane z to LDS for the in-plane neighbours
+#pragma unroll
+        for (int r = 0; r < RY; r++) p0[ty * RY + r][tx] = a_c[r];
+        __syncthreads();
+        // level 1 at plane z (zero on the grid's halo)
+        const bool zhalo = z <= 0 || z >= nz + 1;
+        float l1[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            float v = 0.0f;
+            if (act1[r] && !zero1[r] && !zhalo) {
+                const int ly = ty * RY + r;
+                float sum = p0[ly][tx - 1] + p0[ly][tx + 1];
+                sum = sum + p0[ly - 1][tx];
+                sum = sum + p0[ly + 1][tx];
+                sum = sum + a_m[r];
+                sum = sum + a_p[r];
+                const float c = a_c[r];
+                v = c + alpha * (sum - 6.0f * c);
+            }
+            l1[r] = v;
+            p1[cur][ty * RY + r][tx] = v;
+        }
+        // level 2 at plane z-1 from level-1 planes z-2 (b_m), z-1 (LDS, b_c), z (l1)
+        if (z - 1 >= z0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                if (act2[r]) {
+                    const int ly = ty * RY + r;
+                    const float(*q)[WX] = p1[1 - cur];
+                    float sum = q[ly][tx - 1] + q[ly][tx + 1];
+                    sum = sum + q[ly - 1][tx];
+                    sum = sum + q[ly + 1][tx];
+                    sum = sum + b_m[r];
+                    sum = sum + l1[r];
+                    const float c = b_c[r];
+                    out[col[r] + sxy * (int64_t)(z - 1)] = c + alpha * (sum - 6.0f * c);
+                }
+            }
+        }
+        __syncthreads();
+        // advance the plane registers
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            a_m[r] = a_c[r];
+            a_c[r] = a_p[r];
+            if (z + 1 <= z1) a_p[r] = load(r, z + 2);
+            b_m[r] = b_c[r];
+            b_c[r] = l1[r];
+        }
+        cur = 1 - cur;
     }
 }
 
@@ -223,58 +345,147 @@ void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, i
     default_tile(nx, ny, nz, bx, by, ry, zc);
 }
 
+// Two fused sweeps in -> out (heat_tb2): window height 4*ry rows (ry 4 or 8),
+// z-chunk zc.
+int heat3d_sweep2(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, int ry, int zc,
+                  hipStream_t stream)
+{
+    PAT_REQUIRE(ry == 4 || ry == 8, PAT_ERR_UNSUPPORTED, "heat3d: fused window of %d rows per thread not instantiated",
+                ry);
+    PAT_REQUIRE(zc > 0, PAT_ERR_INVALID, "heat3d: z-chunk must be positive");
+    const int oy = 4 * ry - 4;
+    const int tiles_x = (int)((nx + 59) / 60);
+    const int tiles_y = (int)((ny + oy - 1) / oy);
+    const int tiles_z = (int)((nz + zc - 1) / zc);
+    const int64_t nblocks = (int64_t)tiles_x * tiles_y * tiles_z;
+    PAT_REQUIRE(nblocks < (1LL << 31), PAT_ERR_INVALID, "heat3d: grid too large");
+    auto k = ry == 4 ? heat_tb2<4> : heat_tb2<8>;
+    hipLaunchKernelGGL(k, dim3((unsigned)nblocks), dim3(64, 4), 0, stream, in, out, (int)nx, (int)ny, (int)nz, zc,
+                       alpha, tiles_x, tiles_y, tiles_z);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// One launch of tile t: a single sweep (fused == 0) or two (fused == 1).
+int heat3d_launch(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, const HeatTile &t,
+                  int variant, hipStream_t stream)
+{
+    if (t.fused) return heat3d_sweep2(in, out, nx, ny, nz, alpha, t.ry, t.zc, stream);
+    return heat3d_sweep(in, out, nx, ny, nz, alpha, variant, t.bx, t.by, t.ry, t.zc, stream);
+}
+
+// nsteps sweeps ping-ponging buf[start] -> buf[1-start] ...: fused pairs
+// while two or more steps remain, a plain sweep for an odd last one.
+// Returns the buffer index holding the result in *end.
+int heat3d_steps(float *const buf[2], int start, int64_t nx, int64_t ny, int64_t nz, float alpha, const HeatTile &t,
+                 int variant, int nsteps, hipStream_t stream, int *end)
+{
+    int c = start;
+    int k = 0;
+    if (t.fused) {
+        for (; k + 2 <= nsteps; k += 2) {
+            PAT_TRY(heat3d_sweep2(buf[c], buf[1 - c], nx, ny, nz, alpha, t.ry, t.zc, stream));
+            c = 1 - c;
+        }
+    }
+    if (k < nsteps) {
+        HeatTile p = t;
+        if (t.fused) {  // the odd step runs on the plain kernel
+            p.fused = 0;
+            default_tile(nx, ny, nz, &p.bx, &p.by, &p.ry, &p.zc);
+        }
+        for (; k < nsteps; k++) {
+            PAT_TRY(heat3d_sweep(buf[c], buf[1 - c], nx, ny, nz, alpha, variant, p.bx, p.by, p.ry, p.zc, stream));
+            c = 1 - c;
+        }
+    }
+    *end = c;
+    return PAT_OK;
+}
+
 // Autotuner: times every instantiated tile shape with z-chunks of 8..64
 // planes on the caller's grid (in -> out; out is scratch until the first
 // real sweep writes it, and no kernel writes the halo), keeps the fastest,
 // and caches it per grid shape for the process.
-static std::map<std::tuple<int64_t, int64_t, int64_t>, std::tuple<int, int, int, int>> g_tile_cache;
+static std::map<std::tuple<int64_t, int64_t, int64_t, int>, HeatTile> g_tile_cache;
 static std::mutex g_tile_mutex;
 
-int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
-                      int *bx, int *by, int *ry, int *zc)
+// allow_fused: whether two-sweep tiles may be chosen (the caller runs
+// pairs of steps); their time is compared per step.
+int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
+                 bool allow_fused, HeatTile *t)
 {
-    const auto key = std::make_tuple(nx, ny, nz);
+    const auto key = std::make_tuple(nx, ny, nz, (int)allow_fused);
     {
         std::lock_guard<std::mutex> g(g_tile_mutex);
         auto it = g_tile_cache.find(key);
         if (it != g_tile_cache.end()) {
-            std::tie(*bx, *by, *ry, *zc) = it->second;
+            *t = it->second;
             return PAT_OK;
         }
     }
-    default_tile(nx, ny, nz, bx, by, ry, zc);
+    t->fused = 0;
+    default_tile(nx, ny, nz, &t->bx, &t->by, &t->ry, &t->zc);
     if (!pat::tuning_enabled()) return PAT_OK;
     hipEvent_t a, b;
     PAT_HIP(hipEventCreate(&a));
     PAT_HIP(hipEventCreate(&b));
     float best = 1e30f;
+    auto time_it = [&](const HeatTile &c, float *per_step) -> int {
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            PAT_HIP(hipEventRecord(a, st));
+            PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, c, PAT_STENCIL_LDS25D, st));
+            PAT_HIP(hipEventRecord(b, st));
+            PAT_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            PAT_HIP(hipEventElapsedTime(&ms, a, b));
+            if (rep > 0) tmin = std::min(tmin, ms);
+        }
+        *per_step = c.fused ? tmin / 2 : tmin;
+        return PAT_OK;
+    };
     for (const auto &sh : kShapes) {
         for (int z : {8, 16, 32, 64}) {
             if (z > nz && z != 8) continue;
-            const int zz = (int)std::min<int64_t>(z, nz);
-            float tmin = 1e30f;
-            for (int rep = 0; rep < 3; rep++) {
-                PAT_HIP(hipEventRecord(a, st));
-                PAT_TRY(heat3d_sweep(in, out, nx, ny, nz, alpha, PAT_STENCIL_LDS25D, sh.bx, sh.by, sh.ry, zz, st));
-                PAT_HIP(hipEventRecord(b, st));
-                PAT_HIP(hipEventSynchronize(b));
-                float ms = 0;
-                PAT_HIP(hipEventElapsedTime(&ms, a, b));
-                if (rep > 0) tmin = std::min(tmin, ms);
-            }
-            if (tmin < best) {
-                best = tmin;
-                *bx = sh.bx;
-                *by = sh.by;
-                *ry = sh.ry;
-                *zc = zz;
+            const HeatTile c{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)};
+            float ms;
+            PAT_TRY(time_it(c, &ms));
+            if (ms < best) {
+                best = ms;
+                *t = c;
             }
         }
+    }
+    if (allow_fused) {
+        for (int r : {4, 8})
+            for (int z : {16, 32, 64, 128}) {
+                if (z > nz && z != 16) continue;
+                const HeatTile c{1, 64, 4, r, (int)std::min<int64_t>(z, nz)};
+                float ms;
+                PAT_TRY(time_it(c, &ms));
+                if (ms < best) {
+                    best = ms;
+                    *t = c;
+                }
+            }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     std::lock_guard<std::mutex> g(g_tile_mutex);
-    g_tile_cache[key] = std::make_tuple(*bx, *by, *ry, *zc);
+    g_tile_cache[key] = *t;
+    return PAT_OK;
+}
+
+int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
+                      int *bx, int *by, int *ry, int *zc)
+{
+    HeatTile t;
+    PAT_TRY(heat3d_tuned(in, out, nx, ny, nz, alpha, st, false, &t));
+    *bx = t.bx;
+    *by = t.by;
+    *ry = t.ry;
+    *zc = t.zc;
     return PAT_OK;
 }
 
@@ -299,47 +510,41 @@ extern "C" int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, in
     PAT_REQUIRE(a && b && a != b, PAT_ERR_INVALID, "pat_heat3d_run_dev: need two distinct device grids");
     PAT_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nsteps >= 0, PAT_ERR_INVALID, "pat_heat3d_run_dev: bad sizes");
     PAT_REQUIRE((nx + 2) * (ny + 2) * (nz + 2) < (1LL << 40), PAT_ERR_INVALID, "pat_heat3d_run_dev: grid too large");
-    PAT_REQUIRE(variant == PAT_STENCIL_AUTO || variant == PAT_STENCIL_LDS25D || variant == PAT_STENCIL_GENERIC,
-                PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: unknown variant %d", variant);
-    int dbx, dby, dry, dzc;
-    if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0)
-        PAT_TRY(pat::heat3d_tuned_tile(a, b, nx, ny, nz, alpha, (hipStream_t)stream, &dbx, &dby, &dry, &dzc));
-    else
-        default_tile(nx, ny, nz, &dbx, &dby, &dry, &dzc);
-    // 'by' at the C ABI is the tile height in rows; split it into thread rows
-    // and rows per thread using the instantiated shapes.
-    int tbx = bx > 0 ? bx : dbx;
-    int tby = dby, tr = dry;
-    if (by > 0) {
-        bool found = false;
-        for (const auto &s : kShapes)
-            if (s.bx == tbx && s.by * s.ry == by) {
-                tby = s.by;
-                tr = s.ry;
-                found = true;
-                break;
-            }
-        PAT_REQUIRE(found, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: tile %dx%d not instantiated", tbx, by);
-    } else if (bx > 0 && bx != dbx) {
-        bool found = false;
-        for (const auto &s : kShapes)
-            if (s.bx == tbx) {
-                tby = s.by;
-                tr = s.ry;
-                found = true;
-                break;
-            }
-        PAT_REQUIRE(found, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: tile width %d not instantiated", tbx);
-    }
-    int tzc = zc > 0 ? zc : dzc;
+    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_TEMPORAL2, PAT_ERR_UNSUPPORTED,
+                "pat_heat3d_run_dev: unknown variant %d", variant);
     hipStream_t st = (hipStream_t)stream;
-    float *src = a, *dst = b;
-    for (int s = 0; s < nsteps; s++) {
-        PAT_TRY(pat::heat3d_sweep(src, dst, nx, ny, nz, alpha, variant, tbx, tby, tr, tzc, st));
-        float *t = src;
-        src = dst;
-        dst = t;
+    pat::HeatTile t;
+    if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0) {
+        PAT_TRY(pat::heat3d_tuned(a, b, nx, ny, nz, alpha, st, true, &t));
+    } else if (variant == PAT_STENCIL_TEMPORAL2) {
+        PAT_REQUIRE(bx == 0 || bx == 64, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: fused windows are 64 wide");
+        PAT_REQUIRE(by == 0 || by == 16 || by == 32, PAT_ERR_UNSUPPORTED,
+                    "pat_heat3d_run_dev: fused windows are 16 or 32 rows");
+        t = pat::HeatTile{1, 64, 4, by == 16 ? 4 : 8, zc > 0 ? zc : (int)std::min<int64_t>(64, nz)};
+    } else {
+        t.fused = 0;
+        default_tile(nx, ny, nz, &t.bx, &t.by, &t.ry, &t.zc);
+        // 'by' at the C ABI is the tile height in rows; split it into thread
+        // rows and rows per thread using the instantiated shapes.
+        const int tbx = bx > 0 ? bx : t.bx;
+        if (by > 0 || tbx != t.bx) {
+            bool found = false;
+            for (const auto &sh : kShapes)
+                if (sh.bx == tbx && (by <= 0 || sh.by * sh.ry == by)) {
+                    t.by = sh.by;
+                    t.ry = sh.ry;
+                    found = true;
+                    break;
+                }
+            PAT_REQUIRE(found, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: tile %dx%d not instantiated", tbx, by);
+        }
+        t.bx = tbx;
+        if (zc > 0) t.zc = zc;
     }
-    if (result_in_b) *result_in_b = (src == b) ? 1 : 0;
+    const int v = variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
+    float *const buf[2] = {a, b};
+    int end = 0;
+    PAT_TRY(pat::heat3d_steps(buf, 0, nx, ny, nz, alpha, t, v, nsteps, st, &end));
+    if (result_in_b) *result_in_b = end;
     return PAT_OK;
 }

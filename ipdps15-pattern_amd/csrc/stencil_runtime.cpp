@@ -12,14 +12,7 @@
 
 #include "pat_common.hpp"
 
-namespace pat {
-int heat3d_sweep(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, int variant, int bx,
-                 int by, int ry, int zc, hipStream_t stream);
-void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, int *ry, int *zc);
-int heat3d_shapes(const int **triples);
-int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
-                      int *bx, int *by, int *ry, int *zc);
-}  // namespace pat
+#include "stencil_dev.hpp"
 
 struct pat_stencil {
     int ndims = 3;
@@ -34,14 +27,15 @@ struct pat_stencil {
     int cur = 0;  // buffer holding the current field
     bool loaded = false;
     hipStream_t stream = nullptr;
-    // captured step sequences keyed by (nsteps, starting buffer)
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    // captured step sequences keyed by (nsteps, starting buffer), with the
+    // buffer each leaves the result in
+    std::map<std::pair<int, int>, std::pair<hipGraphExec_t, int>> graphs;
 
     int64_t cells() const { return (n[0] + 2 * width) * (n[1] + 2 * width) * (n[2] + 2 * width); }
 
     void drop_graphs()
     {
-        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second.first);
         graphs.clear();
     }
 
@@ -52,32 +46,36 @@ struct pat_stencil {
         return a;
     }
 
-    int resolve_tile(int *tbx, int *tby, int *tr, int *tzc)
+    // The tile the runs use: tuned (plain or fused) for AUTO without an
+    // explicit tile, else the variant's default with the set fields.
+    int resolve_tile(pat::HeatTile *t)
     {
-        int dbx, dby, dry, dzc;
-        if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0 && loaded)
-            PAT_TRY(pat::heat3d_tuned_tile(buf[cur], buf[1 - cur], n[0], n[1], n[2], alpha(), stream, &dbx, &dby, &dry,
-                                           &dzc));
-        else
-            pat::heat3d_default_tile(n[0], n[1], n[2], &dbx, &dby, &dry, &dzc);
-        *tbx = bx > 0 ? bx : dbx;
-        *tby = by > 0 ? by : dby;
-        *tr = ry > 0 ? ry : dry;
-        *tzc = zc > 0 ? zc : dzc;
+        if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0 && loaded) {
+            PAT_TRY(pat::heat3d_tuned(buf[cur], buf[1 - cur], n[0], n[1], n[2], alpha(), stream, true, t));
+            return PAT_OK;
+        }
+        if (variant == PAT_STENCIL_TEMPORAL2) {
+            *t = pat::HeatTile{1, 64, 4, ry > 0 ? ry : 8, zc > 0 ? zc : (int)std::min<int64_t>(64, n[2])};
+            return PAT_OK;
+        }
+        t->fused = 0;
+        pat::heat3d_default_tile(n[0], n[1], n[2], &t->bx, &t->by, &t->ry, &t->zc);
+        if (bx > 0) t->bx = bx;
+        if (by > 0) t->by = by;
+        if (ry > 0) t->ry = ry;
+        if (zc > 0) t->zc = zc;
         return PAT_OK;
     }
 
-    int enqueue(int nsteps, int start)
+    int kernel_variant() const { return variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D; }
+
+    // *end = the buffer holding the result (fused launches advance two
+    // steps per buffer swap)
+    int enqueue(int nsteps, int start, int *end)
     {
-        int tbx, tby, tr, tzc;
-        PAT_TRY(resolve_tile(&tbx, &tby, &tr, &tzc));
-        int v = variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
-        int s = start;
-        for (int k = 0; k < nsteps; k++) {
-            PAT_TRY(pat::heat3d_sweep(buf[s], buf[1 - s], n[0], n[1], n[2], alpha(), v, tbx, tby, tr, tzc, stream));
-            s = 1 - s;
-        }
-        return PAT_OK;
+        pat::HeatTile t;
+        PAT_TRY(resolve_tile(&t));
+        return pat::heat3d_steps(buf, start, n[0], n[1], n[2], alpha(), t, kernel_variant(), nsteps, stream, end);
     }
 };
 
@@ -132,10 +130,15 @@ int pat_stencil_set_idx(pat_stencil *s, int idx)
 int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
 {
     PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_set_variant: null handle");
-    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_GENERIC, PAT_ERR_UNSUPPORTED,
+    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_TEMPORAL2, PAT_ERR_UNSUPPORTED,
                 "pat_stencil_set_variant: unknown variant %d", variant);
     int tby = 0, tr = 0;
-    if (by > 0) {
+    if (variant == PAT_STENCIL_TEMPORAL2) {
+        PAT_REQUIRE((bx == 0 || bx == 64) && (by == 0 || by == 16 || by == 32) && zc >= 0, PAT_ERR_UNSUPPORTED,
+                    "pat_stencil_set_variant: fused windows are 64 wide and 16 or 32 rows");
+        tby = by > 0 ? 4 : 0;
+        tr = by == 16 ? 4 : (by == 32 ? 8 : 0);
+    } else if (by > 0) {
         const int *sh;
         int ns = pat::heat3d_shapes(&sh);
         int wbx = bx > 0 ? bx : 64;
@@ -180,11 +183,12 @@ int pat_stencil_run(pat_stencil *s, int nsteps)
     auto key = std::make_pair(nsteps, s->cur);
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
-        int tbx, tby, tr, tzc;
-        PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));  // tunes outside the capture
+        pat::HeatTile t;
+        PAT_TRY(s->resolve_tile(&t));  // tunes outside the capture
         hipGraph_t g;
+        int end = s->cur;
         PAT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int rc = s->enqueue(nsteps, s->cur);
+        int rc = s->enqueue(nsteps, s->cur, &end);
         hipError_t ec = hipStreamEndCapture(s->stream, &g);
         if (rc != PAT_OK) return rc;
         PAT_HIP(ec);
@@ -192,11 +196,11 @@ int pat_stencil_run(pat_stencil *s, int nsteps)
         hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         PAT_HIP(ei);
-        it = s->graphs.emplace(key, ge).first;
+        it = s->graphs.emplace(key, std::make_pair(ge, end)).first;
     }
-    PAT_HIP(hipGraphLaunch(it->second, s->stream));
+    PAT_HIP(hipGraphLaunch(it->second.first, s->stream));
     PAT_HIP(hipStreamSynchronize(s->stream));
-    if (nsteps % 2) s->cur = 1 - s->cur;
+    s->cur = it->second.second;
     return PAT_OK;
 }
 
@@ -207,16 +211,29 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_stencil_run_timed: need at least one step");
     std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
     for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
-    int tbx, tby, tr, tzc;
-    PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));
-    const int v = s->variant == PAT_STENCIL_GENERIC ? PAT_STENCIL_GENERIC : PAT_STENCIL_LDS25D;
+    pat::HeatTile t;
+    PAT_TRY(s->resolve_tile(&t));
+    const int v = s->kernel_variant();
+    // one launch per step, or per pair of steps for a fused tile (then each
+    // step of the pair is reported as half the launch); an odd last step
+    // of a fused run uses the plain kernel
+    pat::HeatTile plain = t;
+    if (t.fused) {
+        plain.fused = 0;
+        pat::heat3d_default_tile(s->n[0], s->n[1], s->n[2], &plain.bx, &plain.by, &plain.ry, &plain.zc);
+    }
+    std::vector<int> steps_of;
     int rc = PAT_OK;
     int c = s->cur;
-    for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
-        (void)hipEventRecord(ev[2 * k], s->stream);
-        rc = pat::heat3d_sweep(s->buf[c], s->buf[1 - c], s->n[0], s->n[1], s->n[2], s->alpha(), v, tbx, tby, tr, tzc,
-                               s->stream);
-        (void)hipEventRecord(ev[2 * k + 1], s->stream);
+    int launches = 0;
+    for (int k = 0; k < nsteps && rc == PAT_OK; launches++) {
+        const bool two = t.fused && k + 2 <= nsteps;
+        (void)hipEventRecord(ev[2 * launches], s->stream);
+        rc = pat::heat3d_launch(s->buf[c], s->buf[1 - c], s->n[0], s->n[1], s->n[2], s->alpha(), two ? t : plain, v,
+                                s->stream);
+        (void)hipEventRecord(ev[2 * launches + 1], s->stream);
+        steps_of.push_back(two ? 2 : 1);
+        k += two ? 2 : 1;
         c = 1 - c;
     }
     if (rc == PAT_OK && hipStreamSynchronize(s->stream) != hipSuccess) {
@@ -224,9 +241,15 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
         rc = PAT_ERR_HIP;
     }
     if (rc == PAT_OK) {
-        (void)hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps - 1]);
-        if (kernel_ms)
-            for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
+        (void)hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)launches - 1]);
+        if (kernel_ms) {
+            int k = 0;
+            for (int l = 0; l < launches; l++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]);
+                for (int j = 0; j < steps_of[l]; j++) kernel_ms[k++] = ms / steps_of[l];
+            }
+        }
         s->cur = c;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -245,11 +268,11 @@ int pat_stencil_download(pat_stencil *s, void *host)
 int pat_stencil_tile(pat_stencil *s, int *bx, int *by, int *zc)
 {
     PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_tile: null handle");
-    int tbx, tby, tr, tzc;
-    PAT_TRY(s->resolve_tile(&tbx, &tby, &tr, &tzc));
-    if (bx) *bx = tbx;
-    if (by) *by = tby * tr;
-    if (zc) *zc = tzc;
+    pat::HeatTile t;
+    PAT_TRY(s->resolve_tile(&t));
+    if (bx) *bx = t.bx;
+    if (by) *by = t.by * t.ry;
+    if (zc) *zc = t.zc;
     return PAT_OK;
 }
 

@@ -17,6 +17,7 @@ PAT_STENCIL_HEAT = 0
 PAT_STENCIL_AUTO = 0
 PAT_STENCIL_LDS25D = 1
 PAT_STENCIL_GENERIC = 2
+PAT_STENCIL_TEMPORAL2 = 3
 PAT_IR_LJ_MINIMAGE = 0
 PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0

@@ -38,6 +38,35 @@ def test_lds25d_tiles(pa, orc, gpu, bx, by, zc):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 4), (60, 12, 5), (61, 13, 17), (130, 29, 2), (64, 57, 33),
+                                   (200, 33, 12)])
+@pytest.mark.parametrize("steps", [1, 2, 5])
+def test_temporal2_bitexact_ragged(pa, orc, gpu, shape, steps):
+    """Two fused sweeps per pass: every window and chunk edge, odd step counts."""
+    nx, ny, nz = shape
+    g = pa.gen_heat3d(nx, ny, nz, seed=21)
+    ref = orc.heat3d(g, steps)
+    for by, zc in ((16, 1), (32, 3), (16, 64)):
+        got = run_dev(pa, g, steps, variant=pa.PAT_STENCIL_TEMPORAL2, bx=64, by=by, zc=zc)
+        assert np.array_equal(got, ref), (by, zc)
+
+
+def test_temporal2_handle_replay(pa, orc, gpu):
+    nx, ny, nz = 70, 50, 40
+    g = pa.gen_heat3d(nx, ny, nz, seed=15)
+    s = pa.Stencil(nx, ny, nz)
+    s.set_variant(pa.PAT_STENCIL_TEMPORAL2, 64, 32, 8)
+    s.upload(g)
+    s.run(7)
+    s.run(7)  # replays the captured graph from the other buffer
+    s.run(6)
+    tot, kms = s.run_timed(3)
+    got = s.download()
+    s.close()
+    assert len(kms) == 3 and np.all(kms > 0)
+    assert np.array_equal(got, orc.heat3d(g, 23))
+
+
 def test_naive_bitexact(pa, orc, gpu):
     g = pa.gen_heat3d(45, 30, 20, seed=13)
     ref = orc.heat3d(g, 4)
