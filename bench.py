@@ -9,8 +9,9 @@ to both endpoints, velocity-Verlet with dt = 0.005, FP32.  One step = one
 force evaluation plus the velocity-Verlet update; value = pairs * steps / s
 with the handle's default node-ordered rows (edge-ordered rows and the
 atomic kernel are reported beside it).  Algorithmic bytes per step:
-4 B per CSR entry (2 per pair) + 60 B per node (position read, velocity
-read and write, position and force writes, 12 B each).
+4 B per CSR entry (2 per pair) + 48 B per node (position read, velocity
+read and write, position write, 12 B each; a kick-drift step's force only
+feeds its own half kicks and is not stored).
 
 Secondary engines in the same JSON line: the heat3D stencil (128^3 and 384^3
 FP32, 100 steps) and, when built, kmeans.  roofline: the dominant kernel's
@@ -70,7 +71,7 @@ def time_md(md, steps, warmup, n, E):
     wall = time.perf_counter() - w0
     _, kms = md.run_timed(steps)
     kavg = float(np.mean(kms))
-    alg = 8 * E + 60 * n
+    alg = 8 * E + 48 * n
     return {"pairs_per_s": E * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_us": kavg * 1e3,
             "achieved_gbs": alg / (kavg * 1e-3) / 1e9, "kernel": LANES_NAME.get(md.variant()[1], "")}
 
@@ -80,7 +81,7 @@ def bench_lj(steps, warmup, do_cpu):
     pos, box, edges = pa.moldyn_system(40, seed=42)
     n, E = len(pos), len(edges)
     log(f"[lj] moldyn 40^3: {n} molecules, {E} pairs, set-up {time.perf_counter() - t0:.1f}s")
-    out = {"pairs": E, "nodes": n, "alg_bytes": 8 * E + 60 * n}
+    out = {"pairs": E, "nodes": n, "alg_bytes": 8 * E + 48 * n}
     for name, order in (("node_order", pa.PAT_IR_ORDER_NODE), ("edge_order", pa.PAT_IR_ORDER_EDGE)):
         md = pa.MD(pos, None, edges, box, 2.5, 0.005, order=order)
         r = time_md(md, steps, warmup, n, E)

@@ -284,6 +284,14 @@ int pat_md_set_row_order(pat_md *h, int order);
 /* The kernel in use: variant, lanes per row (after tuning; PAT_IR_SLICED =
  * the sliced layout) and row order. */
 int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order);
+/* Selects the sliced layout with loop form 0 (plain), 1 (software-pipelined
+ * by 4), 2 (by 8), or 3, 4 (neighbour positions staged per 256-row block in
+ * LDS, 4 or 8 entries ahead; PAT_ERR_UNSUPPORTED when a block's neighbour
+ * list does not fit) in place of the tuner's choice; recomputes the stored
+ * force.  All forms give the same bits.  Tuning override of
+ * pat_md_set_variant(h, PAT_IR_CSR, PAT_IR_SLICED); no reference
+ * counterpart (the reference has one kernel form). */
+int pat_md_set_sliced_form(pat_md *h, int form);
 /* Current half-pair count, steps run, neighbour rebuilds and their total
  * host-measured time. */
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);

@@ -30,6 +30,7 @@
 // row entry is exactly the negation the serial scatter applies.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pat_common.hpp"
@@ -228,6 +229,8 @@ struct LJParams {
     float rc2;     // cutoff squared
     float dt, hdt; // time step and half step (hdt = 0.5f*dt rounded once)
     int pbc;
+    float W0;      // least float w with w*w >= rc2 (rounded product)
+    int *inbox;    // 1 while every position lies in [0, L] (may be null)
 };
 
 // 1.0f / b, correctly rounded.  For b in [2^-126, 2^125] one Newton step on
@@ -245,6 +248,17 @@ __device__ __forceinline__ float exact_rcp(float b)
     return r;
 }
 
+// The same refined reciprocal without the range guard, for r^2 inside a
+// cutoff (rc^2 < 2^31).  Below 2^-126 -- two nodes closer than 2^-63 -- the
+// serial code's r^-4 already overflows and its force is NaN, and so is this
+// one's (r^-1 is then >= 2^125 or inf, r^-4 inf or NaN).
+__device__ __forceinline__ float rcp_in_cutoff(float b)
+{
+    const float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
+}
+
 // Minimum image of one component, the functor's two conditional moves
 // (kmeans/irregular.h:55-60) as one subtraction: for |d| < L at most one of
 // them applies -- d < -hL or d > hL, i.e. |d| > hL -- and the move is
@@ -255,32 +269,110 @@ __device__ __forceinline__ float min_image_sel(float d, float L, float hL)
     return d - s;
 }
 
-// Force on i from j (both float4 positions); returns 0 beyond the cutoff.
-__device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams &p, float &fx, float &fy, float &fz)
+// The same move when every position lies in [0, L]: a row with x_i < L/2
+// can only see d = x_i - x_j < -L/2 (x_j > L is impossible), one with
+// x_i >= L/2 only d > L/2, so the move is d + s with s = +L or -L fixed
+// per row (the image sign), and d + (-L) is exactly d - L.
+__device__ __forceinline__ float min_image_dir(float d, float s, float hL)
+{
+    return fabsf(d) > hL ? d + s : d;
+}
+
+struct ImageSign {
+    float x, y, z;  // +L or -L, see min_image_dir
+};
+
+__device__ __forceinline__ ImageSign image_sign(float4 pi, const LJParams &p)
+{
+    return ImageSign{pi.x < p.hL ? p.L : -p.L, pi.y < p.hL ? p.L : -p.L, pi.z < p.hL ? p.L : -p.L};
+}
+
+// Force on i from j (both float4 positions), the minimum image taken on the
+// axes set in WM (bit 0 x, 1 y, 2 z): in the general two-sided form when
+// bit 3 is set, else in the per-row form of min_image_dir (signs sg).  Branch-free: beyond the cutoff the
+// reciprocal is replaced by 0, so F = 0 and each component is +0 or -0;
+// added to a running sum that started at +0 either leaves every bit of it
+// unchanged (a sum started at +0 is never -0 under round-to-nearest), so the
+// sums equal the serial code's, which skips such pairs.
+// rrd7 - 0.5*rrd4 is one fma: 0.5*rrd4 is exact for rrd4 >= 2^-125, which
+// holds inside any cutoff with rc^2 < 2^31.
+template <int WM>
+__device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams &p, const ImageSign &sg, float &fx,
+                                           float &fy, float &fz)
 {
     float dx = pi.x - pj.x;
     float dy = pi.y - pj.y;
     float dz = pi.z - pj.z;
-    if (p.pbc) {
-        dx = min_image_sel(dx, p.L, p.hL);
-        dy = min_image_sel(dy, p.L, p.hL);
-        dz = min_image_sel(dz, p.L, p.hL);
+    if (WM & 8) {
+        if (WM & 1) dx = min_image_sel(dx, p.L, p.hL);
+        if (WM & 2) dy = min_image_sel(dy, p.L, p.hL);
+        if (WM & 4) dz = min_image_sel(dz, p.L, p.hL);
+    } else {
+        if (WM & 1) dx = min_image_dir(dx, sg.x, p.hL);
+        if (WM & 2) dy = min_image_dir(dy, sg.y, p.hL);
+        if (WM & 4) dz = min_image_dir(dz, sg.z, p.hL);
     }
     float rd = dx * dx + dy * dy;
     rd = rd + dz * dz;
-    if (rd < p.rc2) {
-        const float rrd = exact_rcp(rd);
-        const float rrd2 = rrd * rrd;
-        const float rrd4 = rrd2 * rrd2;
-        const float rrd6 = rrd2 * rrd4;
-        const float rrd7 = rrd6 * rrd;
-        const float F = 48.0f * (rrd7 - 0.5f * rrd4);
-        fx = dx * F;
-        fy = dy * F;
-        fz = dz * F;
-    } else {
-        fx = fy = fz = 0.0f;
-    }
+    const float rrd = rd < p.rc2 ? rcp_in_cutoff(rd) : 0.0f;
+    const float rrd2 = rrd * rrd;
+    const float rrd4 = rrd2 * rrd2;
+    const float rrd6 = rrd2 * rrd4;
+    const float rrd7 = rrd6 * rrd;
+    const float F = 48.0f * __builtin_fmaf(-0.5f, rrd4, rrd7);
+    fx = dx * F;
+    fy = dy * F;
+    fz = dz * F;
+}
+
+__device__ __forceinline__ void pair_force(float4 pi, float4 pj, const LJParams &p, float &fx, float &fy, float &fz)
+{
+    const ImageSign none{0.0f, 0.0f, 0.0f};
+    if (p.pbc) pair_force<15>(pi, pj, p, none, fx, fy, fz);
+    else pair_force<0>(pi, pj, p, none, fx, fy, fz);
+}
+
+// Axes on which a wavefront's pairs may need the minimum image (the WM of
+// pair_force).  When every position lies in [0, L] (the inbox flag) and all
+// 64 rows' own coordinate on axis a lies in [W, L - W], W = W0 + L*2^-20,
+// no pair of these rows needs the image on a, or needs it only where both
+// forms lie beyond the cutoff: |d| > L/2 >= W0 without the move, and
+// L - |d| >= L - max(x_i, L - x_i) >= W minus a few roundings of L (each
+// under L*2^-24) >= W0 with it -- both give r^2 >= rc^2, a zero force.  The
+// remaining axes take the per-row form.  Without the flag: all axes, two-
+// sided form.  The sentinel node stays beyond the cutoff in every form.
+template <bool SENT>
+__device__ __forceinline__ int axes_to_wrap(float4 pi, const LJParams &p)
+{
+    if (!p.pbc) return 0;
+    if (!SENT || !p.inbox || !*p.inbox) return 15;
+    const float W = p.W0 + p.L * 0x1p-20f, hi = p.L - W;
+    int wm = 0;
+    if (!__all(pi.x >= W && pi.x <= hi)) wm |= 1;
+    if (!__all(pi.y >= W && pi.y <= hi)) wm |= 2;
+    if (!__all(pi.z >= W && pi.z <= hi)) wm |= 4;
+    return wm;
+}
+
+// Class of a node for the slice order: bit a set when its coordinate on
+// axis a lies within W of a face (W as in axes_to_wrap).  Slices filled
+// with one class take the minimum image on exactly those axes.
+__device__ __forceinline__ int row_class(float4 x, float L, float W0)
+{
+    const float W = W0 + L * 0x1p-20f, hi = L - W;
+    return (x.x >= W && x.x <= hi ? 0 : 1) | (x.y >= W && x.y <= hi ? 0 : 2) | (x.z >= W && x.z <= hi ? 0 : 4);
+}
+
+// Periodic wrap of a drifted position; clears the inbox flag when a
+// coordinate still lies outside [0, L] (a step longer than the box).
+__device__ __forceinline__ void drift_wrap(float4 &x, const LJParams &p)
+{
+    if (!p.pbc) return;
+    x.x = pat::wrap_coord(x.x, p.L);
+    x.y = pat::wrap_coord(x.y, p.L);
+    x.z = pat::wrap_coord(x.z, p.L);
+    if (p.inbox && !(x.x >= 0.0f && x.x <= p.L && x.y >= 0.0f && x.y <= p.L && x.z >= 0.0f && x.z <= p.L))
+        *p.inbox = 0;
 }
 
 // Potential of one pair (4(r^-12 - r^-6)), 0 beyond the cutoff.
@@ -307,7 +399,7 @@ __device__ __forceinline__ float pair_phi(float4 pi, float4 pj, const LJParams &
 enum : int {
     kForceOnly = 0,   // write f
     kKick = 1,        // f; v += hdt*f
-    kKickDrift = 2,   // f; v += hdt*f; v += hdt*f; x_out = wrap(x + dt*v)
+    kKickDrift = 2,   // v += hdt*f; v += hdt*f; x_out = wrap(x + dt*v) (f itself not stored)
 };
 
 // Tree-order variant: rows handled by groups of LPR lanes that stride the
@@ -365,11 +457,7 @@ __global__ __launch_bounds__(256) void lj_csr_tree(const float4 *__restrict__ po
                 x.y = pi.y + p.dt * v.y;
                 x.z = pi.z + p.dt * v.z;
                 x.w = pi.w;
-                if (p.pbc) {
-                    x.x = pat::wrap_coord(x.x, p.L);
-                    x.y = pat::wrap_coord(x.y, p.L);
-                    x.z = pat::wrap_coord(x.z, p.L);
-                }
+                drift_wrap(x, p);
                 pos_out[row] = x;
             }
             vel[row] = v;
@@ -377,14 +465,17 @@ __global__ __launch_bounds__(256) void lj_csr_tree(const float4 *__restrict__ po
     }
 }
 
-// Shared epilogue: force write and the fused velocity-Verlet update.
-__device__ __forceinline__ void lj_epilogue(int row, float4 pi, float fx, float fy, float fz, const LJParams &p,
-                                            float4 *__restrict__ force, float4 *__restrict__ vel,
-                                            float4 *__restrict__ pos_out, int mode)
+// Shared epilogue: the fused velocity-Verlet update and the force write,
+// v being the node's velocity (loaded by the caller; unused for
+// kForceOnly).  A kick-drift step's force only feeds its own two half
+// kicks, so it is not stored; a force evaluation or the closing kKick step
+// of a run stores it (the next run's first half kick reads it).
+__device__ __forceinline__ void lj_finish(int row, float4 pi, float4 v, float fx, float fy, float fz,
+                                          const LJParams &p, float4 *__restrict__ force, float4 *__restrict__ vel,
+                                          float4 *__restrict__ pos_out, int mode)
 {
-    force[row] = make_float4(fx, fy, fz, 0.0f);
+    if (mode != kKickDrift) force[row] = make_float4(fx, fy, fz, 0.0f);
     if (mode == kForceOnly) return;
-    float4 v = vel[row];
     v.x = v.x + p.hdt * fx;
     v.y = v.y + p.hdt * fy;
     v.z = v.z + p.hdt * fz;
@@ -397,14 +488,18 @@ __device__ __forceinline__ void lj_epilogue(int row, float4 pi, float fx, float 
         x.y = pi.y + p.dt * v.y;
         x.z = pi.z + p.dt * v.z;
         x.w = pi.w;
-        if (p.pbc) {
-            x.x = pat::wrap_coord(x.x, p.L);
-            x.y = pat::wrap_coord(x.y, p.L);
-            x.z = pat::wrap_coord(x.z, p.L);
-        }
+        drift_wrap(x, p);
         pos_out[row] = x;
     }
     vel[row] = v;
+}
+
+__device__ __forceinline__ void lj_epilogue(int row, float4 pi, float fx, float fy, float fz, const LJParams &p,
+                                            float4 *__restrict__ force, float4 *__restrict__ vel,
+                                            float4 *__restrict__ pos_out, int mode)
+{
+    const float4 v = mode != kForceOnly ? vel[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    lj_finish(row, pi, v, fx, fy, fz, p, force, vel, pos_out, mode);
 }
 
 // In-order row sums, one lane per row: the lane walks its row in CSR order
@@ -534,68 +629,97 @@ __global__ __launch_bounds__(256) void rcp_check(uint32_t lo, uint32_t hi, unsig
 // entries repeat the row's own index (rows past N repeat node N-1) and are
 // masked out of the sum.
 
-__global__ void sell_width(const int *__restrict__ rowptr, int N, int nsl, int *__restrict__ width)
+// Slice s holds rows perm[64s .. 64s+63] (rows 64s.. when perm is null;
+// entries >= N are empty lanes).
+__device__ __forceinline__ int slice_row(const int *__restrict__ perm, int i)
+{
+    return perm ? perm[i] : i;
+}
+
+__global__ void sell_width(const int *__restrict__ rowptr, const int *__restrict__ perm, int N, int nsl,
+                           int *__restrict__ width)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsl) return;
-    const int r0 = s * kWave, r1 = min(r0 + kWave, N);
     int w = 0;
-    for (int r = r0; r < r1; r++) w = max(w, rowptr[r + 1] - rowptr[r]);
+    for (int i = s * kWave; i < s * kWave + kWave; i++) {
+        const int r = slice_row(perm, i);
+        if (r < N) w = max(w, rowptr[r + 1] - rowptr[r]);
+    }
     width[s] = w;
 }
 
-// pad < 0: padding repeats the row's own index; otherwise it is 'pad' (a
-// sentinel node placed beyond every cutoff).
-__global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr, const int *__restrict__ cols, int N,
-                                                 int nsl, const int *__restrict__ sptr, int *__restrict__ scol, int pad)
+// pad < 0: entries hold node indices and padding repeats the row's own
+// index; otherwise entries hold byte offsets 16*j into the float4 position
+// array and padding is 16*pad (a sentinel node placed beyond every cutoff).
+__global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr, const int *__restrict__ cols,
+                                                 const int *__restrict__ perm, int N, int nsl,
+                                                 const int *__restrict__ sptr, int *__restrict__ scol, int pad)
 {
     const int s = blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (s >= nsl) return;
-    const int row = s * kWave + lane;
+    const int row = slice_row(perm, s * kWave + lane);
     const int b = row < N ? rowptr[row] : 0, e = row < N ? rowptr[row + 1] : 0;
-    const int fill = pad < 0 ? min(row, N - 1) : pad;
+    const int fill = pad < 0 ? min(row, N - 1) : (int)((uint32_t)pad << 4);
+    const int sh = pad < 0 ? 0 : 4;
     const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
-    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? cols[b + k] : fill;
+    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? (int)((uint32_t)cols[b + k] << sh) : fill;
 }
 
-// Neighbour position j.  SENT kernels address with a 32-bit byte offset
-// (the handle keeps fewer than 2^28 nodes), the others with 64-bit indices.
+// Slice order by row class (row_class) within windows of 256 rows: each
+// block sorts its rows by class, stably (ballot ranks), so the window's
+// interior rows fill whole slices of their own while every slice keeps
+// rows from the same stretch of the node order (the same neighbourhood,
+// and the same share of boundary rows per block of the grid).  Rows >= N
+// sort last and become empty lanes (N) where they fall below total.
+__global__ __launch_bounds__(256) void perm_block_sort(const float4 *__restrict__ pos, int N, int total, float L,
+                                                       float W0, int pbc, int *__restrict__ perm)
+{
+    __shared__ int wc[4][9];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    int c = 8;
+    if (i < N) {
+        const float4 x = pos[i];
+        c = row_class(x, pbc == 2 ? x.w : L, W0);
+    }
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const unsigned long long m = __ballot(c == k);
+        if (lane == 0) wc[w][k] = __popcll(m);
+        if (c == k) rank = __popcll(m & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < c; k++) off += wc[0][k] + wc[1][k] + wc[2][k] + wc[3][k];
+    for (int v = 0; v < w; v++) off += wc[v][c];
+    const int dst = blockIdx.x * 256 + off + rank;
+    if (dst < total) perm[dst] = i < N ? i : N;
+}
+
+// Neighbour position.  SENT layouts store the byte offset j*16 of node j
+// (the handle keeps fewer than 2^28 nodes), the others the index.
 template <bool SENT>
 __device__ __forceinline__ float4 gather_pos(const float4 *__restrict__ pos, int j)
 {
-    if (SENT) return *(const float4 *)((const char *)pos + ((uint32_t)j << 4));
+    if (SENT) return *(const float4 *)((const char *)pos + (uint32_t)j);
     return pos[j];
 }
 
-// In-order row sums over the sliced layout, one wavefront per slice and one
-// lane per row: the lane adds its row's pair forces in CSR order, U
-// entries' column reads and position gathers issued ahead of their
-// arithmetic.  Padding entries either repeat the row's own index and are
-// masked (SENT = false), or point at a sentinel node N whose position is
-// beyond every cutoff (SENT = true): its pair force is exactly +0, and
-// adding +0 to a sum that started at +0 leaves every bit unchanged, so no
-// mask is needed.  Bit-identical to lj_csr_seq either way.
-template <int U, bool SENT>
-__global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
-                                               const int *__restrict__ sptr, const int *__restrict__ scol, int N,
-                                               LJParams p, float4 *__restrict__ force, float4 *__restrict__ vel,
-                                               float4 *__restrict__ pos_out, int mode)
+// Row sums of one slice, lane = row: entries k = 0 .. w-1 of the lane's row
+// at c[64k], U entries' column reads and position gathers issued ahead of
+// their arithmetic.  Padding entries either repeat the row's own index and
+// are masked (SENT = false: entries at or past deg), or point at a sentinel
+// node N whose position is beyond every cutoff (SENT = true): its pair force
+// is +0 or -0, and adding it to a sum that started at +0 leaves every bit
+// unchanged, so no mask is needed.
+template <int U, bool SENT, int WM>
+__device__ __forceinline__ void sell_rows(const float4 *__restrict__ pos, const int *__restrict__ c, int w, int deg,
+                                          float4 pi, const LJParams &p, const ImageSign &sg, float &fx, float &fy,
+                                          float &fz)
 {
-    const int nsl = (N + kWave - 1) / kWave;
-    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
-    if (s >= nsl) return;  // whole wavefronts
-    const int lane = threadIdx.x % kWave;
-    const int row = s * kWave + lane;
-    const float4 pi = pos[min(row, N - 1)];
-    if (p.pbc == 2) {
-        p.L = pi.w;
-        p.hL = 0.5f * pi.w;
-    }
-    const int deg = SENT ? 0 : (row < N ? rowptr[row + 1] - rowptr[row] : 0);
-    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
-    const int *__restrict__ c = scol + base + lane;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     int k = 0;
     for (; k + U <= w; k += U) {
         int j[U];
@@ -607,7 +731,7 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
 #pragma unroll
         for (int u = 0; u < U; u++) {
             float ax, ay, az;
-            pair_force(pi, q[u], p, ax, ay, az);
+            pair_force<WM>(pi, q[u], p, sg, ax, ay, az);
             if (SENT || k + u < deg) {
                 fx = fx + ax;
                 fy = fy + ay;
@@ -617,46 +741,27 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
     }
     for (; k < w; k++) {
         float ax, ay, az;
-        pair_force(pi, gather_pos<SENT>(pos, c[k * kWave]), p, ax, ay, az);
+        pair_force<WM>(pi, gather_pos<SENT>(pos, c[k * kWave]), p, sg, ax, ay, az);
         if (SENT || k < deg) {
             fx = fx + ax;
             fy = fy + ay;
             fz = fz + az;
         }
     }
-    if (row < N) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
 }
 
 // The same sums, software-pipelined: while a group of U pairs is evaluated,
 // the positions of the next group and the column indices of the one after
 // are already in flight (two register sets used alternately), so a lane
-// always has a group's memory latency to hide behind arithmetic.
-// Bit-identical to lj_sell.
-template <int U, bool SENT>
-__global__ __launch_bounds__(256) void lj_sell_pipe(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
-                                                    const int *__restrict__ sptr, const int *__restrict__ scol, int N,
-                                                    LJParams p, float4 *__restrict__ force, float4 *__restrict__ vel,
-                                                    float4 *__restrict__ pos_out, int mode)
+// always has a group's memory latency to hide behind arithmetic.  Past the
+// slice width a lane reads a valid dummy column (its own row, masked, or
+// the sentinel).
+template <int U, bool SENT, int WM>
+__device__ __forceinline__ void sell_rows_pipe(const float4 *__restrict__ pos, const int *__restrict__ c, int w,
+                                               int deg, int dummy, float4 pi, const LJParams &p, const ImageSign &sg,
+                                               float &fx, float &fy, float &fz)
 {
-    const int nsl = (N + kWave - 1) / kWave;
-    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
-    if (s >= nsl) return;  // whole wavefronts
-    const int lane = threadIdx.x % kWave;
-    const int row = s * kWave + lane;
-    const int self = min(row, N - 1);
-    const float4 pi = pos[self];
-    if (p.pbc == 2) {
-        p.L = pi.w;
-        p.hL = 0.5f * pi.w;
-    }
-    const int deg = SENT ? 0 : (row < N ? rowptr[row + 1] - rowptr[row] : 0);
-    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
-    const int *__restrict__ c = scol + base + lane;
-    // column of entry k; past the slice width a valid dummy whose force is
-    // masked (or zero, for the sentinel)
-    const int dummy = SENT ? N : self;
     auto col_at = [&](int k) { return k < w ? c[k * kWave] : dummy; };
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     // evaluates group k from qc, loads the positions of group k+U (from jx)
     // into qx and the columns of group k+2U into jy
     auto step = [&](int k, const float4 (&qc)[U], const int (&jx)[U], float4 (&qx)[U], int (&jy)[U]) {
@@ -667,7 +772,7 @@ __global__ __launch_bounds__(256) void lj_sell_pipe(const float4 *__restrict__ p
 #pragma unroll
         for (int u = 0; u < U; u++) {
             float ax, ay, az;
-            pair_force(pi, qc[u], p, ax, ay, az);
+            pair_force<WM>(pi, qc[u], p, sg, ax, ay, az);
             if (SENT || k + u < deg) {
                 fx = fx + ax;
                 fy = fy + ay;
@@ -687,7 +792,304 @@ __global__ __launch_bounds__(256) void lj_sell_pipe(const float4 *__restrict__ p
         step(k, qA, jB, qB, jA);
         if (k + U < w) step(k + U, qB, jA, qA, jB);
     }
-    if (row < N) lj_epilogue(row, pi, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
+// In-order row sums over the sliced layout, one wavefront per slice and one
+// lane per row (PIPE: the software-pipelined loop).  The loop is
+// instantiated for the axis sets {} (no minimum image), {x} and {x, y, z}
+// and each wavefront takes the least one covering axes_to_wrap.
+// Bit-identical to lj_csr_seq.
+template <int U, bool SENT, bool PIPE>
+__global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, const int *__restrict__ rowptr,
+                                               const int *__restrict__ sptr, const int *__restrict__ scol,
+                                               const int *__restrict__ perm, int N, LJParams p,
+                                               float4 *__restrict__ force, float4 *__restrict__ vel,
+                                               float4 *__restrict__ pos_out, int mode)
+{
+    const int nsl = (N + kWave - 1) / kWave;
+    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
+    if (s >= nsl) return;  // whole wavefronts
+    const int lane = threadIdx.x % kWave;
+    const int row = slice_row(perm, s * kWave + lane);
+    const int self = min(row, N - 1);
+    const float4 pi = pos[self];
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int deg = SENT ? 0 : (row < N ? rowptr[row + 1] - rowptr[row] : 0);
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    const int *__restrict__ c = scol + base + lane;
+    const int dummy = SENT ? (int)((uint32_t)N << 4) : self;
+    // the velocity is read ahead of the loop, its latency hidden by the sums
+    const float4 v0 = mode != kForceOnly && row < N ? vel[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int wm = axes_to_wrap<SENT>(pi, p);
+    const ImageSign sg = image_sign(pi, p);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    auto rows = [&](auto tag) {
+        constexpr int WM = decltype(tag)::value;
+        if (PIPE) sell_rows_pipe<U, SENT, WM>(pos, c, w, deg, dummy, pi, p, sg, fx, fy, fz);
+        else sell_rows<U, SENT, WM>(pos, c, w, deg, pi, p, sg, fx, fy, fz);
+    };
+    if (wm == 0) {
+        rows(std::integral_constant<int, 0>());
+    } else if (wm == 15 || !SENT) {
+        rows(std::integral_constant<int, 15>());
+    } else if constexpr (SENT) {
+        switch (wm) {
+        case 1: rows(std::integral_constant<int, 1>()); break;
+        case 2: rows(std::integral_constant<int, 2>()); break;
+        case 3: rows(std::integral_constant<int, 3>()); break;
+        case 4: rows(std::integral_constant<int, 4>()); break;
+        case 5: rows(std::integral_constant<int, 5>()); break;
+        case 6: rows(std::integral_constant<int, 6>()); break;
+        default: rows(std::integral_constant<int, 7>()); break;
+        }
+    }
+    if (row < N) lj_finish(row, pi, v0, fx, fy, fz, p, force, vel, pos_out, mode);
+}
+
+// ------------------------------------------------------------ halo layout
+//
+// The sentinel layout's slices read every neighbour position from global
+// memory, one 16-byte gather per pair, and the texture data path that
+// returns those gathers, not the arithmetic, bounds the kernel.  The halo
+// layout stages positions through LDS instead: for each block of 256 rows
+// (4 slices) the distinct neighbour nodes are listed once (hidx, sorted by
+// their offset from the block in the periodic index space), the block
+// loads their positions into LDS at its start, and the slices' entries
+// become 16-bit LDS byte offsets (local index * 16; the sentinel is the
+// slot after the list).  Same pairs in the same order: the same bits.
+//
+// A block's neighbour set is found with a bitmap over a window of the node
+// index space centred on the block (modular: columns across the periodic
+// index wrap fall in it too), at most kHaloBits wide; a block whose window
+// or list is larger fails the build and the handle keeps the global form.
+constexpr int kHaloBits = 1 << 18;
+constexpr int kHaloCap = 4095;  // list entries per block (16-bit byte offsets, sentinel slot included)
+constexpr int kHaloWords = kHaloBits / 32;
+
+// Signed offset of column col from center in the periodic index space
+// [0, N), in (-N/2, N/2].
+__device__ __forceinline__ int mod_delta(int col, int center, int N)
+{
+    int d = col - center;
+    if (d > N / 2) d -= N;
+    else if (d <= -((N + 1) / 2)) d += N;
+    return d;
+}
+
+// Window of block b: base and width (in bits) from the block's largest
+// column offset m; shared by the count and fill passes.
+struct HaloWindow {
+    int base, width;
+};
+
+__device__ HaloWindow halo_window(int m, int center, int N)
+{
+    if (2 * (int64_t)m + 1 >= N) return HaloWindow{0, N};
+    int base = center - m;
+    if (base < 0) base += N;
+    return HaloWindow{base, 2 * m + 1};
+}
+
+// Builds the block's bitmap in LDS (words [0, ceil(width/32))) and returns
+// its window; win.width = 0 when the window is too wide.
+__device__ HaloWindow halo_bitmap(const int *__restrict__ rowptr, const int *__restrict__ cols,
+                                  const int *__restrict__ perm, int N, int nsl, uint32_t *bits, int *red)
+{
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int s = b * 4 + t / kWave;
+    const int row = s < nsl ? slice_row(perm, s * kWave + t % kWave) : N;
+    const int center = min(b * 256 + 128, N - 1);
+    int m = 0;
+    const int e0 = row < N ? rowptr[row] : 0, e1 = row < N ? rowptr[row + 1] : 0;
+    for (int k = e0; k < e1; k++) m = max(m, abs(mod_delta(cols[k], center, N)));
+    if (t == 0) *red = 0;
+    __syncthreads();
+    atomicMax(red, m);
+    __syncthreads();
+    const HaloWindow win = halo_window(*red, center, N);
+    if (win.width > kHaloBits) return HaloWindow{0, 0};
+    const int nw = (win.width + 31) / 32;
+    for (int i = t; i < nw; i += 256) bits[i] = 0u;
+    __syncthreads();
+    for (int k = e0; k < e1; k++) {
+        int x = cols[k] - win.base;
+        if (x < 0) x += N;
+        atomicOr(&bits[x >> 5], 1u << (x & 31));
+    }
+    __syncthreads();
+    return win;
+}
+
+// Pass 1: hcount[b] = the block's distinct neighbours, or -1 when the block
+// does not fit the halo form.
+__global__ __launch_bounds__(256) void halo_count(const int *__restrict__ rowptr, const int *__restrict__ cols,
+                                                  const int *__restrict__ perm, int N, int nsl,
+                                                  int *__restrict__ hcount)
+{
+    __shared__ uint32_t bits[kHaloWords];
+    __shared__ int red, tot;
+    const HaloWindow win = halo_bitmap(rowptr, cols, perm, N, nsl, bits, &red);
+    if (win.width == 0) {
+        if (threadIdx.x == 0) hcount[blockIdx.x] = -1;
+        return;
+    }
+    int c = 0;
+    for (int i = threadIdx.x; i < (win.width + 31) / 32; i += 256) c += __popc(bits[i]);
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    atomicAdd(&tot, c);
+    __syncthreads();
+    if (threadIdx.x == 0) hcount[blockIdx.x] = tot < kHaloCap ? tot : -1;
+}
+
+// Pass 2: the block's list (hidx from hptr[b], in window order) and its
+// slices' entries as LDS byte offsets.
+__global__ __launch_bounds__(256) void halo_fill(const int *__restrict__ rowptr, const int *__restrict__ cols,
+                                                 const int *__restrict__ perm, int N, int nsl,
+                                                 const int *__restrict__ sptr, const int *__restrict__ hptr,
+                                                 int *__restrict__ hidx, uint16_t *__restrict__ scol16)
+{
+    __shared__ uint32_t bits[kHaloWords];
+    __shared__ int wbase[kHaloWords];
+    __shared__ int red, scan[256], carry;
+    const HaloWindow win = halo_bitmap(rowptr, cols, perm, N, nsl, bits, &red);
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int nw = (win.width + 31) / 32;
+    // exclusive prefix of the words' bit counts, 256 words at a time
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nw; c0 += 256) {
+        const int i = c0 + t;
+        const int v = i < nw ? __popc(bits[i]) : 0;
+        scan[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int a = t >= o ? scan[t - o] : 0;
+            __syncthreads();
+            scan[t] += a;
+            __syncthreads();
+        }
+        if (i < nw) wbase[i] = carry + scan[t] - v;
+        __syncthreads();
+        if (t == 255) carry += scan[255];
+        __syncthreads();
+    }
+    const int h0 = hptr[b], nh = hptr[b + 1] - h0;
+    for (int i = t; i < nw; i += 256) {
+        uint32_t m = bits[i];
+        int r = wbase[i];
+        while (m) {
+            const int bit = __ffs(m) - 1;
+            m &= m - 1;
+            int g = win.base + i * 32 + bit;
+            if (g >= N) g -= N;
+            hidx[h0 + r++] = g;
+        }
+    }
+    // entries of this thread's row: slice s = 4b + wave, lane = row slot
+    const int s = b * 4 + t / kWave, lane = t % kWave;
+    if (s >= nsl) return;
+    const int row = slice_row(perm, s * kWave + lane);
+    const int e0 = row < N ? rowptr[row] : 0, e1 = row < N ? rowptr[row + 1] : 0;
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    for (int k = 0; k < w; k++) {
+        int local = nh;  // sentinel slot
+        if (e0 + k < e1) {
+            int x = cols[e0 + k] - win.base;
+            if (x < 0) x += N;
+            const uint32_t word = bits[x >> 5];
+            local = wbase[x >> 5] + __popc(word & ((1u << (x & 31)) - 1u));
+        }
+        scol16[base + k * kWave + lane] = (uint16_t)(local << 4);
+    }
+}
+
+// Row sums of one slice from the block's LDS positions (see sell_rows).
+template <int U, int WM>
+__device__ __forceinline__ void sell_rows_lds(const float4 *hpos, const uint16_t *__restrict__ c, int w, float4 pi,
+                                              const LJParams &p, const ImageSign &sg, float &fx, float &fy,
+                                              float &fz)
+{
+    auto at = [&](uint32_t off) { return *(const float4 *)((const char *)hpos + off); };
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        uint32_t j[U];
+        float4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) j[u] = c[(k + u) * kWave];
+#pragma unroll
+        for (int u = 0; u < U; u++) q[u] = at(j[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            float ax, ay, az;
+            pair_force<WM>(pi, q[u], p, sg, ax, ay, az);
+            fx = fx + ax;
+            fy = fy + ay;
+            fz = fz + az;
+        }
+    }
+    for (; k < w; k++) {
+        float ax, ay, az;
+        pair_force<WM>(pi, at(c[k * kWave]), p, sg, ax, ay, az);
+        fx = fx + ax;
+        fy = fy + ay;
+        fz = fz + az;
+    }
+}
+
+// The sliced kernel over the halo layout: the block's neighbour positions
+// are loaded into LDS (dynamic, nh+1 float4) first, then each wavefront
+// sums its slice as lj_sell does.  Bit-identical to lj_csr_seq.
+template <int U>
+__global__ __launch_bounds__(256) void lj_sell_halo(const float4 *__restrict__ pos, const int *__restrict__ sptr,
+                                                    const uint16_t *__restrict__ scol16,
+                                                    const int *__restrict__ hptr, const int *__restrict__ hidx,
+                                                    const int *__restrict__ perm, int N, LJParams p,
+                                                    float4 *__restrict__ force, float4 *__restrict__ vel,
+                                                    float4 *__restrict__ pos_out, int mode)
+{
+    extern __shared__ float4 hpos[];
+    const int nsl = (N + kWave - 1) / kWave;
+    const int b = pat::xcd_swizzle(blockIdx.x, gridDim.x);
+    const int h0 = hptr[b], nh = hptr[b + 1] - h0;
+    for (int i = threadIdx.x; i < nh; i += 256) hpos[i] = pos[hidx[h0 + i]];
+    if (threadIdx.x == 0) hpos[nh] = pos[N];  // the sentinel
+    const int s = b * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int row = s < nsl ? slice_row(perm, s * kWave + lane) : N;
+    const int self = min(row, N - 1);
+    const float4 pi = pos[self];
+    const float4 v0 = mode != kForceOnly && row < N ? vel[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    if (s >= nsl) return;  // whole wavefronts, after the barrier
+    if (p.pbc == 2) {
+        p.L = pi.w;
+        p.hL = 0.5f * pi.w;
+    }
+    const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
+    const uint16_t *__restrict__ c = scol16 + base + lane;
+    const int wm = axes_to_wrap<true>(pi, p);
+    const ImageSign sg = image_sign(pi, p);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    auto rows = [&](auto tag) {
+        constexpr int WM = decltype(tag)::value;
+        sell_rows_lds<U, WM>(hpos, c, w, pi, p, sg, fx, fy, fz);
+    };
+    switch (wm) {
+    case 0: rows(std::integral_constant<int, 0>()); break;
+    case 1: rows(std::integral_constant<int, 1>()); break;
+    case 2: rows(std::integral_constant<int, 2>()); break;
+    case 3: rows(std::integral_constant<int, 3>()); break;
+    case 4: rows(std::integral_constant<int, 4>()); break;
+    case 5: rows(std::integral_constant<int, 5>()); break;
+    case 6: rows(std::integral_constant<int, 6>()); break;
+    case 7: rows(std::integral_constant<int, 7>()); break;
+    default: rows(std::integral_constant<int, 15>()); break;
+    }
+    if (row < N) lj_finish(row, pi, v0, fx, fy, fz, p, force, vel, pos_out, mode);
 }
 
 // Per-node half potential 0.5*sum_j phi(i, j) over the full CSR row.
@@ -763,13 +1165,23 @@ __global__ void vv_kick_drift(float4 *__restrict__ pos, float4 *__restrict__ vel
     x.x = x.x + p.dt * v.x;
     x.y = x.y + p.dt * v.y;
     x.z = x.z + p.dt * v.z;
-    if (p.pbc) {
-        x.x = pat::wrap_coord(x.x, p.L);
-        x.y = pat::wrap_coord(x.y, p.L);
-        x.z = pat::wrap_coord(x.z, p.L);
-    }
+    drift_wrap(x, p);
     vel[i] = v;
     pos[i] = x;
+}
+
+__global__ void box_flag_set(int *flag)
+{
+    if (threadIdx.x == 0) *flag = 1;
+}
+
+__global__ void box_check(const float4 *__restrict__ pos, int N, float L, int pbc, int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 x = pos[i];
+    const float b = pbc == 2 ? x.w : L;
+    if (!(x.x >= 0.0f && x.x <= b && x.y >= 0.0f && x.y <= b && x.z >= 0.0f && x.z <= b)) *flag = 0;
 }
 
 __global__ void vv_kick(float4 *__restrict__ vel, const float4 *__restrict__ force, int N, LJParams p)
@@ -794,7 +1206,7 @@ struct LJConfig {
     float dt;
 };
 
-static LJParams make_params(float box, int pbc, float rc2, float dt)
+static LJParams make_params(float box, int pbc, float rc2, float dt, int *inbox = nullptr)
 {
     LJParams p;
     p.L = box;
@@ -803,7 +1215,21 @@ static LJParams make_params(float box, int pbc, float rc2, float dt)
     p.dt = dt;
     p.hdt = 0.5f * dt;
     p.pbc = pbc;
+    // least w with fl(w*w) >= rc2, in the device's (IEEE) float product
+    volatile float w = std::sqrt(std::max(rc2, 0.0f));
+    while (w * w < rc2) w = std::nextafter((float)w, INFINITY);
+    p.W0 = w;
+    p.inbox = inbox;
     return p;
+}
+
+// The pair functor's reciprocal and single-fma force factor hold for
+// cutoffs with rc^2 below 2^31.
+static int check_cutoff(float rc2, const char *who)
+{
+    PAT_REQUIRE(rc2 >= 0.0f && rc2 < 0x1p31f, PAT_ERR_UNSUPPORTED, "%s: cutoff^2 %g outside [0, 2^31)", who,
+                (double)rc2);
+    return PAT_OK;
 }
 
 int64_t csr_workspace_bytes(int64_t N, int64_t E)
@@ -902,8 +1328,10 @@ static CsrKernel csr_kernel_for(int lpr, int tree)
 }
 
 int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
-                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out, hipStream_t st)
+                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out, int *inbox,
+                 hipStream_t st)
 {
+    PAT_TRY(check_cutoff(rc2, "lj_force_csr"));
     // negative lpr selects the tree-order variant
     const int tree = lpr < 0;
     if (tree) lpr = -lpr;
@@ -914,7 +1342,7 @@ int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, 
     const int rpb = 256 / lpr;
     const int64_t blocks = (N + rpb - 1) / rpb;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, st, pos, (const int *)rowptr, (const int *)cols, (int)N,
-                       make_params(box, pbc, rc2, dt), force, vel, pos_out, mode);
+                       make_params(box, pbc, rc2, dt, inbox), force, vel, pos_out, mode);
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }
@@ -925,12 +1353,12 @@ int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, 
 // padded total (capacity *cap).  Refuses (PAT_ERR_UNSUPPORTED, nothing
 // changed) when padding would exceed max_pad times the entry count plus one
 // slice of 64 entries per slice.
-int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
-               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st)
+int sell_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, int64_t N, double max_pad,
+               int32_t *width_d, int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st)
 {
     const int64_t nsl = (N + kWave - 1) / kWave;
-    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr, (int)N,
-                       (int)nsl, (int *)width_d);
+    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
+                       (const int *)perm, (int)N, (int)nsl, (int *)width_d);
     PAT_LAUNCH_CHECK();
     std::vector<int32_t> w((size_t)nsl);
     std::vector<int32_t> sp((size_t)nsl + 1);
@@ -961,7 +1389,7 @@ int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max
     }
     PAT_HIP(hipMemcpyAsync(sptr, sp.data(), sizeof(int32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(sell_fill, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, (const int *)rowptr,
-                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)*scol, pad);
+                       (const int *)cols, (const int *)perm, (int)N, (int)nsl, (const int *)sptr, (int *)*scol, pad);
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipStreamSynchronize(st));  // sp leaves scope
     *total = acc;
@@ -976,8 +1404,8 @@ int sell_build_into(const int32_t *rowptr, const int32_t *cols, int64_t N, int32
 {
     PAT_REQUIRE(N > 0 && N < (1LL << 31) - kWave, PAT_ERR_INVALID, "sell_build: bad node count");
     const int64_t nsl = (N + kWave - 1) / kWave;
-    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr, (int)N,
-                       (int)nsl, (int *)sptr);
+    hipLaunchKernelGGL(sell_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
+                       (const int *)nullptr, (int)N, (int)nsl, (int *)sptr);
     PAT_LAUNCH_CHECK();
     std::vector<int32_t> sp((size_t)nsl + 1);
     PAT_HIP(hipMemcpyAsync(sp.data(), sptr, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost, st));
@@ -995,31 +1423,119 @@ int sell_build_into(const int32_t *rowptr, const int32_t *cols, int64_t N, int32
                 (long long)cap);
     PAT_HIP(hipMemcpyAsync(sptr, sp.data(), sizeof(int32_t) * (nsl + 1), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(sell_fill, dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, (const int *)rowptr,
-                       (const int *)cols, (int)N, (int)nsl, (const int *)sptr, (int *)scol, -1);
+                       (const int *)cols, (const int *)nullptr, (int)N, (int)nsl, (const int *)sptr, (int *)scol, -1);
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipStreamSynchronize(st));  // sp leaves scope
     return PAT_OK;
 }
 
-// kind: 0 = lj_sell<4>, 1 = lj_sell_pipe<4>, 2 = lj_sell_pipe<8> over a
-// layout whose padding repeats the row; 3, 4, 5 = the same over a layout
-// padded with the sentinel node N (pos then holds N+1 entries, the last
-// beyond every cutoff, and N < 2^28).  Same sums for all.
-int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
-                  float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
-                  int kind, hipStream_t st)
+// Slice order by row class from positions pos (see perm_block_sort): perm
+// gets nsl*64 entries.
+int sell_perm_build(const float4 *pos, int64_t N, float box, int pbc, float rc2, int32_t *perm, hipStream_t st)
+{
+    const int64_t nsl = (N + kWave - 1) / kWave, total = nsl * kWave;
+    const LJParams p = make_params(box, pbc, rc2, 0.f);
+    hipLaunchKernelGGL(perm_block_sort, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pos, (int)N,
+                       (int)total, box, p.W0, pbc, (int *)perm);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// Halo layout of the sliced layout (sptr, perm as built for the sentinel
+// kinds): hptr gets nblk+1 offsets (nblk = ceil(nsl/4)); hidx and scol16
+// are (re)allocated as needed (capacities in *hcap, *scap16).  Returns
+// PAT_ERR_UNSUPPORTED when some block does not fit the halo form; *max_nh
+// = the largest list.
+int halo_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, const int32_t *sptr, int64_t N,
+               int64_t stotal, int32_t *hptr, int32_t **hidx, int64_t *hcap, uint16_t **scol16, int64_t *scap16,
+               int *max_nh, hipStream_t st)
+{
+    const int64_t nsl = (N + kWave - 1) / kWave, nblk = (nsl + 3) / 4;
+    hipLaunchKernelGGL(halo_count, dim3((unsigned)nblk), dim3(256), 0, st, (const int *)rowptr, (const int *)cols,
+                       (const int *)perm, (int)N, (int)nsl, (int *)hptr);
+    PAT_LAUNCH_CHECK();
+    std::vector<int32_t> h((size_t)nblk + 1);
+    PAT_HIP(hipMemcpyAsync(h.data(), hptr, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    int64_t acc = 0;
+    int mx = 0;
+    for (int64_t b = 0; b < nblk; b++) {
+        const int c = h[b];
+        if (c < 0) {
+            pat::set_error("halo_build: block %lld does not fit the halo form", (long long)b);
+            return PAT_ERR_UNSUPPORTED;
+        }
+        h[b] = (int32_t)acc;
+        acc += c;
+        mx = std::max(mx, c);
+    }
+    h[nblk] = (int32_t)acc;
+    PAT_REQUIRE(acc < (1LL << 31), PAT_ERR_UNSUPPORTED, "halo_build: %lld list entries", (long long)acc);
+    auto grow = [](void **ptr, int64_t *cap, int64_t need, size_t elem) -> int {
+        if (need <= *cap) return PAT_OK;
+        const int64_t want = need + need / 4 + 4096;
+        void *n = nullptr;
+        if (pat::dev_malloc(&n, elem * (size_t)want) != hipSuccess) {
+            (void)hipGetLastError();
+            pat::set_error("halo_build: hipMalloc of %lld entries failed", (long long)want);
+            return PAT_ERR_NOMEM;
+        }
+        if (*ptr) (void)pat::dev_free(*ptr);
+        *ptr = n;
+        *cap = want;
+        return PAT_OK;
+    };
+    PAT_TRY(grow((void **)hidx, hcap, acc, sizeof(int32_t)));
+    PAT_TRY(grow((void **)scol16, scap16, stotal, sizeof(uint16_t)));
+    PAT_HIP(hipMemcpyAsync(hptr, h.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(halo_fill, dim3((unsigned)nblk), dim3(256), 0, st, (const int *)rowptr, (const int *)cols,
+                       (const int *)perm, (int)N, (int)nsl, (const int *)sptr, (const int *)hptr, (int *)*hidx,
+                       *scol16);
+    PAT_LAUNCH_CHECK();
+    PAT_HIP(hipStreamSynchronize(st));  // h leaves scope
+    *max_nh = mx;
+    return PAT_OK;
+}
+
+// Forces over the halo layout; U = 4 or 8 entries' loads ahead.
+int lj_force_halo(const float4 *pos, const int32_t *sptr, const uint16_t *scol16, const int32_t *hptr,
+                  const int32_t *hidx, const int32_t *perm, int64_t N, int max_nh, float box, int pbc, float rc2,
+                  float dt, int mode, float4 *force, float4 *vel, float4 *pos_out, int U, int *inbox, hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 28) - 1, PAT_ERR_INVALID, "lj_force_halo: bad node count");
+    PAT_REQUIRE(max_nh >= 0 && max_nh < kHaloCap, PAT_ERR_INVALID, "lj_force_halo: list of %d entries", max_nh);
+    PAT_TRY(check_cutoff(rc2, "lj_force_halo"));
+    const int64_t nsl = (N + kWave - 1) / kWave, nblk = (nsl + 3) / 4;
+    const size_t lds = sizeof(float4) * (size_t)(max_nh + 1);
+    auto k = U == 8 ? lj_sell_halo<8> : lj_sell_halo<4>;
+    hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(256), lds, st, pos, (const int *)sptr, scol16,
+                       (const int *)hptr, (const int *)hidx, (const int *)perm, (int)N,
+                       make_params(box, pbc, rc2, dt, inbox), force, vel, pos_out, mode);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// kind: 0 = lj_sell<4>, 1 = pipelined <4>, 2 = pipelined <8> over an
+// index layout whose padding repeats the row; 3, 4, 5 = the same over a
+// byte-offset layout padded with the sentinel node N (pos then holds N+1
+// entries, the last beyond every cutoff, and N < 2^28), which may skip the
+// minimum image per wavefront while *inbox is set.  Same sums for all.
+int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol,
+                  const int32_t *perm, int64_t N, float box, int pbc, float rc2, float dt, int mode, float4 *force,
+                  float4 *vel, float4 *pos_out, int kind, int *inbox, hipStream_t st)
 {
     PAT_REQUIRE(N > 0 && N < (1LL << 31) - kWave, PAT_ERR_INVALID, "lj_force_sell: bad node count");
-    using K = void (*)(const float4 *, const int *, const int *, const int *, int, LJParams, float4 *, float4 *,
-                       float4 *, int);
-    const K ks[] = {lj_sell<4, false>, lj_sell_pipe<4, false>, lj_sell_pipe<8, false>,
-                    lj_sell<4, true>,  lj_sell_pipe<4, true>,  lj_sell_pipe<8, true>};
+    PAT_TRY(check_cutoff(rc2, "lj_force_sell"));
+    using K = void (*)(const float4 *, const int *, const int *, const int *, const int *, int, LJParams, float4 *,
+                       float4 *, float4 *, int);
+    const K ks[] = {lj_sell<4, false, false>, lj_sell<4, false, true>, lj_sell<8, false, true>,
+                    lj_sell<4, true, false>,  lj_sell<4, true, true>,  lj_sell<8, true, true>};
     PAT_REQUIRE(kind >= 0 && kind < 6, PAT_ERR_INVALID, "lj_force_sell: kind %d", kind);
     PAT_REQUIRE(kind < 3 || N < (1LL << 28) - 1, PAT_ERR_INVALID, "lj_force_sell: sentinel kinds need N < 2^28 - 1");
     const int64_t nsl = (N + kWave - 1) / kWave;
     hipLaunchKernelGGL(ks[kind], dim3((unsigned)((nsl + 3) / 4)), dim3(256), 0, st, pos, (const int *)rowptr,
-                       (const int *)sptr, (const int *)scol, (int)N, make_params(box, pbc, rc2, dt), force, vel,
-                       pos_out, mode);
+                       (const int *)sptr, (const int *)scol, (const int *)perm, (int)N,
+                       make_params(box, pbc, rc2, dt, inbox), force, vel, pos_out, mode);
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }
@@ -1068,6 +1584,7 @@ int lj_energy_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols,
 int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t N, float box, int pbc, float rc2,
                     float4 *force, hipStream_t st)
 {
+    PAT_TRY(check_cutoff(rc2, "lj_force_atomic"));
     hipLaunchKernelGGL(fill_zero4, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, force, (int)N);
     PAT_LAUNCH_CHECK();
     if (E > 0) {
@@ -1080,11 +1597,25 @@ int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t 
 }
 
 int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
-                      hipStream_t st)
+                      int *inbox, hipStream_t st)
 {
     hipLaunchKernelGGL(vv_kick_drift, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, pos, vel, force, (int)N,
-                       make_params(box, pbc, 0.f, dt));
+                       make_params(box, pbc, 0.f, dt, inbox));
     PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// *flag = 1 when every position of nodes [0, N) lies in [0, L] on all
+// three axes (L = box, or each node's w for pbc 2), else 0.
+int box_check_dev(const float4 *pos, int64_t N, float box, int pbc, int *flag, hipStream_t st)
+{
+    hipLaunchKernelGGL(box_flag_set, dim3(1), dim3(64), 0, st, flag);
+    PAT_LAUNCH_CHECK();
+    if (pbc && N > 0) {
+        hipLaunchKernelGGL(box_check, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, pos, (int)N, box, pbc,
+                           flag);
+        PAT_LAUNCH_CHECK();
+    }
     return PAT_OK;
 }
 

@@ -22,20 +22,28 @@ namespace pat {
 int64_t csr_workspace_bytes(int64_t N, int64_t E);
 int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st);
-int sell_build(const int32_t *rowptr, const int32_t *cols, int64_t N, double max_pad, int32_t *width_d,
-               int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st);
-int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol, int64_t N,
-                  float box, int pbc, float rc2, float dt, int mode, float4 *force, float4 *vel, float4 *pos_out,
-                  int kind, hipStream_t st);
+int sell_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, int64_t N, double max_pad,
+               int32_t *width_d, int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st);
+int sell_perm_build(const float4 *pos, int64_t N, float box, int pbc, float rc2, int32_t *perm, hipStream_t st);
+int halo_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, const int32_t *sptr, int64_t N,
+               int64_t stotal, int32_t *hptr, int32_t **hidx, int64_t *hcap, uint16_t **scol16, int64_t *scap16,
+               int *max_nh, hipStream_t st);
+int lj_force_halo(const float4 *pos, const int32_t *sptr, const uint16_t *scol16, const int32_t *hptr,
+                  const int32_t *hidx, const int32_t *perm, int64_t N, int max_nh, float box, int pbc, float rc2,
+                  float dt, int mode, float4 *force, float4 *vel, float4 *pos_out, int U, int *inbox, hipStream_t st);
+int lj_force_sell(const float4 *pos, const int32_t *rowptr, const int32_t *sptr, const int32_t *scol,
+                  const int32_t *perm, int64_t N, float box, int pbc, float rc2, float dt, int mode, float4 *force,
+                  float4 *vel, float4 *pos_out, int kind, int *inbox, hipStream_t st);
 int lj_force_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
-                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out,
+                 float rc2, float dt, int lpr, int mode, float4 *force, float4 *vel, float4 *pos_out, int *inbox,
                  hipStream_t st);
+int box_check_dev(const float4 *pos, int64_t N, float box, int pbc, int *flag, hipStream_t st);
 int lj_energy_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
                   float rc2, double *node_e, hipStream_t st);
 int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t N, float box, int pbc, float rc2,
                     float4 *force, hipStream_t st);
 int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
-                      hipStream_t st);
+                      int *inbox, hipStream_t st);
 int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
 int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
                   int32_t *grp, int32_t *gcl, hipStream_t st);
@@ -69,7 +77,12 @@ constexpr double kSellMaxPad = 1.5;
 // Tuned lanes per row, cached per (node count, mean degree, row order) class
 // for the process: the in-order variants give bit-identical sums, so the
 // choice only changes speed.
-std::map<std::tuple<int64_t, int, int>, std::pair<int, int>> g_lpr_cache;  // -> (lanes, sliced kind)
+struct KernelChoice {
+    int lanes;      // lanes per row, or PAT_IR_SLICED
+    int kind;       // sliced kernel form (lj_force_sell kind)
+    bool by_class;  // sliced rows in row-class order
+};
+std::map<std::tuple<int64_t, int, int>, KernelChoice> g_lpr_cache;
 std::mutex g_lpr_mutex;
 
 // Device state shared by the IrregularRuntime handle and the MD handle.
@@ -90,7 +103,26 @@ struct IrState {
     int32_t *sptr = nullptr, *scol = nullptr, *swidth = nullptr;
     int64_t scap = 0, stotal = 0;
     bool sell_ok = false;
-    int sell_kind = 4;  // sliced kernel form (lj_force_sell kind), chosen by tune()
+    int sell_kind = 3;  // sliced kernel form (lj_force_sell kind), chosen by tune()
+    // device flag, 1 while every position lies inside the box (set by
+    // check_box, cleared by any drift that leaves it); the sentinel kinds
+    // skip the minimum image per wavefront only while it is set
+    int32_t *inbox = nullptr;
+    // slice order of the sentinel layout: rows of each 256-row window
+    // grouped by row class (nodes within a cutoff of a face, per axis) so
+    // most wavefronts take the minimum image on no axis; identity (perm_on
+    // false) unless the tuner finds it faster
+    int32_t *perm = nullptr;
+    bool perm_on = false;
+    bool want_perm = false;  // tuned: whether the class order pays off
+    // halo copy of the sentinel layout (kinds 6, 7): per 256-row block the
+    // distinct neighbours (hptr/hidx) and the entries as LDS offsets
+    int32_t *hptr = nullptr, *hidx = nullptr;
+    uint16_t *scol16 = nullptr;
+    int64_t hcap = 0, scap16 = 0;
+    int halo_max = 0;
+    bool halo_ok = false;
+    bool want_halo = false;  // built with every sliced layout while set
 
     // Sliced kernels padded with a sentinel node: positions hold N+1
     // entries, the last far beyond any cutoff (set at allocation, never
@@ -106,6 +138,7 @@ struct IrState {
     {
         N = n;
         E = e;
+        sell_kind = sentinel() ? 3 : 0;
         PAT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         auto am = [](void **p, size_t b) -> int {
             if (b == 0) b = 16;
@@ -122,6 +155,8 @@ struct IrState {
         PAT_TRY(am((void **)&pos[0], sizeof(float) * 4 * (size_t)(N + 1)));
         PAT_TRY(am((void **)&force, sizeof(float) * 4 * (size_t)N));
         PAT_TRY(am((void **)&node_e, sizeof(double) * (size_t)N));
+        PAT_TRY(am((void **)&inbox, sizeof(int32_t)));
+        PAT_HIP(hipMemsetAsync(inbox, 0, sizeof(int32_t), stream));
         if (md) {
             PAT_TRY(am((void **)&pos[1], sizeof(float) * 4 * (size_t)(N + 1)));
             PAT_TRY(am((void **)&vel, sizeof(float) * 4 * (size_t)N));
@@ -133,6 +168,8 @@ struct IrState {
         PAT_HIP(hipStreamSynchronize(stream));
         const int64_t nsl = (N + 63) / 64;
         PAT_TRY(am((void **)&sptr, sizeof(int32_t) * (size_t)(nsl + 1)));
+        PAT_TRY(am((void **)&perm, sizeof(int32_t) * (size_t)(64 * nsl)));
+        PAT_TRY(am((void **)&hptr, sizeof(int32_t) * (size_t)((nsl + 3) / 4 + 1)));
         PAT_TRY(am((void **)&swidth, sizeof(int32_t) * (size_t)nsl));
         if (E > 0) PAT_HIP(hipMemcpyAsync(edges, host_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, stream));
         lpr = 1;
@@ -160,13 +197,21 @@ struct IrState {
     }
 
     // Sliced copy of the current CSR; left unavailable (in-order kernels on
-    // the plain CSR only) when its padding would exceed kSellMaxPad.
-    int build_sell()
+    // the plain CSR only) when its padding would exceed kSellMaxPad.  With
+    // positions p4 (periodic box, sentinel layout) the slices follow the
+    // row-class order computed from them.
+    int build_sell(const float *p4 = nullptr, float box = 0.f, int pbc = 0, float rc2 = 0.f)
     {
         layout_gen++;
+        perm_on = false;
+        halo_ok = false;
+        if (want_perm && p4 && pbc && sentinel()) {
+            PAT_TRY(pat::sell_perm_build((const float4 *)p4, N, box, pbc, rc2, perm, stream));
+            perm_on = true;
+        }
         // padded with the sentinel node N when the handle can hold it
-        int rc = pat::sell_build(rowptr, cols, N, kSellMaxPad, swidth, sptr, &scol, &scap, &stotal,
-                                 sentinel() ? (int)N : -1, stream);
+        int rc = pat::sell_build(rowptr, cols, perm_on ? perm : nullptr, N, kSellMaxPad, swidth, sptr, &scol, &scap,
+                                 &stotal, sentinel() ? (int)N : -1, stream);
         if (rc == PAT_ERR_UNSUPPORTED) {
             pat::clear_error();
             sell_ok = false;
@@ -175,6 +220,24 @@ struct IrState {
         }
         if (rc != PAT_OK) return rc;
         sell_ok = true;
+        return want_halo ? build_halo() : PAT_OK;
+    }
+
+    // Halo copy of the current sentinel layout; halo_ok false when some
+    // block does not fit it (the global-gather kinds remain).
+    int build_halo()
+    {
+        halo_ok = false;
+        if (!sell_ok || !sentinel()) return PAT_OK;
+        int rc = pat::halo_build(rowptr, cols, perm_on ? perm : nullptr, sptr, N, stotal, hptr, &hidx, &hcap,
+                                 &scol16, &scap16, &halo_max, stream);
+        if (rc == PAT_ERR_UNSUPPORTED) {
+            pat::clear_error();
+            if (lpr == PAT_IR_SLICED && sell_kind >= 6) sell_kind = 3;  // same sums, bit for bit
+            return PAT_OK;
+        }
+        if (rc != PAT_OK) return rc;
+        halo_ok = true;
         return PAT_OK;
     }
 
@@ -182,7 +245,8 @@ struct IrState {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)edges, (void *)rowptr, (void *)cols, (void *)pos[0], (void *)pos[1], (void *)vel,
-                        (void *)force, (void *)node_e, (void *)sptr, (void *)scol, (void *)swidth})
+                        (void *)force, (void *)node_e, (void *)sptr, (void *)scol, (void *)swidth, (void *)inbox,
+                        (void *)perm, (void *)hptr, (void *)hidx, (void *)scol16})
             if (p) (void)pat::dev_free(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -239,11 +303,36 @@ struct IrState {
     {
         if (l == PAT_IR_SLICED) {
             PAT_REQUIRE(sell_ok, PAT_ERR_STATE, "irregular: the sliced layout is not built");
-            return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, N, box, pbc, rc2, dt, mode,
-                                      (float4 *)force, (float4 *)vel, (float4 *)pos_out, sell_kind, s);
+            // kinds 3..7 read the sentinel layout (6, 7 its halo copy), 0..2 indices
+            PAT_REQUIRE((sell_kind >= 3) == sentinel(), PAT_ERR_STATE, "irregular: sliced kind %d does not fit the layout",
+                        sell_kind);
+            if (sell_kind >= 6) {
+                PAT_REQUIRE(halo_ok, PAT_ERR_STATE, "irregular: the halo layout is not built");
+                return pat::lj_force_halo((const float4 *)p4, sptr, scol16, hptr, hidx, perm_on ? perm : nullptr, N,
+                                          halo_max, box, pbc, rc2, dt, mode, (float4 *)force, (float4 *)vel,
+                                          (float4 *)pos_out, sell_kind == 7 ? 8 : 4, inbox, s);
+            }
+            return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, perm_on ? perm : nullptr, N, box, pbc,
+                                      rc2, dt, mode, (float4 *)force, (float4 *)vel, (float4 *)pos_out, sell_kind,
+                                      inbox, s);
         }
         return pat::lj_force_csr((const float4 *)p4, rowptr, cols, N, box, pbc, rc2, dt, l, mode, (float4 *)force,
-                                 (float4 *)vel, (float4 *)pos_out, s);
+                                 (float4 *)vel, (float4 *)pos_out, inbox, s);
+    }
+
+    // Sets the inbox flag from positions p4 (stream-ordered before what
+    // follows on s).
+    int check_box(const float *p4, float box, int pbc, hipStream_t s)
+    {
+        return pat::box_check_dev((const float4 *)p4, N, box, pbc, inbox, s);
+    }
+
+    // Kick-drift of positions p4 in place, after checking them into the
+    // inbox flag: the start of every run of steps.
+    int begin_steps(float *p4, float box, int pbc, float dt, hipStream_t s)
+    {
+        PAT_TRY(check_box(p4, box, pbc, s));
+        return pat::vv_kick_drift_dev((float4 *)p4, (float4 *)vel, (const float4 *)force, N, box, pbc, dt, inbox, s);
     }
 
     int csr_force(const float *p4, float box, int pbc, float rc2, float dt, int mode, float *pos_out, hipStream_t s)
@@ -253,65 +342,118 @@ struct IrState {
 
     int force_eval(const float *p4, float box, int pbc, float rc2)
     {
+        PAT_TRY(check_box(p4, box, pbc, stream));
         if (variant == PAT_IR_ATOMIC)
             return pat::lj_force_atomic((const float4 *)p4, edges, E, N, box, pbc, rc2, (float4 *)force, stream);
         return csr_force(p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream);
     }
 
-    // Times the in-order force kernel for each lanes-per-row candidate on
-    // this CSR and keeps the fastest (cached by size class).  Overwrites the
-    // force buffer: callers recompute the force afterwards.
+    // Sliced layout in class order (by_class) or node order, with or
+    // without its halo copy; rebuilt from positions p4 where it differs.
+    int set_layout(bool by_class, bool halo, const float *p4, float box, int pbc, float rc2)
+    {
+        want_perm = by_class && pbc && sentinel();
+        want_halo = halo && sentinel();
+        if (!sell_ok) return PAT_OK;
+        if (want_perm != perm_on) return build_sell(p4, box, pbc, rc2);  // with the halo when wanted
+        if (want_halo && !halo_ok) return build_halo();
+        return PAT_OK;
+    }
+
+    // Times the in-order force kernel for each lanes-per-row candidate, each
+    // sliced form, and the sliced forms again over the class-ordered layout,
+    // on this CSR, and keeps the fastest (cached by size class).  Overwrites
+    // the force buffer: callers recompute the force afterwards.
     int tune(const float *p4, float box, int pbc, float rc2)
     {
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
-        const auto key = std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg, order);
+        const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
+        const auto key =
+            std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg, order + 2 * k0);
         layout_gen++;
         if (!pat::tuning_enabled()) {
             lpr = sell_ok ? PAT_IR_SLICED : 1;
-            sell_kind = sentinel() ? 4 : 1;
-            return PAT_OK;
+            sell_kind = k0;
+            return set_layout(false, false, p4, box, pbc, rc2);
         }
         {
             std::lock_guard<std::mutex> g(g_lpr_mutex);
             auto it = g_lpr_cache.find(key);
-            if (it != g_lpr_cache.end() && (it->second.first != PAT_IR_SLICED || sell_ok)) {
-                lpr = it->second.first;
-                sell_kind = it->second.second;
-                return PAT_OK;
+            if (it != g_lpr_cache.end() && (it->second.lanes != PAT_IR_SLICED || sell_ok)) {
+                lpr = it->second.lanes;
+                sell_kind = it->second.kind;
+                return set_layout(it->second.by_class, it->second.kind >= 6, p4, box, pbc, rc2);
             }
         }
+        PAT_TRY(check_box(p4, box, pbc, stream));
+        PAT_TRY(set_layout(false, true, p4, box, pbc, rc2));
         hipEvent_t a, b;
         PAT_HIP(hipEventCreate(&a));
         PAT_HIP(hipEventCreate(&b));
-        float best = 1e30f;
-        int best_l = 1, best_kind = sell_kind;
-        const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
-        const std::pair<int, int> cands[] = {{1, 0}, {4, 0}, {8, 0}, {16, 0}, {32, 0},
-                                             {PAT_IR_SLICED, k0}, {PAT_IR_SLICED, k0 + 1}, {PAT_IR_SLICED, k0 + 2}};
-        for (const auto &cand : cands) {
-            const int l = cand.first;
-            if (l == PAT_IR_SLICED && !sell_ok) continue;
-            sell_kind = cand.second;
-            for (int rep = 0; rep < 4; rep++) {
-                PAT_HIP(hipEventRecord(a, stream));
-                PAT_TRY(csr_force_l(l, p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream));
-                PAT_HIP(hipEventRecord(b, stream));
-                PAT_HIP(hipEventSynchronize(b));
-                float ms = 0;
-                PAT_HIP(hipEventElapsedTime(&ms, a, b));
-                if (rep > 0 && ms < best) {  // first launch of each is warm-up
-                    best = ms;
-                    best_l = l;
-                    best_kind = cand.second;
+        struct Cand {
+            int lanes, kind;
+            bool by_class;
+        };
+        std::vector<Cand> cands = {{1, k0, false}, {4, k0, false}, {8, k0, false}, {16, k0, false}, {32, k0, false}};
+        if (sell_ok)
+            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, false});
+        if (halo_ok)
+            for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, false});
+        std::vector<float> t(cands.size(), 1e30f);
+        auto launch = [&](size_t c) {
+            sell_kind = cands[c].kind;
+            return csr_force_l(cands[c].lanes, p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream);
+        };
+        // times candidates [c0, c1): rounds that time every one in turn,
+        // each keeping its best round
+        auto time_range = [&](size_t c0, size_t c1) -> int {
+            constexpr int kRounds = 5, kReps = 3;
+            for (int r = 0; r < kRounds; r++)
+                for (size_t c = c0; c < c1; c++) {
+                    PAT_HIP(hipEventRecord(a, stream));
+                    for (int i = 0; i < kReps; i++) PAT_TRY(launch(c));
+                    PAT_HIP(hipEventRecord(b, stream));
+                    PAT_HIP(hipEventSynchronize(b));
+                    float ms = 0;
+                    PAT_HIP(hipEventElapsedTime(&ms, a, b));
+                    t[c] = std::min(t[c], ms);
                 }
+            return PAT_OK;
+        };
+        // warm-up of a few milliseconds first (clocks ramp up under load)
+        auto w0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < 1000; i++) {
+            PAT_TRY(launch(cands.size() - 1));
+            if (i % 16 == 15) {
+                PAT_HIP(hipStreamSynchronize(stream));
+                if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(5)) break;
             }
+        }
+        PAT_TRY(time_range(0, cands.size()));
+        if (sell_ok && pbc && sentinel()) {  // the sliced forms over the class-ordered layout
+            PAT_TRY(set_layout(true, true, p4, box, pbc, rc2));
+            const size_t c0 = cands.size();
+            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, true});
+            if (halo_ok)
+                for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, true});
+            t.resize(cands.size(), 1e30f);
+            PAT_TRY(time_range(c0, cands.size()));
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        lpr = best_l;
-        sell_kind = best_kind;
+        size_t bi = 0;
+        for (size_t c = 1; c < cands.size(); c++)
+            if (t[c] < t[bi]) bi = c;
+        if (pat::tune_log())
+            for (size_t c = 0; c < cands.size(); c++)
+                fprintf(stderr, "[tune lj N=%lld] lanes %d kind %d class-order %d: %.2f us%s\n", (long long)N,
+                        cands[c].lanes, cands[c].kind, (int)cands[c].by_class, 1000.0f * t[c] / 3,
+                        c == bi ? "  <- chosen" : "");
+        lpr = cands[bi].lanes;
+        sell_kind = lpr == PAT_IR_SLICED ? cands[bi].kind : k0;
+        PAT_TRY(set_layout(cands[bi].by_class, lpr == PAT_IR_SLICED && sell_kind >= 6, p4, box, pbc, rc2));
         std::lock_guard<std::mutex> g(g_lpr_mutex);
-        g_lpr_cache[key] = std::make_pair(best_l, best_kind);
+        g_lpr_cache[key] = KernelChoice{lpr, sell_kind, cands[bi].by_class};
         return PAT_OK;
     }
 };
@@ -396,7 +538,7 @@ struct pat_md {
                                     &total, st.stream));
         PAT_HIP(hipStreamSynchronize(st.stream));
         st.E = total / 2;
-        PAT_TRY(st.build_sell());
+        PAT_TRY(st.build_sell(st.pos[c], box, pbc, rc2));
         rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         rebuilds++;
         return PAT_OK;
@@ -413,8 +555,7 @@ struct pat_md {
             for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
         }
         int c = cur;
-        int rc = pat::vv_kick_drift_dev((float4 *)st.pos[c], (float4 *)st.vel, (const float4 *)st.force, st.N, box,
-                                        pbc, dt, s);
+        int rc = st.begin_steps(st.pos[c], box, pbc, dt, s);
         for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
             const bool last = k == nsteps - 1;
             if ((steps + k + 1) % every == 0) rc = rebuild(c);
@@ -452,15 +593,14 @@ struct pat_md {
         if (st.variant == PAT_IR_ATOMIC) {
             for (int k = 0; k < nsteps; k++) {
                 PAT_TRY(pat::vv_kick_drift_dev((float4 *)st.pos[start], (float4 *)st.vel, (const float4 *)st.force,
-                                               st.N, box, pbc, dt, s));
+                                               st.N, box, pbc, dt, nullptr, s));
                 PAT_TRY(pat::lj_force_atomic((const float4 *)st.pos[start], st.edges, st.E, st.N, box, pbc, rc2,
                                              (float4 *)st.force, s));
                 PAT_TRY(pat::vv_kick_dev((float4 *)st.vel, (const float4 *)st.force, st.N, dt, s));
             }
             return PAT_OK;
         }
-        PAT_TRY(pat::vv_kick_drift_dev((float4 *)st.pos[start], (float4 *)st.vel, (const float4 *)st.force, st.N,
-                                       box, pbc, dt, s));
+        PAT_TRY(st.begin_steps(st.pos[start], box, pbc, dt, s));
         int c = start;
         for (int k = 0; k < nsteps; k++) {
             const bool last = k == nsteps - 1;
@@ -498,7 +638,8 @@ int pat_lj_force_csr_dev(const float *pos4, const int32_t *rowptr, const int32_t
     PAT_REQUIRE(pos4 && rowptr && cols && force4, PAT_ERR_INVALID, "pat_lj_force_csr_dev: null argument");
     PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_csr_dev: box must exceed twice the cutoff");
     return pat::lj_force_csr((const float4 *)pos4, rowptr, cols, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff, 0.f,
-                             lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, (hipStream_t)stream);
+                             lanes_per_row, kForceOnly, (float4 *)force4, nullptr, nullptr, nullptr,
+                             (hipStream_t)stream);
 }
 
 int pat_ir_sell_build_dev(const int32_t *rowptr, const int32_t *cols, int64_t num_nodes, int32_t *sptr,
@@ -515,8 +656,9 @@ int pat_lj_force_sell_dev(const float *pos4, const int32_t *rowptr, const int32_
     PAT_REQUIRE(pos4 && rowptr && sptr && scol && force4, PAT_ERR_INVALID, "pat_lj_force_sell_dev: null argument");
     PAT_REQUIRE(kind >= 0 && kind < 3, PAT_ERR_INVALID, "pat_lj_force_sell_dev: kind %d not in 0..2", kind);
     PAT_REQUIRE(!pbc || box > 2 * cutoff, PAT_ERR_INVALID, "pat_lj_force_sell_dev: box must exceed twice the cutoff");
-    return pat::lj_force_sell((const float4 *)pos4, rowptr, sptr, scol, num_nodes, box, pbc ? 1 : 0, cutoff * cutoff,
-                              0.f, kForceOnly, (float4 *)force4, nullptr, nullptr, kind, (hipStream_t)stream);
+    return pat::lj_force_sell((const float4 *)pos4, rowptr, sptr, scol, nullptr, num_nodes, box, pbc ? 1 : 0,
+                              cutoff * cutoff, 0.f, kForceOnly, (float4 *)force4, nullptr, nullptr, kind, nullptr,
+                              (hipStream_t)stream);
 }
 
 int pat_lj_rcp_selftest(float lo, float hi, int64_t *bad_one_step, int64_t *bad_two_step)
@@ -800,6 +942,26 @@ int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
     return PAT_OK;
 }
 
+int pat_md_set_sliced_form(pat_md *h, int form)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_sliced_form: null handle");
+    PAT_REQUIRE(form >= 0 && form < 5, PAT_ERR_INVALID, "pat_md_set_sliced_form: form %d not in 0..4", form);
+    PAT_TRY(check_sliced(h->st, PAT_IR_CSR, PAT_IR_SLICED));
+    if (form >= 3) {
+        PAT_REQUIRE(h->st.sentinel(), PAT_ERR_UNSUPPORTED, "pat_md_set_sliced_form: the halo forms need N < 2^28 - 1");
+        PAT_TRY(h->st.set_layout(h->st.want_perm, true, h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+        PAT_REQUIRE(h->st.halo_ok, PAT_ERR_UNSUPPORTED, "pat_md_set_sliced_form: the list does not fit the halo form");
+    }
+    h->st.variant = PAT_IR_CSR;
+    h->st.lpr = PAT_IR_SLICED;
+    h->st.sell_kind = form >= 3 ? form + 3 : (h->st.sentinel() ? 3 : 0) + form;
+    h->st.layout_gen++;
+    h->drop_graphs();
+    PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    return PAT_OK;
+}
+
 int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_get_variant: null handle");
@@ -897,7 +1059,7 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
     if (h->st.variant == PAT_IR_ATOMIC) {
         for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
             rc = pat::vv_kick_drift_dev((float4 *)h->st.pos[c], (float4 *)h->st.vel, (const float4 *)h->st.force,
-                                        h->st.N, h->box, h->pbc, h->dt, s);
+                                        h->st.N, h->box, h->pbc, h->dt, nullptr, s);
             if (rc == PAT_OK) rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
             if (rc == PAT_OK)
                 rc = pat::lj_force_atomic((const float4 *)h->st.pos[c], h->st.edges, h->st.E, h->st.N, h->box, h->pbc,
@@ -906,8 +1068,7 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
             if (rc == PAT_OK) rc = pat::vv_kick_dev((float4 *)h->st.vel, (const float4 *)h->st.force, h->st.N, h->dt, s);
         }
     } else {
-        rc = pat::vv_kick_drift_dev((float4 *)h->st.pos[c], (float4 *)h->st.vel, (const float4 *)h->st.force, h->st.N,
-                                    h->box, h->pbc, h->dt, s);
+        rc = h->st.begin_steps(h->st.pos[c], h->box, h->pbc, h->dt, s);
         for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
             const bool last = k == nsteps - 1;
             rc = hipEventRecord(ev[1 + 2 * k], s) == hipSuccess ? PAT_OK : PAT_ERR_HIP;
@@ -1035,7 +1196,7 @@ struct pat_mdb {
         }
         PAT_HIP(hipStreamSynchronize(md.st.stream));
         md.st.E = entries / 2;
-        PAT_TRY(md.st.build_sell());
+        PAT_TRY(md.st.build_sell(md.st.pos[c], 1.0f, 2, md.rc2));
         md.rebuilds++;
         md.rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return PAT_OK;
@@ -1083,8 +1244,7 @@ struct pat_mdb {
     {
         hipStream_t s = md.st.stream;
         int c = md.cur;
-        PAT_TRY(pat::vv_kick_drift_dev((float4 *)md.st.pos[c], (float4 *)md.st.vel, (const float4 *)md.st.force,
-                                       md.st.N, 1.0f, 2, md.dt, s));
+        PAT_TRY(md.st.begin_steps(md.st.pos[c], 1.0f, 2, md.dt, s));
         int k = 0;
         while (k < nsteps) {
             if ((md.steps + k + 1) % md.every == 0) PAT_TRY(rebuild_all(c));
@@ -1180,7 +1340,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     if (rc == PAT_OK) rc = h->rebuild_all(0);
     if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK)
-        rc = m.st.csr_force(m.st.pos[0], 1.0f, 2, m.rc2, 0.f, kForceOnly, nullptr, m.st.stream);
+        rc = m.st.force_eval(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc != PAT_OK) {
         if (h->rp_s) (void)pat::dev_free(h->rp_s);
@@ -1217,6 +1377,7 @@ int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
     h->md.st.layout_gen++;
     h->drop_seg_graphs();
     // the stored force must come from the same summation order
+    PAT_TRY(h->md.st.check_box(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.st.stream));
     PAT_TRY(h->md.st.csr_force(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2, 0.f, kForceOnly, nullptr,
                                h->md.st.stream));
     PAT_HIP(hipStreamSynchronize(h->md.st.stream));

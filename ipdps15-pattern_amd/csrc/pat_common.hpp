@@ -53,6 +53,9 @@ size_t arena_largest_free(const Arena *a);
 // PAT_NO_TUNE in the environment, or per thread, as the serving loop's
 // workers do: timings taken beside other jobs would be noise).
 bool tuning_enabled();
+// PAT_TUNE_LOG in the environment: the autotuners print their candidates'
+// times to stderr.
+bool tune_log();
 void set_thread_tuning(bool on);
 
 }  // namespace pat

@@ -59,6 +59,7 @@ int current_device() { return g_device; }
 static thread_local bool t_tuning = true;
 
 bool tuning_enabled() { return t_tuning && !getenv("PAT_NO_TUNE"); }
+bool tune_log() { return getenv("PAT_TUNE_LOG") != nullptr; }
 
 void set_thread_tuning(bool on) { t_tuning = on; }
 

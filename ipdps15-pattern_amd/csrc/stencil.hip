@@ -22,6 +22,7 @@
 // halo-inclusive (nx+2)(ny+2)(nz+2), x fastest.
 // Compiled with -ffp-contract=off: each cell's arithmetic rounds exactly as
 // written, so results are bit-identical to a plain C sweep of the same rule.
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -427,49 +428,53 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
     t->fused = 0;
     default_tile(nx, ny, nz, &t->bx, &t->by, &t->ry, &t->zc);
     if (!pat::tuning_enabled()) return PAT_OK;
+    std::vector<HeatTile> cands;
+    for (const auto &sh : kShapes)
+        for (int z : {8, 16, 32, 64}) {
+            if (z > nz && z != 8) continue;
+            cands.push_back(HeatTile{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)});
+        }
+    if (allow_fused)
+        for (int r : {4, 8})
+            for (int z : {16, 32, 64, 128}) {
+                if (z > nz && z != 16) continue;
+                cands.push_back(HeatTile{1, 64, 4, r, (int)std::min<int64_t>(z, nz)});
+            }
     hipEvent_t a, b;
     PAT_HIP(hipEventCreate(&a));
     PAT_HIP(hipEventCreate(&b));
-    float best = 1e30f;
-    auto time_it = [&](const HeatTile &c, float *per_step) -> int {
-        float tmin = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {
+    // a few milliseconds of warm-up on the default tile (clocks ramp up
+    // under load), then rounds that time every candidate in turn; each
+    // candidate keeps its best round, fused ones counted per step
+    const HeatTile def = *t;
+    auto w0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < 2000; i++) {
+        PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, def, PAT_STENCIL_LDS25D, st));
+        if (i % 8 == 7) {
+            PAT_HIP(hipStreamSynchronize(st));
+            if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(5)) break;
+        }
+    }
+    std::vector<float> tm(cands.size(), 1e30f);
+    for (int r = 0; r < 3; r++)
+        for (size_t c = 0; c < cands.size(); c++) {
             PAT_HIP(hipEventRecord(a, st));
-            PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, c, PAT_STENCIL_LDS25D, st));
+            PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, cands[c], PAT_STENCIL_LDS25D, st));
             PAT_HIP(hipEventRecord(b, st));
             PAT_HIP(hipEventSynchronize(b));
             float ms = 0;
             PAT_HIP(hipEventElapsedTime(&ms, a, b));
-            if (rep > 0) tmin = std::min(tmin, ms);
+            tm[c] = std::min(tm[c], cands[c].fused ? ms / 2 : ms);
         }
-        *per_step = c.fused ? tmin / 2 : tmin;
-        return PAT_OK;
-    };
-    for (const auto &sh : kShapes) {
-        for (int z : {8, 16, 32, 64}) {
-            if (z > nz && z != 8) continue;
-            const HeatTile c{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)};
-            float ms;
-            PAT_TRY(time_it(c, &ms));
-            if (ms < best) {
-                best = ms;
-                *t = c;
-            }
-        }
-    }
-    if (allow_fused) {
-        for (int r : {4, 8})
-            for (int z : {16, 32, 64, 128}) {
-                if (z > nz && z != 16) continue;
-                const HeatTile c{1, 64, 4, r, (int)std::min<int64_t>(z, nz)};
-                float ms;
-                PAT_TRY(time_it(c, &ms));
-                if (ms < best) {
-                    best = ms;
-                    *t = c;
-                }
-            }
-    }
+    size_t bi = 0;
+    for (size_t c = 1; c < cands.size(); c++)
+        if (tm[c] < tm[bi]) bi = c;
+    if (pat::tune_log())
+        for (size_t c = 0; c < cands.size(); c++)
+            fprintf(stderr, "[tune heat3d %lldx%lldx%lld] fused %d bx %d by %d ry %d zc %d: %.2f us/step%s\n",
+                    (long long)nx, (long long)ny, (long long)nz, cands[c].fused, cands[c].bx, cands[c].by,
+                    cands[c].ry, cands[c].zc, 1000.0f * tm[c], c == bi ? "  <- chosen" : "");
+    *t = cands[bi];
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     std::lock_guard<std::mutex> g(g_tile_mutex);

@@ -219,6 +219,54 @@ def test_moldyn_workload(pa, orc, gpu, order):
     assert np.max(np.abs(f.astype(np.float64).sum(axis=0))) / np.max(np.abs(f)) < 1e-3
 
 
+@pytest.mark.parametrize("cells", [6, 16])
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
+def test_sliced_forms_vs_oracle(pa, orc, gpu, cells, form):
+    """Each loop form on the handle's sentinel layout (byte-offset columns, the
+    minimum image taken per wavefront only on axes where a row lies within a
+    cutoff of a face; forms 3 and 4 stage each block's neighbour positions in
+    LDS) gives the serial scatter's bits over a trajectory."""
+    pos, box, edges = pa.moldyn_system(cells, seed=33)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=5)
+    md = pa.MD(pos, vel, edges, box, 2.5, 0.005)
+    md.set_sliced_form(form)
+    assert md.variant()[1] == pa.PAT_IR_SLICED
+    md.run(7)
+    md.run(8)
+    p, v, f = md.get()
+    md.close()
+    rp, rv, rf, _ = orc.md_run(pos, vel, node_sorted(edges), box, 2.5, 0.005, 15)
+    assert np.array_equal(f, rf) and np.array_equal(v, rv) and np.array_equal(p, rp)
+
+
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
+def test_minimum_image_kept_for_positions_outside_the_box(pa, orc, gpu, form):
+    """Rows 0..63 (one wavefront) lie well inside the box; node 64 starts at
+    x = -2, outside it.  Pair (0, 64) is 9 apart in x unwrapped and 1 apart
+    through the periodic image: its force is only right if the handle sees
+    that not every position lies in [0, L] and keeps the minimum image for
+    row 0's wavefront.  After the first drift node 64 is wrapped to x ~ 8,
+    inside, and later runs may skip the image -- same bits throughout."""
+    L, n = 10.0, 128
+    pos = np.zeros((n, 3), np.float32)
+    pos[:64] = np.stack([np.full(64, 7.0), 2.6 + 0.07 * np.arange(64), np.full(64, 5.0)], 1)
+    pos[64] = (-2.0, 2.6, 5.0)
+    pos[65:] = np.stack([np.full(63, 5.0), np.full(63, 7.0), 2.6 + 0.07 * np.arange(63)], 1)
+    edges = np.array([[0, 64], [65, 80]], np.int32)
+    md = pa.MD(pos, None, edges, L, 2.5, 0.001)
+    md.set_sliced_form(form)
+    _, _, f0 = md.get()
+    rf0, _ = orc.lj_force(pos, edges, L, 2.5)
+    assert np.array_equal(f0, rf0) and f0[0, 0] != 0.0
+    md.run(3)
+    md.run(3)
+    p, v, f = md.get()
+    md.close()
+    rp, rv, rf, _ = orc.md_run(pos, np.zeros_like(pos), edges, L, 2.5, 0.001, 6)
+    assert 7.5 < rp[64, 0] < 8.5
+    assert np.array_equal(f, rf) and np.array_equal(v, rv) and np.array_equal(p, rp)
+
+
 def test_reciprocal_matches_ieee_division(pa, gpu):
     """The force kernels' reciprocal (one Newton step on v_rcp_f32) equals the
     IEEE quotient 1.0f / b for every FP32 value in [2^-126, 2^125]: checked
