@@ -1055,8 +1055,23 @@ __global__ __launch_bounds__(256) void lj_sell_halo(const float4 *__restrict__ p
     const int nsl = (N + kWave - 1) / kWave;
     const int b = pat::xcd_swizzle(blockIdx.x, gridDim.x);
     const int h0 = hptr[b], nh = hptr[b + 1] - h0;
-    for (int i = threadIdx.x; i < nh; i += 256) hpos[i] = pos[hidx[h0 + i]];
-    if (threadIdx.x == 0) hpos[nh] = pos[N];  // the sentinel
+    // list entries (and the sentinel, node N, at entry nh) four per thread
+    // at a time: the four index reads, then the four position reads, so
+    // the loads overlap instead of queueing
+    for (int i0 = threadIdx.x; i0 <= nh; i0 += 4 * 256) {
+        int g[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = i0 + 256 * r;
+            g[r] = i < nh ? hidx[h0 + i] : N;
+        }
+        float4 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = pos[g[r]];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (i0 + 256 * r <= nh) hpos[i0 + 256 * r] = v[r];
+    }
     const int s = b * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int row = s < nsl ? slice_row(perm, s * kWave + lane) : N;

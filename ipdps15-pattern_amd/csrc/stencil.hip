@@ -404,7 +404,7 @@ int heat3d_steps(float *const buf[2], int start, int64_t nx, int64_t ny, int64_t
     return PAT_OK;
 }
 
-// Autotuner: times every instantiated tile shape with z-chunks of 8..64
+// Autotuner: times every instantiated tile shape with z-chunks of 2..64
 // planes on the caller's grid (in -> out; out is scratch until the first
 // real sweep writes it, and no kernel writes the halo), keeps the fastest,
 // and caches it per grid shape for the process.
@@ -430,14 +430,14 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
     if (!pat::tuning_enabled()) return PAT_OK;
     std::vector<HeatTile> cands;
     for (const auto &sh : kShapes)
-        for (int z : {8, 16, 32, 64}) {
-            if (z > nz && z != 8) continue;
+        for (int z : {2, 4, 8, 16, 32, 64}) {
+            if (z > nz && z != 2) continue;
             cands.push_back(HeatTile{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)});
         }
     if (allow_fused)
         for (int r : {4, 8})
-            for (int z : {16, 32, 64, 128}) {
-                if (z > nz && z != 16) continue;
+            for (int z : {2, 4, 8, 16, 32, 64, 128}) {
+                if (z > nz && z != 2) continue;
                 cands.push_back(HeatTile{1, 64, 4, r, (int)std::min<int64_t>(z, nz)});
             }
     hipEvent_t a, b;
