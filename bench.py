@@ -52,6 +52,14 @@ def cpu_threads():
 
 LANES_NAME = {1: "csr one lane per row", 4: "csr 4 lanes per row", 8: "csr 8 lanes per row",
               16: "csr 16 lanes per row", 32: "csr 32 lanes per row", 64: "sliced (SELL-64) one lane per row"}
+FORM_NAME = {0: "global gathers, 4 ahead", 1: "global gathers, pipelined by 4", 2: "global gathers, pipelined by 8",
+             3: "LDS-staged block neighbours, 4 ahead", 4: "LDS-staged block neighbours, 8 ahead"}
+
+
+def kernel_name(md):
+    lanes, form = md.variant()[1], md.sliced_form()
+    name = LANES_NAME.get(lanes, "")
+    return f"{name} ({FORM_NAME[form]})" if form >= 0 else name
 
 
 def node_sorted(edges):
@@ -73,10 +81,10 @@ def time_md(md, steps, warmup, n, E):
     kavg = float(np.mean(kms))
     alg = 8 * E + 48 * n
     return {"pairs_per_s": E * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_us": kavg * 1e3,
-            "achieved_gbs": alg / (kavg * 1e-3) / 1e9, "kernel": LANES_NAME.get(md.variant()[1], "")}
+            "achieved_gbs": alg / (kavg * 1e-3) / 1e9, "kernel": kernel_name(md), "form": md.sliced_form()}
 
 
-def bench_lj(steps, warmup, do_cpu):
+def bench_lj(steps, warmup, do_cpu, forms=()):
     t0 = time.perf_counter()
     pos, box, edges = pa.moldyn_system(40, seed=42)
     n, E = len(pos), len(edges)
@@ -89,6 +97,14 @@ def bench_lj(steps, warmup, do_cpu):
         out[name] = r
         log(f"[lj] {name}: {r['ms_per_step'] * 1e3:.1f} us/step ({r['pairs_per_s'] / 1e9:.2f} G pairs/s), force "
             f"kernel {r['kernel_us']:.1f} us [{r['kernel']}], {r['achieved_gbs']:.0f} GB/s algorithmic")
+    for f in forms:  # node order with each requested sliced form in place of the tuner's choice
+        md = pa.MD(pos, None, edges, box, 2.5, 0.005, order=pa.PAT_IR_ORDER_NODE)
+        md.set_sliced_form(f)
+        r = time_md(md, steps, warmup, n, E)
+        md.close()
+        out.setdefault("forms", {})[str(f)] = r
+        log(f"[lj] node_order, form {f}: {r['ms_per_step'] * 1e3:.1f} us/step ({r['pairs_per_s'] / 1e9:.2f} G pairs/s), "
+            f"force kernel {r['kernel_us']:.1f} us [{r['kernel']}]")
     mda = pa.MD(pos, None, edges, box, 2.5, 0.005, variant=pa.PAT_IR_ATOMIC)
     mda.run_timed(warmup)
     pa.call("pat_device_sync")
@@ -282,6 +298,8 @@ def main():
     ap.add_argument("--engines", default="lj,stencil,kmeans,serve",
                     help="comma-separated subset of lj,stencil,kmeans,serve,copy (lj carries value)")
     ap.add_argument("--serve-jobs", type=int, default=600, help="jobs in the serving mix (the config's is 10000)")
+    ap.add_argument("--lj-forms", default="", help="comma-separated sliced LJ loop forms (0-4) to time beside the "
+                                                    "tuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every engine here runs its whole domain on a single MI355X, and no "
@@ -306,7 +324,8 @@ def main():
     if "lj" not in which:
         print(json.dumps({"engines": engines}))
         return
-    lj = bench_lj(args.steps, args.warmup, not args.no_cpu_baseline)
+    forms = [int(f) for f in args.lj_forms.split(",") if f != ""]
+    lj = bench_lj(args.steps, args.warmup, not args.no_cpu_baseline, forms)
     engines["lj_moldyn"] = {k: v for k, v in lj.items() if k != "cpu"}
     head = lj["node_order"]
 
@@ -329,7 +348,8 @@ def main():
                    "pairs": lj["pairs"], "nodes": lj["nodes"],
                    "kernel": f"in-order CSR force sums in node order, {head['kernel']}, VV update fused"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic("lj_step_bytes")},
+                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(f"lj_step_bytes_form{head['form']}")},
         "cpu_baseline": lj["cpu"],
         "engines": engines,
     }

@@ -292,6 +292,9 @@ int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order);
  * pat_md_set_variant(h, PAT_IR_CSR, PAT_IR_SLICED); no reference
  * counterpart (the reference has one kernel form). */
 int pat_md_set_sliced_form(pat_md *h, int form);
+/* The sliced loop form in use (0..4 as above), or -1 when the handle runs
+ * another kernel.  No reference counterpart. */
+int pat_md_get_sliced_form(pat_md *h, int *form);
 /* Current half-pair count, steps run, neighbour rebuilds and their total
  * host-measured time. */
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);

@@ -962,6 +962,16 @@ int pat_md_set_sliced_form(pat_md *h, int form)
     return PAT_OK;
 }
 
+int pat_md_get_sliced_form(pat_md *h, int *form)
+{
+    PAT_REQUIRE(h && form, PAT_ERR_INVALID, "pat_md_get_sliced_form: null argument");
+    const IrState &st = h->st;
+    if (st.variant != PAT_IR_CSR || st.lpr != PAT_IR_SLICED) *form = -1;
+    else if (st.sell_kind >= 6) *form = st.sell_kind - 3;
+    else *form = st.sell_kind - (st.sentinel() ? 3 : 0);
+    return PAT_OK;
+}
+
 int pat_md_get_variant(pat_md *h, int *variant, int *lanes_per_row, int *order)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_get_variant: null handle");

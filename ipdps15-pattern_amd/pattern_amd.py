@@ -138,6 +138,7 @@ SIGNATURES = {
     "pat_md_set_row_order": (c_int, [c_vp, c_int]),
     "pat_md_get_variant": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_md_set_sliced_form": (c_int, [c_vp, c_int]),
+    "pat_md_get_sliced_form": (c_int, [c_vp, c_intp]),
     "pat_mdb_create": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_double, c_int, c_double,
                                ctypes.POINTER(c_vp)]),
     "pat_mdb_set_lanes": (c_int, [c_vp, c_int]),
@@ -527,8 +528,14 @@ class MD:
         call("pat_md_set_variant", self.h, variant, lanes_per_row)
 
     def set_sliced_form(self, form):
-        """Sliced layout with loop form 0, 1 or 2 (same bits for all)."""
+        """Sliced layout with loop form 0..4 (same bits for all)."""
         call("pat_md_set_sliced_form", self.h, form)
+
+    def sliced_form(self):
+        """The sliced loop form in use, or -1."""
+        f = c_int(0)
+        call("pat_md_get_sliced_form", self.h, ctypes.byref(f))
+        return f.value
 
     def variant(self):
         """(variant, lanes per row in use, row order)."""
