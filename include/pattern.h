@@ -379,6 +379,38 @@ int pat_kmeans_get_object(pat_kmeans *h, int64_t *sums, int64_t *counts, int64_t
 int pat_kmeans_stats(pat_kmeans *h, int64_t *rescored_total, float *maxabs);
 int pat_kmeans_destroy(pat_kmeans *h);
 
+/* Generalized reduction with the applications' additive map/reduce pairs
+ * (reduce adds the emitted values), the RegularRuntime work beside kmeans:
+ * RegularRuntime(num_procs, input, input_size, offsets, n, params,
+ * param_size) + RegularInit + set_map_idx/set_reduce_idx + RegularStart +
+ * RegularGetOutput (kmeans/regular/regular_runtime.h:79-105) with
+ *   PAT_GR_VSTATS (0): kmeans/regular.h:30-73 map/reduce as moldyn runs it
+ *     (kmeans/moldyn.cu:440-470): units = velocities (double[3]), param =
+ *     float vaverh; per unit {(float)|v|, |v| > vaverh}; 1 key, 2 values.
+ *   PAT_GR_LJ_ENERGY (2): miniMD/regular.h:139-181 map2/reduce2: units =
+ *     edges (int[2]), param = {double cutsq, double x[3N]}; per edge within
+ *     the cutoff phi = sr6 (sr6 - 1); 1 key, 1 value.
+ * The reference's ANY_KEY buckets are folded into key 0 and the values are
+ * summed in FP64 in a fixed order (the same bits every run).  Units and
+ * parameter are copied at create.  kmeans map1 (index 1) runs on
+ * pat_kmeans_*. */
+#define PAT_GR_VSTATS 0
+#define PAT_GR_LJ_ENERGY 2
+typedef struct pat_gr pat_gr;
+int pat_gr_create(const void *units, int64_t num_units, int64_t unit_bytes, int map_idx, const void *param,
+                  int64_t param_bytes, pat_gr **out);
+/* set_map_idx / set_reduce_idx: the handle's own map and its adding reduce
+ * only (PAT_ERR_UNSUPPORTED otherwise: units are laid out per map). */
+int pat_gr_set_map_idx(pat_gr *h, int idx);
+int pat_gr_set_reduce_idx(pat_gr *h, int idx);
+int pat_gr_set_param(pat_gr *h, const void *param, int64_t param_bytes);
+/* RegularStart: one map + reduce pass over every unit.  Blocking. */
+int pat_gr_run(pat_gr *h);
+/* RegularGetOutput: the reduction object, keys x values FP64 sums
+ * (values may be NULL) and the emits per key (may be NULL). */
+int pat_gr_get_output(pat_gr *h, int *num_keys, int *num_values, double *values, int64_t *emits);
+int pat_gr_destroy(pat_gr *h);
+
 /* ------------------------------------------------------------------------
  * Serving loop.  No counterpart in the reference, whose mains run one
  * application per process (RuntimeInit .. RuntimeFinalize,

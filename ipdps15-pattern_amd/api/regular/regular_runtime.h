@@ -1,12 +1,20 @@
 // RegularRuntime (kmeans/regular/regular_runtime.h:11-106): the
 // generalized-reduction runtime.  Same constructor and calls; the map/reduce
 // functor pair selected by set_map_idx/set_reduce_idx runs on the GPU through
-// libpattern.  Table of the kmeans application (kmeans/regular.h:143-145):
-// index 1 = map1/reduce1 (nearest centre, emit {x, 1, min_dist}).
-// The parameter is the K x D centre array; RegularStart() is one map+reduce
-// pass over every unit; RegularGetOutput() returns, per key present, the
-// reduced value {sum x_0..x_{D-1}, count, sum min_dist} as floats, the
-// layout reduce1 produces (kmeans/regular.h:121-136).
+// libpattern.  Functor table (kmeans/regular.h:143-145, miniMD/regular.h):
+//   index 0 = map/reduce (kmeans/regular.h:30-73): units are velocities
+//     (double[3]), the parameter is float vaverh; output: one key, the value
+//     {sum |v|, count of |v| > vaverh} as float[2] (reduce's layout);
+//   index 1 = map1/reduce1 (nearest centre, emit {x, 1, min_dist}): the
+//     parameter is the K x D centre array; output per key present
+//     {sum x_0..x_{D-1}, count, sum min_dist} as floats (kmeans/regular.h:
+//     121-136);
+//   index 2 = map2/reduce2 (miniMD/regular.h:139-181): units are edges
+//     (int[2]), the parameter is {double cutsq, double x[3N]}; output: one
+//     key, the value {sum phi} as double[1].
+// The reference spreads indices 0 and 2 over ANY_KEY buckets; here they are
+// one key (the buckets' sum).  RegularStart() is one map+reduce pass over
+// every unit.
 #ifndef REGULAR_RUNTIME_H_
 #define REGULAR_RUNTIME_H_
 
@@ -41,6 +49,7 @@ class RegularRuntime {
     ~RegularRuntime()
     {
         if (handle_) pat_kmeans_destroy(handle_);
+        if (gr_) pat_gr_destroy(gr_);
     }
 
     // Validates the units and uploads them.  Units of equal size laid out
@@ -69,8 +78,12 @@ class RegularRuntime {
 
     void RegularStart()
     {
-        if (map_idx_ != 1 || reduce_idx_ != 1)
-            fail("RegularStart: only the kmeans map1/reduce1 pair (index 1) has a GPU engine");
+        if (map_idx_ != reduce_idx_ || map_idx_ < 0 || map_idx_ > 2)
+            fail("RegularStart: functor pairs 0/0, 1/1 and 2/2 have GPU engines");
+        if (map_idx_ != 1) {
+            start_additive();
+            return;
+        }
         if (parameter_size_ <= 0 || parameter_size_ % (int)(sizeof(float) * dim_))
             fail("RegularStart: the parameter must be K x D floats");
         const int k = parameter_size_ / (int)(sizeof(float) * dim_);
@@ -87,6 +100,7 @@ class RegularRuntime {
 
     struct output RegularGetOutput()
     {
+        if (gr_) return additive_output();
         if (!handle_) fail("RegularGetOutput: RegularStart has not run");
         std::vector<int64_t> sums((size_t)k_ * dim_), counts(k_), dsums(k_);
         int s = 0, ds = 0;
@@ -125,6 +139,48 @@ class RegularRuntime {
     }
 
   private:
+    // Indices 0 and 2: the units back to back (unit = the map's record), the
+    // parameter as given; one handle per runtime, the parameter refreshed on
+    // every start.
+    void start_additive()
+    {
+        const size_t usz = map_idx_ == 0 ? 3 * sizeof(double) : 2 * sizeof(int);
+        for (size_t i = 0; i < num_offsets_; i++)
+            if (offsets_[i].size != usz || offsets_[i].offset != i * usz)
+                fail("RegularStart: map 0 / 2 units must be back-to-back double[3] / int[2] records");
+        if (!gr_)
+            PAT_API_CALL(pat_gr_create(input_, (int64_t)num_offsets_, (int64_t)usz, map_idx_, parameter_,
+                                       parameter_size_, &gr_));
+        else
+            PAT_API_CALL(pat_gr_set_param(gr_, parameter_, parameter_size_));
+        PAT_API_CALL(pat_gr_run(gr_));
+    }
+
+    struct output additive_output()
+    {
+        double vals[2] = {0.0, 0.0};
+        int nk = 0, nv = 0;
+        PAT_API_CALL(pat_gr_get_output(gr_, &nk, &nv, vals, nullptr));
+        keys_.assign(1, 0u);
+        kidx_.assign(1, 0u);
+        vidx_.assign(1, 0u);
+        if (map_idx_ == 0) {
+            vals_.assign({(float)vals[0], (float)vals[1]});
+            out_.output_vals = (char *)vals_.data();
+            out_.value_size = 2 * sizeof(float);
+        } else {
+            dvals_.assign(1, vals[0]);
+            out_.output_vals = (char *)dvals_.data();
+            out_.value_size = sizeof(double);
+        }
+        out_.output_keys = (char *)keys_.data();
+        out_.key_index = kidx_.data();
+        out_.val_index = vidx_.data();
+        out_.key_num = 1;
+        out_.key_size = sizeof(unsigned int);
+        return out_;
+    }
+
     static void fail(const char *msg)
     {
         fprintf(stderr, "pattern error: %s\n", msg);
@@ -136,6 +192,8 @@ class RegularRuntime {
     std::vector<float> packed_;
     std::vector<unsigned int> keys_, kidx_, vidx_;
     std::vector<float> vals_;
+    std::vector<double> dvals_;
+    pat_gr *gr_ = nullptr;
     struct output out_;
 };
 

@@ -13,6 +13,7 @@
 #include "irregular.h"
 #include "include/runtime.h"
 #include "irregular/irregular_runtime.h"
+#include "regular/regular_runtime.h"
 
 int main(int argc, char *argv[])
 {
@@ -90,6 +91,32 @@ int main(int argc, char *argv[])
         px += v[3 * i];
     }
     printf("moldyn: %d steps, kinetic energy %.6e, total x momentum %.2e\n", steps, ekin, px);
+
+    // velocity statistics through the generalized-reduction API, as moldyn
+    // runs them (kmeans/moldyn.cu:440-470): map 0 emits {|v|, |v| > vaverh}
+    // per molecule, reduce 0 adds them; checked against the host loop
+    double vsum = 0.0;
+    for (long i = 0; i < n; i++)
+        vsum += std::sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    float vaverh = (float)(vsum / n);
+    std::vector<Offset> offsets(n);
+    for (long i = 0; i < n; i++) offsets[i] = Offset{(size_t)i * 3 * sizeof(double), 3 * sizeof(double)};
+    RegularRuntime rr(1, v.data(), sizeof(double) * 3 * n, offsets.data(), n, &vaverh, sizeof(float));
+    rr.RegularInit();
+    rr.set_map_idx(0);
+    rr.set_reduce_idx(0);
+    rr.RegularStart();
+    struct output o = rr.RegularGetOutput();
+    const float *val = (const float *)o.output_vals;
+    double hs = 0.0, hc = 0.0;
+    for (long i = 0; i < n; i++) {
+        const float sq = (float)std::sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        if (sq > vaverh) hc += 1.0;
+        hs += sq;
+    }
+    printf("moldyn: velocity statistics sum |v| %.6e (host %.6e), above vaverh %.0f (host %.0f)\n", val[0], hs,
+           val[1], hc);
+    if (std::fabs(val[0] - hs) > 1e-6 * hs || val[1] != (float)hc) return 1;
     RuntimeFinalize();
     return 0;
 }

@@ -18,6 +18,8 @@ PAT_STENCIL_AUTO = 0
 PAT_STENCIL_LDS25D = 1
 PAT_STENCIL_GENERIC = 2
 PAT_STENCIL_TEMPORAL2 = 3
+PAT_GR_VSTATS = 0
+PAT_GR_LJ_ENERGY = 2
 PAT_IR_LJ_MINIMAGE = 0
 PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0
@@ -138,6 +140,13 @@ SIGNATURES = {
     "pat_md_set_row_order": (c_int, [c_vp, c_int]),
     "pat_md_get_variant": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_md_set_sliced_form": (c_int, [c_vp, c_int]),
+    "pat_gr_create": (c_int, [c_vp, c_i64, c_i64, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
+    "pat_gr_set_map_idx": (c_int, [c_vp, c_int]),
+    "pat_gr_set_reduce_idx": (c_int, [c_vp, c_int]),
+    "pat_gr_set_param": (c_int, [c_vp, c_vp, c_i64]),
+    "pat_gr_run": (c_int, [c_vp]),
+    "pat_gr_get_output": (c_int, [c_vp, c_intp, c_intp, c_vp, c_vp]),
+    "pat_gr_destroy": (c_int, [c_vp]),
     "pat_md_get_sliced_form": (c_int, [c_vp, c_intp]),
     "pat_mdb_create": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_double, c_int, c_double,
                                ctypes.POINTER(c_vp)]),
@@ -848,3 +857,46 @@ class Serve:
             self.close()
         except Exception:
             pass
+
+
+class GR:
+    """Generalized reduction with an additive map/reduce pair (pat_gr_*):
+    PAT_GR_VSTATS over velocities (n x 3 FP64) with vaverh, or
+    PAT_GR_LJ_ENERGY over edges (E x 2 int32) with (cutsq, positions)."""
+
+    def __init__(self, map_idx, units, param):
+        self.map_idx = map_idx
+        u = np.ascontiguousarray(units, dtype=np.float64 if map_idx == PAT_GR_VSTATS else np.int32)
+        self._units = u
+        p = self._param_bytes(param)
+        h = c_vp()
+        call("pat_gr_create", ptr(u), u.shape[0], u.itemsize * u.shape[1], map_idx, ptr(p), p.nbytes,
+             ctypes.byref(h))
+        self.h = h
+
+    def _param_bytes(self, param):
+        if self.map_idx == PAT_GR_VSTATS:
+            return np.array([param], dtype=np.float32)
+        cutsq, x = param
+        return np.concatenate([[float(cutsq)], np.ascontiguousarray(x, dtype=np.float64).ravel()])
+
+    def set_param(self, param):
+        p = self._param_bytes(param)
+        call("pat_gr_set_param", self.h, ptr(p), p.nbytes)
+
+    def run(self):
+        call("pat_gr_run", self.h)
+
+    def output(self):
+        """(values: keys x values FP64, emits per key)."""
+        k, v = c_int(0), c_int(0)
+        call("pat_gr_get_output", self.h, ctypes.byref(k), ctypes.byref(v), None, None)
+        vals = np.zeros((k.value, v.value), dtype=np.float64)
+        em = np.zeros(k.value, dtype=np.int64)
+        call("pat_gr_get_output", self.h, None, None, ptr(vals), ptr(em))
+        return vals, em
+
+    def close(self):
+        if self.h:
+            call("pat_gr_destroy", self.h)
+            self.h = None

@@ -831,3 +831,67 @@ void orc_kmeans_lloyd_f64(int64_t N, int D, int K, const float *x, int iters, do
     free(sums);
     free(counts);
 }
+
+/* ======================================================================
+ * Generalized reduction with additive functors (map/reduce pairs whose
+ * reduce adds values), the work the reference runs through RegularRuntime
+ * beside kmeans.  Keys: the reference spreads emits over ANY_KEY buckets
+ * (threadIdx % 41 on the GPU, 0 on the CPU, kmeans/regular/
+ * gpu_kernel_tools.cu:68-71) and the caller sums them; here every emit goes
+ * to key 0, and the sums are FP64 in unit order.
+ * ====================================================================== */
+
+/*
+ * Velocity statistics, map/reduce of kmeans/regular.h:30-73 as moldyn calls
+ * it (kmeans/moldyn.cu:440-470): per unit (double[3] velocity) value[0] =
+ * (float)sqrt(vx*vx + vy*vy + vz*vz) in FP64, value[1] = value[0] > vaverh
+ * (FP32 compare); reduce adds both.  out[0] = sum of value[0] (FP64, in unit
+ * order), out[1] = the count; per-unit values go to per (float, 2 per unit)
+ * when not NULL.
+ */
+void orc_gr_vstats(int64_t n, const double *vel, float vaverh, double *out, float *per)
+{
+    double s = 0.0, c = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        const double *v = vel + 3 * i;
+        float value0 = (float)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        float value1 = value0 > vaverh ? 1.0f : 0.0f;
+        if (per) {
+            per[2 * i] = value0;
+            per[2 * i + 1] = value1;
+        }
+        s += (double)value0;
+        c += (double)value1;
+    }
+    out[0] = s;
+    out[1] = c;
+}
+
+/*
+ * Pair energy, map2/reduce2 of miniMD/regular.h:139-181: per unit (int[2]
+ * edge i, j) with parameter {cutsq, x[3N]} (FP64): rsq = dx*dx + dy*dy +
+ * dz*dz (no minimum image), and when rsq < cutsq emit phi = sr6*(sr6 - 1),
+ * sr2 = 1/rsq, sr6 = sr2*sr2*sr2; reduce adds.  Returns the FP64 sum in unit
+ * order; *emits = the number of pairs within the cutoff.
+ */
+double orc_gr_lj_energy(int64_t e, const int32_t *edges, const double *x, double cutsq, int64_t *emits)
+{
+    double s = 0.0;
+    int64_t c = 0;
+    for (int64_t k = 0; k < e; k++) {
+        const int64_t i = edges[2 * k], j = edges[2 * k + 1];
+        double delx = x[3 * i] - x[3 * j];
+        double dely = x[3 * i + 1] - x[3 * j + 1];
+        double delz = x[3 * i + 2] - x[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < cutsq) {
+            double sr2 = 1.0 / rsq;
+            double sr6 = sr2 * sr2 * sr2;
+            double phi = sr6 * (sr6 - 1.0);
+            s += phi;
+            c++;
+        }
+    }
+    *emits = c;
+    return s;
+}

@@ -41,6 +41,8 @@ _SIG = {
     "orc_kmeans_update": (None, [c_int, c_int, c_vp, c_vp, c_int, c_vp]),
     "orc_kmeans_lloyd": (None, [c_i64, c_int, c_int, c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_int]),
     "orc_kmeans_lloyd_f64": (None, [c_i64, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_int]),
+    "orc_gr_vstats": (None, [c_i64, c_vp, c_float, c_vp, c_vp]),
+    "orc_gr_lj_energy": (c_double, [c_i64, c_vp, c_vp, c_double, ctypes.POINTER(c_i64)]),
 }
 
 _lib = None
@@ -269,3 +271,25 @@ def ref_minimd_system(n, rho=0.8442, nbin=None, cutneigh=2.8, cutforce=2.5):
     if rc != 0:
         raise RuntimeError(f"ref_minimd_system failed ({rc})")
     return pos[:nat.value], force[:nat.value], pairs[:npairs.value]
+
+
+# ---------------------------------------------------------------- additive generalized reductions
+
+def gr_vstats(vel, vaverh, per_unit=False):
+    """moldyn velocity statistics (sum of |v| as FP32 values, count above
+    vaverh); returns (sum, count) or ((sum, count), per-unit values)."""
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    out = np.zeros(2, dtype=np.float64)
+    per = np.zeros((len(v), 2), dtype=np.float32) if per_unit else None
+    lib().orc_gr_vstats(len(v), _p(v), np.float32(vaverh), _p(out), _p(per))
+    return ((out[0], out[1]), per) if per_unit else (out[0], out[1])
+
+
+def gr_lj_energy(edges, x, cutsq):
+    """miniMD map2 pair energy sum over the edges within cutsq (FP64, no
+    minimum image); returns (sum, pairs within the cutoff)."""
+    e = np.ascontiguousarray(edges, dtype=np.int32)
+    p = np.ascontiguousarray(x, dtype=np.float64)
+    n = c_i64(0)
+    s = lib().orc_gr_lj_energy(len(e), _p(e), _p(p), float(cutsq), ctypes.byref(n))
+    return s, n.value

@@ -31,7 +31,7 @@ def test_kmeans_app(gpu):
 
 def test_moldyn_app(gpu):
     out = run("moldyn", 8, 20)
-    assert "first pass" in out and "kinetic energy" in out
+    assert "first pass" in out and "kinetic energy" in out and "velocity statistics" in out
 
 
 def test_minimd_app(gpu):
