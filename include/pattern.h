@@ -412,6 +412,44 @@ int pat_gr_get_output(pat_gr *h, int *num_keys, int *num_values, double *values,
 int pat_gr_destroy(pat_gr *h);
 
 /* ------------------------------------------------------------------------
+ * Reference input files (host only), so reference-shaped datasets replay.
+ * pat_read_mold: moldyn's parameter file, which kmeans/moldyn.cu:193-201
+ *   scanf()s from stdin: nB (4 nB^3 molecules), time steps, neighbour
+ *   update period, cutoff, mesh and xyz file names, percent, start time,
+ *   update frequency.
+ * pat_read_ljin: miniMD's lj.in (miniMD/input.cpp:39-72): two header lines,
+ *   then problem size, steps, bins, dt, temperature, density, reneighbour
+ *   period, cutoffs, thermo period, one group per line.
+ * pat_read_mesh: ReadInputGraph (kmeans/misc.c:76-145): leading '%' lines,
+ *   "#nodes #edges", then per node its neighbours (1-based) up to a
+ *   non-positive value or the line end; each edge once, from the node with
+ *   the smaller number (neighbour >= own number, as the reference counts).
+ *   Edges are 1-based, or 0-based with zero_based (the reference's
+ *   CSTYLE).  edges = NULL: counts only.  The edge count must equal the
+ *   header's.
+ * pat_read_xyz: ReadCoord (misc.c:226-247): whitespace-separated triples,
+ *   exactly num_nodes of them. */
+typedef struct pat_mold {
+    int box_cells, steps, neigh_update;
+    double cutoff;
+    char mesh[512], xyz[512];
+    double percent;
+    int start_time, update_freq;
+} pat_mold;
+typedef struct pat_ljin {
+    int nx, ny, nz, ntimes, nbinx, nbiny, nbinz;
+    double dt, t_request, rho;
+    int every;
+    double cutforce, cutneigh;
+    int nstat;
+} pat_ljin;
+int pat_read_mold(const char *path, pat_mold *out);
+int pat_read_ljin(const char *path, pat_ljin *out);
+int pat_read_mesh(const char *path, int zero_based, int64_t *num_nodes, int64_t *num_edges, int32_t *edges,
+                  int64_t capacity);
+int pat_read_xyz(const char *path, int64_t num_nodes, double *xyz);
+
+/* ------------------------------------------------------------------------
  * Serving loop.  No counterpart in the reference, whose mains run one
  * application per process (RuntimeInit .. RuntimeFinalize,
  * kmeans/include/runtime.cpp:4-13): here heat3D, LJ and kmeans jobs of any

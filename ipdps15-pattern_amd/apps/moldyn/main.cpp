@@ -2,12 +2,14 @@
 // kmeans/moldyn.cu:331-403 (host coordinate update, reset_node_data,
 // IrregularStart, get_reduction_result, host velocity update) as
 // velocity-Verlet, on an FCC lattice of M^3 cells (default 16: 16,384
-// molecules) at density 0.8442, jitter 0.05, cutoff 2.5, dt 0.005, 50 steps.
+// molecules) at density 0.8442, jitter 0.05, cutoff 2.5, dt 0.005, 50 steps;
+// or, with -f <parameter file> [steps], a reference dataset (.mesh + .xyz).
 // Node data is double3 as moldyn's point_data.  The first force pass is
 // checked against the host map functor.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "irregular.h"
@@ -17,26 +19,63 @@
 
 int main(int argc, char *argv[])
 {
-    const int M = argc > 1 ? atoi(argv[1]) : 16;
-    const int steps = argc > 2 ? atoi(argv[2]) : 50;
     const double dt = 0.005, hdt = 0.5 * dt;
     RuntimeInit(argc, argv);
 
-    const long n = 4L * M * M * M;
-    std::vector<float> pos3(3 * n);
-    double box[3];
-    PAT_API_CALL(pat_gen_fcc(M, M, M, 0.8442, 0.05, 42, pos3.data(), box));
+    long n = 0;
+    int steps = 50;
+    double cutoff = 2.5, box[3];
+    std::vector<float> pos3;
+    std::vector<point_data> x;
+    std::vector<EDGE> edges;
     int64_t ne = 0;
-    PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], 2.5, 1, 43, nullptr, &ne));
-    std::vector<EDGE> edges(ne);
-    PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], 2.5, 1, 43, (int32_t *)edges.data(), &ne));
-
-    std::vector<point_data> x(n);
-    for (long i = 0; i < n; i++)
-        for (int c = 0; c < 3; c++) x[i].points[c] = pos3[3 * i + c];
+    if (argc > 2 && std::string(argv[1]) == "-f") {
+        // a reference dataset: the parameter file moldyn reads from stdin
+        // (kmeans/moldyn.cu:193-201) naming its .mesh graph and .xyz
+        // coordinates (paths relative to the parameter file); the box side
+        // from the molecule count at moldyn's density, as InitSettings
+        // computes it (kmeans/moldyn.cu:493-499, DENSITY kmeans/moldyn.h:81)
+        pat_mold m;
+        PAT_API_CALL(pat_read_mold(argv[2], &m));
+        std::string dir(argv[2]);
+        dir = dir.find('/') == std::string::npos ? std::string(".") : dir.substr(0, dir.rfind('/'));
+        const std::string mesh = m.mesh[0] == '/' ? m.mesh : dir + "/" + m.mesh;
+        const std::string xyz = m.xyz[0] == '/' ? m.xyz : dir + "/" + m.xyz;
+        int64_t nn = 0;
+        PAT_API_CALL(pat_read_mesh(mesh.c_str(), 1, &nn, &ne, nullptr, 0));
+        edges.resize(ne);
+        PAT_API_CALL(pat_read_mesh(mesh.c_str(), 1, &nn, &ne, (int32_t *)edges.data(), ne));
+        n = (long)nn;
+        std::vector<double> xyzv(3 * n);
+        PAT_API_CALL(pat_read_xyz(xyz.c_str(), n, xyzv.data()));
+        steps = argc > 3 ? atoi(argv[3]) : m.steps;
+        cutoff = m.cutoff;
+        box[0] = box[1] = box[2] = std::pow((double)n / 0.83134, 0.3333333);
+        pos3.resize(3 * n);
+        x.resize(n);
+        for (long i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) {
+                x[i].points[c] = xyzv[3 * i + c];
+                pos3[3 * i + c] = (float)xyzv[3 * i + c];
+            }
+        printf("moldyn: dataset %s: %ld molecules, %lld pairs, box %.4f, cutoff %.2f\n", argv[2], n, (long long)ne,
+               box[0], cutoff);
+    } else {
+        const int M = argc > 1 ? atoi(argv[1]) : 16;
+        steps = argc > 2 ? atoi(argv[2]) : 50;
+        n = 4L * M * M * M;
+        pos3.resize(3 * n);
+        PAT_API_CALL(pat_gen_fcc(M, M, M, 0.8442, 0.05, 42, pos3.data(), box));
+        PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], cutoff, 1, 43, nullptr, &ne));
+        edges.resize(ne);
+        PAT_API_CALL(pat_gen_pairs(n, pos3.data(), box[0], cutoff, 1, 43, (int32_t *)edges.data(), &ne));
+        x.resize(n);
+        for (long i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) x[i].points[c] = pos3[3 * i + c];
+    }
     std::vector<double> v(3 * n, 0.0);
     std::vector<VALUE> f(n);
-    struct para pa = {box[0], 2.5};
+    struct para pa = {box[0], cutoff};
 
     IrregularRuntime runtime(ne, n, edges.data(), nullptr, x.data(), 0, sizeof(point_data), sizeof(VALUE), 3, nullptr,
                              0, &pa, sizeof(pa), 1, steps);

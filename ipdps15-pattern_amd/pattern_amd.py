@@ -140,6 +140,10 @@ SIGNATURES = {
     "pat_md_set_row_order": (c_int, [c_vp, c_int]),
     "pat_md_get_variant": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_md_set_sliced_form": (c_int, [c_vp, c_int]),
+    "pat_read_mold": (c_int, [ctypes.c_char_p, c_vp]),
+    "pat_read_ljin": (c_int, [ctypes.c_char_p, c_vp]),
+    "pat_read_mesh": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), c_vp, c_i64]),
+    "pat_read_xyz": (c_int, [ctypes.c_char_p, c_i64, c_vp]),
     "pat_gr_create": (c_int, [c_vp, c_i64, c_i64, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "pat_gr_set_map_idx": (c_int, [c_vp, c_int]),
     "pat_gr_set_reduce_idx": (c_int, [c_vp, c_int]),
@@ -900,3 +904,52 @@ class GR:
         if self.h:
             call("pat_gr_destroy", self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- reference input files
+
+class Mold(ctypes.Structure):
+    _fields_ = [("box_cells", c_int), ("steps", c_int), ("neigh_update", c_int), ("cutoff", ctypes.c_double),
+                ("mesh", ctypes.c_char * 512), ("xyz", ctypes.c_char * 512), ("percent", ctypes.c_double),
+                ("start_time", c_int), ("update_freq", c_int)]
+
+
+class LJIn(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("ntimes", c_int), ("nbinx", c_int),
+                ("nbiny", c_int), ("nbinz", c_int), ("dt", ctypes.c_double), ("t_request", ctypes.c_double),
+                ("rho", ctypes.c_double), ("every", c_int), ("cutforce", ctypes.c_double),
+                ("cutneigh", ctypes.c_double), ("nstat", c_int)]
+
+
+def _struct_dict(st):
+    return {k: (getattr(st, k).decode() if isinstance(getattr(st, k), bytes) else getattr(st, k))
+            for k, _ in st._fields_}
+
+
+def read_mold(path):
+    """moldyn's parameter file as a dict."""
+    m = Mold()
+    call("pat_read_mold", os.fsencode(path), ctypes.byref(m))
+    return _struct_dict(m)
+
+
+def read_ljin(path):
+    """miniMD's lj.in as a dict."""
+    m = LJIn()
+    call("pat_read_ljin", os.fsencode(path), ctypes.byref(m))
+    return _struct_dict(m)
+
+
+def read_mesh(path, zero_based=True):
+    """(num_nodes, edges (E x 2 int32)) from a moldyn .mesh graph file."""
+    nn, ne = c_i64(0), c_i64(0)
+    call("pat_read_mesh", os.fsencode(path), int(zero_based), ctypes.byref(nn), ctypes.byref(ne), None, 0)
+    e = np.zeros((ne.value, 2), dtype=np.int32)
+    call("pat_read_mesh", os.fsencode(path), int(zero_based), ctypes.byref(nn), ctypes.byref(ne), ptr(e), ne.value)
+    return nn.value, e
+
+
+def read_xyz(path, num_nodes):
+    x = np.zeros((num_nodes, 3), dtype=np.float64)
+    call("pat_read_xyz", os.fsencode(path), num_nodes, ptr(x))
+    return x

@@ -5,6 +5,7 @@ run on the GPU and agree with their own host functors."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -37,3 +38,27 @@ def test_moldyn_app(gpu):
 def test_minimd_app(gpu):
     out = run("minimd", 10)
     assert "5 passes match" in out
+
+
+def test_moldyn_app_replays_a_reference_dataset(pa, gpu, tmp_path):
+    """A dataset in the reference's shape: the parameter file moldyn reads
+    (kmeans/moldyn.cu:193-201) naming a .mesh graph and .xyz coordinates
+    (kmeans/misc.c:63-75, 226-247), written here for a 4^3-cell FCC system
+    at moldyn's density (box side (N/0.83134)^(1/3))."""
+    n = 4 * 4 ** 3
+    side = (n / 0.83134) ** 0.3333333
+    pos, box = pa.gen_fcc(4, 4, 4, rho=n / side ** 3, jitter=0.02, seed=5)
+    e = pa.gen_pairs(pos, box[0], 2.5)
+    (tmp_path / "input").mkdir()
+    adj = [[] for _ in range(n)]
+    for i, j in e.tolist():
+        adj[i].append(j + 1)
+        adj[j].append(i + 1)
+    with open(tmp_path / "input" / "g.mesh", "w") as f:
+        f.write(f"% fcc 4^3\n{n} {len(e)}\n")
+        for a in adj:
+            f.write(" ".join(map(str, a)) + "\n")
+    np.savetxt(tmp_path / "input" / "g.xyz", pos.astype(np.float64), fmt="%.9g")
+    (tmp_path / "mold").write_text(f"4\n10\n20\n2.5\ninput/g.mesh\ninput/g.xyz\n10\n0\n1\n")
+    out = run("moldyn", "-f", str(tmp_path / "mold"))
+    assert f"{n} molecules, {len(e)} pairs" in out and "first pass" in out and "velocity statistics" in out
