@@ -37,7 +37,17 @@ def test_moldyn_app(gpu):
 
 def test_minimd_app(gpu):
     out = run("minimd", 10)
-    assert "5 passes match" in out
+    assert "5 passes match" in out and "pair energy sum" in out
+
+
+def test_minimd_app_reads_lj_in(gpu, tmp_path):
+    """The reference's own lj.in, with the problem size cut to 12^3 cells."""
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_inputs", "lj.in")
+    lines = open(src).read().split("\n")
+    lines[2] = "12 12 12        size of problem"
+    (tmp_path / "lj.in").write_text("\n".join(lines))
+    out = run("minimd", "-f", str(tmp_path / "lj.in"))
+    assert "12 x 12 x 12 cells, density 0.8442, cutoffs 2.5 / 2.8" in out and "5 passes match" in out
 
 
 def test_moldyn_app_replays_a_reference_dataset(pa, gpu, tmp_path):
