@@ -337,7 +337,9 @@ constexpr float kFar = 3.0e38f;
 // from the registers holding its half rows.
 // TAIL = false: D == 16*KS and the rows are 16-byte aligned, so every
 // K-step is full and the loads need no per-element guard.
-template <int KS, int KB, int NT, bool TAIL>
+// GC: the BF16 rows are read from global memory (they stay in L2) instead
+// of being staged in LDS -- the form for K x D too large for LDS.
+template <int KS, int KB, int NT, bool TAIL, bool GC>
 __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
 {
@@ -351,12 +353,13 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     __shared__ float red[16];
     if (a.use_lds_acc) zero_lds_object(a, acc_o, cnt_o, dsum_o);
     // stage the BF16 centre matrix in LDS (16-byte copies)
-    {
+    if (!GC) {
         const int n16 = K_pad * KW / 8;
         const bf16x8 *src = (const bf16x8 *)cb;
         bf16x8 *dst = (bf16x8 *)cbs;
         for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
     }
+    const __bf16 *__restrict__ cfr = GC ? cb : cbs;
     // max ||c|| over the centres (for the bound)
     float cm = 0.0f;
     for (int i = threadIdx.x; i < a.K; i += blockDim.x) cm = fmaxf(cm, cnorm[i]);
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     const int kblocks = K_pad / 32;
     auto chunk_values = [&](int ch, const bf16x8 (&bx)[NS], f32x16 (&acc)[KB]) {
         constexpr int T = KB * NS, P = 3;
-        const bf16x8 *ab = (const bf16x8 *)cbs + lane;
+        const bf16x8 *ab = (const bf16x8 *)cfr + lane;
         auto frag = [&](int t) {
             const int blk = ch * KB + t / NS;
             return ab[((int64_t)(blk < kblocks ? blk : 0) * NS + t % NS) * kWave];
@@ -742,15 +745,19 @@ int kmeans_prepare_kernels()
     PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     g_lds_limit = lim - 256;  // room for the kernels' static LDS
     const void *fns[] = {(const void *)km_plain<8, 1024>, (const void *)km_plain<16, 1024>, (const void *)km_plain<32, 1024>,
-                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1, true>,
-                         (const void *)km_mfma<2, 8, 1, true>,  (const void *)km_mfma<4, 4, 1, true>,
-                         (const void *)km_mfma<8, 2, 1, true>,  (const void *)km_mfma<1, 8, 3, true>,
-                         (const void *)km_mfma<2, 8, 3, true>,  (const void *)km_mfma<4, 4, 3, true>,
-                         (const void *)km_mfma<8, 2, 3, true>,  (const void *)km_mfma<1, 8, 1, false>,
-                         (const void *)km_mfma<2, 8, 1, false>, (const void *)km_mfma<4, 4, 1, false>,
-                         (const void *)km_mfma<8, 2, 1, false>, (const void *)km_mfma<1, 8, 3, false>,
-                         (const void *)km_mfma<2, 8, 3, false>, (const void *)km_mfma<4, 4, 3, false>,
-                         (const void *)km_mfma<8, 2, 3, false>};
+                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1, true, false>,
+                         (const void *)km_mfma<2, 8, 1, true, false>,  (const void *)km_mfma<4, 4, 1, true, false>,
+                         (const void *)km_mfma<8, 2, 1, true, false>,  (const void *)km_mfma<1, 8, 3, true, false>,
+                         (const void *)km_mfma<2, 8, 3, true, false>,  (const void *)km_mfma<4, 4, 3, true, false>,
+                         (const void *)km_mfma<8, 2, 3, true, false>,  (const void *)km_mfma<1, 8, 1, false, false>,
+                         (const void *)km_mfma<2, 8, 1, false, false>, (const void *)km_mfma<4, 4, 1, false, false>,
+                         (const void *)km_mfma<8, 2, 1, false, false>, (const void *)km_mfma<1, 8, 3, false, false>,
+                         (const void *)km_mfma<2, 8, 3, false, false>, (const void *)km_mfma<4, 4, 3, false, false>,
+                         (const void *)km_mfma<8, 2, 3, false, false>,
+                         (const void *)km_mfma<1, 8, 3, true, true>,   (const void *)km_mfma<2, 8, 3, true, true>,
+                         (const void *)km_mfma<4, 4, 3, true, true>,   (const void *)km_mfma<8, 2, 3, true, true>,
+                         (const void *)km_mfma<1, 8, 3, false, true>,  (const void *)km_mfma<2, 8, 3, false, true>,
+                         (const void *)km_mfma<4, 4, 3, false, true>,  (const void *)km_mfma<8, 2, 3, false, true>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
@@ -820,11 +827,11 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
 
 // LDS bytes of the MFMA kernel with nt terms (staged BF16 rows plus the
 // reduction object when it is kept in LDS).
-static int64_t mfma_lds_bytes(int K, int D, int nt)
+static int64_t mfma_lds_bytes(int K, int D, int nt, bool gc = false)
 {
     const int ks = kmeans_ks_for(D);
     const int K_pad = (K + 31) / 32 * 32;
-    const int64_t cbb = 2LL * K_pad * 16 * (nt * ks + 1);
+    const int64_t cbb = gc ? 0 : 2LL * K_pad * 16 * (nt * ks + 1);
     const int64_t obj = object_bytes(K, D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024)
                             ? object_bytes(K, D) : 0;
     return cbb + obj;
@@ -840,14 +847,19 @@ int kmeans_nt_for(int K, int D)
     return 0;
 }
 
-bool kmeans_mfma_fits(int K, int D) { return kmeans_nt_for(K, D) != 0; }
+// Whether the MFMA path reads its rows from global memory: when neither
+// LDS form fits (it then uses three terms).
+bool kmeans_gc_for(int K, int D) { return kmeans_ks_for(D) != 0 && kmeans_nt_for(K, D) == 0; }
+
+bool kmeans_mfma_fits(int K, int D) { return kmeans_ks_for(D) != 0 && K <= 65536; }
 
 int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
 {
     KmArgs a = make_args(d);
     PAT_REQUIRE(d.NT == 1 || d.NT == 3, PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K,
                 d.D);
-    const int64_t lds = mfma_lds_bytes(d.K, d.D, d.NT);
+    PAT_REQUIRE(!d.gc || d.NT == 3, PAT_ERR_STATE, "kmeans: global-row MFMA form needs three terms");
+    const int64_t lds = mfma_lds_bytes(d.K, d.D, d.NT, d.gc != 0);
     PAT_REQUIRE(lds <= g_lds_limit, PAT_ERR_UNSUPPORTED, "kmeans: K=%d, D=%d too large for the MFMA path", d.K, d.D);
     // Error-bound constants.  u = 2^-8 (BF16 unit roundoff); g = FP32
     // accumulation over every term of the MFMA chain, with a factor 2 of
@@ -870,18 +882,30 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
     // no tail: every K-step full and 16-byte aligned rows
     const bool tail = d.D != 16 * d.KS || ((uintptr_t)d.x % 16) != 0;
-#define PAT_KM(KS_, KB_, NT_)                                                                                          \
+#define PAT_KM_GC(KS_, KB_, NT_, GC_)                                                                                  \
     do {                                                                                                               \
         if (tail) {                                                                                                    \
-            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, true>, lds));                                       \
-            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, true>), dim3(grid), dim3(512), (size_t)lds, st, a,              \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, true, GC_>, lds));                                  \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, true, GC_>), dim3(grid), dim3(512), (size_t)lds, st, a,         \
                                (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
         } else {                                                                                                       \
-            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, false>, lds));                                      \
-            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, false>), dim3(grid), dim3(512), (size_t)lds, st, a,             \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, false, GC_>, lds));                                 \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, false, GC_>), dim3(grid), dim3(512), (size_t)lds, st, a,        \
                                (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
         }                                                                                                              \
     } while (0)
+#define PAT_KM(KS_, KB_, NT_) PAT_KM_GC(KS_, KB_, NT_, false)
+    if (d.gc) {
+        switch (d.KS) {
+        case 1: PAT_KM_GC(1, 8, 3, true); break;
+        case 2: PAT_KM_GC(2, 8, 3, true); break;
+        case 4: PAT_KM_GC(4, 4, 3, true); break;
+        case 8: PAT_KM_GC(8, 2, 3, true); break;
+        default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
+        }
+        PAT_LAUNCH_CHECK();
+        return PAT_OK;
+    }
     switch (d.KS * 4 + d.NT) {
     case 1 * 4 + 1: PAT_KM(1, 8, 1); break;
     case 2 * 4 + 1: PAT_KM(2, 8, 1); break;
@@ -894,6 +918,7 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
     }
 #undef PAT_KM
+#undef PAT_KM_GC
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }

@@ -13,6 +13,7 @@ struct KmDev {
     int64_t N;
     int D, K, K_pad, KS;  // K_pad: K rounded up to 32; KS: 16-wide K-steps of the data
     int NT;               // MFMA data terms: 3 (split BF16), 1 (BF16), 0 (no MFMA path)
+    int gc;               // MFMA rows read from global memory (K x D too large for LDS)
     int shift, dshift;    // fixed-point scales 2^shift (coordinates), 2^dshift (distances)
     int32_t *assign;      // N
     float *mindist;       // N
@@ -27,6 +28,7 @@ int kmeans_fixed_shift(int64_t n, float maxabs);
 int kmeans_ks_for(int D);
 bool kmeans_mfma_fits(int K, int D);
 int kmeans_nt_for(int K, int D);
+bool kmeans_gc_for(int K, int D);
 int kmeans_maxabs(const float *x, int64_t n, unsigned int *out_bits, hipStream_t st);
 int kmeans_finalize(const KmDev &d, int update, hipStream_t st);
 int kmeans_assign_plain(const KmDev &d, hipStream_t st);

@@ -89,7 +89,8 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     D.K = k;
     D.K_pad = (k + 31) / 32 * 32;
     D.KS = pat::kmeans_ks_for(d);
-    D.NT = std::max(1, pat::kmeans_nt_for(k, d));  // rows are written even when only the plain path fits
+    D.gc = pat::kmeans_gc_for(k, d) ? 1 : 0;
+    D.NT = D.gc ? 3 : std::max(1, pat::kmeans_nt_for(k, d));  // rows are written even when only the plain path runs
     int dev = 0;
     PAT_HIP(hipGetDevice(&dev));
     PAT_HIP(hipDeviceGetAttribute(&D.cus, hipDeviceAttributeMultiprocessorCount, dev));

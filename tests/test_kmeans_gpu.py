@@ -27,7 +27,9 @@ def oracle_iters(orc, pa, x, k, iters):
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("n,d,k", [(1, 3, 1), (31, 3, 5), (1000, 3, 40), (4099, 8, 17), (20000, 16, 64),
                                    (5003, 32, 256), (3000, 33, 70), (2000, 64, 100), (1500, 128, 33),
-                                   (3000, 32, 300)])
+                                   (3000, 32, 300),
+                                   # rows too large for LDS: the MFMA path reads them from global memory
+                                   (6000, 128, 1024), (4000, 64, 1000), (3000, 100, 700), (9000, 16, 8000)])
 def test_single_pass_bitexact(pa, orc, gpu, mode, n, d, k):
     x = pa.gen_gmm(n, d, max(1, k // 2), sigma=1.0, seed=n + d + k)
     km = pa.KMeans(x, k, mode=mode)
@@ -56,9 +58,12 @@ def adversarial(kind, rng):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["duplicates", "lattice", "large", "close"])
+@pytest.mark.parametrize("kind", ["duplicates", "lattice", "large", "close", "close_wide"])
 def test_mfma_adversarial_bitexact(pa, orc, gpu, kind):
-    x, k = adversarial(kind, np.random.default_rng(7))
+    if kind == "close_wide":  # a tight cluster and K x D beyond LDS (rows from global memory)
+        x, k = (np.random.default_rng(8).normal(size=(3000, 128)) * 1e-3).astype(np.float32), 1024
+    else:
+        x, k = adversarial(kind, np.random.default_rng(7))
     out = []
     for mode in (0, 1):
         km = pa.KMeans(x, k, mode=mode)
