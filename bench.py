@@ -137,6 +137,50 @@ def bench_lj(steps, warmup, do_cpu, forms=()):
     return out
 
 
+def bench_minimd(steps=100):
+    """miniMD: FCC 64^3 cells (1,048,576 atoms), density 0.8442, Maxwell
+    velocities at T = 1.44, cutoff 2.5, skin 0.3, binned neighbour rebuild
+    every 20 steps on the device, dt 0.005; then 64 independent systems of
+    8^3-32^3 cells (log-uniform) advanced as one batched launch per step.
+    pairs/s counts the half pairs within the neighbour cutoff per step."""
+    pos, box = pa.gen_fcc(64, 64, 64, rho=0.8442, jitter=0.0, seed=7)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=8)
+    t0 = time.perf_counter()
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    setup = time.perf_counter() - t0
+    md.run(20)
+    _, _, rb0, rbms0 = md.stats()
+    pa.call("pat_device_sync")
+    w0 = time.perf_counter()
+    md.run(steps)
+    wall = time.perf_counter() - w0
+    npairs, _, rb1, rbms1 = md.stats()
+    _, kms = md.run_timed(20)
+    md.close()
+    out = {"atoms": len(pos), "half_pairs": npairs, "steps": steps, "setup_s": setup,
+           "pairs_per_s": npairs * steps / wall, "ms_per_step": wall * 1e3 / steps,
+           "force_kernel_us": float(np.mean(kms)) * 1e3, "rebuilds": rb1 - rb0,
+           "rebuild_ms_each": (rbms1 - rbms0) / max(rb1 - rb0, 1)}
+    log(f"[minimd] 64^3: {len(pos)} atoms, {npairs} half pairs, {out['ms_per_step'] * 1e3:.1f} us/step with "
+        f"rebuilds ({out['pairs_per_s'] / 1e9:.1f} G pairs/s), force kernel {out['force_kernel_us']:.1f} us, "
+        f"{out['rebuilds']} rebuilds at {out['rebuild_ms_each']:.2f} ms")
+    systems = pa.batch_systems(64, 8, 32, seed=99)
+    b = pa.MDBatch(systems)
+    b.run(20)
+    pa.call("pat_device_sync")
+    w0 = time.perf_counter()
+    b.run(steps)
+    bwall = time.perf_counter() - w0
+    bpairs, _, _, _ = b.stats()
+    b.close()
+    natoms = sum(len(p) for p, _, _ in systems)
+    out["batch"] = {"systems": 64, "atoms": natoms, "half_pairs": bpairs, "pairs_per_s": bpairs * steps / bwall,
+                    "ms_per_step": bwall * 1e3 / steps}
+    log(f"[minimd] batch of 64 systems (8^3-32^3 cells): {natoms} atoms, {bwall * 1e3 / steps * 1e3:.1f} us/step "
+        f"({bpairs * steps / bwall / 1e9:.1f} G pairs/s)")
+    return out
+
+
 def bench_stencil(n, steps):
     g = pa.gen_heat3d(n, n, n, seed=42)
     s = pa.Stencil(n, n, n, alpha=0.1)
@@ -295,8 +339,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--engines", default="lj,stencil,kmeans,serve",
-                    help="comma-separated subset of lj,stencil,kmeans,serve,copy (lj carries value)")
+    ap.add_argument("--engines", default="lj,stencil,kmeans,minimd,serve",
+                    help="comma-separated subset of lj,stencil,kmeans,minimd,serve,copy (lj carries value)")
     ap.add_argument("--serve-jobs", type=int, default=600, help="jobs in the serving mix (the config's is 10000)")
     ap.add_argument("--lj-forms", default="", help="comma-separated sliced LJ loop forms (0-4) to time beside the "
                                                     "tuner's choice")
@@ -315,6 +359,8 @@ def main():
         engines["serving"] = bench_serving(args.serve_jobs)
     if "kmeans" in which:
         engines["kmeans"] = bench_kmeans()
+    if "minimd" in which:
+        engines["minimd"] = bench_minimd()
     if "stencil" in which:
         for n in (128, 384):
             st = bench_stencil(n, 100)

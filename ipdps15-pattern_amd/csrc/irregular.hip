@@ -123,24 +123,61 @@ __global__ __launch_bounds__(256) void csr_row_sort_by_edge(const int *__restric
     }
 }
 
-// One wavefront per row: rank sort of the row's columns (ties broken by
-// position), writing the sorted row from tmp into cols.
-__global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowptr, int64_t N, const int *__restrict__ tmp,
-                                                    int *__restrict__ cols)
+// Compare-exchange of the bitonic network between this lane's value and
+// lane ^ j's: the lower lane of the pair keeps the smaller value when up.
+__device__ __forceinline__ unsigned bitonic_step(unsigned v, int lane, int j, bool up)
+{
+    const unsigned o = (unsigned)__shfl_xor((int)v, j, kWave);
+    const bool low = (lane & j) == 0;
+    return (low == up) ? min(v, o) : max(v, o);
+}
+
+// One wavefront per row of at most 128 columns: bitonic sort of the row
+// held two values per lane (entry lane and entry 64 + lane; padding sorts
+// last), writing the sorted row from tmp into cols.  Longer rows take the
+// rank sort.  Columns of a row are distinct, so any correct sort gives the
+// rank sort's order.
+__global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowptr, int64_t N,
+                                                    const int *__restrict__ tmp, int *__restrict__ cols)
 {
     const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x % kWave;
     if (row >= N) return;
-    const int b = rowptr[row], e = rowptr[row + 1];
-    for (int k = b + lane; k < e; k += kWave) {
-        const int v = tmp[k];
-        int rank = 0;
-        for (int m = b; m < e; m++) {
-            const int u = tmp[m];
-            rank += (u < v) || (u == v && m < k);
+    const int b = rowptr[row], e = rowptr[row + 1], n = e - b;
+    if (n > 2 * kWave) {  // uniform per wavefront
+        for (int k = b + lane; k < e; k += kWave) {
+            const int v = tmp[k];
+            int rank = 0;
+            for (int m = b; m < e; m++) {
+                const int u = tmp[m];
+                rank += (u < v) || (u == v && m < k);
+            }
+            cols[b + rank] = v;
         }
-        cols[b + rank] = v;
+        return;
     }
+    // values as unsigned with padding 0xffffffff (columns are >= 0)
+    unsigned a0 = lane < n ? (unsigned)tmp[b + lane] : 0xffffffffu;
+    unsigned a1 = kWave + lane < n ? (unsigned)tmp[b + kWave + lane] : 0xffffffffu;
+    // sort the 128-element sequence s[i] (i < 64: a0 of lane i; else a1 of
+    // lane i - 64): stages k = 2..128, distances j = k/2..1; direction of
+    // element i is ascending when (i & k) == 0
+    for (int k = 2; k <= 2 * kWave; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == kWave) {  // pairs (lane, 64 + lane): both in this lane
+                const unsigned lo = min(a0, a1), hi = max(a0, a1);
+                a0 = lo;  // k = 128: ascending for every pair
+                a1 = hi;
+            } else {
+                const bool up0 = (lane & k) == 0;                  // element lane
+                const bool up1 = ((kWave + lane) & k) == 0;        // element 64 + lane
+                a0 = bitonic_step(a0, lane, j, up0);
+                a1 = bitonic_step(a1, lane, j, up1);
+            }
+        }
+    }
+    if (lane < n) cols[b + lane] = (int)a0;
+    if (kWave + lane < n) cols[b + kWave + lane] = (int)a1;
 }
 
 // ------------------------------------------------------------ neighbour build
@@ -158,19 +195,41 @@ struct BinGeom {
     int nb;                  // bins per axis (>= 3)
 };
 
-__device__ __forceinline__ int bin_of(float x, const BinGeom &g)
+// One periodic cube of a (possibly batched) build: its atoms are
+// [atom0, atom1) of the concatenated positions and its nb^3 bins are
+// [cell0, cell0 + nb^3) of the concatenated bin arrays.
+struct NbSys {
+    float L, hL, inv;  // box, half box, nb / L
+    int nb, cell0, atom0, atom1, pad;
+};
+
+__device__ __forceinline__ int bin_of(float x, float inv, int nb)
 {
-    int c = (int)floorf(x * g.inv);
-    return c < 0 ? 0 : (c >= g.nb ? g.nb - 1 : c);
+    int c = (int)floorf(x * inv);
+    return c < 0 ? 0 : (c >= nb ? nb - 1 : c);
 }
 
-__global__ void nb_bin_atoms(const float4 *__restrict__ pos, int N, BinGeom g, int *__restrict__ cell,
-                             int *__restrict__ count)
+// Index of the system holding atom i (by atom0) or bin q (by cell0).
+template <bool BY_CELL>
+__device__ __forceinline__ int nb_sys_of(const NbSys *__restrict__ sys, int nsys, int v)
+{
+    int lo = 0, hi = nsys;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((BY_CELL ? sys[mid].cell0 : sys[mid].atom0) <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void nb_bin_atoms(const float4 *__restrict__ pos, int N, const NbSys *__restrict__ sys, int nsys,
+                             int *__restrict__ cell, int *__restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    const NbSys S = sys[nb_sys_of<false>(sys, nsys, i)];
     const float4 p = pos[i];
-    const int c = bin_of(p.x, g) + g.nb * (bin_of(p.y, g) + g.nb * bin_of(p.z, g));
+    const int c = S.cell0 + bin_of(p.x, S.inv, S.nb) + S.nb * (bin_of(p.y, S.inv, S.nb) + S.nb * bin_of(p.z, S.inv, S.nb));
     cell[i] = c;
     atomicAdd(&count[c], 1);
 }
@@ -193,33 +252,104 @@ __device__ __forceinline__ bool nb_within(float4 a, float4 b, const BinGeom &g)
     return r2 < g.cut2;
 }
 
-// One thread per atom over the 27 neighbouring bins: with fill == 0 counts
-// the row length into deg, with fill != 0 writes the row into tmp.
-__global__ void nb_rows(const float4 *__restrict__ pos, int N, BinGeom g, const int *__restrict__ cell,
-                        const int *__restrict__ cstart, const int *__restrict__ catoms, int *__restrict__ deg,
-                        const int *__restrict__ rowptr, int *__restrict__ tmp, int fill)
+// Candidate slots per lane of nb_cells: a wavefront holds 64 * kNbSlots
+// atoms of a bin's 27-bin neighbourhood in registers at a time (about 510
+// at the FCC 0.8442 density and 2.8-wide bins; larger ones take more
+// passes over the bin's atoms).
+constexpr int kNbSlots = 12;
+
+// One wavefront per bin q: the atoms of the 27 bins around q (periodic
+// wrap; nb >= 3 keeps them distinct) are laid out as one flat candidate
+// range, gathered into registers slice by slice, and every atom i of q is
+// tested against them, all lanes at once.  !FILL: deg[i] = row length;
+// FILL: the row goes to tmp[rowptr[i] ...], compacted by ballot (order
+// within a row is left to the row sort).
+template <bool FILL>
+__global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, const NbSys *__restrict__ sys,
+                                                int nsys, int ncell, float cut2, const int *__restrict__ cstart,
+                                                const int *__restrict__ catoms, int *__restrict__ deg,
+                                                const int *__restrict__ rowptr, int *__restrict__ tmp)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const float4 pi = pos[i];
-    const int c = cell[i];
-    const int cx = c % g.nb, cy = (c / g.nb) % g.nb, cz = c / (g.nb * g.nb);
-    int n = 0;
-    int w = fill ? rowptr[i] : 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int qx = (cx + dx + g.nb) % g.nb, qy = (cy + dy + g.nb) % g.nb, qz = (cz + dz + g.nb) % g.nb;
-                const int q = qx + g.nb * (qy + g.nb * qz);
-                for (int s = cstart[q]; s < cstart[q + 1]; s++) {
-                    const int j = catoms[s];
-                    if (j != i && nb_within(pi, pos[j], g)) {
-                        if (fill) tmp[w++] = j;
-                        n++;
-                    }
+    const int q = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (q >= ncell) return;  // whole wavefronts
+    const NbSys S = sys[nb_sys_of<true>(sys, nsys, q)];
+    const int nb = S.nb, local = q - S.cell0;
+    const int cx = local % nb, cy = (local / nb) % nb, cz = local / (nb * nb);
+    // lane k < 27: neighbour bin k, its first atom slot and atom count
+    int ns = 0, nc = 0;
+    if (lane < 27) {
+        const int qx = (cx + lane % 3 - 1 + nb) % nb;
+        const int qy = (cy + (lane / 3) % 3 - 1 + nb) % nb;
+        const int qz = (cz + lane / 9 - 1 + nb) % nb;
+        const int nq = S.cell0 + qx + nb * (qy + nb * qz);
+        ns = cstart[nq];
+        nc = cstart[nq + 1] - ns;
+    }
+    int incl = nc;  // inclusive scan over lanes 0..31
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
+    }
+    const int excl = incl - nc;
+    const int total = __shfl(incl, 26, kWave);
+    BinGeom g;
+    g.L = S.L;
+    g.hL = S.hL;
+    g.cut2 = cut2;
+    g.inv = S.inv;
+    g.nb = nb;
+    const int a_beg = cstart[q], a_end = cstart[q + 1];
+    for (int a0 = a_beg; a0 < a_end; a0 += kWave) {
+        const int na = min(kWave, a_end - a0);
+        int acc = 0;  // lane l: entries found so far for atom slot a0 + l
+        for (int base = 0; base < total; base += kWave * kNbSlots) {
+            float4 pj[kNbSlots];
+            int jj[kNbSlots];
+#pragma unroll
+            for (int r = 0; r < kNbSlots; r++) {
+                const int f = base + r * kWave + lane;
+                // largest k <= 26 with excl_k <= f (excl is nondecreasing,
+                // so bin k is non-empty and holds f when f < total)
+                int k = 0;
+#pragma unroll
+                for (int step = 16; step > 0; step >>= 1) {
+                    const int e = __shfl(excl, min(k + step, 26), kWave);
+                    k = (k + step <= 26 && e <= f) ? k + step : k;
+                }
+                const int ex = __shfl(excl, k, kWave), s0 = __shfl(ns, k, kWave);
+                jj[r] = -1;
+                pj[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (f < total) {
+                    jj[r] = catoms[s0 + f - ex];
+                    pj[r] = pos[jj[r]];
                 }
             }
-    if (!fill) deg[i] = n;
+            for (int a = a0; a < a0 + na; a++) {
+                const int i = catoms[a];
+                const float4 pi = pos[i];
+                int w = 0, wend = 0;
+                if (FILL) {
+                    w = rowptr[i] + __shfl(acc, a - a0, kWave);
+                    wend = rowptr[i + 1];  // the counting pass sized the row; never write past it
+                }
+                int found = 0;
+#pragma unroll
+                for (int r = 0; r < kNbSlots; r++) {
+                    const bool hit = jj[r] >= 0 && jj[r] != i && nb_within(pi, pj[r], g);
+                    const unsigned long long m = __ballot(hit);
+                    if (FILL && hit) {
+                        const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
+                        if (o < wend) tmp[o] = jj[r];
+                    }
+                    found += __popcll(m);
+                }
+                if (lane == a - a0) acc += found;
+            }
+        }
+        if (!FILL && lane < na) deg[catoms[a0 + lane]] = acc;
+    }
 }
 
 // ------------------------------------------------------------------ force
@@ -649,22 +779,51 @@ __global__ void sell_width(const int *__restrict__ rowptr, const int *__restrict
     width[s] = w;
 }
 
-// pad < 0: entries hold node indices and padding repeats the row's own
-// index; otherwise entries hold byte offsets 16*j into the float4 position
-// array and padding is 16*pad (a sentinel node placed beyond every cutoff).
+// Columns per LDS tile of sell_fill.
+constexpr int kFillK = 32;
+
+// Orders this wavefront's LDS writes before its later LDS reads (and the
+// reads before later overwrites).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per slice.  pad < 0: entries hold node indices and padding
+// repeats the row's own index; otherwise entries hold byte offsets 16*j into
+// the float4 position array and padding is 16*pad (a sentinel node placed
+// beyond every cutoff).  The slice is transposed through LDS 32 columns at a
+// time: half-wavefronts read 32 consecutive entries of one row (coalesced),
+// then every lane writes its row's entry k (one 256-byte store per k).
 __global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr, const int *__restrict__ cols,
                                                  const int *__restrict__ perm, int N, int nsl,
                                                  const int *__restrict__ sptr, int *__restrict__ scol, int pad)
 {
-    const int s = blockIdx.x * 4 + threadIdx.x / kWave;
+    __shared__ int tile[4][kWave][kFillK + 1];
+    const int wv = threadIdx.x / kWave;
+    const int s = blockIdx.x * 4 + wv;
     const int lane = threadIdx.x % kWave;
-    if (s >= nsl) return;
+    if (s >= nsl) return;  // whole wavefronts
     const int row = slice_row(perm, s * kWave + lane);
     const int b = row < N ? rowptr[row] : 0, e = row < N ? rowptr[row + 1] : 0;
     const int fill = pad < 0 ? min(row, N - 1) : (int)((uint32_t)pad << 4);
     const int sh = pad < 0 ? 0 : 4;
     const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
-    for (int k = 0; k < w; k++) scol[base + k * kWave + lane] = b + k < e ? (int)((uint32_t)cols[b + k] << sh) : fill;
+    const int half = lane >> 5, t = lane & 31;
+    for (int k0 = 0; k0 < w; k0 += kFillK) {
+        for (int m = 0; m < kWave / 2; m++) {
+            const int rr = 2 * m + half;  // lanes 0-31: row 2m, lanes 32-63: row 2m+1
+            const int rb = __shfl(b, rr, kWave), re = __shfl(e, rr, kWave), rf = __shfl(fill, rr, kWave);
+            const int k = rb + k0 + t;
+            tile[wv][rr][t] = k < re ? (int)((uint32_t)cols[k] << sh) : rf;
+        }
+        wave_lds_sync();
+        const int kn = min(kFillK, w - k0);
+        for (int kk = 0; kk < kn; kk++) scol[base + (k0 + kk) * kWave + lane] = tile[wv][lane][kk];
+        wave_lds_sync();
+    }
 }
 
 // Slice order by row class (row_class) within windows of 256 rows: each
@@ -1149,17 +1308,6 @@ __global__ __launch_bounds__(256) void lj_atomic(const float4 *__restrict__ pos,
     }
 }
 
-// Copies one system's CSR into the batched CSR, shifting node and entry
-// indices by the system's offsets.
-__global__ void csr_place(const int *__restrict__ rp, const int *__restrict__ cl, int n, int node_off, int entry_off,
-                          int *__restrict__ grp, int *__restrict__ gcl)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t <= n) grp[node_off + t] = rp[t] + entry_off;
-    const int ne = rp[n];
-    for (int64_t k = t; k < ne; k += (int64_t)gridDim.x * blockDim.x) gcl[entry_off + k] = cl[k] + node_off;
-}
-
 __global__ void fill_zero4(float4 *__restrict__ a, int N)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1577,16 +1725,6 @@ int rcp_selftest(uint32_t lo, uint32_t hi, int64_t *bad_one, int64_t *bad_two)
     return PAT_OK;
 }
 
-int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
-                  int32_t *grp, int32_t *gcl, hipStream_t st)
-{
-    const unsigned grid = (unsigned)std::min<int64_t>((n + 256) / 256 + 1, 4096);
-    hipLaunchKernelGGL(csr_place, dim3(grid), dim3(256), 0, st, (const int *)rp, (const int *)cl, (int)n, (int)node_off,
-                       (int)entry_off, (int *)grp, (int *)gcl);
-    PAT_LAUNCH_CHECK();
-    return PAT_OK;
-}
-
 int lj_energy_csr(const float4 *pos, const int32_t *rowptr, const int32_t *cols, int64_t N, float box, int pbc,
                   float rc2, double *node_e, hipStream_t st)
 {
@@ -1648,16 +1786,19 @@ namespace pat {
 
 // Device buffers of the binned neighbour build, kept between rebuilds.
 struct NbWork {
-    int64_t N = 0, ncell = 0, cap = 0;  // cap: capacity of cols/coltmp
-    int nb = 0;
+    int64_t N = 0, ncell = 0, cap = 0;  // N, ncell: capacities of the per-atom / per-bin buffers; cap: coltmp
+    int64_t nsys_cap = 0;
     int *cell = nullptr, *cstart = nullptr, *cursor = nullptr, *atmp = nullptr, *catoms = nullptr, *deg = nullptr,
         *ws = nullptr, *coltmp = nullptr;
+    NbSys *sys = nullptr;
+    std::vector<NbSys> sys_h;  // source of the last upload (read before the build returns)
 };
 
 static void nb_free(NbWork &w)
 {
     for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp})
         if (p) (void)pat::dev_free(p);
+    if (w.sys) (void)pat::dev_free(w.sys);
     w = NbWork{};
 }
 
@@ -1680,48 +1821,77 @@ static int nb_alloc(int **p, int64_t n)
     return PAT_OK;
 }
 
-// Rebuilds the full, row-sorted neighbour CSR of N float4 positions in a
-// periodic cube of side L for pairs closer than cut.  *cols (capacity
-// *cap entries) is reallocated when too small.  Synchronises the stream once
-// (to size the column array).
-int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr, int32_t **cols,
-                   int64_t *cap, int64_t *total, hipStream_t st)
+// Rebuilds the full, row-sorted neighbour CSR of nsys independent periodic
+// cubes at once: system s holds atoms [off[s], off[s+1]) of the float4
+// positions in a cube of side Ls[s]; pairs closer than cut (within one
+// system) land in both rows, with global atom indices.  *cols (capacity
+// *cap entries) is reallocated when too small.  Synchronises the stream
+// once (to size the column array).
+int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const int64_t *off, double cut, NbWork *w,
+                         int32_t *rowptr, int32_t **cols, int64_t *cap, int64_t *total, hipStream_t st)
 {
-    PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1 && L > 0 && cut > 0, PAT_ERR_INVALID, "neighbour build: bad arguments");
-    const int nb = (int)std::floor(L / (cut * 1.0001));
-    PAT_REQUIRE(nb >= 3, PAT_ERR_UNSUPPORTED, "neighbour build: box %.3f holds fewer than 3 bins of %.3f", L, cut);
-    PAT_REQUIRE((int64_t)nb * nb * nb < (1LL << 30), PAT_ERR_INVALID, "neighbour build: too many bins");
-    const int64_t ncell = (int64_t)nb * nb * nb;
-    if (w->N != N || w->ncell != ncell) {
+    PAT_REQUIRE(nsys >= 1 && Ls && off && off[0] == 0 && cut > 0, PAT_ERR_INVALID, "neighbour build: bad arguments");
+    const int64_t N = off[nsys];
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1, PAT_ERR_INVALID, "neighbour build: bad atom count");
+    w->sys_h.resize((size_t)nsys);
+    int64_t ncell = 0;
+    for (int s = 0; s < nsys; s++) {
+        const double L = Ls[s];
+        PAT_REQUIRE(L > 0 && off[s + 1] > off[s], PAT_ERR_INVALID, "neighbour build: system %d is empty", s);
+        const int nb = (int)std::floor(L / (cut * 1.0001));
+        PAT_REQUIRE(nb >= 3, PAT_ERR_UNSUPPORTED, "neighbour build: box %.3f holds fewer than 3 bins of %.3f", L,
+                    cut);
+        PAT_REQUIRE(nb < 1024, PAT_ERR_INVALID, "neighbour build: too many bins");
+        NbSys &S = w->sys_h[(size_t)s];
+        S.L = (float)L;
+        S.hL = 0.5f * S.L;
+        S.inv = (float)nb / S.L;
+        S.nb = nb;
+        S.cell0 = (int)ncell;
+        S.atom0 = (int)off[s];
+        S.atom1 = (int)off[s + 1];
+        S.pad = 0;
+        ncell += (int64_t)nb * nb * nb;
+        PAT_REQUIRE(ncell < (1LL << 30), PAT_ERR_INVALID, "neighbour build: too many bins");
+    }
+    // the buffers only grow: rebuilds of other sizes reuse them (a free
+    // here would wait for the whole device)
+    if (w->N < N || w->ncell < ncell || w->nsys_cap < nsys) {
         const int64_t cap_keep = w->cap;
         int *coltmp = w->coltmp;
         w->coltmp = nullptr;
+        const int64_t Nn = std::max(w->N, N), nc = std::max(w->ncell, ncell), ns = std::max<int64_t>(w->nsys_cap, nsys);
+        std::vector<NbSys> keep = std::move(w->sys_h);
         nb_free(*w);
+        w->sys_h = std::move(keep);
         w->coltmp = coltmp;
         w->cap = cap_keep;
-        const int64_t m = std::max<int64_t>(N, ncell) + 1;
+        const int64_t m = std::max<int64_t>(Nn, nc) + 1;
         const int64_t scan_ws = 2 * ((m + kScanBlock - 1) / kScanBlock) + 2 * kScanBlock + 4;
-        PAT_TRY(nb_alloc(&w->cell, N));
-        PAT_TRY(nb_alloc(&w->cstart, ncell + 1));
-        PAT_TRY(nb_alloc(&w->cursor, ncell + 1));
-        PAT_TRY(nb_alloc(&w->atmp, N));
-        PAT_TRY(nb_alloc(&w->catoms, N));
-        PAT_TRY(nb_alloc(&w->deg, N + 1));
+        PAT_TRY(nb_alloc(&w->cell, Nn));
+        PAT_TRY(nb_alloc(&w->cstart, nc + 1));
+        PAT_TRY(nb_alloc(&w->cursor, nc + 1));
+        PAT_TRY(nb_alloc(&w->atmp, Nn));
+        PAT_TRY(nb_alloc(&w->catoms, Nn));
+        PAT_TRY(nb_alloc(&w->deg, Nn + 1));
         PAT_TRY(nb_alloc(&w->ws, scan_ws));
-        w->N = N;
-        w->ncell = ncell;
-        w->nb = nb;
+        if (pat::dev_malloc((void **)&w->sys, sizeof(NbSys) * (size_t)ns) != hipSuccess) {
+            (void)hipGetLastError();
+            pat::set_error("neighbour build: hipMalloc of %lld systems failed", (long long)ns);
+            return PAT_ERR_NOMEM;
+        }
+        w->N = Nn;
+        w->ncell = nc;
+        w->nsys_cap = ns;
     }
-    BinGeom g;
-    g.L = (float)L;
-    g.hL = 0.5f * g.L;
-    g.cut2 = (float)(cut * cut);
-    g.nb = nb;
-    g.inv = (float)nb / g.L;
+    const float cut2 = (float)(cut * cut);
+    PAT_HIP(hipMemcpyAsync(w->sys, w->sys_h.data(), sizeof(NbSys) * (size_t)nsys, hipMemcpyHostToDevice, st));
     const unsigned gN = (unsigned)((N + 255) / 256);
+    const unsigned gC = (unsigned)((ncell + 3) / 4);
     // bin the atoms and sort each bin's list by atom index
     PAT_HIP(hipMemsetAsync(w->cursor, 0, sizeof(int) * (ncell + 1), st));
-    hipLaunchKernelGGL(nb_bin_atoms, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cursor);
+    hipLaunchKernelGGL(nb_bin_atoms, dim3(gN), dim3(256), 0, st, pos, (int)N, (const NbSys *)w->sys, nsys, w->cell,
+                       w->cursor);
     PAT_LAUNCH_CHECK();
     PAT_TRY(scan_exclusive(w->cursor, w->cstart, ncell + 1, w->ws, st));
     PAT_HIP(hipMemcpyAsync(w->cursor, w->cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, st));
@@ -1731,8 +1901,8 @@ int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w
                        w->atmp, w->catoms);
     PAT_LAUNCH_CHECK();
     // row lengths, then row pointers
-    hipLaunchKernelGGL(nb_rows, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cstart, w->catoms, w->deg,
-                       (const int *)nullptr, (int *)nullptr, 0);
+    hipLaunchKernelGGL(nb_cells<false>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys, (int)ncell, cut2,
+                       (const int *)w->cstart, (const int *)w->catoms, w->deg, (const int *)nullptr, (int *)nullptr);
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
     PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
@@ -1751,14 +1921,22 @@ int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w
         *cap = want;
         w->cap = want;
     }
-    hipLaunchKernelGGL(nb_rows, dim3(gN), dim3(256), 0, st, pos, (int)N, g, w->cell, w->cstart, w->catoms, w->deg,
-                       (const int *)rowptr, w->coltmp, 1);
+    hipLaunchKernelGGL(nb_cells<true>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys, (int)ncell, cut2,
+                       (const int *)w->cstart, (const int *)w->catoms, w->deg, (const int *)rowptr, w->coltmp);
     PAT_LAUNCH_CHECK();
     hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((N * kWave + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
                        N, w->coltmp, *cols);
     PAT_LAUNCH_CHECK();
     *total = tot;
     return PAT_OK;
+}
+
+// The same for one cube of N atoms and side L.
+int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr, int32_t **cols,
+                   int64_t *cap, int64_t *total, hipStream_t st)
+{
+    const int64_t off[2] = {0, N};
+    return neighbor_build_multi(pos, 1, &L, off, cut, w, rowptr, cols, cap, total, st);
 }
 
 }  // namespace pat

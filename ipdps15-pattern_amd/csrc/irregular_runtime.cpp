@@ -45,14 +45,14 @@ int lj_force_atomic(const float4 *pos, const int32_t *edges, int64_t E, int64_t 
 int vv_kick_drift_dev(float4 *pos, float4 *vel, const float4 *force, int64_t N, float box, int pbc, float dt,
                       int *inbox, hipStream_t st);
 int vv_kick_dev(float4 *vel, const float4 *force, int64_t N, float dt, hipStream_t st);
-int csr_place_dev(const int32_t *rp, const int32_t *cl, int64_t n, int64_t node_off, int64_t entry_off,
-                  int32_t *grp, int32_t *gcl, hipStream_t st);
 int rcp_selftest(uint32_t lo, uint32_t hi, int64_t *bad_one, int64_t *bad_two);
 int sell_build_into(const int32_t *rowptr, const int32_t *cols, int64_t N, int32_t *sptr, int32_t *scol, int64_t cap,
                     int64_t *total, hipStream_t st);
 struct NbWork;
 NbWork *nb_create();
 void nb_destroy(NbWork *w);
+int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const int64_t *off, double cut, NbWork *w,
+                         int32_t *rowptr, int32_t **cols, int64_t *cap, int64_t *total, hipStream_t st);
 int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr, int32_t **cols,
                    int64_t *cap, int64_t *total, hipStream_t st);
 }  // namespace pat
@@ -123,6 +123,8 @@ struct IrState {
     int halo_max = 0;
     bool halo_ok = false;
     bool want_halo = false;  // built with every sliced layout while set
+    int rebuild_every = 0;   // steps between neighbour rebuilds (0: fixed list); the tuner charges
+                             // the halo layout's build time over that many force evaluations
 
     // Sliced kernels padded with a sentinel node: positions hold N+1
     // entries, the last far beyond any cutoff (set at allocation, never
@@ -368,8 +370,8 @@ struct IrState {
     {
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
         const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
-        const auto key =
-            std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg, order + 2 * k0);
+        const auto key = std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg,
+                                         order + 2 * k0 + 8 * std::min(rebuild_every, 1 << 20));
         layout_gen++;
         if (!pat::tuning_enabled()) {
             lpr = sell_ok ? PAT_IR_SLICED : 1;
@@ -399,6 +401,17 @@ struct IrState {
             for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, false});
         if (halo_ok)
             for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, false});
+        // a layout rebuilt with the neighbour list costs its build time over
+        // rebuild_every force evaluations
+        float halo_share = 0.0f;  // per kReps launches
+        if (halo_ok && rebuild_every > 0) {
+            PAT_HIP(hipStreamSynchronize(stream));
+            auto h0 = std::chrono::steady_clock::now();
+            PAT_TRY(build_halo());
+            PAT_HIP(hipStreamSynchronize(stream));
+            const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - h0).count();
+            halo_share = 3.0f * ms / (float)rebuild_every;
+        }
         std::vector<float> t(cands.size(), 1e30f);
         auto launch = [&](size_t c) {
             sell_kind = cands[c].kind;
@@ -416,7 +429,7 @@ struct IrState {
                     PAT_HIP(hipEventSynchronize(b));
                     float ms = 0;
                     PAT_HIP(hipEventElapsedTime(&ms, a, b));
-                    t[c] = std::min(t[c], ms);
+                    t[c] = std::min(t[c], ms + (cands[c].kind >= 6 ? halo_share : 0.0f));
                 }
             return PAT_OK;
         };
@@ -444,6 +457,9 @@ struct IrState {
         size_t bi = 0;
         for (size_t c = 1; c < cands.size(); c++)
             if (t[c] < t[bi]) bi = c;
+        if (pat::tune_log() && halo_share > 0.0f)
+            fprintf(stderr, "[tune lj N=%lld] kinds 6-7 include %.2f us of halo build per step (rebuild every %d)\n",
+                    (long long)N, 1000.0f * halo_share / 3, rebuild_every);
         if (pat::tune_log())
             for (size_t c = 0; c < cands.size(); c++)
                 fprintf(stderr, "[tune lj N=%lld] lanes %d kind %d class-order %d: %.2f us%s\n", (long long)N,
@@ -870,6 +886,7 @@ int md_create_binned(int64_t num_nodes, const float *pos, const float *vel, bool
         std::vector<float> zero(3 * (size_t)num_nodes, 0.0f);
         rc = h->st.upload3(h->st.vel, vel ? vel : zero.data(), 12);
     }
+    h->st.rebuild_every = every;
     if (rc == PAT_OK) rc = h->rebuild(0);
     if (rc == PAT_OK) rc = h->st.tune(h->st.pos[0], h->box, h->pbc, h->rc2);
     if (rc == PAT_OK) rc = h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2);
@@ -1168,42 +1185,16 @@ struct pat_mdb {
     pat_md md;                      // the concatenated system (pbc mode 2)
     std::vector<int64_t> off;       // node offsets, nsys + 1
     std::vector<double> box;        // per system
-    int32_t *rp_s = nullptr, *cl_s = nullptr;  // one system's CSR during a rebuild
-    int64_t cap_s = 0;
 
+    // One binned build over every system (global atom indices, so the CSR
+    // is the concatenated one), then the sliced layout.
     int rebuild_all(int c)
     {
         auto t0 = std::chrono::steady_clock::now();
-        const int nsys = (int)box.size();
-        std::vector<int64_t> totals(nsys);
         int64_t entries = 0;
-        // pass 1: build each system's CSR and place it
-        for (int s = 0; s < nsys; s++) {
-            const int64_t n = off[s + 1] - off[s];
-            int64_t tot = 0;
-            PAT_TRY(pat::neighbor_build((const float4 *)md.st.pos[c] + off[s], n, box[s], md.cutneigh, md.nbw, rp_s,
-                                        &cl_s, &cap_s, &tot, md.st.stream));
-            if (entries + tot > md.cap) {
-                // grow the batched column array, keeping what is placed
-                const int64_t want = (int64_t)((entries + tot) * 1.3) + 4096;
-                int32_t *nc = nullptr;
-                if (pat::dev_malloc((void **)&nc, sizeof(int32_t) * (size_t)want) != hipSuccess) {
-                    (void)hipGetLastError();
-                    pat::set_error("batched md: hipMalloc of %lld column entries failed", (long long)want);
-                    return PAT_ERR_NOMEM;
-                }
-                if (entries > 0 && md.st.cols)
-                    PAT_HIP(hipMemcpyAsync(nc, md.st.cols, sizeof(int32_t) * entries, hipMemcpyDeviceToDevice,
-                                           md.st.stream));
-                PAT_HIP(hipStreamSynchronize(md.st.stream));
-                if (md.st.cols) (void)pat::dev_free(md.st.cols);
-                md.st.cols = nc;
-                md.cap = want;
-            }
-            PAT_TRY(pat::csr_place_dev(rp_s, cl_s, n, off[s], entries, md.st.rowptr, md.st.cols, md.st.stream));
-            entries += tot;
-            totals[s] = tot;
-        }
+        PAT_TRY(pat::neighbor_build_multi((const float4 *)md.st.pos[c], (int)box.size(), box.data(), off.data(),
+                                          md.cutneigh, md.nbw, md.st.rowptr, &md.st.cols, &md.cap, &entries,
+                                          md.st.stream));
         PAT_HIP(hipStreamSynchronize(md.st.stream));
         md.st.E = entries / 2;
         PAT_TRY(md.st.build_sell(md.st.pos[c], 1.0f, 2, md.rc2));
@@ -1344,17 +1335,13 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
         std::vector<float> zero(3 * (size_t)N, 0.0f);
         rc = m.st.upload3(m.st.vel, vel ? vel : zero.data(), 12);
     }
-    if (rc == PAT_OK) {
-        if (pat::dev_malloc((void **)&h->rp_s, sizeof(int32_t) * (size_t)(N + 1)) != hipSuccess) rc = PAT_ERR_NOMEM;
-    }
+    m.st.rebuild_every = every;
     if (rc == PAT_OK) rc = h->rebuild_all(0);
     if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK)
         rc = m.st.force_eval(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK && hipStreamSynchronize(m.st.stream) != hipSuccess) rc = PAT_ERR_HIP;
     if (rc != PAT_OK) {
-        if (h->rp_s) (void)pat::dev_free(h->rp_s);
-        if (h->cl_s) (void)pat::dev_free(h->cl_s);
         pat::nb_destroy(m.nbw);
         m.nbw = nullptr;
         m.st.release();
@@ -1422,8 +1409,6 @@ int pat_mdb_stats(pat_mdb *h, int64_t *num_pairs, int64_t *steps, int64_t *rebui
 int pat_mdb_destroy(pat_mdb *h)
 {
     if (!h) return PAT_OK;
-    if (h->rp_s) (void)pat::dev_free(h->rp_s);
-    if (h->cl_s) (void)pat::dev_free(h->cl_s);
     if (h->md.nbw) pat::nb_destroy(h->md.nbw);
     h->md.nbw = nullptr;
     h->drop_seg_graphs();

@@ -36,6 +36,26 @@ def test_binned_csr_equals_host_pairs(pa, gpu, cells, seed):
     assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
 
 
+def test_binned_csr_dense_and_clustered(pa, gpu):
+    """Bins of more than 64 atoms, bin neighbourhoods of more than 768
+    candidates (several register slices per wavefront) and rows longer than
+    128 (the rank-sort path): a jittered simple cubic lattice at density 2.5
+    with a 5^3 cluster at spacing 0.5 inside one bin."""
+    rng = np.random.default_rng(11)
+    L, m = 17.0, 23
+    g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3) * (L / m)
+    g = g + rng.uniform(-0.1, 0.1, g.shape)
+    g = g[~np.all((g > 2.6) & (g < 5.4), axis=1)]
+    c = np.stack(np.meshgrid(*[np.arange(5)] * 3, indexing="ij"), -1).reshape(-1, 3) * 0.5 + 3.0
+    pos = np.mod(np.concatenate([g, c]), L).astype(np.float32)
+    md = pa.MD(pos, None, None, L, 2.5, 0.005, skin=0.3, every=20)
+    rp, cl = md.csr()
+    md.close()
+    erp, ecl = host_csr(len(pos), pa.gen_pairs(pos, L, 2.8))
+    assert np.diff(erp).max() > 128
+    assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
+
+
 @pytest.mark.parametrize("lanes", [0, 1, 64])
 def test_binned_md_vs_oracle(pa, orc, gpu, lanes):
     """Rebuilds before the force of steps 20 and 40: bit-identical to the
