@@ -295,6 +295,17 @@ int pat_md_set_sliced_form(pat_md *h, int form);
 /* The sliced loop form in use (0..4 as above), or -1 when the handle runs
  * another kernel.  No reference counterpart. */
 int pat_md_get_sliced_form(pat_md *h, int *form);
+/* Order of rows in the sliced layout's 64-row slices: 0 node order, 1 row
+ * class (nodes near a box face grouped within 256-row windows), 2 spatial
+ * (rows sorted by the Morton code of a half-cutoff bin, so each slice's
+ * rows share one compact gather neighbourhood; recomputed at every
+ * rebuild).  Rows keep their own entries in order, so every order gives the
+ * same bits.  In place of the tuner's choice; PAT_ERR_UNSUPPORTED without
+ * the sliced sentinel layout of a periodic box.  No reference counterpart
+ * (the reference has no device layout). */
+int pat_md_set_slice_order(pat_md *h, int order);
+/* The slice order of the current layout (0, 1, 2 as above). */
+int pat_md_get_slice_order(pat_md *h, int *order);
 /* Current half-pair count, steps run, neighbour rebuilds and their total
  * host-measured time. */
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);
@@ -315,6 +326,9 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
                    double cutoff, double skin, int every, double dt, pat_mdb **out);
 /* Lanes per CSR row (0 = default); recomputes the stored force. */
 int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row);
+/* pat_md_set_slice_order for the batch (the spatial order sorts system by
+ * system). */
+int pat_mdb_set_slice_order(pat_mdb *h, int order);
 /* nsteps steps (blocking); total_ms = host wall time, kernel_ms[k] = step
  * k's force kernel when non-NULL (then launched one by one with events). */
 int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms);
@@ -358,6 +372,14 @@ typedef struct pat_kmeans pat_kmeans;
  * device pointer the handle reads in place. */
 int pat_kmeans_create(const float *x, int64_t n, int d, int k, int mode, int x_on_device, pat_kmeans **out);
 int pat_kmeans_set_mode(pat_kmeans *h, int mode);
+/* BF16 terms of the MFMA path's distance values: 3 (x and -2c split into
+ * high and low parts) or 1 (rounded once, a wider rescoring margin).  Both
+ * give the plain path's assignments bit for bit; by default the faster one
+ * on the handle's data is kept (timed at creation, when both fit in LDS).
+ * PAT_ERR_UNSUPPORTED when the shape's rows do not fit the requested form.
+ * No reference counterpart (the reference has no MFMA path). */
+int pat_kmeans_set_terms(pat_kmeans *h, int terms);
+int pat_kmeans_get_terms(pat_kmeans *h, int *terms);
 /* Replaces the centres (k x d host array). */
 int pat_kmeans_set_centroids(pat_kmeans *h, const float *c);
 /* iters x (RegularStart: assign + reduce; then the Lloyd update

@@ -33,6 +33,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "pat_common.hpp"
 #include "pat_device.hpp"
 
@@ -858,6 +860,57 @@ __global__ __launch_bounds__(256) void perm_block_sort(const float4 *__restrict_
     if (dst < total) perm[dst] = i < N ? i : N;
 }
 
+// Spatial slice order: nodes sorted by the Morton code of a bin about h
+// wide (at most 1024 per axis), system by system, ties in node order, so a
+// slice's 64 rows sit in a compact region and share one small gather
+// neighbourhood.  Rows keep their own entries and summation order.
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// keys[i] = (system << 30) | morton(bin of node i); vals[i] = i.  The box
+// side is box, or each node's w for pbc 2; sysoff (nsys + 1 offsets, null
+// for one system) gives the systems' node ranges.
+__global__ void morton_keys(const float4 *__restrict__ pos, int N, float box, int pbc, float h,
+                            const int *__restrict__ sysoff, int nsys, uint64_t *__restrict__ keys,
+                            int *__restrict__ vals)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 p = pos[i];
+    const float L = pbc == 2 ? p.w : box;
+    const int nb = (int)fminf(1024.0f, fmaxf(1.0f, floorf(L / h)));
+    const float inv = (float)nb / L;
+    const uint32_t bx = (uint32_t)min(max((int)floorf(p.x * inv), 0), nb - 1);
+    const uint32_t by = (uint32_t)min(max((int)floorf(p.y * inv), 0), nb - 1);
+    const uint32_t bz = (uint32_t)min(max((int)floorf(p.z * inv), 0), nb - 1);
+    int s = 0;
+    if (sysoff) {
+        int lo = 0, hi = nsys;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sysoff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        s = lo;
+    }
+    keys[i] = ((uint64_t)s << 30) | (uint64_t)(spread3(bx) | (spread3(by) << 1) | (spread3(bz) << 2));
+    vals[i] = i;
+}
+
+// perm[k] = sorted node k for k < N, the empty lane N up to total.
+__global__ void perm_pad(const int *__restrict__ sorted, int N, int total, int *__restrict__ perm)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < total) perm[k] = k < N ? sorted[k] : N;
+}
+
 // Neighbour position.  SENT layouts store the byte offset j*16 of node j
 // (the handle keeps fewer than 2^28 nodes), the others the index.
 template <bool SENT>
@@ -1600,6 +1653,72 @@ int sell_perm_build(const float4 *pos, int64_t N, float box, int pbc, float rc2,
     const LJParams p = make_params(box, pbc, rc2, 0.f);
     hipLaunchKernelGGL(perm_block_sort, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pos, (int)N,
                        (int)total, box, p.W0, pbc, (int *)perm);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// Buffers of the spatial slice order, grown as needed.
+struct SortWork {
+    int64_t n = 0;
+    size_t tmp_bytes = 0;
+    uint64_t *keys = nullptr, *keys2 = nullptr;
+    int *vals = nullptr, *vals2 = nullptr;
+    void *tmp = nullptr;
+};
+
+SortWork *sort_create() { return new SortWork(); }
+void sort_destroy(SortWork *w)
+{
+    if (!w) return;
+    for (void *p : {(void *)w->keys, (void *)w->keys2, (void *)w->vals, (void *)w->vals2, w->tmp})
+        if (p) (void)pat::dev_free(p);
+    delete w;
+}
+
+// Spatial slice order of N nodes (see morton_keys): perm gets nsl*64
+// entries.  sysoff: nsys + 1 node offsets on the device (null: one system).
+int sell_perm_spatial(const float4 *pos, int64_t N, float box, int pbc, float h, const int32_t *sysoff, int nsys,
+                      int32_t *perm, SortWork *w, hipStream_t st)
+{
+    PAT_REQUIRE(pbc != 0 && h > 0 && nsys >= 1 && nsys < (1 << 20), PAT_ERR_INVALID,
+                "spatial slice order: needs a periodic box, h > 0 and 1 <= systems < 2^20");
+    const int64_t nsl = (N + kWave - 1) / kWave, total = nsl * kWave;
+    int end_bit = 30;
+    while ((1LL << (end_bit - 30)) < nsys) end_bit++;
+    size_t need = 0;
+    PAT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+                                               (const int *)nullptr, (int *)nullptr, (int)N, 0, end_bit, st));
+    if (w->n < N || w->tmp_bytes < need) {
+        for (void **p : {(void **)&w->keys, (void **)&w->keys2, (void **)&w->vals, (void **)&w->vals2, &w->tmp})
+            if (*p) {
+                (void)pat::dev_free(*p);
+                *p = nullptr;
+            }
+        const int64_t n = std::max(w->n, N);
+        const size_t tb = std::max(w->tmp_bytes, need);
+        if (pat::dev_malloc((void **)&w->keys, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
+            pat::dev_malloc((void **)&w->keys2, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
+            pat::dev_malloc((void **)&w->vals, sizeof(int) * (size_t)n) != hipSuccess ||
+            pat::dev_malloc((void **)&w->vals2, sizeof(int) * (size_t)n) != hipSuccess ||
+            pat::dev_malloc(&w->tmp, tb > 0 ? tb : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            w->n = 0;
+            w->tmp_bytes = 0;
+            pat::set_error("spatial slice order: hipMalloc for %lld nodes failed", (long long)N);
+            return PAT_ERR_NOMEM;
+        }
+        w->n = n;
+        w->tmp_bytes = tb;
+    }
+    const unsigned g = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(morton_keys, dim3(g), dim3(256), 0, st, pos, (int)N, box, pbc, h, (const int *)sysoff, nsys,
+                       w->keys, w->vals);
+    PAT_LAUNCH_CHECK();
+    size_t tb = w->tmp_bytes;
+    PAT_HIP(hipcub::DeviceRadixSort::SortPairs(w->tmp, tb, (const uint64_t *)w->keys, w->keys2,
+                                               (const int *)w->vals, w->vals2, (int)N, 0, end_bit, st));
+    hipLaunchKernelGGL(perm_pad, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, (const int *)w->vals2,
+                       (int)N, (int)total, (int *)perm);
     PAT_LAUNCH_CHECK();
     return PAT_OK;
 }

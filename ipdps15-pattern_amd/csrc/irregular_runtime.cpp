@@ -25,6 +25,11 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *ro
 int sell_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, int64_t N, double max_pad,
                int32_t *width_d, int32_t *sptr, int32_t **scol, int64_t *cap, int64_t *total, int pad, hipStream_t st);
 int sell_perm_build(const float4 *pos, int64_t N, float box, int pbc, float rc2, int32_t *perm, hipStream_t st);
+struct SortWork;
+SortWork *sort_create();
+void sort_destroy(SortWork *w);
+int sell_perm_spatial(const float4 *pos, int64_t N, float box, int pbc, float h, const int32_t *sysoff, int nsys,
+                      int32_t *perm, SortWork *w, hipStream_t st);
 int halo_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, const int32_t *sptr, int64_t N,
                int64_t stotal, int32_t *hptr, int32_t **hidx, int64_t *hcap, uint16_t **scol16, int64_t *scap16,
                int *max_nh, hipStream_t st);
@@ -80,7 +85,7 @@ constexpr double kSellMaxPad = 1.5;
 struct KernelChoice {
     int lanes;      // lanes per row, or PAT_IR_SLICED
     int kind;       // sliced kernel form (lj_force_sell kind)
-    bool by_class;  // sliced rows in row-class order
+    int slice_order;  // sliced rows in node (0), row-class (1) or spatial (2) order
 };
 std::map<std::tuple<int64_t, int, int>, KernelChoice> g_lpr_cache;
 std::mutex g_lpr_mutex;
@@ -108,13 +113,19 @@ struct IrState {
     // check_box, cleared by any drift that leaves it); the sentinel kinds
     // skip the minimum image per wavefront only while it is set
     int32_t *inbox = nullptr;
-    // slice order of the sentinel layout: rows of each 256-row window
-    // grouped by row class (nodes within a cutoff of a face, per axis) so
-    // most wavefronts take the minimum image on no axis; identity (perm_on
-    // false) unless the tuner finds it faster
+    // slice order of the sentinel layout (perm): 0 node order (identity);
+    // 1 row class: rows of each 256-row window grouped by row class (nodes
+    // within a cutoff of a face, per axis) so most wavefronts take the
+    // minimum image on no axis; 2 spatial: rows sorted by the Morton code
+    // of a half-cutoff bin, so a slice's rows share one small gather
+    // neighbourhood.  Rows keep their entries, so every order gives the
+    // same sums; the tuner picks the fastest.
     int32_t *perm = nullptr;
-    bool perm_on = false;
-    bool want_perm = false;  // tuned: whether the class order pays off
+    int order_on = 0;    // order of the current sliced layout
+    int want_order = 0;  // order wanted for the next build
+    pat::SortWork *sortw = nullptr;
+    std::vector<int32_t> sysoff_h;  // batched systems' node offsets (empty: one system)
+    int32_t *sysoff_d = nullptr;
     // halo copy of the sentinel layout (kinds 6, 7): per 256-row block the
     // distinct neighbours (hptr/hidx) and the entries as LDS offsets
     int32_t *hptr = nullptr, *hidx = nullptr;
@@ -125,6 +136,10 @@ struct IrState {
     bool want_halo = false;  // built with every sliced layout while set
     int rebuild_every = 0;   // steps between neighbour rebuilds (0: fixed list); the tuner charges
                              // the halo layout's build time over that many force evaluations
+    // kernel choice made by tune() (not fixed by the caller); with
+    // rebuilds, tuned once more (fresh) at the first rebuild after creation,
+    // when the positions have left their initial lattice
+    bool choice_tuned = false, retuned = false;
 
     // Sliced kernels padded with a sentinel node: positions hold N+1
     // entries, the last far beyond any cutoff (set at allocation, never
@@ -205,14 +220,31 @@ struct IrState {
     int build_sell(const float *p4 = nullptr, float box = 0.f, int pbc = 0, float rc2 = 0.f)
     {
         layout_gen++;
-        perm_on = false;
+        order_on = 0;
         halo_ok = false;
-        if (want_perm && p4 && pbc && sentinel()) {
-            PAT_TRY(pat::sell_perm_build((const float4 *)p4, N, box, pbc, rc2, perm, stream));
-            perm_on = true;
+        if (want_order && p4 && pbc && sentinel()) {
+            if (want_order == 1) {
+                PAT_TRY(pat::sell_perm_build((const float4 *)p4, N, box, pbc, rc2, perm, stream));
+            } else {
+                const int nsys = sysoff_h.size() > 2 ? (int)sysoff_h.size() - 1 : 1;
+                if (!sortw) sortw = pat::sort_create();
+                if (nsys > 1 && !sysoff_d) {
+                    if (pat::dev_malloc((void **)&sysoff_d, sizeof(int32_t) * sysoff_h.size()) != hipSuccess) {
+                        (void)hipGetLastError();
+                        pat::set_error("irregular: hipMalloc of %zu system offsets failed", sysoff_h.size());
+                        return PAT_ERR_NOMEM;
+                    }
+                    PAT_HIP(hipMemcpyAsync(sysoff_d, sysoff_h.data(), sizeof(int32_t) * sysoff_h.size(),
+                                           hipMemcpyHostToDevice, stream));
+                    PAT_HIP(hipStreamSynchronize(stream));
+                }
+                PAT_TRY(pat::sell_perm_spatial((const float4 *)p4, N, box, pbc, 0.5f * std::sqrt(rc2),
+                                               nsys > 1 ? sysoff_d : nullptr, nsys, perm, sortw, stream));
+            }
+            order_on = want_order;
         }
         // padded with the sentinel node N when the handle can hold it
-        int rc = pat::sell_build(rowptr, cols, perm_on ? perm : nullptr, N, kSellMaxPad, swidth, sptr, &scol, &scap,
+        int rc = pat::sell_build(rowptr, cols, order_on ? perm : nullptr, N, kSellMaxPad, swidth, sptr, &scol, &scap,
                                  &stotal, sentinel() ? (int)N : -1, stream);
         if (rc == PAT_ERR_UNSUPPORTED) {
             pat::clear_error();
@@ -231,7 +263,7 @@ struct IrState {
     {
         halo_ok = false;
         if (!sell_ok || !sentinel()) return PAT_OK;
-        int rc = pat::halo_build(rowptr, cols, perm_on ? perm : nullptr, sptr, N, stotal, hptr, &hidx, &hcap,
+        int rc = pat::halo_build(rowptr, cols, order_on ? perm : nullptr, sptr, N, stotal, hptr, &hidx, &hcap,
                                  &scol16, &scap16, &halo_max, stream);
         if (rc == PAT_ERR_UNSUPPORTED) {
             pat::clear_error();
@@ -248,8 +280,10 @@ struct IrState {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)edges, (void *)rowptr, (void *)cols, (void *)pos[0], (void *)pos[1], (void *)vel,
                         (void *)force, (void *)node_e, (void *)sptr, (void *)scol, (void *)swidth, (void *)inbox,
-                        (void *)perm, (void *)hptr, (void *)hidx, (void *)scol16})
+                        (void *)perm, (void *)hptr, (void *)hidx, (void *)scol16, (void *)sysoff_d})
             if (p) (void)pat::dev_free(p);
+        pat::sort_destroy(sortw);
+        sortw = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -310,11 +344,11 @@ struct IrState {
                         sell_kind);
             if (sell_kind >= 6) {
                 PAT_REQUIRE(halo_ok, PAT_ERR_STATE, "irregular: the halo layout is not built");
-                return pat::lj_force_halo((const float4 *)p4, sptr, scol16, hptr, hidx, perm_on ? perm : nullptr, N,
+                return pat::lj_force_halo((const float4 *)p4, sptr, scol16, hptr, hidx, order_on ? perm : nullptr, N,
                                           halo_max, box, pbc, rc2, dt, mode, (float4 *)force, (float4 *)vel,
                                           (float4 *)pos_out, sell_kind == 7 ? 8 : 4, inbox, s);
             }
-            return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, perm_on ? perm : nullptr, N, box, pbc,
+            return pat::lj_force_sell((const float4 *)p4, rowptr, sptr, scol, order_on ? perm : nullptr, N, box, pbc,
                                       rc2, dt, mode, (float4 *)force, (float4 *)vel, (float4 *)pos_out, sell_kind,
                                       inbox, s);
         }
@@ -350,14 +384,14 @@ struct IrState {
         return csr_force(p4, box, pbc, rc2, 0.f, kForceOnly, nullptr, stream);
     }
 
-    // Sliced layout in class order (by_class) or node order, with or
-    // without its halo copy; rebuilt from positions p4 where it differs.
-    int set_layout(bool by_class, bool halo, const float *p4, float box, int pbc, float rc2)
+    // Sliced layout in slice order slice_order (see perm), with or without
+    // its halo copy; rebuilt from positions p4 where it differs.
+    int set_layout(int slice_order, bool halo, const float *p4, float box, int pbc, float rc2)
     {
-        want_perm = by_class && pbc && sentinel();
+        want_order = pbc && sentinel() ? slice_order : 0;
         want_halo = halo && sentinel();
         if (!sell_ok) return PAT_OK;
-        if (want_perm != perm_on) return build_sell(p4, box, pbc, rc2);  // with the halo when wanted
+        if (want_order != order_on) return build_sell(p4, box, pbc, rc2);  // with the halo when wanted
         if (want_halo && !halo_ok) return build_halo();
         return PAT_OK;
     }
@@ -366,41 +400,44 @@ struct IrState {
     // sliced form, and the sliced forms again over the class-ordered layout,
     // on this CSR, and keeps the fastest (cached by size class).  Overwrites
     // the force buffer: callers recompute the force afterwards.
-    int tune(const float *p4, float box, int pbc, float rc2)
+    // fresh: time the candidates even when the process cache holds a
+    // choice for this class (and replace it), e.g. once the positions of a
+    // run have moved away from their initial lattice.
+    int tune(const float *p4, float box, int pbc, float rc2, bool fresh = false)
     {
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
         const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
+        choice_tuned = true;
         const auto key = std::make_tuple((int64_t)(63 - __builtin_clzll((unsigned long long)N | 1)), mdeg,
                                          order + 2 * k0 + 8 * std::min(rebuild_every, 1 << 20));
         layout_gen++;
         if (!pat::tuning_enabled()) {
             lpr = sell_ok ? PAT_IR_SLICED : 1;
             sell_kind = k0;
-            return set_layout(false, false, p4, box, pbc, rc2);
+            return set_layout(0, false, p4, box, pbc, rc2);
         }
-        {
+        if (!fresh) {
             std::lock_guard<std::mutex> g(g_lpr_mutex);
             auto it = g_lpr_cache.find(key);
             if (it != g_lpr_cache.end() && (it->second.lanes != PAT_IR_SLICED || sell_ok)) {
                 lpr = it->second.lanes;
                 sell_kind = it->second.kind;
-                return set_layout(it->second.by_class, it->second.kind >= 6, p4, box, pbc, rc2);
+                return set_layout(it->second.slice_order, it->second.kind >= 6, p4, box, pbc, rc2);
             }
         }
         PAT_TRY(check_box(p4, box, pbc, stream));
-        PAT_TRY(set_layout(false, true, p4, box, pbc, rc2));
+        PAT_TRY(set_layout(0, true, p4, box, pbc, rc2));
         hipEvent_t a, b;
         PAT_HIP(hipEventCreate(&a));
         PAT_HIP(hipEventCreate(&b));
         struct Cand {
-            int lanes, kind;
-            bool by_class;
+            int lanes, kind, slice_order;
         };
-        std::vector<Cand> cands = {{1, k0, false}, {4, k0, false}, {8, k0, false}, {16, k0, false}, {32, k0, false}};
+        std::vector<Cand> cands = {{1, k0, 0}, {4, k0, 0}, {8, k0, 0}, {16, k0, 0}, {32, k0, 0}};
         if (sell_ok)
-            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, false});
+            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, 0});
         if (halo_ok)
-            for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, false});
+            for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, 0});
         // a layout rebuilt with the neighbour list costs its build time over
         // rebuild_every force evaluations
         float halo_share = 0.0f;  // per kReps launches
@@ -443,12 +480,14 @@ struct IrState {
             }
         }
         PAT_TRY(time_range(0, cands.size()));
-        if (sell_ok && pbc && sentinel()) {  // the sliced forms over the class-ordered layout
-            PAT_TRY(set_layout(true, true, p4, box, pbc, rc2));
+        // the sliced forms over the class-ordered and the spatially ordered
+        // layouts (the halo copy rebuilt for each; its share re-measured)
+        for (int so = 1; so <= 2 && sell_ok && pbc && sentinel(); so++) {
+            PAT_TRY(set_layout(so, true, p4, box, pbc, rc2));
             const size_t c0 = cands.size();
-            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, true});
+            for (int k = k0; k < k0 + 3; k++) cands.push_back({PAT_IR_SLICED, k, so});
             if (halo_ok)
-                for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, true});
+                for (int k = 6; k < 8; k++) cands.push_back({PAT_IR_SLICED, k, so});
             t.resize(cands.size(), 1e30f);
             PAT_TRY(time_range(c0, cands.size()));
         }
@@ -462,14 +501,14 @@ struct IrState {
                     (long long)N, 1000.0f * halo_share / 3, rebuild_every);
         if (pat::tune_log())
             for (size_t c = 0; c < cands.size(); c++)
-                fprintf(stderr, "[tune lj N=%lld] lanes %d kind %d class-order %d: %.2f us%s\n", (long long)N,
-                        cands[c].lanes, cands[c].kind, (int)cands[c].by_class, 1000.0f * t[c] / 3,
+                fprintf(stderr, "[tune lj N=%lld] lanes %d kind %d slice-order %d: %.2f us%s\n", (long long)N,
+                        cands[c].lanes, cands[c].kind, cands[c].slice_order, 1000.0f * t[c] / 3,
                         c == bi ? "  <- chosen" : "");
         lpr = cands[bi].lanes;
         sell_kind = lpr == PAT_IR_SLICED ? cands[bi].kind : k0;
-        PAT_TRY(set_layout(cands[bi].by_class, lpr == PAT_IR_SLICED && sell_kind >= 6, p4, box, pbc, rc2));
+        PAT_TRY(set_layout(cands[bi].slice_order, lpr == PAT_IR_SLICED && sell_kind >= 6, p4, box, pbc, rc2));
         std::lock_guard<std::mutex> g(g_lpr_mutex);
-        g_lpr_cache[key] = KernelChoice{lpr, sell_kind, cands[bi].by_class};
+        g_lpr_cache[key] = KernelChoice{lpr, sell_kind, cands[bi].slice_order};
         return PAT_OK;
     }
 };
@@ -556,7 +595,10 @@ struct pat_md {
         st.E = total / 2;
         PAT_TRY(st.build_sell(st.pos[c], box, pbc, rc2));
         rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        rebuilds++;
+        if (rebuilds++ >= 1 && st.choice_tuned && !st.retuned) {
+            st.retuned = true;
+            PAT_TRY(st.tune(st.pos[c], box, pbc, rc2, true));
+        }
         return PAT_OK;
     }
 
@@ -949,6 +991,7 @@ int pat_md_set_variant(pat_md *h, int variant, int lanes_per_row)
     PAT_TRY(check_sliced(h->st, variant, lanes_per_row));
     h->st.variant = variant;
     h->st.layout_gen++;
+    h->st.choice_tuned = false;
     if (lanes_per_row) h->st.lpr = lanes_per_row;
     else if (variant == PAT_IR_CSR_TREE) h->st.lpr = default_lpr(h->st.N, h->st.E);
     else PAT_TRY(h->st.tune(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
@@ -966,16 +1009,39 @@ int pat_md_set_sliced_form(pat_md *h, int form)
     PAT_TRY(check_sliced(h->st, PAT_IR_CSR, PAT_IR_SLICED));
     if (form >= 3) {
         PAT_REQUIRE(h->st.sentinel(), PAT_ERR_UNSUPPORTED, "pat_md_set_sliced_form: the halo forms need N < 2^28 - 1");
-        PAT_TRY(h->st.set_layout(h->st.want_perm, true, h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+        PAT_TRY(h->st.set_layout(h->st.want_order, true, h->st.pos[h->cur], h->box, h->pbc, h->rc2));
         PAT_REQUIRE(h->st.halo_ok, PAT_ERR_UNSUPPORTED, "pat_md_set_sliced_form: the list does not fit the halo form");
     }
     h->st.variant = PAT_IR_CSR;
     h->st.lpr = PAT_IR_SLICED;
+    h->st.choice_tuned = false;
     h->st.sell_kind = form >= 3 ? form + 3 : (h->st.sentinel() ? 3 : 0) + form;
     h->st.layout_gen++;
     h->drop_graphs();
     PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
+    return PAT_OK;
+}
+
+int pat_md_set_slice_order(pat_md *h, int order)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_set_slice_order: null handle");
+    PAT_REQUIRE(order >= 0 && order <= 2, PAT_ERR_INVALID, "pat_md_set_slice_order: order %d not in 0..2", order);
+    PAT_REQUIRE(h->st.sell_ok && h->st.sentinel() && h->pbc, PAT_ERR_UNSUPPORTED,
+                "pat_md_set_slice_order: needs the sliced sentinel layout of a periodic box");
+    h->st.choice_tuned = false;
+    PAT_TRY(h->st.set_layout(order, h->st.want_halo, h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    h->st.layout_gen++;
+    h->drop_graphs();
+    PAT_TRY(h->st.force_eval(h->st.pos[h->cur], h->box, h->pbc, h->rc2));
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    return PAT_OK;
+}
+
+int pat_md_get_slice_order(pat_md *h, int *order)
+{
+    PAT_REQUIRE(h && order, PAT_ERR_INVALID, "pat_md_get_slice_order: null argument");
+    *order = h->st.order_on;
     return PAT_OK;
 }
 
@@ -1198,8 +1264,11 @@ struct pat_mdb {
         PAT_HIP(hipStreamSynchronize(md.st.stream));
         md.st.E = entries / 2;
         PAT_TRY(md.st.build_sell(md.st.pos[c], 1.0f, 2, md.rc2));
-        md.rebuilds++;
         md.rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (md.rebuilds++ >= 1 && md.st.choice_tuned && !md.st.retuned) {
+            md.st.retuned = true;
+            PAT_TRY(md.st.tune(md.st.pos[c], 1.0f, 2, md.rc2, true));
+        }
         return PAT_OK;
     }
 
@@ -1336,6 +1405,7 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
         rc = m.st.upload3(m.st.vel, vel ? vel : zero.data(), 12);
     }
     m.st.rebuild_every = every;
+    m.st.sysoff_h.assign(offsets, offsets + nsys + 1);
     if (rc == PAT_OK) rc = h->rebuild_all(0);
     if (rc == PAT_OK) rc = m.st.tune(m.st.pos[0], 1.0f, 2, m.rc2);
     if (rc == PAT_OK)
@@ -1369,6 +1439,7 @@ int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_set_lanes: null handle");
     PAT_TRY(check_variant(PAT_IR_CSR, lanes_per_row));
     PAT_TRY(check_sliced(h->md.st, PAT_IR_CSR, lanes_per_row));
+    h->md.st.choice_tuned = false;
     if (lanes_per_row) h->md.st.lpr = lanes_per_row;
     else PAT_TRY(h->md.st.tune(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2));
     h->md.st.layout_gen++;
@@ -1378,6 +1449,23 @@ int pat_mdb_set_lanes(pat_mdb *h, int lanes_per_row)
     PAT_TRY(h->md.st.csr_force(h->md.st.pos[h->md.cur], 1.0f, 2, h->md.rc2, 0.f, kForceOnly, nullptr,
                                h->md.st.stream));
     PAT_HIP(hipStreamSynchronize(h->md.st.stream));
+    return PAT_OK;
+}
+
+int pat_mdb_set_slice_order(pat_mdb *h, int order)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_set_slice_order: null handle");
+    PAT_REQUIRE(order >= 0 && order <= 2, PAT_ERR_INVALID, "pat_mdb_set_slice_order: order %d not in 0..2", order);
+    IrState &st = h->md.st;
+    PAT_REQUIRE(st.sell_ok && st.sentinel(), PAT_ERR_UNSUPPORTED,
+                "pat_mdb_set_slice_order: needs the sliced sentinel layout");
+    st.choice_tuned = false;
+    PAT_TRY(st.set_layout(order, st.want_halo, st.pos[h->md.cur], 1.0f, 2, h->md.rc2));
+    st.layout_gen++;
+    h->drop_seg_graphs();
+    PAT_TRY(st.check_box(st.pos[h->md.cur], 1.0f, 2, st.stream));
+    PAT_TRY(st.csr_force(st.pos[h->md.cur], 1.0f, 2, h->md.rc2, 0.f, kForceOnly, nullptr, st.stream));
+    PAT_HIP(hipStreamSynchronize(st.stream));
     return PAT_OK;
 }
 

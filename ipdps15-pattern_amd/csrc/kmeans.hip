@@ -414,14 +414,12 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 
     const int64_t ntiles = (a.N + 31) / 32;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; tile < ntiles;
-         tile += nw) {
-        const int64_t p = tile * 32 + col;
+    // this lane's half of tile t's point row (dimensions 16s + 8*half + e);
+    // dead lanes read a valid row and keep zeros
+    auto load_half_row = [&](int64_t t, float (&xo)[8 * KS]) {
+        const int64_t p = t * 32 + col;
         const bool live = p < a.N;
-        const float *xp = a.x + (live ? p : a.N - 1) * D;  // dead lanes read a valid row
-
-        // this lane's half of the row (dimensions 16s + 8*half + e)
-        float xo[8 * KS];
+        const float *xp = a.x + (live ? p : a.N - 1) * D;
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int d0 = 16 * s + 8 * half;
@@ -439,6 +437,29 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
 #pragma unroll
                 for (int e = 0; e < 8; e++) xo[8 * s + e] = (live && d0 + e < D) ? xp[d0 + e] : 0.0f;
             }
+        }
+    };
+    // With the centre fragments in LDS the next tile's rows are loaded
+    // while this tile computes (its loads return during the MFMAs); the
+    // global-fragment form loads them in place, since vector loads complete
+    // in order and the fragment loads would wait behind them, and so do rows
+    // of more than 64 dimensions (the extra registers would spill).
+    constexpr bool PF = !GC && KS <= 4;
+    const int64_t tile0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    float xnext[8 * KS];
+    if (PF && tile0 < ntiles) load_half_row(tile0, xnext);
+    for (int64_t tile = tile0; tile < ntiles; tile += nw) {
+        const int64_t p = tile * 32 + col;
+        const bool live = p < a.N;
+        const float *xp = a.x + (live ? p : a.N - 1) * D;  // dead lanes read a valid row
+
+        float xo[8 * KS];
+        if (!PF) {
+            load_half_row(tile, xo);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8 * KS; i++) xo[i] = xnext[i];
+            if (tile + nw < ntiles) load_half_row(tile + nw, xnext);
         }
         // B operand: point p, K elements [16s + 8*half, +8) of
         // [x, 1, 1, 1, 0...] or [xh, xh, xl, 1, 1, 1, 0...]

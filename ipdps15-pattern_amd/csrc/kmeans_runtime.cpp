@@ -11,9 +11,13 @@
 // assignment+accumulation kernel and one update kernel on the handle's
 // stream.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 
 #include "pat_common.hpp"
 
@@ -78,6 +82,65 @@ int amalloc(void **p, size_t b)
     return PAT_OK;
 }
 
+// Terms of the MFMA path (1 or 3) chosen per (log2 N, D, K) class for the
+// process; both give the same assignments bit for bit, so the choice only
+// changes speed.
+std::map<std::tuple<int, int, int>, int> g_terms_cache;
+std::mutex g_terms_mutex;
+
+// When the three-term rows fit in LDS the one-term form fits too: it runs
+// fewer than half the MFMAs and fragment reads but has a wider margin, so
+// more points may need rescoring -- which wins depends on the data.  Times
+// one assignment pass of each (interleaved rounds, best of each) on the
+// current centres, keeps the faster, and clears what the passes
+// accumulated.
+int tune_terms(pat_kmeans *h)
+{
+    auto &D = h->d;
+    const auto key = std::make_tuple(63 - __builtin_clzll((unsigned long long)D.N | 1), D.D, D.K);
+    int nt = 0;
+    {
+        std::lock_guard<std::mutex> g(g_terms_mutex);
+        auto it = g_terms_cache.find(key);
+        if (it != g_terms_cache.end()) nt = it->second;
+    }
+    if (!nt) {
+        hipEvent_t e0, e1;
+        PAT_HIP(hipEventCreate(&e0));
+        PAT_HIP(hipEventCreate(&e1));
+        const int cand[2] = {3, 1};
+        float best[2] = {1e30f, 1e30f};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 2; c++) {
+                D.NT = cand[c];
+                PAT_TRY(pat::kmeans_finalize(D, 0, h->stream));
+                PAT_HIP(hipEventRecord(e0, h->stream));
+                PAT_TRY(pat::kmeans_assign_mfma(D, h->stream));
+                PAT_HIP(hipEventRecord(e1, h->stream));
+                PAT_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                PAT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best[c] = std::min(best[c], ms);
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        nt = best[1] < best[0] ? 1 : 3;
+        if (pat::tune_log())
+            fprintf(stderr, "[tune kmeans N=%lld D=%d K=%d] three terms %.3f ms, one term %.3f ms -> %d\n",
+                    (long long)D.N, D.D, D.K, best[0], best[1], nt);
+        std::lock_guard<std::mutex> g(g_terms_mutex);
+        g_terms_cache[key] = nt;
+    }
+    D.NT = nt;
+    PAT_TRY(pat::kmeans_zero(D.gsum, (int64_t)D.K * D.D, h->stream));
+    PAT_TRY(pat::kmeans_zero(D.gcnt, D.K, h->stream));
+    PAT_TRY(pat::kmeans_zero(D.gdsum, D.K, h->stream));
+    PAT_TRY(pat::kmeans_zero(D.rescored, 1, h->stream));
+    PAT_TRY(pat::kmeans_finalize(D, 0, h->stream));
+    PAT_HIP(hipStreamSynchronize(h->stream));
+    return PAT_OK;
+}
+
 int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int x_on_device)
 {
     PAT_TRY(pat::kmeans_prepare_kernels());
@@ -136,6 +199,7 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     PAT_TRY(pat::kmeans_zero(D.rescored, 1, h->stream));
     PAT_TRY(pat::kmeans_finalize(D, 0, h->stream));
     PAT_HIP(hipStreamSynchronize(h->stream));
+    if (mode == PAT_KM_MFMA && !D.gc && D.NT == 3 && pat::tuning_enabled()) PAT_TRY(tune_terms(h));
     return PAT_OK;
 }
 
@@ -183,6 +247,30 @@ int pat_kmeans_set_mode(pat_kmeans *h, int mode)
                     "pat_kmeans_set_mode: shape too large for the MFMA path");
     h->mode = mode;
     h->drop_graphs();
+    return PAT_OK;
+}
+
+int pat_kmeans_set_terms(pat_kmeans *h, int terms)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_kmeans_set_terms: null handle");
+    PAT_REQUIRE(terms == 1 || terms == 3, PAT_ERR_INVALID, "pat_kmeans_set_terms: terms %d not 1 or 3", terms);
+    auto &D = h->d;
+    const int fits = pat::kmeans_nt_for(D.K, D.D);  // 3: both forms fit, 1: only one term, 0: neither
+    PAT_REQUIRE(!D.gc && (fits == 3 || (fits == 1 && terms == 1)), PAT_ERR_UNSUPPORTED,
+                "pat_kmeans_set_terms: %d-term rows for K = %d, D = %d do not fit in LDS", terms, D.K, D.D);
+    if (D.NT != terms) {
+        D.NT = terms;
+        PAT_TRY(pat::kmeans_finalize(D, 0, h->stream));
+        PAT_HIP(hipStreamSynchronize(h->stream));
+        h->drop_graphs();
+    }
+    return PAT_OK;
+}
+
+int pat_kmeans_get_terms(pat_kmeans *h, int *terms)
+{
+    PAT_REQUIRE(h && terms, PAT_ERR_INVALID, "pat_kmeans_get_terms: null argument");
+    *terms = h->d.NT;
     return PAT_OK;
 }
 

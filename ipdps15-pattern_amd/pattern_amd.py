@@ -152,6 +152,9 @@ SIGNATURES = {
     "pat_gr_get_output": (c_int, [c_vp, c_intp, c_intp, c_vp, c_vp]),
     "pat_gr_destroy": (c_int, [c_vp]),
     "pat_md_get_sliced_form": (c_int, [c_vp, c_intp]),
+    "pat_md_set_slice_order": (c_int, [c_vp, c_int]),
+    "pat_md_get_slice_order": (c_int, [c_vp, c_intp]),
+    "pat_mdb_set_slice_order": (c_int, [c_vp, c_int]),
     "pat_mdb_create": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_double, c_int, c_double,
                                ctypes.POINTER(c_vp)]),
     "pat_mdb_set_lanes": (c_int, [c_vp, c_int]),
@@ -169,6 +172,8 @@ SIGNATURES = {
     "pat_kmeans_fixed_shift": (c_int, [c_i64, c_float, c_intp]),
     "pat_kmeans_create": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, ctypes.POINTER(c_vp)]),
     "pat_kmeans_set_mode": (c_int, [c_vp, c_int]),
+    "pat_kmeans_set_terms": (c_int, [c_vp, c_int]),
+    "pat_kmeans_get_terms": (c_int, [c_vp, c_intp]),
     "pat_kmeans_set_centroids": (c_int, [c_vp, c_vp]),
     "pat_kmeans_run": (c_int, [c_vp, c_int]),
     "pat_kmeans_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
@@ -550,6 +555,16 @@ class MD:
         call("pat_md_get_sliced_form", self.h, ctypes.byref(f))
         return f.value
 
+    def set_slice_order(self, order):
+        """Rows of the sliced layout in node (0), row-class (1) or spatial
+        (2) order (same bits for all)."""
+        call("pat_md_set_slice_order", self.h, order)
+
+    def slice_order(self):
+        o = c_int(0)
+        call("pat_md_get_slice_order", self.h, ctypes.byref(o))
+        return o.value
+
     def variant(self):
         """(variant, lanes per row in use, row order)."""
         a, b, c = c_int(0), c_int(0), c_int(0)
@@ -636,6 +651,9 @@ class MDBatch:
         call("pat_mdb_run", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
         return tot.value, k
 
+    def set_slice_order(self, order):
+        call("pat_mdb_set_slice_order", self.h, order)
+
     def get(self, sys):
         n = self.sizes[sys]
         p, v, f = (np.empty((n, 3), dtype=np.float32) for _ in range(3))
@@ -709,6 +727,15 @@ class KMeans:
 
     def set_mode(self, mode):
         call("pat_kmeans_set_mode", self.h, mode)
+
+    def set_terms(self, terms):
+        """BF16 terms of the MFMA distance values (1 or 3; same bits)."""
+        call("pat_kmeans_set_terms", self.h, terms)
+
+    def terms(self):
+        t = c_int(0)
+        call("pat_kmeans_get_terms", self.h, ctypes.byref(t))
+        return t.value
 
     def set_centroids(self, c):
         c = np.ascontiguousarray(c, dtype=np.float32)

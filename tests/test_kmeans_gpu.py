@@ -58,8 +58,10 @@ def adversarial(kind, rng):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["duplicates", "lattice", "large", "close", "close_wide"])
-def test_mfma_adversarial_bitexact(pa, orc, gpu, kind):
+@pytest.mark.parametrize("kind,terms", [("duplicates", 3), ("lattice", 3), ("large", 3), ("close", 3),
+                                        ("duplicates", 1), ("lattice", 1), ("large", 1), ("close", 1),
+                                        ("close_wide", 3)])
+def test_mfma_adversarial_bitexact(pa, orc, gpu, kind, terms):
     if kind == "close_wide":  # a tight cluster and K x D beyond LDS (rows from global memory)
         x, k = (np.random.default_rng(8).normal(size=(3000, 128)) * 1e-3).astype(np.float32), 1024
     else:
@@ -67,6 +69,8 @@ def test_mfma_adversarial_bitexact(pa, orc, gpu, kind):
     out = []
     for mode in (0, 1):
         km = pa.KMeans(x, k, mode=mode)
+        if mode == 1 and kind != "close_wide":
+            km.set_terms(terms)
         km.run(3)
         out.append(km.get() + km.get_object()[:3])
         km.close()
@@ -74,6 +78,26 @@ def test_mfma_adversarial_bitexact(pa, orc, gpu, kind):
         assert np.array_equal(p, m)
     ref = list(oracle_iters(orc, pa, x, k, 3))[-1]
     assert np.array_equal(out[1][1], ref[1])
+
+
+@pytest.mark.parametrize("terms", [1, 3])
+def test_mfma_terms_bitexact(pa, orc, gpu, terms):
+    """Either number of BF16 terms (one: a wider margin, more rescoring)
+    gives the oracle's assignments and centres bit for bit, also when the
+    form changes between runs."""
+    x = pa.gen_gmm(50000, 32, 64, sigma=1.0, lo=-10, hi=10, seed=13)
+    km = pa.KMeans(x, 64, mode=1)
+    km.set_terms(terms)
+    assert km.terms() == terms
+    km.run(3)
+    km.set_terms(4 - terms)
+    km.run(2)
+    c, a, md = km.get()
+    km.close()
+    *_, last = oracle_iters(orc, pa, x, 64, 5)
+    assert np.array_equal(a, last[1]) and np.array_equal(md, last[2])
+    c4 = list(oracle_iters(orc, pa, x, 64, 6))[-1][0]
+    assert np.array_equal(c, c4)
 
 
 def test_lloyd_20_iterations_midsize(pa, orc, gpu):

@@ -221,16 +221,19 @@ def test_moldyn_workload(pa, orc, gpu, order):
 
 @pytest.mark.parametrize("cells", [6, 16])
 @pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
-def test_sliced_forms_vs_oracle(pa, orc, gpu, cells, form):
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_sliced_forms_vs_oracle(pa, orc, gpu, cells, form, order):
     """Each loop form on the handle's sentinel layout (byte-offset columns, the
     minimum image taken per wavefront only on axes where a row lies within a
     cutoff of a face; forms 3 and 4 stage each block's neighbour positions in
-    LDS) gives the serial scatter's bits over a trajectory."""
+    LDS), with the slices' rows in node, row-class or spatial order, gives
+    the serial scatter's bits over a trajectory."""
     pos, box, edges = pa.moldyn_system(cells, seed=33)
     vel = pa.gen_velocities(len(pos), 1.44, seed=5)
     md = pa.MD(pos, vel, edges, box, 2.5, 0.005)
     md.set_sliced_form(form)
-    assert md.variant()[1] == pa.PAT_IR_SLICED
+    md.set_slice_order(order)
+    assert md.variant()[1] == pa.PAT_IR_SLICED and md.slice_order() == order
     md.run(7)
     md.run(8)
     p, v, f = md.get()
@@ -240,7 +243,8 @@ def test_sliced_forms_vs_oracle(pa, orc, gpu, cells, form):
 
 
 @pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
-def test_minimum_image_kept_for_positions_outside_the_box(pa, orc, gpu, form):
+@pytest.mark.parametrize("order", [0, 2])
+def test_minimum_image_kept_for_positions_outside_the_box(pa, orc, gpu, form, order):
     """Rows 0..63 (one wavefront) lie well inside the box; node 64 starts at
     x = -2, outside it.  Pair (0, 64) is 9 apart in x unwrapped and 1 apart
     through the periodic image: its force is only right if the handle sees
@@ -255,6 +259,7 @@ def test_minimum_image_kept_for_positions_outside_the_box(pa, orc, gpu, form):
     edges = np.array([[0, 64], [65, 80]], np.int32)
     md = pa.MD(pos, None, edges, L, 2.5, 0.001)
     md.set_sliced_form(form)
+    md.set_slice_order(order)
     _, _, f0 = md.get()
     rf0, _ = orc.lj_force(pos, edges, L, 2.5)
     assert np.array_equal(f0, rf0) and f0[0, 0] != 0.0

@@ -31,6 +31,22 @@ def test_batch_equals_alone(pa, gpu, lanes):
     b.close()
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_batch_slice_orders_equal_alone(pa, gpu, order):
+    """Row-class and spatial slice orders over the batch (the spatial sort
+    keeps each system's rows together) change no bits."""
+    systems = [pa.batch_systems(1, m, m, seed=20 + m)[0] for m in (9, 8, 11)]
+    b = pa.MDBatch(systems)
+    b.set_slice_order(order)
+    b.run(45)
+    for s, (pos, vel, box) in enumerate(systems):
+        got = b.get(s)
+        ref = alone(pa, pos, vel, box, 45, 1)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r)
+    b.close()
+
+
 def test_batch_workload_64_systems(pa, gpu):
     """64 systems of 8^3-32^3 FCC cells (log-uniform), T = 1.44, 100 steps
     with rebuilds every 20, as one batched launch per step."""

@@ -75,6 +75,27 @@ def test_binned_md_vs_oracle(pa, orc, gpu, lanes):
     assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
 
 
+@pytest.mark.parametrize("form", [0, 2, 3])
+@pytest.mark.parametrize("order", [1, 2])
+def test_binned_md_slice_orders(pa, orc, gpu, order, form):
+    """Sliced layout with its rows in row-class or spatial (Morton) slice
+    order, the latter recomputed at each rebuild: bit-identical to the
+    oracle across the rebuilds at steps 20 and 40."""
+    pos, box = pa.gen_fcc(9, 9, 9, rho=0.8442, jitter=0.05, seed=6)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=7)
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    md.set_sliced_form(form)
+    md.set_slice_order(order)
+    assert md.slice_order() == order
+    md.run(23)
+    md.run(22)
+    assert md.slice_order() == order and md.sliced_form() == form
+    p, v, f = md.get()
+    md.close()
+    rp, rv, rf, _ = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 20, 0.005, 45)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
+
+
 def test_minimd_workload(pa, orc, gpu):
     """miniMD: FCC 64^3 cells (1,048,576 atoms), density 0.8442, Maxwell
     velocities at T = 1.44, cutoff 2.5, skin 0.3, binned rebuild every 20
