@@ -456,10 +456,13 @@ struct IrState {
         };
         // times candidates [c0, c1): rounds that time every one in turn,
         // each keeping its best round
+        // (a candidate over 2.5x the best so far after its first round is
+        // not timed again)
         auto time_range = [&](size_t c0, size_t c1) -> int {
             constexpr int kRounds = 5, kReps = 3;
             for (int r = 0; r < kRounds; r++)
                 for (size_t c = c0; c < c1; c++) {
+                    if (r > 0 && t[c] > 2.5f * *std::min_element(t.begin(), t.end())) continue;
                     PAT_HIP(hipEventRecord(a, stream));
                     for (int i = 0; i < kReps; i++) PAT_TRY(launch(c));
                     PAT_HIP(hipEventRecord(b, stream));
