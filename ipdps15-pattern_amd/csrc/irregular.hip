@@ -136,22 +136,25 @@ __device__ __forceinline__ unsigned bitonic_step(unsigned v, int lane, int j, bo
 
 // One wavefront per row of at most 128 columns: bitonic sort of the row
 // held two values per lane (entry lane and entry 64 + lane; padding sorts
-// last), writing the sorted row from tmp into cols.  Longer rows take the
-// rank sort.  Columns of a row are distinct, so any correct sort gives the
-// rank sort's order.
+// last), writing the sorted row into cols.  Longer rows take the rank sort.
+// Columns of a row are distinct, so any correct sort gives the rank sort's
+// order.  The unsorted row is tmp[rowptr[row] ...] (slot == 0) or
+// tmp[row*slot ...] (rows of at most slot entries; longer ones are skipped).
 __global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowptr, int64_t N,
-                                                    const int *__restrict__ tmp, int *__restrict__ cols)
+                                                    const int *__restrict__ tmp, int *__restrict__ cols, int slot)
 {
     const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x % kWave;
     if (row >= N) return;
     const int b = rowptr[row], e = rowptr[row + 1], n = e - b;
+    if (slot && n > slot) return;  // never produced by a slot pass
+    const int *__restrict__ src = tmp + (slot ? row * slot : (int64_t)b);
     if (n > 2 * kWave) {  // uniform per wavefront
-        for (int k = b + lane; k < e; k += kWave) {
-            const int v = tmp[k];
+        for (int k = lane; k < n; k += kWave) {
+            const int v = src[k];
             int rank = 0;
-            for (int m = b; m < e; m++) {
-                const int u = tmp[m];
+            for (int m = 0; m < n; m++) {
+                const int u = src[m];
                 rank += (u < v) || (u == v && m < k);
             }
             cols[b + rank] = v;
@@ -159,8 +162,8 @@ __global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowp
         return;
     }
     // values as unsigned with padding 0xffffffff (columns are >= 0)
-    unsigned a0 = lane < n ? (unsigned)tmp[b + lane] : 0xffffffffu;
-    unsigned a1 = kWave + lane < n ? (unsigned)tmp[b + kWave + lane] : 0xffffffffu;
+    unsigned a0 = lane < n ? (unsigned)src[lane] : 0xffffffffu;
+    unsigned a1 = kWave + lane < n ? (unsigned)src[kWave + lane] : 0xffffffffu;
     // sort the 128-element sequence s[i] (i < 64: a0 of lane i; else a1 of
     // lane i - 64): stages k = 2..128, distances j = k/2..1; direction of
     // element i is ascending when (i & k) == 0
@@ -260,18 +263,31 @@ __device__ __forceinline__ bool nb_within(float4 a, float4 b, const BinGeom &g)
 // passes over the bin's atoms).
 constexpr int kNbSlots = 12;
 
+// Entries per atom slot of the one-pass neighbour build (rows up to this
+// long take the register bitonic sort).
+constexpr int kNbSlot = 128;
+
+enum NbMode {
+    kNbCount = 0,  // deg[i] = row length
+    kNbFill = 1,   // row into tmp[rowptr[i] ...]
+    kNbSlotFill = 2,  // deg[i] = row length and the row into tmp[i*kNbSlot ...] (at most kNbSlot
+                      // entries; *overflow = 1 when a row is longer)
+};
+
 // One wavefront per bin q: the atoms of the 27 bins around q (periodic
 // wrap; nb >= 3 keeps them distinct) are laid out as one flat candidate
 // range, gathered into registers slice by slice, and every atom i of q is
-// tested against them, all lanes at once.  !FILL: deg[i] = row length;
-// FILL: the row goes to tmp[rowptr[i] ...], compacted by ballot (order
-// within a row is left to the row sort).
-template <bool FILL>
+// tested against them, all lanes at once.  Rows are written compacted by
+// ballot (order within a row is left to the row sort), never past their
+// end (row pointer or slot).
+template <int MODE>
 __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, const NbSys *__restrict__ sys,
                                                 int nsys, int ncell, float cut2, const int *__restrict__ cstart,
                                                 const int *__restrict__ catoms, int *__restrict__ deg,
-                                                const int *__restrict__ rowptr, int *__restrict__ tmp)
+                                                const int *__restrict__ rowptr, int *__restrict__ tmp,
+                                                int *__restrict__ overflow)
 {
+    constexpr bool FILL = MODE != kNbCount;
     const int q = blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (q >= ncell) return;  // whole wavefronts
@@ -332,9 +348,12 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                 const int i = catoms[a];
                 const float4 pi = pos[i];
                 int w = 0, wend = 0;
-                if (FILL) {
+                if (MODE == kNbFill) {
                     w = rowptr[i] + __shfl(acc, a - a0, kWave);
                     wend = rowptr[i + 1];  // the counting pass sized the row; never write past it
+                } else if (MODE == kNbSlotFill) {
+                    w = i * kNbSlot + __shfl(acc, a - a0, kWave);
+                    wend = (i + 1) * kNbSlot;
                 }
                 int found = 0;
 #pragma unroll
@@ -350,7 +369,8 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                 if (lane == a - a0) acc += found;
             }
         }
-        if (!FILL && lane < na) deg[catoms[a0 + lane]] = acc;
+        if (MODE != kNbFill && lane < na) deg[catoms[a0 + lane]] = acc;
+        if (MODE == kNbSlotFill && lane < na && acc > kNbSlot) *overflow = 1;
     }
 }
 
@@ -1511,7 +1531,7 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *ro
         if (order == 0)
             hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, tmpe, cols);
         else
-            hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols);
+            hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols, 0);
         PAT_LAUNCH_CHECK();
     }
     return PAT_OK;
@@ -1910,12 +1930,15 @@ struct NbWork {
     int *cell = nullptr, *cstart = nullptr, *cursor = nullptr, *atmp = nullptr, *catoms = nullptr, *deg = nullptr,
         *ws = nullptr, *coltmp = nullptr;
     NbSys *sys = nullptr;
+    int *slot = nullptr, *flag = nullptr;  // one-pass rows: kNbSlot entries per atom; overflow flag
+    int64_t slot_cap = 0;
+    bool slot_fail = false;  // a row outgrew its slot: two passes from then on
     std::vector<NbSys> sys_h;  // source of the last upload (read before the build returns)
 };
 
 static void nb_free(NbWork &w)
 {
-    for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp})
+    for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp, w.slot, w.flag})
         if (p) (void)pat::dev_free(p);
     if (w.sys) (void)pat::dev_free(w.sys);
     w = NbWork{};
@@ -2017,34 +2040,66 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
     hipLaunchKernelGGL(nb_scatter_atoms, dim3(gN), dim3(256), 0, st, w->cell, (int)N, w->cursor, w->atmp);
     PAT_LAUNCH_CHECK();
     hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((ncell * kWave + 255) / 256)), dim3(256), 0, st, w->cstart, ncell,
-                       w->atmp, w->catoms);
+                       w->atmp, w->catoms, 0);
     PAT_LAUNCH_CHECK();
-    // row lengths, then row pointers
-    hipLaunchKernelGGL(nb_cells<false>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys, (int)ncell, cut2,
-                       (const int *)w->cstart, (const int *)w->catoms, w->deg, (const int *)nullptr, (int *)nullptr);
+    // One pass: row lengths, and each row into a fixed slot of kNbSlot
+    // entries per atom; the rows are then sorted straight out of the slots.
+    // A row longer than its slot sets a flag, and that build (and every
+    // later one on this workspace) fills the rows again at their final
+    // offsets instead.  Without slots (or for more than 2^31 slot entries):
+    // a counting pass first.
+    const bool slots = !w->slot_fail && N * kNbSlot < (1LL << 31);
+    if (slots) {
+        if (w->slot_cap < N * kNbSlot) {
+            if (w->slot) (void)pat::dev_free(w->slot);
+            w->slot = nullptr;
+            w->slot_cap = 0;
+            PAT_TRY(nb_alloc(&w->slot, N * kNbSlot));
+            w->slot_cap = N * kNbSlot;
+        }
+        if (!w->flag) PAT_TRY(nb_alloc(&w->flag, 1));
+        PAT_HIP(hipMemsetAsync(w->flag, 0, sizeof(int), st));
+        hipLaunchKernelGGL(nb_cells<kNbSlotFill>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys, (int)ncell,
+                           cut2, (const int *)w->cstart, (const int *)w->catoms, w->deg, (const int *)nullptr, w->slot,
+                           w->flag);
+    } else {
+        hipLaunchKernelGGL(nb_cells<kNbCount>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys,
+                           (int)ncell, cut2, (const int *)w->cstart, (const int *)w->catoms, w->deg,
+                           (const int *)nullptr, (int *)nullptr, (int *)nullptr);
+    }
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
     PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
-    int tot = 0;
+    int tot = 0, over = 0;
     PAT_HIP(hipMemcpyAsync(&tot, rowptr + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (slots) PAT_HIP(hipMemcpyAsync(&over, w->flag, sizeof(int), hipMemcpyDeviceToHost, st));
     PAT_HIP(hipStreamSynchronize(st));
     PAT_REQUIRE(tot >= 0, PAT_ERR_INVALID, "neighbour build: pair count overflows int32");
-    if (tot > *cap || tot > w->cap) {
+    if (over) w->slot_fail = true;
+    const bool from_slots = slots && !over;
+    if (tot > *cap) {
         const int64_t want = (int64_t)(tot * 1.25) + 1024;
         if (*cols) (void)pat::dev_free(*cols);
-        if (w->coltmp) (void)pat::dev_free(w->coltmp);
         *cols = nullptr;
-        w->coltmp = nullptr;
         PAT_TRY(nb_alloc(cols, want));
-        PAT_TRY(nb_alloc(&w->coltmp, want));
         *cap = want;
+    }
+    if (!from_slots && tot > w->cap) {
+        const int64_t want = (int64_t)(tot * 1.25) + 1024;
+        if (w->coltmp) (void)pat::dev_free(w->coltmp);
+        w->coltmp = nullptr;
+        w->cap = 0;
+        PAT_TRY(nb_alloc(&w->coltmp, want));
         w->cap = want;
     }
-    hipLaunchKernelGGL(nb_cells<true>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys, (int)ncell, cut2,
-                       (const int *)w->cstart, (const int *)w->catoms, w->deg, (const int *)rowptr, w->coltmp);
-    PAT_LAUNCH_CHECK();
+    if (!from_slots) {
+        hipLaunchKernelGGL(nb_cells<kNbFill>, dim3(gC), dim3(256), 0, st, pos, (const NbSys *)w->sys, nsys,
+                           (int)ncell, cut2, (const int *)w->cstart, (const int *)w->catoms, w->deg,
+                           (const int *)rowptr, w->coltmp, (int *)nullptr);
+        PAT_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((N * kWave + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
-                       N, w->coltmp, *cols);
+                       N, from_slots ? (const int *)w->slot : (const int *)w->coltmp, *cols, from_slots ? kNbSlot : 0);
     PAT_LAUNCH_CHECK();
     *total = tot;
     return PAT_OK;

@@ -88,6 +88,9 @@ struct KernelChoice {
     int slice_order;  // sliced rows in node (0), row-class (1) or spatial (2) order
 };
 std::map<std::tuple<int64_t, int, int>, KernelChoice> g_lpr_cache;
+// choices made on positions that have moved away from their initial lattice
+// (the re-tune of a binned handle at its first rebuild)
+std::map<std::tuple<int64_t, int, int>, KernelChoice> g_lpr_cache_moved;
 std::mutex g_lpr_mutex;
 
 // Device state shared by the IrregularRuntime handle and the MD handle.
@@ -400,11 +403,12 @@ struct IrState {
     // sliced form, and the sliced forms again over the class-ordered layout,
     // on this CSR, and keeps the fastest (cached by size class).  Overwrites
     // the force buffer: callers recompute the force afterwards.
-    // fresh: time the candidates even when the process cache holds a
-    // choice for this class (and replace it), e.g. once the positions of a
-    // run have moved away from their initial lattice.
-    int tune(const float *p4, float box, int pbc, float rc2, bool fresh = false)
+    // moved: the positions have moved away from their initial lattice (the
+    // re-tune of a binned handle at its first rebuild): the choice comes
+    // from, and goes to, a cache of its own.
+    int tune(const float *p4, float box, int pbc, float rc2, bool moved = false)
     {
+        auto &cache = moved ? g_lpr_cache_moved : g_lpr_cache;
         const int mdeg = N > 0 ? (int)((2 * E + N / 2) / N) : 0;
         const int k0 = sentinel() ? 3 : 0;  // sliced kinds for this layout
         choice_tuned = true;
@@ -416,10 +420,10 @@ struct IrState {
             sell_kind = k0;
             return set_layout(0, false, p4, box, pbc, rc2);
         }
-        if (!fresh) {
+        {
             std::lock_guard<std::mutex> g(g_lpr_mutex);
-            auto it = g_lpr_cache.find(key);
-            if (it != g_lpr_cache.end() && (it->second.lanes != PAT_IR_SLICED || sell_ok)) {
+            auto it = cache.find(key);
+            if (it != cache.end() && (it->second.lanes != PAT_IR_SLICED || sell_ok)) {
                 lpr = it->second.lanes;
                 sell_kind = it->second.kind;
                 return set_layout(it->second.slice_order, it->second.kind >= 6, p4, box, pbc, rc2);
@@ -511,7 +515,7 @@ struct IrState {
         sell_kind = lpr == PAT_IR_SLICED ? cands[bi].kind : k0;
         PAT_TRY(set_layout(cands[bi].slice_order, lpr == PAT_IR_SLICED && sell_kind >= 6, p4, box, pbc, rc2));
         std::lock_guard<std::mutex> g(g_lpr_mutex);
-        g_lpr_cache[key] = KernelChoice{lpr, sell_kind, cands[bi].slice_order};
+        cache[key] = KernelChoice{lpr, sell_kind, cands[bi].slice_order};
         return PAT_OK;
     }
 };

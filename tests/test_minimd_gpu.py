@@ -48,11 +48,20 @@ def test_binned_csr_dense_and_clustered(pa, gpu):
     g = g[~np.all((g > 2.6) & (g < 5.4), axis=1)]
     c = np.stack(np.meshgrid(*[np.arange(5)] * 3, indexing="ij"), -1).reshape(-1, 3) * 0.5 + 3.0
     pos = np.mod(np.concatenate([g, c]), L).astype(np.float32)
-    md = pa.MD(pos, None, None, L, 2.5, 0.005, skin=0.3, every=20)
+    md = pa.MD(pos, None, None, L, 2.5, 1e-5, skin=0.3, every=20)
     rp, cl = md.csr()
-    md.close()
     erp, ecl = host_csr(len(pos), pa.gen_pairs(pos, L, 2.8))
     assert np.diff(erp).max() > 128
+    assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
+    # rows longer than their one-pass slot: later rebuilds count first.  The
+    # rebuild before step 20's force uses the positions after its drift,
+    # which are the positions the run ends with.
+    md.run(20)
+    p20 = md.get()[0]
+    rp, cl = md.csr()
+    assert md.stats()[2] == 2
+    md.close()
+    erp, ecl = host_csr(len(pos), pa.gen_pairs(p20, L, 2.8))
     assert np.array_equal(rp, erp) and np.array_equal(cl, ecl)
 
 
