@@ -88,6 +88,7 @@ struct KmArgs {
     unsigned long long *gcnt;   // K
     unsigned long long *gdsum;  // K
     int use_lds_acc;        // 1: LDS reduction object, 0: global atomics directly
+    int defer_acc;          // 1: assignments only; the bucketed reduction (km_bucket_*) sums afterwards
     int rs;                 // row stride (8-byte words) of the coordinate sums: D in global memory,
                             // odd in LDS so that rows of different keys start in different banks
 };
@@ -259,9 +260,10 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
             if (a.mindist) a.mindist[p] = min_dist;
         }
         const int np = (int)min<int64_t>(kWave, a.N - base);
-        accumulate_points(
-            a, base, np, [&](int j) { return (int)__shfl((int)key, j, kWave); },
-            [&](int j) { return __shfl(min_dist, j, kWave); }, acc, cnt, dsum);
+        if (!a.defer_acc)
+            accumulate_points(
+                a, base, np, [&](int j) { return (int)__shfl((int)key, j, kWave); },
+                [&](int j) { return __shfl(min_dist, j, kWave); }, acc, cnt, dsum);
     }
     if (a.use_lds_acc) {
         __syncthreads();
@@ -337,8 +339,14 @@ constexpr float kFar = 3.0e38f;
 // from the registers holding its half rows.
 // TAIL = false: D == 16*KS and the rows are 16-byte aligned, so every
 // K-step is full and the loads need no per-element guard.
-// GC: the BF16 rows are read from global memory (they stay in L2) instead
-// of being staged in LDS -- the form for K x D too large for LDS.
+// GC: the form for K x D too large for LDS.  The workgroup's wavefronts
+// step through the centre chunks together: before each chunk the
+// workgroup copies its KB blocks of BF16 rows from global memory into one
+// LDS chunk buffer (between two barriers) and every wavefront computes its
+// own tile against it, so each copy of the rows serves 8 tiles.  Tiles are
+// taken block-uniformly (a wavefront past the last tile computes with every
+// lane dead) and the recompute path is a workgroup vote, so every wavefront
+// reaches every barrier.
 template <int KS, int KB, int NT, bool TAIL, bool GC>
 __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
@@ -352,14 +360,15 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     __bf16 *cbs = (__bf16 *)(smem + (a.use_lds_acc ? object_words(a.K, a.D) : 0));
     __shared__ float red[16];
     if (a.use_lds_acc) zero_lds_object(a, acc_o, cnt_o, dsum_o);
-    // stage the BF16 centre matrix in LDS (16-byte copies)
+    // stage the BF16 centre matrix in LDS (16-byte copies); the GC form
+    // stages one chunk at a time (stage_chunk)
     if (!GC) {
         const int n16 = K_pad * KW / 8;
         const bf16x8 *src = (const bf16x8 *)cb;
         bf16x8 *dst = (bf16x8 *)cbs;
         for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
     }
-    const __bf16 *__restrict__ cfr = GC ? cb : cbs;
+    const __bf16 *__restrict__ cfr = cbs;
     // max ||c|| over the centres (for the bound)
     float cm = 0.0f;
     for (int i = threadIdx.x; i < a.K; i += blockDim.x) cm = fmaxf(cm, cnorm[i]);
@@ -392,6 +401,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         constexpr int T = KB * NS, P = 3;
         const bf16x8 *ab = (const bf16x8 *)cfr + lane;
         auto frag = [&](int t) {
+            if (GC) return ab[t * kWave];  // the staged chunk: block t / NS, K-step t % NS
             const int blk = ch * KB + t / NS;
             return ab[((int64_t)(blk < kblocks ? blk : 0) * NS + t % NS) * kWave];
         };
@@ -410,6 +420,18 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     auto centre_of = [&](int ch, int idx) {
         const int b = idx >> 4, i = idx & 15;
         return (ch * KB + b) * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+    };
+    // GC: chunk ch's blocks into the LDS chunk buffer (blocks past the last
+    // centre row are not copied; their values are never scanned).  Called
+    // by every thread of the workgroup.
+    auto stage_chunk = [&](int ch) {
+        __syncthreads();  // every wavefront is done with the previous chunk
+        const int nblk = min(KB, kblocks - ch * KB);
+        const int n16 = nblk * NS * kWave;
+        const bf16x8 *src = (const bf16x8 *)cb + (int64_t)ch * KB * NS * kWave;
+        bf16x8 *dst = (bf16x8 *)cbs;
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
     };
 
     const int64_t ntiles = (a.N + 31) / 32;
@@ -448,7 +470,9 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     const int64_t tile0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     float xnext[8 * KS];
     if (PF && tile0 < ntiles) load_half_row(tile0, xnext);
-    for (int64_t tile = tile0; tile < ntiles; tile += nw) {
+    // GC: the workgroup's loop ends when its first wavefront's tile does
+    const int64_t block_tile0 = (int64_t)blockIdx.x * (blockDim.x / kWave);
+    for (int64_t tile = tile0; (GC ? block_tile0 + (tile - tile0) : tile) < ntiles; tile += nw) {
         const int64_t p = tile * 32 + col;
         const bool live = p < a.N;
         const float *xp = a.x + (live ? p : a.N - 1) * D;  // dead lanes read a valid row
@@ -492,6 +516,7 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         float k1 = kFar, k2 = kFar;
         int ch1 = 0;
         for (int ch = 0; ch < nchunks; ch++) {
+            if (GC) stage_chunk(ch);
             chunk_values(ch, bx, acc);
             float c1 = kFar, c2 = kFar;
 #pragma unroll
@@ -534,9 +559,14 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
         // Several candidates: rescore every key under the threshold.  Taken
         // by the whole wavefront (a vote), because recomputing a chunk's
         // values needs every lane's operands.
-        if (__any(multi)) {
+        if (GC ? __syncthreads_or(multi && live) != 0 : __any(multi)) {
             for (int ch = 0; ch < nchunks; ch++) {
-                if (nchunks > 1) chunk_values(ch, bx, acc);
+                if (GC) {
+                    stage_chunk(ch);
+                    chunk_values(ch, bx, acc);
+                } else if (nchunks > 1) {
+                    chunk_values(ch, bx, acc);
+                }
                 if (!multi) continue;
                 uint64_t w0 = 0, w1 = 0;
 #pragma unroll
@@ -574,7 +604,13 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
             best_k = ok;
         }
         if (best_k == 0x7fffffff) best_k = 0;  // unreachable for live points
-        if (live) {
+        if (live && a.defer_acc) {
+            if (half == 0) {
+                a.assign[p] = best_k;
+                a.mindist[p] = best_d;
+                if (multi) my_rescored++;
+            }
+        } else if (live) {
             if (half == 0) {
                 a.assign[p] = best_k;
                 if (a.mindist) a.mindist[p] = best_d;
@@ -681,7 +717,136 @@ __global__ void km_maxabs(const float *__restrict__ x, int64_t n, unsigned int *
     if (threadIdx.x % kWave == 0) atomicMax(out, __float_as_uint(m));
 }
 
+// ------------------------------------------------------------ bucketed reduction
+//
+// For a reduction object too large for LDS the assignment kernels only
+// write assign/mindist, and the object is summed afterwards bucket by
+// bucket: points are counted per centre, the counts scanned, the point
+// indices scattered into their centre's bucket, and wavefronts walk the
+// buckets in runs of kRun positions, summing each centre's rows in
+// registers (the same fixed-point integers as the direct path, so the sums
+// are the same bit for bit) and adding them to the object once per centre
+// run -- about (N / kRun + K) * D atomics instead of N * (D + 2).
+
+constexpr int kRun = 256;
+
+// cnt[k] += points assigned to k (LDS histogram per workgroup when K fits).
+__global__ __launch_bounds__(256) void km_bucket_count(const int32_t *__restrict__ assign, int64_t N, int K,
+                                                       int *__restrict__ cnt, int lds_hist)
+{
+    extern __shared__ int hist[];
+    if (lds_hist) {
+        for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
+        __syncthreads();
+    }
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        const int k = assign[p];
+        if (lds_hist) atomicAdd(&hist[k], 1);
+        else atomicAdd(&cnt[k], 1);
+    }
+    if (lds_hist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < K; k += blockDim.x)
+            if (hist[k]) atomicAdd(&cnt[k], hist[k]);
+    }
+}
+
+// One workgroup of 1024 threads: off[k] = exclusive prefix sum of cnt
+// (K <= 65536 entries; off has K + 1), and cur = off (scatter cursors).
+__global__ __launch_bounds__(1024) void km_bucket_scan(const int *__restrict__ cnt, int K, int *__restrict__ off,
+                                                       int *__restrict__ cur)
+{
+    __shared__ int part[1024];
+    const int per = (K + 1023) / 1024;
+    const int k0 = threadIdx.x * per, k1 = min(K, k0 + per);
+    int sum = 0;
+    for (int k = k0; k < k1; k++) sum += cnt[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+        const int v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;
+    for (int k = k0; k < k1; k++) {
+        off[k] = run;
+        cur[k] = run;
+        run += cnt[k];
+    }
+    if (threadIdx.x == 1023) off[K] = part[1023];
+}
+
+// idx[slot] = p, each point into its centre's bucket (order within a bucket
+// is arbitrary: the sums are integers).
+__global__ __launch_bounds__(256) void km_bucket_scatter(const int32_t *__restrict__ assign, int64_t N,
+                                                         int *__restrict__ cur, int *__restrict__ idx)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x)
+        idx[atomicAdd(&cur[assign[p]], 1)] = (int)p;
+}
+
+// One wavefront per run of kRun bucket positions: lanes hold dimensions
+// lane + 64t; each centre's points in the run are summed in registers and
+// added to the object when the centre changes (and at the run's end).
+template <int DT>
+__global__ __launch_bounds__(256) void km_bucket_sum(const float *__restrict__ x, const int *__restrict__ idx,
+                                                     const int32_t *__restrict__ assign,
+                                                     const float *__restrict__ mindist, int64_t N, int D, float scale,
+                                                     float dscale, unsigned long long *__restrict__ gsum,
+                                                     unsigned long long *__restrict__ gcnt,
+                                                     unsigned long long *__restrict__ gdsum)
+{
+    const int64_t run = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t j0 = run * kRun;
+    if (j0 >= N) return;  // whole wavefronts
+    const int64_t j1 = min<int64_t>(N, j0 + kRun);
+    unsigned long long acc[DT], cnt = 0, dsum = 0;
+#pragma unroll
+    for (int t = 0; t < DT; t++) acc[t] = 0;
+    int kc = assign[idx[j0]];
+    auto flush = [&](int k) {
+#pragma unroll
+        for (int t = 0; t < DT; t++) {
+            const int d = lane + kWave * t;
+            if (d < D && acc[t]) atomicAdd(&gsum[(int64_t)k * D + d], acc[t]);
+            acc[t] = 0;
+        }
+        if (lane == 0) {
+            atomicAdd(&gcnt[k], cnt);
+            atomicAdd(&gdsum[k], dsum);
+        }
+        cnt = 0;
+        dsum = 0;
+    };
+    for (int64_t j = j0; j < j1; j++) {
+        const int p = idx[j];
+        const int k = assign[p];
+        if (k != kc) {  // uniform: every lane reads the same p
+            flush(kc);
+            kc = k;
+        }
+        const float *xp = x + (int64_t)p * D;
+#pragma unroll
+        for (int t = 0; t < DT; t++) {
+            const int d = lane + kWave * t;
+            if (d < D) acc[t] += to_fixed(xp[d], scale);
+        }
+        cnt++;
+        if (lane == 0) dsum += to_fixed(mindist[p], dscale);
+    }
+    flush(kc);
+}
+
 __global__ void km_zero_u64(unsigned long long *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void km_zero_i32(int *__restrict__ p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -747,7 +912,8 @@ static KmArgs make_args(const KmDev &d)
     a.gsum = d.gsum;
     a.gcnt = d.gcnt;
     a.gdsum = d.gdsum;
-    a.use_lds_acc = object_bytes(d.K, d.D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024);
+    a.use_lds_acc = kmeans_object_in_lds(d.K, d.D);
+    a.defer_acc = d.bucketed;
     a.rs = a.use_lds_acc ? object_stride(d.D) : d.D;
     return a;
 }
@@ -783,6 +949,44 @@ int kmeans_prepare_kernels()
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
     done = 1;
+    return PAT_OK;
+}
+
+// Whether the reduction object (K x D sums, counts, distance sums) is kept
+// in LDS by the assignment kernels; otherwise a handle with fewer than 2^31
+// points sums it with the bucketed reduction.
+bool kmeans_object_in_lds(int K, int D)
+{
+    return object_bytes(K, D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024);
+}
+
+// The bucketed reduction after an assignment pass (see km_bucket_sum).
+static int bucket_reduce(const KmDev &d, hipStream_t st)
+{
+    const int K = d.K;
+    hipLaunchKernelGGL(km_zero_i32, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, d.bcnt, (int64_t)K);
+    PAT_LAUNCH_CHECK();
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((d.N + 255) / 256, 8LL * d.cus));
+    const int lds_hist = K <= 8192;
+    hipLaunchKernelGGL(km_bucket_count, dim3(g), dim3(256), lds_hist ? (size_t)4 * K : 0, st, (const int32_t *)d.assign,
+                       d.N, K, d.bcnt, lds_hist);
+    PAT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(km_bucket_scan, dim3(1), dim3(1024), 0, st, (const int *)d.bcnt, K, d.boff, d.bcur);
+    PAT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(km_bucket_scatter, dim3(g), dim3(256), 0, st, (const int32_t *)d.assign, d.N, d.bcur, d.bidx);
+    PAT_LAUNCH_CHECK();
+    const int64_t runs = (d.N + kRun - 1) / kRun;
+    const unsigned gs = (unsigned)((runs + 3) / 4);
+    const float scale = ldexpf(1.0f, d.shift), dscale = ldexpf(1.0f, d.dshift);
+    if (d.D <= 64)
+        hipLaunchKernelGGL(km_bucket_sum<1>, dim3(gs), dim3(256), 0, st, d.x, (const int *)d.bidx,
+                           (const int32_t *)d.assign, (const float *)d.mindist, d.N, d.D, scale, dscale, d.gsum, d.gcnt,
+                           d.gdsum);
+    else
+        hipLaunchKernelGGL(km_bucket_sum<2>, dim3(gs), dim3(256), 0, st, d.x, (const int *)d.bidx,
+                           (const int32_t *)d.assign, (const float *)d.mindist, d.N, d.D, scale, dscale, d.gsum, d.gcnt,
+                           d.gdsum);
+    PAT_LAUNCH_CHECK();
     return PAT_OK;
 }
 
@@ -843,7 +1047,7 @@ int kmeans_assign_plain(const KmDev &d, hipStream_t st)
     }
 #undef PAT_KP
     PAT_LAUNCH_CHECK();
-    return PAT_OK;
+    return d.bucketed ? bucket_reduce(d, st) : PAT_OK;
 }
 
 // LDS bytes of the MFMA kernel with nt terms (staged BF16 rows plus the
@@ -852,7 +1056,10 @@ static int64_t mfma_lds_bytes(int K, int D, int nt, bool gc = false)
 {
     const int ks = kmeans_ks_for(D);
     const int K_pad = (K + 31) / 32 * 32;
-    const int64_t cbb = gc ? 0 : 2LL * K_pad * 16 * (nt * ks + 1);
+    // all rows, or (gc) one chunk: KB blocks of 32 rows (KB = 8, 8, 4, 2 for
+    // KS = 1, 2, 4, 8)
+    const int kb = kmeans_kb_for(ks);
+    const int64_t cbb = gc ? 2LL * std::min(K_pad, 32 * kb) * 16 * (nt * ks + 1) : 2LL * K_pad * 16 * (nt * ks + 1);
     const int64_t obj = object_bytes(K, D) <= std::min<int64_t>(96 * 1024, g_lds_limit - 32 * 1024)
                             ? object_bytes(K, D) : 0;
     return cbb + obj;
@@ -925,7 +1132,7 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
         default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
         }
         PAT_LAUNCH_CHECK();
-        return PAT_OK;
+        return d.bucketed ? bucket_reduce(d, st) : PAT_OK;
     }
     switch (d.KS * 4 + d.NT) {
     case 1 * 4 + 1: PAT_KM(1, 8, 1); break;
@@ -941,7 +1148,7 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
 #undef PAT_KM
 #undef PAT_KM_GC
     PAT_LAUNCH_CHECK();
-    return PAT_OK;
+    return d.bucketed ? bucket_reduce(d, st) : PAT_OK;
 }
 
 int kmeans_zero(unsigned long long *p, int64_t n, hipStream_t st)

@@ -22,6 +22,8 @@ struct KmDev {
     float *cnorm;         // ||c_k|| (rounded up)
     unsigned long long *rescored;  // points with more than one candidate
     int cus;              // compute units
+    int bucketed;         // reduction object too large for LDS: assignments first, then bucket sums
+    int *bidx, *bcnt, *boff, *bcur;  // N bucketed point indices; K counts, K+1 offsets, K cursors
 };
 
 int kmeans_fixed_shift(int64_t n, float maxabs);
@@ -29,6 +31,7 @@ int kmeans_ks_for(int D);
 bool kmeans_mfma_fits(int K, int D);
 int kmeans_nt_for(int K, int D);
 bool kmeans_gc_for(int K, int D);
+bool kmeans_object_in_lds(int K, int D);
 int kmeans_maxabs(const float *x, int64_t n, unsigned int *out_bits, hipStream_t st);
 int kmeans_finalize(const KmDev &d, int update, hipStream_t st);
 int kmeans_assign_plain(const KmDev &d, hipStream_t st);

@@ -63,7 +63,8 @@ struct pat_kmeans {
         drop_graphs();
         for (void *p : {(void *)xbuf, (void *)d.c, (void *)d.assign, (void *)d.mindist, (void *)d.gsum, (void *)d.gcnt,
                         (void *)d.gdsum, (void *)d.csnap, rowbuf, (void *)d.cnorm, (void *)d.rescored,
-                        (void *)maxbits, (void *)snap})
+                        (void *)maxbits, (void *)snap, (void *)d.bidx, (void *)d.bcnt, (void *)d.boff,
+                        (void *)d.bcur})
             if (p) (void)pat::dev_free(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -175,6 +176,14 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     PAT_TRY(amalloc(&h->rowbuf, 2 * (size_t)D.K_pad * 16 * (D.NT * D.KS + 1)));
     D.cb = h->rowbuf;
     PAT_TRY(amalloc((void **)&D.cnorm, sizeof(float) * (size_t)k));
+    // a reduction object too large for LDS is summed bucket by bucket
+    D.bucketed = !pat::kmeans_object_in_lds(k, d) && n < (1LL << 31);
+    if (D.bucketed) {
+        PAT_TRY(amalloc((void **)&D.bidx, sizeof(int) * (size_t)n));
+        PAT_TRY(amalloc((void **)&D.bcnt, sizeof(int) * (size_t)k));
+        PAT_TRY(amalloc((void **)&D.boff, sizeof(int) * ((size_t)k + 1)));
+        PAT_TRY(amalloc((void **)&D.bcur, sizeof(int) * (size_t)k));
+    }
     PAT_TRY(amalloc((void **)&D.rescored, 8));
     PAT_TRY(amalloc((void **)&h->maxbits, 8));
     PAT_TRY(amalloc((void **)&h->snap, 8 * ((size_t)k * d + 2 * (size_t)k)));

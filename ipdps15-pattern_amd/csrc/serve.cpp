@@ -90,7 +90,9 @@ struct pat_serve {
         }
         case PAT_JOB_KMEANS: {
             const int64_t kd = (int64_t)j.k * j.dim;
-            return j.size * j.dim * 4 + 8 * j.size + 40 * kd + 64 * (int64_t)j.k + MB;
+            // points, assignments and distances, bucket indices (bucketed
+            // reduction), rows and the reduction object, per-centre arrays
+            return j.size * j.dim * 4 + 12 * j.size + 40 * kd + 80 * (int64_t)j.k + MB;
         }
         default: return 0;
         }
