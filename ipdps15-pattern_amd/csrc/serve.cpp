@@ -85,8 +85,12 @@ struct pat_serve {
             return 2 * s * s * s * 4 + MB;
         }
         case PAT_JOB_LJ: {
+            // per atom: positions, velocity, force (~80 B), CSR and its
+            // sliced and halo copies (~1.1 KB at ~80 entries per row), the
+            // rebuild's per-atom slots (512 B) or second column array, sort
+            // keys of the spatial slice order
             const int64_t n = 4 * j.size * j.size * j.size;
-            return 1600 * n + 2 * MB;
+            return 2400 * n + 2 * MB;
         }
         case PAT_JOB_KMEANS: {
             const int64_t kd = (int64_t)j.k * j.dim;
