@@ -347,8 +347,11 @@ constexpr float kFar = 3.0e38f;
 // taken block-uniformly (a wavefront past the last tile computes with every
 // lane dead) and the recompute path is a workgroup vote, so every wavefront
 // reaches every barrier.
-template <int KS, int KB, int NT, bool TAIL, bool GC>
-__global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
+// BS: threads per workgroup (the LDS-resident centre rows and reduction
+// object are shared by all its wavefronts, so larger workgroups raise the
+// occupancy the LDS allows when the registers permit).
+template <int KS, int KB, int NT, bool TAIL, bool GC, int BS = 512>
+__global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict__ cb, const float *__restrict__ cnorm,
                                                int K_pad, MfmaBound bd, unsigned long long *__restrict__ rescored)
 {
     constexpr int NS = NT * KS + 1;  // K-steps per centre row
@@ -465,8 +468,9 @@ __global__ __launch_bounds__(512) void km_mfma(KmArgs a, const __bf16 *__restric
     // while this tile computes (its loads return during the MFMAs); the
     // global-fragment form loads them in place, since vector loads complete
     // in order and the fragment loads would wait behind them, and so do rows
-    // of more than 64 dimensions (the extra registers would spill).
-    constexpr bool PF = !GC && KS <= 4;
+    // of more than 64 dimensions or in workgroups over 512 threads (the
+    // extra registers would spill).
+    constexpr bool PF = !GC && KS <= 4 && BS <= 512;
     const int64_t tile0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     float xnext[8 * KS];
     if (PF && tile0 < ntiles) load_half_row(tile0, xnext);
@@ -935,16 +939,17 @@ int kmeans_prepare_kernels()
                          (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1, true, false>,
                          (const void *)km_mfma<2, 8, 1, true, false>,  (const void *)km_mfma<4, 4, 1, true, false>,
                          (const void *)km_mfma<8, 2, 1, true, false>,  (const void *)km_mfma<1, 8, 3, true, false>,
-                         (const void *)km_mfma<2, 8, 3, true, false>,  (const void *)km_mfma<4, 4, 3, true, false>,
+                         (const void *)km_mfma<4, 4, 3, true, false>,
                          (const void *)km_mfma<8, 2, 3, true, false>,  (const void *)km_mfma<1, 8, 1, false, false>,
                          (const void *)km_mfma<2, 8, 1, false, false>, (const void *)km_mfma<4, 4, 1, false, false>,
                          (const void *)km_mfma<8, 2, 1, false, false>, (const void *)km_mfma<1, 8, 3, false, false>,
-                         (const void *)km_mfma<2, 8, 3, false, false>, (const void *)km_mfma<4, 4, 3, false, false>,
+                         (const void *)km_mfma<4, 4, 3, false, false>,
                          (const void *)km_mfma<8, 2, 3, false, false>,
                          (const void *)km_mfma<1, 8, 3, true, true>,   (const void *)km_mfma<2, 8, 3, true, true>,
                          (const void *)km_mfma<4, 4, 3, true, true>,   (const void *)km_mfma<8, 2, 3, true, true>,
                          (const void *)km_mfma<1, 8, 3, false, true>,  (const void *)km_mfma<2, 8, 3, false, true>,
-                         (const void *)km_mfma<4, 4, 3, false, true>,  (const void *)km_mfma<8, 2, 3, false, true>};
+                         (const void *)km_mfma<4, 4, 3, false, true>,  (const void *)km_mfma<8, 2, 3, false, true>,
+                         (const void *)km_mfma<2, 4, 3, true, false, 768>, (const void *)km_mfma<2, 4, 3, false, false, 768>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
@@ -1106,22 +1111,26 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
         bd.c_dot = (float)((4 * u + 2 * u * u + 4 * g * (1 + u) * (1 + u)) * 1.01);
     bd.c_norm = (float)((u * u * u + 2 * g * (1 + 2 * u)) * 1.01);
     bd.c_ref = (float)((3.0 + d.D + 16.0) * ldexp(1.0, -24) * 1.01);
-    const int64_t want = (d.N + 31) / 32 / 8;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
+    // one workgroup per CU (the LDS holds one), BS_ / 64 wavefronts each
+    auto grid_for = [&](int bs) {
+        const int64_t want = (d.N + 31) / 32 / (bs / 64);
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)d.cus));
+    };
     // no tail: every K-step full and 16-byte aligned rows
     const bool tail = d.D != 16 * d.KS || ((uintptr_t)d.x % 16) != 0;
-#define PAT_KM_GC(KS_, KB_, NT_, GC_)                                                                                  \
+#define PAT_KM_BS(KS_, KB_, NT_, GC_, BS_)                                                                             \
     do {                                                                                                               \
         if (tail) {                                                                                                    \
-            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, true, GC_>, lds));                                  \
-            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, true, GC_>), dim3(grid), dim3(512), (size_t)lds, st, a,         \
-                               (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, true, GC_, BS_>, lds));                             \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, true, GC_, BS_>), dim3(grid_for(BS_)), dim3(BS_), (size_t)lds,  \
+                               st, a, (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                         \
         } else {                                                                                                       \
-            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, false, GC_>, lds));                                 \
-            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, false, GC_>), dim3(grid), dim3(512), (size_t)lds, st, a,        \
-                               (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                                \
+            PAT_TRY(allow_lds((const void *)km_mfma<KS_, KB_, NT_, false, GC_, BS_>, lds));                            \
+            hipLaunchKernelGGL((km_mfma<KS_, KB_, NT_, false, GC_, BS_>), dim3(grid_for(BS_)), dim3(BS_), (size_t)lds, \
+                               st, a, (const __bf16 *)d.cb, d.cnorm, d.K_pad, bd, d.rescored);                         \
         }                                                                                                              \
     } while (0)
+#define PAT_KM_GC(KS_, KB_, NT_, GC_) PAT_KM_BS(KS_, KB_, NT_, GC_, 512)
 #define PAT_KM(KS_, KB_, NT_) PAT_KM_GC(KS_, KB_, NT_, false)
     if (d.gc) {
         switch (d.KS) {
@@ -1140,13 +1149,14 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     case 4 * 4 + 1: PAT_KM(4, 4, 1); break;
     case 8 * 4 + 1: PAT_KM(8, 2, 1); break;
     case 1 * 4 + 3: PAT_KM(1, 8, 3); break;
-    case 2 * 4 + 3: PAT_KM(2, 8, 3); break;
+    case 2 * 4 + 3: PAT_KM_BS(2, 4, 3, false, 768); break;  // 3 wavefronts per SIMD (168 VGPRs)
     case 4 * 4 + 3: PAT_KM(4, 4, 3); break;
     case 8 * 4 + 3: PAT_KM(8, 2, 3); break;
     default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
     }
 #undef PAT_KM
 #undef PAT_KM_GC
+#undef PAT_KM_BS
     PAT_LAUNCH_CHECK();
     return d.bucketed ? bucket_reduce(d, st) : PAT_OK;
 }
