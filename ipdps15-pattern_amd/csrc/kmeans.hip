@@ -935,21 +935,24 @@ int kmeans_prepare_kernels()
     PAT_HIP(hipGetDevice(&dev));
     PAT_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     g_lds_limit = lim - 256;  // room for the kernels' static LDS
-    const void *fns[] = {(const void *)km_plain<8, 1024>, (const void *)km_plain<16, 1024>, (const void *)km_plain<32, 1024>,
-                         (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>, (const void *)km_mfma<1, 8, 1, true, false>,
-                         (const void *)km_mfma<2, 8, 1, true, false>,  (const void *)km_mfma<4, 4, 1, true, false>,
-                         (const void *)km_mfma<8, 2, 1, true, false>,  (const void *)km_mfma<1, 8, 3, true, false>,
-                         (const void *)km_mfma<4, 4, 3, true, false>,
-                         (const void *)km_mfma<8, 2, 3, true, false>,  (const void *)km_mfma<1, 8, 1, false, false>,
-                         (const void *)km_mfma<2, 8, 1, false, false>, (const void *)km_mfma<4, 4, 1, false, false>,
-                         (const void *)km_mfma<8, 2, 1, false, false>, (const void *)km_mfma<1, 8, 3, false, false>,
-                         (const void *)km_mfma<4, 4, 3, false, false>,
-                         (const void *)km_mfma<8, 2, 3, false, false>,
-                         (const void *)km_mfma<1, 8, 3, true, true>,   (const void *)km_mfma<2, 8, 3, true, true>,
-                         (const void *)km_mfma<4, 4, 3, true, true>,   (const void *)km_mfma<8, 2, 3, true, true>,
-                         (const void *)km_mfma<1, 8, 3, false, true>,  (const void *)km_mfma<2, 8, 3, false, true>,
-                         (const void *)km_mfma<4, 4, 3, false, true>,  (const void *)km_mfma<8, 2, 3, false, true>,
-                         (const void *)km_mfma<2, 4, 3, true, false, 768>, (const void *)km_mfma<2, 4, 3, false, false, 768>};
+    const void *fns[] = {
+        (const void *)km_plain<8, 1024>, (const void *)km_plain<16, 1024>, (const void *)km_plain<32, 1024>,
+        (const void *)km_plain<64, 1024>, (const void *)km_plain<128, 512>,
+        // one term
+        (const void *)km_mfma<1, 8, 1, true, false>, (const void *)km_mfma<2, 8, 1, true, false>,
+        (const void *)km_mfma<4, 4, 1, true, false>, (const void *)km_mfma<8, 2, 1, true, false>,
+        (const void *)km_mfma<1, 8, 1, false, false>, (const void *)km_mfma<2, 8, 1, false, false>,
+        (const void *)km_mfma<4, 4, 1, false, false>, (const void *)km_mfma<8, 2, 1, false, false>,
+        // three terms, rows in LDS
+        (const void *)km_mfma<1, 4, 3, true, false, 768>, (const void *)km_mfma<1, 4, 3, false, false, 768>,
+        (const void *)km_mfma<2, 4, 3, true, false, 768>, (const void *)km_mfma<2, 4, 3, false, false, 768>,
+        (const void *)km_mfma<4, 2, 3, true, false, 768>, (const void *)km_mfma<4, 2, 3, false, false, 768>,
+        (const void *)km_mfma<8, 2, 3, true, false>, (const void *)km_mfma<8, 2, 3, false, false>,
+        // three terms, rows staged chunk by chunk
+        (const void *)km_mfma<1, 8, 3, true, true>, (const void *)km_mfma<2, 8, 3, true, true>,
+        (const void *)km_mfma<4, 4, 3, true, true>, (const void *)km_mfma<8, 2, 3, true, true>,
+        (const void *)km_mfma<1, 8, 3, false, true>, (const void *)km_mfma<2, 8, 3, false, true>,
+        (const void *)km_mfma<4, 4, 3, false, true>, (const void *)km_mfma<8, 2, 3, false, true>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g_lds_limit) != hipSuccess)
             (void)hipGetLastError();
@@ -1148,9 +1151,9 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     case 2 * 4 + 1: PAT_KM(2, 8, 1); break;
     case 4 * 4 + 1: PAT_KM(4, 4, 1); break;
     case 8 * 4 + 1: PAT_KM(8, 2, 1); break;
-    case 1 * 4 + 3: PAT_KM(1, 8, 3); break;
+    case 1 * 4 + 3: PAT_KM_BS(1, 4, 3, false, 768); break;  // 3 wavefronts per SIMD
     case 2 * 4 + 3: PAT_KM_BS(2, 4, 3, false, 768); break;  // 3 wavefronts per SIMD (168 VGPRs)
-    case 4 * 4 + 3: PAT_KM(4, 4, 3); break;
+    case 4 * 4 + 3: PAT_KM_BS(4, 2, 3, false, 768); break;  // 3 wavefronts per SIMD
     case 8 * 4 + 3: PAT_KM(8, 2, 3); break;
     default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;
     }
