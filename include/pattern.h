@@ -375,7 +375,8 @@ int pat_kmeans_set_mode(pat_kmeans *h, int mode);
 /* BF16 terms of the MFMA path's distance values: 3 (x and -2c split into
  * high and low parts) or 1 (rounded once, a wider rescoring margin).  Both
  * give the plain path's assignments bit for bit; by default the faster one
- * on the handle's data is kept (timed at creation, when both fit in LDS).
+ * on the handle's data is kept (timed at creation when both fit in LDS and
+ * N >= 2^20; three terms otherwise).
  * PAT_ERR_UNSUPPORTED when the shape's rows do not fit the requested form.
  * No reference counterpart (the reference has no MFMA path). */
 int pat_kmeans_set_terms(pat_kmeans *h, int terms);

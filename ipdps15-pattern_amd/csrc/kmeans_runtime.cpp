@@ -83,9 +83,9 @@ int amalloc(void **p, size_t b)
     return PAT_OK;
 }
 
-// Terms of the MFMA path (1 or 3) chosen per (log2 N, D, K) class for the
-// process; both give the same assignments bit for bit, so the choice only
-// changes speed.
+// Terms of the MFMA path (1 or 3) chosen per (log2 N, D, log2 K) class for
+// the process; both give the same assignments bit for bit, so the choice
+// only changes speed.
 std::map<std::tuple<int, int, int>, int> g_terms_cache;
 std::mutex g_terms_mutex;
 
@@ -98,7 +98,8 @@ std::mutex g_terms_mutex;
 int tune_terms(pat_kmeans *h)
 {
     auto &D = h->d;
-    const auto key = std::make_tuple(63 - __builtin_clzll((unsigned long long)D.N | 1), D.D, D.K);
+    const auto key = std::make_tuple(63 - __builtin_clzll((unsigned long long)D.N | 1), D.D,
+                                     31 - __builtin_clz((unsigned)D.K | 1));
     int nt = 0;
     {
         std::lock_guard<std::mutex> g(g_terms_mutex);
@@ -208,7 +209,8 @@ int setup(pat_kmeans *h, const float *x, int64_t n, int d, int k, int mode, int 
     PAT_TRY(pat::kmeans_zero(D.rescored, 1, h->stream));
     PAT_TRY(pat::kmeans_finalize(D, 0, h->stream));
     PAT_HIP(hipStreamSynchronize(h->stream));
-    if (mode == PAT_KM_MFMA && !D.gc && D.NT == 3 && pat::tuning_enabled()) PAT_TRY(tune_terms(h));
+    // timed only where a pass is long enough to pay for six trial passes
+    if (mode == PAT_KM_MFMA && !D.gc && D.NT == 3 && n >= (1 << 20) && pat::tuning_enabled()) PAT_TRY(tune_terms(h));
     return PAT_OK;
 }
 

@@ -23,6 +23,7 @@
 // Compiled with -ffp-contract=off: each cell's arithmetic rounds exactly as
 // written, so results are bit-identical to a plain C sweep of the same rule.
 #include <chrono>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -407,21 +408,26 @@ int heat3d_steps(float *const buf[2], int start, int64_t nx, int64_t ny, int64_t
 // Autotuner: times every instantiated tile shape with z-chunks of 2..64
 // planes on the caller's grid (in -> out; out is scratch until the first
 // real sweep writes it, and no kernel writes the halo), keeps the fastest,
-// and caches it per grid shape for the process.
-static std::map<std::tuple<int64_t, int64_t, int64_t, int>, HeatTile> g_tile_cache;
+// and caches it for the process per size class: each extent's half octave
+// (floor(2 log2 n)), so a stream of grids of many sizes (a serving mix)
+// tunes once per class; every tile gives the same bits on any grid.
+static std::map<std::tuple<int, int, int, int>, HeatTile> g_tile_cache;
 static std::mutex g_tile_mutex;
+
+static int size_class(int64_t n) { return (int)std::floor(2.0 * std::log2((double)n)); }
 
 // allow_fused: whether two-sweep tiles may be chosen (the caller runs
 // pairs of steps); their time is compared per step.
 int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, hipStream_t st,
                  bool allow_fused, HeatTile *t)
 {
-    const auto key = std::make_tuple(nx, ny, nz, (int)allow_fused);
+    const auto key = std::make_tuple(size_class(nx), size_class(ny), size_class(nz), (int)allow_fused);
     {
         std::lock_guard<std::mutex> g(g_tile_mutex);
         auto it = g_tile_cache.find(key);
         if (it != g_tile_cache.end()) {
             *t = it->second;
+            t->zc = (int)std::min<int64_t>(t->zc, nz);
             return PAT_OK;
         }
     }
