@@ -3,7 +3,7 @@
  * per-partition compute engines (stencil sweep, irregular edge reduction,
  * generalized reduction for kmeans).
  *
- * THIS FILE IS TEST INFRASTRUCTURE.  It is the checker the HIP engines are
+ * THIS FILE IS TEST INFRASTRUCTURE.  It is the oracle the HIP engines are
  * compared against.  Only the test suite, the smoke check and the CPU-baseline
  * leg of the benchmark may load it; the product library never links or calls
  * it, and nothing here is a fallback for the GPU path.

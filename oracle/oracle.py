@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+CPU-baseline leg of bench.py, to compare against.  The product never imports it.
 """
 import ctypes
 import os
@@ -241,7 +241,7 @@ def kmeans_lloyd_f64(x, k, iters, threads=0):
     return c, a
 
 
-# ---------------------------------------------------------------- reference checker
+# ---------------------------------------------------------------- the reference's own miniMD code
 
 REF_PATH = os.path.join(_HERE, "_ref", "libref_minimd.so")
 

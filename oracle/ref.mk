@@ -1,4 +1,4 @@
-# Builds the reference checker oracle/_ref/libref_minimd.so from the
+# Builds oracle/_ref/libref_minimd.so (the oracle's second source) from the
 # reference's own miniMD sources where they lie (never copied), plus the
 # driver oracle/ref_minimd.cpp.  Needs /root/reference and MPICH's mpi.h
 # (setup.cpp/atom.cpp include it; no MPI call is linked in the path used).

@@ -1,4 +1,4 @@
-// Driver for the reference checker: links miniMD's own Atom, Neighbor,
+// Entry point of oracle/_ref/libref_minimd.so: links miniMD's own Atom, Neighbor,
 // Force and setup code (compiled from /root/reference/miniMD by
 // oracle/ref.mk) and exposes one C entry point, so the tests can compare the
 // oracle's LJ force, pair generator and FCC lattice with the reference's

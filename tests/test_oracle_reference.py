@@ -1,13 +1,13 @@
 """Pins the LJ oracle, the pair generator and the FCC generator against the
 reference's own serial miniMD code (create_atoms, Neighbor::build,
 Force::compute), built from the reference sources into oracle/_ref by
-oracle/ref.mk.  Skipped where that checker is not built."""
+oracle/ref.mk.  Skipped where that library is not built."""
 import numpy as np
 import pytest
 
 import oracle as _o
 
-pytestmark = pytest.mark.skipif(not _o.ref_available(), reason="reference checker oracle/_ref not built")
+pytestmark = pytest.mark.skipif(not _o.ref_available(), reason="the reference's miniMD library oracle/_ref is not built")
 
 
 @pytest.mark.parametrize("n,nbin", [(4, 4), (6, 5), (8, 6)])
