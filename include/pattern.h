@@ -124,8 +124,8 @@ int pat_gen_gmm_dev(int64_t n, int d, int ncomp, double sigma, double lo, double
 #define PAT_STENCIL_AUTO 0    /* autotuned choice */
 #define PAT_STENCIL_LDS25D 1  /* LDS-staged 2.5D z-marching tiles */
 #define PAT_STENCIL_GENERIC 2 /* one thread per cell calling the functor */
-#define PAT_STENCIL_TEMPORAL2 3 /* two sweeps fused per pass (temporal blocking, 64-wide windows of 16
-                                   or 32 rows); an odd last step runs the LDS25D kernel.  Same bits */
+#define PAT_STENCIL_TEMPORAL2 3 /* two sweeps fused per pass (temporal blocking, 64-wide windows of 8,
+                                   16 or 32 rows); an odd last step runs the LDS25D kernel.  Same bits */
 
 /* nsteps ping-pong heat sweeps over device grids a, b (halo-inclusive,
  * x fastest, both halos already 0).  *result_in_b = 1 when the final field

@@ -352,8 +352,8 @@ void heat3d_default_tile(int64_t nx, int64_t ny, int64_t nz, int *bx, int *by, i
 int heat3d_sweep2(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, int ry, int zc,
                   hipStream_t stream)
 {
-    PAT_REQUIRE(ry == 4 || ry == 8, PAT_ERR_UNSUPPORTED, "heat3d: fused window of %d rows per thread not instantiated",
-                ry);
+    PAT_REQUIRE(ry == 2 || ry == 4 || ry == 8, PAT_ERR_UNSUPPORTED,
+                "heat3d: fused window of %d rows per thread not instantiated", ry);
     PAT_REQUIRE(zc > 0, PAT_ERR_INVALID, "heat3d: z-chunk must be positive");
     const int oy = 4 * ry - 4;
     const int tiles_x = (int)((nx + 59) / 60);
@@ -361,7 +361,7 @@ int heat3d_sweep2(const float *in, float *out, int64_t nx, int64_t ny, int64_t n
     const int tiles_z = (int)((nz + zc - 1) / zc);
     const int64_t nblocks = (int64_t)tiles_x * tiles_y * tiles_z;
     PAT_REQUIRE(nblocks < (1LL << 31), PAT_ERR_INVALID, "heat3d: grid too large");
-    auto k = ry == 4 ? heat_tb2<4> : heat_tb2<8>;
+    auto k = ry == 2 ? heat_tb2<2> : (ry == 4 ? heat_tb2<4> : heat_tb2<8>);
     hipLaunchKernelGGL(k, dim3((unsigned)nblocks), dim3(64, 4), 0, stream, in, out, (int)nx, (int)ny, (int)nz, zc,
                        alpha, tiles_x, tiles_y, tiles_z);
     PAT_LAUNCH_CHECK();
@@ -441,7 +441,7 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
             cands.push_back(HeatTile{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)});
         }
     if (allow_fused)
-        for (int r : {4, 8})
+        for (int r : {2, 4, 8})
             for (int z : {2, 4, 8, 16, 32, 64, 128}) {
                 if (z > nz && z != 2) continue;
                 cands.push_back(HeatTile{1, 64, 4, r, (int)std::min<int64_t>(z, nz)});
@@ -529,9 +529,9 @@ extern "C" int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, in
         PAT_TRY(pat::heat3d_tuned(a, b, nx, ny, nz, alpha, st, true, &t));
     } else if (variant == PAT_STENCIL_TEMPORAL2) {
         PAT_REQUIRE(bx == 0 || bx == 64, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: fused windows are 64 wide");
-        PAT_REQUIRE(by == 0 || by == 16 || by == 32, PAT_ERR_UNSUPPORTED,
-                    "pat_heat3d_run_dev: fused windows are 16 or 32 rows");
-        t = pat::HeatTile{1, 64, 4, by == 16 ? 4 : 8, zc > 0 ? zc : (int)std::min<int64_t>(64, nz)};
+        PAT_REQUIRE(by == 0 || by == 8 || by == 16 || by == 32, PAT_ERR_UNSUPPORTED,
+                    "pat_heat3d_run_dev: fused windows are 8, 16 or 32 rows");
+        t = pat::HeatTile{1, 64, 4, by == 8 ? 2 : (by == 16 ? 4 : 8), zc > 0 ? zc : (int)std::min<int64_t>(64, nz)};
     } else {
         t.fused = 0;
         default_tile(nx, ny, nz, &t.bx, &t.by, &t.ry, &t.zc);

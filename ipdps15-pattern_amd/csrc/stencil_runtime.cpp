@@ -134,10 +134,10 @@ int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
                 "pat_stencil_set_variant: unknown variant %d", variant);
     int tby = 0, tr = 0;
     if (variant == PAT_STENCIL_TEMPORAL2) {
-        PAT_REQUIRE((bx == 0 || bx == 64) && (by == 0 || by == 16 || by == 32) && zc >= 0, PAT_ERR_UNSUPPORTED,
-                    "pat_stencil_set_variant: fused windows are 64 wide and 16 or 32 rows");
+        PAT_REQUIRE((bx == 0 || bx == 64) && (by == 0 || by == 8 || by == 16 || by == 32) && zc >= 0,
+                    PAT_ERR_UNSUPPORTED, "pat_stencil_set_variant: fused windows are 64 wide and 8, 16 or 32 rows");
         tby = by > 0 ? 4 : 0;
-        tr = by == 16 ? 4 : (by == 32 ? 8 : 0);
+        tr = by == 8 ? 2 : (by == 16 ? 4 : (by == 32 ? 8 : 0));
     } else if (by > 0) {
         const int *sh;
         int ns = pat::heat3d_shapes(&sh);

@@ -46,7 +46,7 @@ def test_temporal2_bitexact_ragged(pa, orc, gpu, shape, steps):
     nx, ny, nz = shape
     g = pa.gen_heat3d(nx, ny, nz, seed=21)
     ref = orc.heat3d(g, steps)
-    for by, zc in ((16, 1), (32, 3), (16, 64)):
+    for by, zc in ((16, 1), (32, 3), (16, 64), (8, 2), (8, 5)):
         got = run_dev(pa, g, steps, variant=pa.PAT_STENCIL_TEMPORAL2, bx=64, by=by, zc=zc)
         assert np.array_equal(got, ref), (by, zc)
 
