@@ -431,9 +431,9 @@ struct IrState {
         }
         PAT_TRY(check_box(p4, box, pbc, stream));
         PAT_TRY(set_layout(0, true, p4, box, pbc, rc2));
-        hipEvent_t a, b;
-        PAT_HIP(hipEventCreate(&a));
-        PAT_HIP(hipEventCreate(&b));
+        pat::EventPair ev;
+        PAT_HIP(ev.create());
+        hipEvent_t a = ev.a, b = ev.b;
         struct Cand {
             int lanes, kind, slice_order;
         };
@@ -498,8 +498,6 @@ struct IrState {
             t.resize(cands.size(), 1e30f);
             PAT_TRY(time_range(c0, cands.size()));
         }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         size_t bi = 0;
         for (size_t c = 1; c < cands.size(); c++)
             if (t[c] < t[bi]) bi = c;
@@ -1294,20 +1292,17 @@ struct pat_mdb {
     {
         for (int j = 0; j < seg; j++) {
             const bool final_step = ends_run && j == seg - 1;
-            hipEvent_t e0 = nullptr, e1 = nullptr;
+            pat::EventPair ev;
             if (kernel_ms) {
-                PAT_HIP(hipEventCreate(&e0));
-                PAT_HIP(hipEventCreate(&e1));
-                PAT_HIP(hipEventRecord(e0, s));
+                PAT_HIP(ev.create());
+                PAT_HIP(hipEventRecord(ev.a, s));
             }
             PAT_TRY(md.st.csr_force(md.st.pos[c], 1.0f, 2, md.rc2, md.dt, final_step ? kKick : kKickDrift,
                                     md.st.pos[1 - c], s));
             if (kernel_ms) {
-                PAT_HIP(hipEventRecord(e1, s));
-                PAT_HIP(hipEventSynchronize(e1));
-                (void)hipEventElapsedTime(&kernel_ms[j], e0, e1);
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
+                PAT_HIP(hipEventRecord(ev.b, s));
+                PAT_HIP(hipEventSynchronize(ev.b));
+                (void)hipEventElapsedTime(&kernel_ms[j], ev.a, ev.b);
             }
             if (!final_step) c = 1 - c;
         }

@@ -107,9 +107,9 @@ int tune_terms(pat_kmeans *h)
         if (it != g_terms_cache.end()) nt = it->second;
     }
     if (!nt) {
-        hipEvent_t e0, e1;
-        PAT_HIP(hipEventCreate(&e0));
-        PAT_HIP(hipEventCreate(&e1));
+        pat::EventPair ev;
+        PAT_HIP(ev.create());
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         const int cand[2] = {3, 1};
         float best[2] = {1e30f, 1e30f};
         for (int r = 0; r < 3; r++)
@@ -124,8 +124,6 @@ int tune_terms(pat_kmeans *h)
                 PAT_HIP(hipEventElapsedTime(&ms, e0, e1));
                 best[c] = std::min(best[c], ms);
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         nt = best[1] < best[0] ? 1 : 3;
         if (pat::tune_log())
             fprintf(stderr, "[tune kmeans N=%lld D=%d K=%d] three terms %.3f ms, one term %.3f ms -> %d\n",

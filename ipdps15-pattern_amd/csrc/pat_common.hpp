@@ -58,6 +58,24 @@ bool tuning_enabled();
 bool tune_log();
 void set_thread_tuning(bool on);
 
+// Two HIP events destroyed on every path out of the scope that owns them.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    hipError_t create()
+    {
+        const hipError_t e = hipEventCreate(&a);
+        return e != hipSuccess ? e : hipEventCreate(&b);
+    }
+};
+
 }  // namespace pat
 
 #define PAT_HIP(call)                                                                                      \

@@ -446,9 +446,9 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
                 if (z > nz && z != 2) continue;
                 cands.push_back(HeatTile{1, 64, 4, r, (int)std::min<int64_t>(z, nz)});
             }
-    hipEvent_t a, b;
-    PAT_HIP(hipEventCreate(&a));
-    PAT_HIP(hipEventCreate(&b));
+    pat::EventPair ev;
+    PAT_HIP(ev.create());
+    hipEvent_t a = ev.a, b = ev.b;
     // a few milliseconds of warm-up on the default tile (clocks ramp up
     // under load), then rounds that time every candidate in turn; each
     // candidate keeps its best round, fused ones counted per step
@@ -481,8 +481,6 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
                     (long long)nx, (long long)ny, (long long)nz, cands[c].fused, cands[c].bx, cands[c].by,
                     cands[c].ry, cands[c].zc, 1000.0f * tm[c], c == bi ? "  <- chosen" : "");
     *t = cands[bi];
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     std::lock_guard<std::mutex> g(g_tile_mutex);
     g_tile_cache[key] = *t;
     return PAT_OK;
