@@ -125,64 +125,128 @@ __global__ __launch_bounds__(256) void csr_row_sort_by_edge(const int *__restric
     }
 }
 
-// Compare-exchange of the bitonic network between this lane's value and
-// lane ^ j's: the lower lane of the pair keeps the smaller value when up.
-__device__ __forceinline__ unsigned bitonic_step(unsigned v, int lane, int j, bool up)
+// Value held by lane ^ M (M < 64), by the cheapest cross-lane form: DPP
+// quad permutes and row mirrors for 1, 2, 3, 7, 15, a bit-mode swizzle
+// inside 32 lanes for the other masks below 32, a permute otherwise.
+template <int M>
+__device__ __forceinline__ unsigned xor_lane(unsigned v)
 {
-    const unsigned o = (unsigned)__shfl_xor((int)v, j, kWave);
-    const bool low = (lane & j) == 0;
-    return (low == up) ? min(v, o) : max(v, o);
+    const int x = (int)v;
+    if constexpr (M == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+    else if constexpr (M == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+    else if constexpr (M == 3) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x1B, 0xF, 0xF, false);  // [3,2,1,0]
+    else if constexpr (M == 7) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // half mirror
+    else if constexpr (M == 15) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);  // mirror
+    else if constexpr (M < 32) return (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x1f | (M << 10));
+    else return (unsigned)__shfl_xor(x, M, kWave);
 }
 
-// One wavefront per row of at most 128 columns: bitonic sort of the row
-// held two values per lane (entry lane and entry 64 + lane; padding sorts
-// last), writing the sorted row into cols.  Longer rows take the rank sort.
-// Columns of a row are distinct, so any correct sort gives the rank sort's
-// order.  The unsorted row is tmp[rowptr[row] ...] (slot == 0) or
-// tmp[row*slot ...] (rows of at most slot entries; longer ones are skipped).
+// Rows sorted per wavefront by csr_row_sort.
+constexpr int kSortRows = 4;
+
+// Compare-exchange of every row's value `first` (0: entry lane, 1: entry
+// 64 + lane) with lane ^ M's; the lane whose bit LOW is clear keeps the
+// smaller value.
+template <int M, int LOW>
+__device__ __forceinline__ void sort_cx(unsigned (&a)[2 * kSortRows], int first, int lane)
+{
+#pragma unroll
+    for (int t = 0; t < kSortRows; t++) {
+        const unsigned v = a[2 * t + first], o = xor_lane<M>(v);
+        a[2 * t + first] = (lane & LOW) ? max(v, o) : min(v, o);
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void sort_half_clean(unsigned (&a)[2 * kSortRows], int lane)
+{
+    if constexpr (J >= 1) {
+        sort_cx<J, J>(a, 0, lane);
+        sort_cx<J, J>(a, 1, lane);
+        sort_half_clean<J / 2>(a, lane);
+    }
+}
+
+// Merge step K of the bitonic network in its flip form (Batcher's network
+// with the first comparison of each merge mirrored, so every comparison
+// sorts ascending): element i against i ^ (K - 1), then half-cleaners at
+// distances K/4 .. 1.  Element i < 64 is value 0 of lane i, element 64 + i
+// value 1 of lane i.
+template <int K>
+__device__ __forceinline__ void sort_merge(unsigned (&a)[2 * kSortRows], int lane)
+{
+    if constexpr (K <= kWave) {
+        sort_cx<K - 1, K / 2>(a, 0, lane);
+        sort_cx<K - 1, K / 2>(a, 1, lane);
+    } else {  // K = 128: element lane against element 127 - lane (value 1 of lane ^ 63)
+#pragma unroll
+        for (int t = 0; t < kSortRows; t++) {
+            const unsigned o1 = xor_lane<63>(a[2 * t + 1]), o0 = xor_lane<63>(a[2 * t]);
+            a[2 * t] = min(a[2 * t], o1);
+            a[2 * t + 1] = max(a[2 * t + 1], o0);
+        }
+    }
+    sort_half_clean<K / 4>(a, lane);
+}
+
+// kSortRows rows per wavefront, each row of at most 128 columns sorted by
+// the bitonic network held two values per lane (padding sorts last); the
+// rows' pointers and values are all loaded before the networks run, which
+// interleave.  Longer rows take the rank sort afterwards.  Columns of a row
+// are distinct, so any correct sort gives the rank sort's order.  The
+// unsorted row is tmp[rowptr[row] ...] (slot == 0) or tmp[row*slot ...]
+// (rows of at most slot entries; longer ones are skipped).
 __global__ __launch_bounds__(256) void csr_row_sort(const int *__restrict__ rowptr, int64_t N,
                                                     const int *__restrict__ tmp, int *__restrict__ cols, int slot)
 {
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t row0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave * kSortRows;
     const int lane = threadIdx.x % kWave;
-    if (row >= N) return;
-    const int b = rowptr[row], e = rowptr[row + 1], n = e - b;
-    if (slot && n > slot) return;  // never produced by a slot pass
-    const int *__restrict__ src = tmp + (slot ? row * slot : (int64_t)b);
-    if (n > 2 * kWave) {  // uniform per wavefront
-        for (int k = lane; k < n; k += kWave) {
-            const int v = src[k];
-            int rank = 0;
-            for (int m = 0; m < n; m++) {
-                const int u = src[m];
-                rank += (u < v) || (u == v && m < k);
-            }
-            cols[b + rank] = v;
-        }
-        return;
+    if (row0 >= N) return;  // whole wavefronts
+    // lane t <= kSortRows holds rowptr[row0 + t] (clamped to rowptr[N])
+    const int rpl = rowptr[min(row0 + min(lane, kSortRows), N)];
+    int b[kSortRows], n[kSortRows];
+    unsigned a[2 * kSortRows];
+#pragma unroll
+    for (int t = 0; t < kSortRows; t++) {
+        b[t] = __shfl(rpl, t, kWave);
+        n[t] = row0 + t < N ? __shfl(rpl, t + 1, kWave) - b[t] : 0;
+        if (slot && n[t] > slot) n[t] = 0;  // never produced by a slot pass
+        const int m = n[t] <= 2 * kWave ? n[t] : 0;  // longer rows: rank sort below
+        const int *__restrict__ src = tmp + (slot ? (row0 + t) * slot : (int64_t)b[t]);
+        a[2 * t] = lane < m ? (unsigned)src[lane] : 0xffffffffu;
+        a[2 * t + 1] = kWave + lane < m ? (unsigned)src[kWave + lane] : 0xffffffffu;
     }
-    // values as unsigned with padding 0xffffffff (columns are >= 0)
-    unsigned a0 = lane < n ? (unsigned)src[lane] : 0xffffffffu;
-    unsigned a1 = kWave + lane < n ? (unsigned)src[kWave + lane] : 0xffffffffu;
-    // sort the 128-element sequence s[i] (i < 64: a0 of lane i; else a1 of
-    // lane i - 64): stages k = 2..128, distances j = k/2..1; direction of
-    // element i is ascending when (i & k) == 0
-    for (int k = 2; k <= 2 * kWave; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j == kWave) {  // pairs (lane, 64 + lane): both in this lane
-                const unsigned lo = min(a0, a1), hi = max(a0, a1);
-                a0 = lo;  // k = 128: ascending for every pair
-                a1 = hi;
-            } else {
-                const bool up0 = (lane & k) == 0;                  // element lane
-                const bool up1 = ((kWave + lane) & k) == 0;        // element 64 + lane
-                a0 = bitonic_step(a0, lane, j, up0);
-                a1 = bitonic_step(a1, lane, j, up1);
+    sort_merge<2>(a, lane);
+    sort_merge<4>(a, lane);
+    sort_merge<8>(a, lane);
+    sort_merge<16>(a, lane);
+    sort_merge<32>(a, lane);
+    sort_merge<64>(a, lane);
+    sort_merge<128>(a, lane);
+#pragma unroll
+    for (int t = 0; t < kSortRows; t++) {
+        if (n[t] > 2 * kWave) {  // uniform per wavefront
+            const int *__restrict__ src = tmp + (slot ? (row0 + t) * slot : (int64_t)b[t]);
+            for (int k = lane; k < n[t]; k += kWave) {
+                const int v = src[k];
+                int rank = 0;
+                for (int q = 0; q < n[t]; q++) {
+                    const int u = src[q];
+                    rank += (u < v) || (u == v && q < k);
+                }
+                cols[b[t] + rank] = v;
             }
+        } else {
+            if (lane < n[t]) cols[b[t] + lane] = (int)a[2 * t];
+            if (kWave + lane < n[t]) cols[b[t] + kWave + lane] = (int)a[2 * t + 1];
         }
     }
-    if (lane < n) cols[b + lane] = (int)a0;
-    if (kWave + lane < n) cols[b + kWave + lane] = (int)a1;
+}
+
+// Grid of csr_row_sort over N rows.
+static inline unsigned row_sort_grid(int64_t N)
+{
+    return (unsigned)(((N + kSortRows - 1) / kSortRows * kWave + 255) / 256);
 }
 
 // ------------------------------------------------------------ neighbour build
@@ -1531,7 +1595,7 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *ro
         if (order == 0)
             hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, tmpe, cols);
         else
-            hipLaunchKernelGGL(csr_row_sort, dim3(g2), dim3(256), 0, st, rowptr, N, tmp, cols, 0);
+            hipLaunchKernelGGL(csr_row_sort, dim3(row_sort_grid(N)), dim3(256), 0, st, rowptr, N, tmp, cols, 0);
         PAT_LAUNCH_CHECK();
     }
     return PAT_OK;
@@ -2039,7 +2103,7 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
     PAT_HIP(hipMemcpyAsync(w->cursor, w->cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(nb_scatter_atoms, dim3(gN), dim3(256), 0, st, w->cell, (int)N, w->cursor, w->atmp);
     PAT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((ncell * kWave + 255) / 256)), dim3(256), 0, st, w->cstart, ncell,
+    hipLaunchKernelGGL(csr_row_sort, dim3(row_sort_grid(ncell)), dim3(256), 0, st, w->cstart, ncell,
                        w->atmp, w->catoms, 0);
     PAT_LAUNCH_CHECK();
     // One pass: row lengths, and each row into a fixed slot of kNbSlot
@@ -2098,7 +2162,7 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
                            (const int *)rowptr, w->coltmp, (int *)nullptr);
         PAT_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(csr_row_sort, dim3((unsigned)((N * kWave + 255) / 256)), dim3(256), 0, st, (const int *)rowptr,
+    hipLaunchKernelGGL(csr_row_sort, dim3(row_sort_grid(N)), dim3(256), 0, st, (const int *)rowptr,
                        N, from_slots ? (const int *)w->slot : (const int *)w->coltmp, *cols, from_slots ? kNbSlot : 0);
     PAT_LAUNCH_CHECK();
     *total = tot;
