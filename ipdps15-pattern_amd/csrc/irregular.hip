@@ -321,6 +321,16 @@ __device__ __forceinline__ bool nb_within(float4 a, float4 b, const BinGeom &g)
     return r2 < g.cut2;
 }
 
+// The same test where every coordinate difference is known to lie in
+// [-L/2, L/2], on which the minimum image returns its argument unchanged.
+__device__ __forceinline__ bool nb_within_near(float4 a, float4 b, const BinGeom &g)
+{
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float r2 = dx * dx + dy * dy;
+    r2 = r2 + dz * dz;
+    return r2 < g.cut2;
+}
+
 // Candidate slots per lane of nb_cells: a wavefront holds 64 * kNbSlots
 // atoms of a bin's 27-bin neighbourhood in registers at a time (about 510
 // at the FCC 0.8442 density and 2.8-wide bins; larger ones take more
@@ -386,6 +396,18 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
     for (int a0 = a_beg; a0 < a_end; a0 += kWave) {
         const int na = min(kWave, a_end - a0);
         int acc = 0;  // lane l: entries found so far for atom slot a0 + l
+        // lane l: atom a0 + l, its position and row start / end, loaded
+        // together and broadcast in the atom loop
+        int my_i = 0, my_w = 0, my_wend = 0;
+        float4 my_p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < na) {
+            my_i = catoms[a0 + lane];
+            my_p = pos[my_i];
+            if (MODE == kNbFill) {
+                my_w = rowptr[my_i];
+                my_wend = rowptr[my_i + 1];
+            }
+        }
         for (int base = 0; base < total; base += kWave * kNbSlots) {
             float4 pj[kNbSlots];
             int jj[kNbSlots];
@@ -408,32 +430,66 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                     pj[r] = pos[jj[r]];
                 }
             }
-            for (int a = a0; a < a0 + na; a++) {
-                const int i = catoms[a];
-                const float4 pi = pos[i];
-                int w = 0, wend = 0;
-                if (MODE == kNbFill) {
-                    w = rowptr[i] + __shfl(acc, a - a0, kWave);
-                    wend = rowptr[i + 1];  // the counting pass sized the row; never write past it
-                } else if (MODE == kNbSlotFill) {
-                    w = i * kNbSlot + __shfl(acc, a - a0, kWave);
-                    wend = (i + 1) * kNbSlot;
-                }
-                int found = 0;
+            // When one pass holds every candidate (the bin's own atoms among
+            // them) and, per axis, the rounded span of their coordinates is
+            // below L/2, the exact span is too (rounding is monotone), so
+            // every difference lies in [-L/2, L/2] and the minimum image is
+            // the identity: skipping it gives the same bits.  Bins whose
+            // neighbourhood wraps around the box keep it.
+            bool near = false;
+            if (total <= kWave * kNbSlots) {  // uniform per wavefront
+                float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
 #pragma unroll
                 for (int r = 0; r < kNbSlots; r++) {
-                    const bool hit = jj[r] >= 0 && jj[r] != i && nb_within(pi, pj[r], g);
-                    const unsigned long long m = __ballot(hit);
-                    if (FILL && hit) {
-                        const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
-                        if (o < wend) tmp[o] = jj[r];
+                    if (jj[r] >= 0) {
+                        lx = fminf(lx, pj[r].x), hx = fmaxf(hx, pj[r].x);
+                        ly = fminf(ly, pj[r].y), hy = fmaxf(hy, pj[r].y);
+                        lz = fminf(lz, pj[r].z), hz = fmaxf(hz, pj[r].z);
                     }
-                    found += __popcll(m);
                 }
-                if (lane == a - a0) acc += found;
+#pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1) {
+                    lx = fminf(lx, __shfl_xor(lx, d, kWave)), hx = fmaxf(hx, __shfl_xor(hx, d, kWave));
+                    ly = fminf(ly, __shfl_xor(ly, d, kWave)), hy = fmaxf(hy, __shfl_xor(hy, d, kWave));
+                    lz = fminf(lz, __shfl_xor(lz, d, kWave)), hz = fmaxf(hz, __shfl_xor(hz, d, kWave));
+                }
+                near = hx - lx < g.hL && hy - ly < g.hL && hz - lz < g.hL;
             }
+            auto test_atoms = [&](auto near_c) {
+                constexpr bool NEAR = decltype(near_c)::value;
+                for (int a = a0; a < a0 + na; a++) {
+                    const int i = __shfl(my_i, a - a0, kWave);
+                    const float4 pi = make_float4(__shfl(my_p.x, a - a0, kWave), __shfl(my_p.y, a - a0, kWave),
+                                                  __shfl(my_p.z, a - a0, kWave), 0.f);
+                    int w = 0, wend = 0;
+                    if (MODE == kNbFill) {
+                        w = __shfl(my_w, a - a0, kWave) + __shfl(acc, a - a0, kWave);
+                        wend = __shfl(my_wend, a - a0, kWave);  // the counting pass sized the row; never write past it
+                    } else if (MODE == kNbSlotFill) {
+                        w = i * kNbSlot + __shfl(acc, a - a0, kWave);
+                        wend = (i + 1) * kNbSlot;
+                    }
+                    int found = 0;
+#pragma unroll
+                    for (int r = 0; r < kNbSlots; r++) {
+                        const bool hit = jj[r] >= 0 && jj[r] != i &&
+                                         (NEAR ? nb_within_near(pi, pj[r], g) : nb_within(pi, pj[r], g));
+                        const unsigned long long m = __ballot(hit);
+                        if (FILL && hit) {
+                            const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
+                            if (o < wend) tmp[o] = jj[r];
+                        }
+                        found += __popcll(m);
+                    }
+                    if (lane == a - a0) acc += found;
+                }
+            };
+            if (near)
+                test_atoms(std::true_type{});
+            else
+                test_atoms(std::false_type{});
         }
-        if (MODE != kNbFill && lane < na) deg[catoms[a0 + lane]] = acc;
+        if (MODE != kNbFill && lane < na) deg[my_i] = acc;
         if (MODE == kNbSlotFill && lane < na && acc > kNbSlot) *overflow = 1;
     }
 }
