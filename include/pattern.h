@@ -338,6 +338,11 @@ int pat_mdb_stats(pat_mdb *h, int64_t *num_pairs, int64_t *steps, int64_t *rebui
 int pat_mdb_destroy(pat_mdb *h);
 /* nsteps steps queued on 'stream' (NULL = the handle's stream), not waited for. */
 int pat_md_run_async(pat_md *h, int nsteps, void *stream);
+/* pat_md_run for a handle without rebuilds, with device_ms = the time
+ * between HIP events recorded on the handle's stream around the replay of
+ * the captured step graph (every force + update launch and the gaps between
+ * them).  Timing only; the reference has no counterpart. */
+int pat_md_run_graph_timed(pat_md *h, int nsteps, float *device_ms);
 /* nsteps steps launched one by one on the handle's stream with HIP events
  * around each force kernel: total_ms = whole run, kernel_ms[k] = step k's
  * force kernel (array of nsteps, may be NULL).  Blocking. */

@@ -1089,12 +1089,12 @@ int pat_md_set_row_order(pat_md *h, int order)
     return PAT_OK;
 }
 
-int pat_md_run(pat_md *h, int nsteps)
+// nsteps steps (no rebuilds) replayed from the handle's captured graph for
+// that step count and start buffer, captured on first use; device_ms, when
+// non-NULL, receives the replay's time between HIP events recorded on the
+// handle's stream around it.  Blocking.
+static int md_run_graph(pat_md *h, int nsteps, float *device_ms)
 {
-    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run: null handle");
-    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run: negative step count");
-    if (nsteps == 0) return PAT_OK;
-    if (h->every > 0) return h->run_binned(nsteps, nullptr);
     // the atomic path updates positions in place; the CSR path ping-pongs
     const int start = h->cur;
     auto key = std::make_pair(nsteps, start);
@@ -1112,11 +1112,35 @@ int pat_md_run(pat_md *h, int nsteps)
         PAT_HIP(ei);
         it = h->graphs.emplace(key, ge).first;
     }
+    pat::EventPair ev;
+    if (device_ms) {
+        PAT_HIP(ev.create());
+        PAT_HIP(hipEventRecord(ev.a, h->st.stream));
+    }
     PAT_HIP(hipGraphLaunch(it->second, h->st.stream));
+    if (device_ms) PAT_HIP(hipEventRecord(ev.b, h->st.stream));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
+    if (device_ms) PAT_HIP(hipEventElapsedTime(device_ms, ev.a, ev.b));
     if (h->st.variant != PAT_IR_ATOMIC && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
     h->steps += nsteps;
     return PAT_OK;
+}
+
+int pat_md_run(pat_md *h, int nsteps)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run: null handle");
+    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run: negative step count");
+    if (nsteps == 0) return PAT_OK;
+    if (h->every > 0) return h->run_binned(nsteps, nullptr);
+    return md_run_graph(h, nsteps, nullptr);
+}
+
+int pat_md_run_graph_timed(pat_md *h, int nsteps, float *device_ms)
+{
+    PAT_REQUIRE(h && device_ms, PAT_ERR_INVALID, "pat_md_run_graph_timed: null argument");
+    PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_md_run_graph_timed: need at least one step");
+    PAT_REQUIRE(h->every == 0, PAT_ERR_UNSUPPORTED, "pat_md_run_graph_timed: binned rebuilds run outside the graph");
+    return md_run_graph(h, nsteps, device_ms);
 }
 
 int pat_md_run_async(pat_md *h, int nsteps, void *stream)

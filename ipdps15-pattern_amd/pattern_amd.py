@@ -165,6 +165,7 @@ SIGNATURES = {
     "pat_md_run": (c_int, [c_vp, c_int]),
     "pat_md_run_async": (c_int, [c_vp, c_int, c_vp]),
     "pat_md_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
+    "pat_md_run_graph_timed": (c_int, [c_vp, c_int, c_fp]),
     "pat_md_get": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pat_md_energy": (c_int, [c_vp, c_dp, c_dp]),
     "pat_md_device_buffers": (c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
@@ -580,6 +581,13 @@ class MD:
         k = np.zeros(nsteps, dtype=np.float32)
         call("pat_md_run_timed", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
         return tot.value, k
+
+    def run_graph_timed(self, nsteps):
+        """run() of a handle without rebuilds; returns the device time (ms)
+        between HIP events around the step graph's replay."""
+        ms = c_float(0)
+        call("pat_md_run_graph_timed", self.h, nsteps, ctypes.byref(ms))
+        return ms.value
 
     def run_async(self, nsteps, stream=None):
         call("pat_md_run_async", self.h, nsteps, stream_ptr(stream))

@@ -195,6 +195,28 @@ def test_md_small_vs_oracle(pa, orc, gpu, small, variant, order):
     md.close()
 
 
+def test_graph_timed_run_matches_run(pa, orc, gpu, small):
+    """The timed graph replay advances the handle like run(): 7 + 7 + 6
+    steps (odd and even step counts, so both start buffers are replayed),
+    bit-identical to the oracle; each call reports a positive device time.
+    A handle with neighbour rebuilds refuses it."""
+    pos, box, edges = small
+    vel = pa.gen_velocities(len(pos), 1.44, seed=6)
+    md = pa.MD(pos, vel, edges, box, 2.5, 0.005)
+    ms = [md.run_graph_timed(7), md.run_graph_timed(7), md.run_graph_timed(6)]
+    p, v, f = md.get()
+    assert md.stats()[1] == 20
+    md.close()
+    assert all(t > 0 for t in ms)
+    rp, rv, rf, _ = orc.md_run(pos, vel, node_sorted(edges), box, 2.5, 0.005, 20)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
+    fp, fbox = pa.gen_fcc(6, 6, 6, rho=0.8442, jitter=0.0, seed=3)
+    mb = pa.MD(fp, None, None, fbox[0], 2.5, 0.005, skin=0.3, every=20)
+    with pytest.raises(pa.PatternError):
+        mb.run_graph_timed(5)
+    mb.close()
+
+
 @pytest.mark.parametrize("order", [1, 0])
 def test_moldyn_workload(pa, orc, gpu, order):
     """moldyn: 40^3 FCC cells (256,000 molecules), density 0.8442, jitter
