@@ -194,18 +194,20 @@ def bench_stencil(n, steps):
     s = pa.Stencil(n, n, n, alpha=0.1)
     s.upload(g)
     s.run_timed(5)
-    tot, kms = s.run_timed(steps)
-    s.run(steps)  # graph path (warm)
+    _, kms = s.run_timed(steps)  # sweeps launched one by one, events around each
+    s.run(steps)  # captures the sweep graph
+    s.run(steps)  # and replays it once: a graph's first two launches carry one-time costs
+    pa.call("pat_device_sync")
     w0 = time.perf_counter()
-    s.run(steps)
+    dev_ms = s.run_graph_timed(steps)
     wall = time.perf_counter() - w0
     s.close()
     cells = n ** 3
-    kavg = float(np.mean(kms))
+    kavg = dev_ms / steps  # device time per sweep inside the graph, gaps between launches included
     return {
         "grid": f"{n}^3", "steps": steps, "gcells_per_s": cells * steps / wall / 1e9,
         "kernel_us": kavg * 1e3, "achieved_gbs": 8 * cells / (kavg * 1e-3) / 1e9,
-        "frac": 8 * cells / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "frac": 8 * cells / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS, "eager_kernel_us": float(np.mean(kms)) * 1e3,
     }
 
 
@@ -373,7 +375,8 @@ def main():
         for n in (128, 384):
             st = bench_stencil(n, 100)
             engines[f"heat3d_{n}"] = st
-            log(f"[stencil] {n}^3: {st['gcells_per_s']:.1f} Gcell/s, kernel {st['kernel_us']:.1f} us, "
+            log(f"[stencil] {n}^3: {st['gcells_per_s']:.1f} Gcell/s, {st['kernel_us']:.1f} us per sweep in the graph "
+                f"({st['eager_kernel_us']:.1f} us launched one by one), "
                 f"{st['achieved_gbs']:.0f} GB/s ({100 * st['frac']:.1f}% HBM)")
     if "lj" not in which:
         print(json.dumps({"engines": engines}))

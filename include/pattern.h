@@ -149,6 +149,11 @@ int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
 int pat_stencil_upload(pat_stencil *s, const void *host);
 /* StencilBegin for nsteps iterations (kmeans/lib/stencil_runtime.cu:576-614). */
 int pat_stencil_run(pat_stencil *s, int nsteps);
+/* pat_stencil_run with device_ms = the time between HIP events recorded on
+ * the handle's stream around the replay of the captured sweep graph (every
+ * launch and the gaps between them).  Timing only; the reference has no
+ * counterpart. */
+int pat_stencil_run_graph_timed(pat_stencil *s, int nsteps, float *device_ms);
 /* nsteps sweeps launched one by one with HIP events around each kernel:
  * total_ms = first kernel start to last kernel end, kernel_ms[k] = sweep k
  * (array of nsteps, may be NULL).  Blocking. */

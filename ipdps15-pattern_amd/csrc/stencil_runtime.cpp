@@ -174,12 +174,12 @@ int pat_stencil_upload(pat_stencil *s, const void *host)
     return PAT_OK;
 }
 
-int pat_stencil_run(pat_stencil *s, int nsteps)
+// nsteps sweeps replayed from the handle's captured graph for that step
+// count and start buffer (captured on first use, after the tile is
+// resolved); device_ms, when non-NULL, receives the replay's time between
+// HIP events recorded on the handle's stream around it.  Blocking.
+static int stencil_run_graph(pat_stencil *s, int nsteps, float *device_ms)
 {
-    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_run: null handle");
-    PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run: no grid uploaded");
-    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_stencil_run: negative step count");
-    if (nsteps == 0) return PAT_OK;
     auto key = std::make_pair(nsteps, s->cur);
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
@@ -198,10 +198,34 @@ int pat_stencil_run(pat_stencil *s, int nsteps)
         PAT_HIP(ei);
         it = s->graphs.emplace(key, std::make_pair(ge, end)).first;
     }
+    pat::EventPair ev;
+    if (device_ms) {
+        PAT_HIP(ev.create());
+        PAT_HIP(hipEventRecord(ev.a, s->stream));
+    }
     PAT_HIP(hipGraphLaunch(it->second.first, s->stream));
+    if (device_ms) PAT_HIP(hipEventRecord(ev.b, s->stream));
     PAT_HIP(hipStreamSynchronize(s->stream));
+    if (device_ms) PAT_HIP(hipEventElapsedTime(device_ms, ev.a, ev.b));
     s->cur = it->second.second;
     return PAT_OK;
+}
+
+int pat_stencil_run(pat_stencil *s, int nsteps)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_run: null handle");
+    PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run: no grid uploaded");
+    PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_stencil_run: negative step count");
+    if (nsteps == 0) return PAT_OK;
+    return stencil_run_graph(s, nsteps, nullptr);
+}
+
+int pat_stencil_run_graph_timed(pat_stencil *s, int nsteps, float *device_ms)
+{
+    PAT_REQUIRE(s && device_ms, PAT_ERR_INVALID, "pat_stencil_run_graph_timed: null argument");
+    PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run_graph_timed: no grid uploaded");
+    PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_stencil_run_graph_timed: need at least one step");
+    return stencil_run_graph(s, nsteps, device_ms);
 }
 
 int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *kernel_ms)

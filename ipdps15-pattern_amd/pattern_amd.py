@@ -111,6 +111,7 @@ SIGNATURES = {
     "pat_stencil_upload": (c_int, [c_vp, c_vp]),
     "pat_stencil_run": (c_int, [c_vp, c_int]),
     "pat_stencil_run_timed": (c_int, [c_vp, c_int, c_fp, c_fp]),
+    "pat_stencil_run_graph_timed": (c_int, [c_vp, c_int, c_fp]),
     "pat_stencil_download": (c_int, [c_vp, c_vp]),
     "pat_stencil_tile": (c_int, [c_vp, c_intp, c_intp, c_intp]),
     "pat_stencil_device_ptr": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
@@ -416,6 +417,13 @@ class Stencil:
         k = np.zeros(nsteps, dtype=np.float32)
         call("pat_stencil_run_timed", self.h, nsteps, ctypes.byref(tot), k.ctypes.data_as(c_fp))
         return tot.value, k
+
+    def run_graph_timed(self, nsteps):
+        """run(); returns the device time (ms) between HIP events around the
+        sweep graph's replay."""
+        ms = c_float(0)
+        call("pat_stencil_run_graph_timed", self.h, nsteps, ctypes.byref(ms))
+        return ms.value
 
     def download(self):
         out = np.empty(self.shape, dtype=np.float32)

@@ -104,6 +104,20 @@ def test_handle_graph_replay(pa, orc, gpu):
     assert np.array_equal(got, orc.heat3d(g, 20))
 
 
+def test_graph_timed_run_matches_run(pa, orc, gpu):
+    """The timed replay advances the grid like run() (7 + 7 + 6 sweeps, so
+    graphs start from both buffers) and reports a positive device time."""
+    nx, ny, nz = 70, 50, 40
+    g = pa.gen_heat3d(nx, ny, nz, seed=15)
+    s = pa.Stencil(nx, ny, nz)
+    s.upload(g)
+    ms = [s.run_graph_timed(7), s.run_graph_timed(7), s.run_graph_timed(6)]
+    got = s.download()
+    s.close()
+    assert all(t > 0 for t in ms)
+    assert np.array_equal(got, orc.heat3d(g, 20))
+
+
 def test_eigenmode_full_size_384(pa, gpu):
     """384^3 (over the 256 MB last-level cache): closed-form decay of a sine mode."""
     n, steps, alpha = 384, 10, 0.1
