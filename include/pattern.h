@@ -312,7 +312,8 @@ int pat_md_set_slice_order(pat_md *h, int order);
 /* The slice order of the current layout (0, 1, 2 as above). */
 int pat_md_get_slice_order(pat_md *h, int *order);
 /* Current half-pair count, steps run, neighbour rebuilds and their total
- * host-measured time. */
+ * device time (HIP events around each rebuild's launches, the read-back of
+ * the pair count between them included). */
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms);
 /* Current CSR by destination (rowptr: N+1, cols: 2 * num_pairs). */
 int pat_md_csr(pat_md *h, int32_t *rowptr, int32_t *cols);

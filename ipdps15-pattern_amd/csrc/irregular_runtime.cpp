@@ -586,20 +586,47 @@ struct pat_md {
     int64_t cap = 0;    // capacity of st.cols
     int64_t steps = 0;  // steps completed
     int64_t rebuilds = 0;
-    float rebuild_ms = 0.0f;  // host-measured time of all rebuilds
+    float rebuild_ms = 0.0f;  // device time of all rebuilds (settled ones)
+    pat::EventPair rb_ev;     // around the last rebuild's launches
+    bool rb_pending = false;  // rb_ev not yet added to rebuild_ms
+
+    // Adds the last rebuild's device time once its end event has passed.
+    int settle_rebuild_time()
+    {
+        if (!rb_pending) return PAT_OK;
+        float ms = 0.0f;
+        PAT_HIP(hipEventSynchronize(rb_ev.b));
+        PAT_HIP(hipEventElapsedTime(&ms, rb_ev.a, rb_ev.b));
+        rebuild_ms += ms;
+        rb_pending = false;
+        return PAT_OK;
+    }
+    int begin_rebuild_time()
+    {
+        PAT_TRY(settle_rebuild_time());
+        if (!rb_ev.a) PAT_HIP(rb_ev.create());
+        PAT_HIP(hipEventRecord(rb_ev.a, st.stream));
+        return PAT_OK;
+    }
+    int end_rebuild_time()
+    {
+        PAT_HIP(hipEventRecord(rb_ev.b, st.stream));
+        rb_pending = true;
+        return PAT_OK;
+    }
     pat::NbWork *nbw = nullptr;
 
     // Rebuilds the CSR from the positions in buffer c.
     int rebuild(int c)
     {
         int64_t total = 0;
-        auto t0 = std::chrono::steady_clock::now();
+        PAT_TRY(begin_rebuild_time());
         PAT_TRY(pat::neighbor_build((const float4 *)st.pos[c], st.N, box, cutneigh, nbw, st.rowptr, &st.cols, &cap,
                                     &total, st.stream));
         PAT_HIP(hipStreamSynchronize(st.stream));
         st.E = total / 2;
         PAT_TRY(st.build_sell(st.pos[c], box, pbc, rc2));
-        rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        PAT_TRY(end_rebuild_time());
         if (rebuilds++ >= 1 && st.choice_tuned && !st.retuned) {
             st.retuned = true;
             PAT_TRY(st.tune(st.pos[c], box, pbc, rc2, true));
@@ -968,6 +995,7 @@ int pat_md_create_binned_dev(int64_t num_nodes, const float *pos4, const float *
 int pat_md_stats(pat_md *h, int64_t *num_pairs, int64_t *steps, int64_t *rebuilds, float *rebuild_ms)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_stats: null handle");
+    PAT_TRY(h->settle_rebuild_time());
     if (num_pairs) *num_pairs = h->st.E;
     if (steps) *steps = h->steps;
     if (rebuilds) *rebuilds = h->rebuilds;
@@ -1285,7 +1313,7 @@ struct pat_mdb {
     // is the concatenated one), then the sliced layout.
     int rebuild_all(int c)
     {
-        auto t0 = std::chrono::steady_clock::now();
+        PAT_TRY(md.begin_rebuild_time());
         int64_t entries = 0;
         PAT_TRY(pat::neighbor_build_multi((const float4 *)md.st.pos[c], (int)box.size(), box.data(), off.data(),
                                           md.cutneigh, md.nbw, md.st.rowptr, &md.st.cols, &md.cap, &entries,
@@ -1293,7 +1321,7 @@ struct pat_mdb {
         PAT_HIP(hipStreamSynchronize(md.st.stream));
         md.st.E = entries / 2;
         PAT_TRY(md.st.build_sell(md.st.pos[c], 1.0f, 2, md.rc2));
-        md.rebuild_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        PAT_TRY(md.end_rebuild_time());
         if (md.rebuilds++ >= 1 && md.st.choice_tuned && !md.st.retuned) {
             md.st.retuned = true;
             PAT_TRY(md.st.tune(md.st.pos[c], 1.0f, 2, md.rc2, true));
