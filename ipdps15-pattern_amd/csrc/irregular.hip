@@ -954,12 +954,25 @@ __global__ __launch_bounds__(256) void sell_fill(const int *__restrict__ rowptr,
     const int sh = pad < 0 ? 0 : 4;
     const int base = sptr[s], w = (sptr[s + 1] - base) / kWave;
     const int half = lane >> 5, t = lane & 31;
+    const int last = rowptr[N] - 1;  // last entry of the CSR (-1: none)
     for (int k0 = 0; k0 < w; k0 += kFillK) {
-        for (int m = 0; m < kWave / 2; m++) {
-            const int rr = 2 * m + half;  // lanes 0-31: row 2m, lanes 32-63: row 2m+1
-            const int rb = __shfl(b, rr, kWave), re = __shfl(e, rr, kWave), rf = __shfl(fill, rr, kWave);
-            const int k = rb + k0 + t;
-            tile[wv][rr][t] = k < re ? (int)((uint32_t)cols[k] << sh) : rf;
+        if (last >= 0) {  // uniform per wavefront
+            // every lane loads (at an index clamped into the CSR) and the
+            // padding is selected afterwards, so the 32 loads are in flight
+            // together instead of one branch and wait each
+#pragma unroll
+            for (int m = 0; m < kWave / 2; m++) {
+                const int rr = 2 * m + half;  // lanes 0-31: row 2m, lanes 32-63: row 2m+1
+                const int rb = __shfl(b, rr, kWave), re = __shfl(e, rr, kWave), rf = __shfl(fill, rr, kWave);
+                const int k = rb + k0 + t;
+                const int v = cols[min(k, last)];
+                tile[wv][rr][t] = k < re ? (int)((uint32_t)v << sh) : rf;
+            }
+        } else {
+            for (int m = 0; m < kWave / 2; m++) {
+                const int rr = 2 * m + half;
+                tile[wv][rr][t] = __shfl(fill, rr, kWave);
+            }
         }
         wave_lds_sync();
         const int kn = min(kFillK, w - k0);
