@@ -6,6 +6,8 @@ generator is itself pinned to miniMD's Neighbor::build and to brute force).
 Dynamics with rebuilds every 20 steps are compared with the FP32 oracle
 rebuilding its pair list at the same steps: 1e-5 relative L-infinity.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -73,12 +75,19 @@ def test_binned_md_vs_oracle(pa, orc, gpu, lanes):
     pos, box = pa.gen_fcc(8, 8, 8, rho=0.8442, jitter=0.0, seed=4)
     vel = pa.gen_velocities(len(pos), 1.44, seed=5)
     md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20, lanes_per_row=lanes)
+    t0 = time.perf_counter()
     md.run(25)
     md.run(15)
+    wall_ms = (time.perf_counter() - t0) * 1e3
     p, v, f = md.get()
-    npairs, steps, rebuilds, _ = md.stats()
+    npairs, steps, rebuilds, rb_ms = md.stats()
+    _, _, _, rb_ms_again = md.stats()
     md.close()
     assert steps == 40 and rebuilds == 3  # at creation, step 20, step 40
+    # device time of the rebuilds (events around each, settled by stats);
+    # the two inside the runs fit in the runs' wall time
+    assert 0 < rb_ms and rb_ms == rb_ms_again
+    assert rb_ms < wall_ms + 1e3
     rp, rv, rf, rn = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 20, 0.005, 40)
     assert npairs == rn
     assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
