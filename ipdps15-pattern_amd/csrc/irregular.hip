@@ -1609,6 +1609,18 @@ int64_t csr_workspace_bytes(int64_t N, int64_t E)
     return (int64_t)sizeof(int) * ((N + 1) + 2 * nb + 2 * nb2 + 2 + 4 * E) + 256;
 }
 
+// Sum of n non-negative ints in 64 bits (into *out, which starts at 0):
+// the neighbour build checks the total entry count with it before the int32
+// row pointers are used, since an int32 scan of more than 2^31 entries wraps.
+__global__ void sum_nonneg_i64(const int *__restrict__ a, int64_t n, unsigned long long *__restrict__ out)
+{
+    unsigned long long s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += (unsigned)a[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
 // Exclusive scan of n ints (n <= kScanBlock^3) using ws for block sums.
 static int scan_exclusive(const int *in, int *out, int64_t n, int *ws, hipStream_t st)
 {
@@ -2064,6 +2076,7 @@ struct NbWork {
         *ws = nullptr, *coltmp = nullptr;
     NbSys *sys = nullptr;
     int *slot = nullptr, *flag = nullptr;  // one-pass rows: kNbSlot entries per atom; overflow flag
+    unsigned long long *tot64 = nullptr;   // 64-bit sum of the row lengths
     int64_t slot_cap = 0;
     bool slot_fail = false;  // a row outgrew its slot: two passes from then on
     std::vector<NbSys> sys_h;  // source of the last upload (read before the build returns)
@@ -2074,8 +2087,28 @@ static void nb_free(NbWork &w)
     for (int *p : {w.cell, w.cstart, w.cursor, w.atmp, w.catoms, w.deg, w.ws, w.coltmp, w.slot, w.flag})
         if (p) (void)pat::dev_free(p);
     if (w.sys) (void)pat::dev_free(w.sys);
+    if (w.tot64) (void)pat::dev_free(w.tot64);
     w = NbWork{};
 }
+
+// The neighbour build's entry count check: the 64-bit sum of the row
+// lengths must fit an int32 row pointer and equal the int32 scan's total.
+extern "C" int pat_internal_nb_check_total(unsigned long long tot64, int tot32);
+int nb_check_total(unsigned long long tot64, int tot32) { return pat_internal_nb_check_total(tot64, tot32); }
+}  // namespace pat
+
+// Exported (outside the public header) so the CPU tests can check it with
+// synthetic totals, including sums that wrap an int32 scan.
+extern "C" int pat_internal_nb_check_total(unsigned long long tot64, int tot32)
+{
+    PAT_REQUIRE(tot64 <= (unsigned long long)INT32_MAX, PAT_ERR_UNSUPPORTED,
+                "neighbour build: %llu list entries exceed the int32 row pointers (at most 2^31-1)", tot64);
+    PAT_REQUIRE((long long)tot32 == (long long)tot64, PAT_ERR_INVALID,
+                "neighbour build: scanned total %d differs from the 64-bit sum %llu", tot32, tot64);
+    return PAT_OK;
+}
+
+namespace pat {
 
 NbWork *nb_create() { return new NbWork(); }
 void nb_destroy(NbWork *w)
@@ -2202,12 +2235,26 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
     }
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
+    if (!w->tot64 && pat::dev_malloc((void **)&w->tot64, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipGetLastError();
+        w->tot64 = nullptr;
+        pat::set_error("neighbour build: hipMalloc of 8 bytes failed");
+        return PAT_ERR_NOMEM;
+    }
+    PAT_HIP(hipMemsetAsync(w->tot64, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(sum_nonneg_i64, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 1024)), dim3(256), 0, st,
+                       (const int *)w->deg, N, w->tot64);
+    PAT_LAUNCH_CHECK();
     PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
     int tot = 0, over = 0;
+    unsigned long long tot64 = 0;
     PAT_HIP(hipMemcpyAsync(&tot, rowptr + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipMemcpyAsync(&tot64, w->tot64, sizeof(tot64), hipMemcpyDeviceToHost, st));
     if (slots) PAT_HIP(hipMemcpyAsync(&over, w->flag, sizeof(int), hipMemcpyDeviceToHost, st));
     PAT_HIP(hipStreamSynchronize(st));
-    PAT_REQUIRE(tot >= 0, PAT_ERR_INVALID, "neighbour build: pair count overflows int32");
+    // Nothing has been written at the scanned offsets yet: refuse before any
+    // fill when the entries do not fit int32 row pointers.
+    PAT_TRY(nb_check_total(tot64, tot));
     if (over) w->slot_fail = true;
     const bool from_slots = slots && !over;
     if (tot > *cap) {

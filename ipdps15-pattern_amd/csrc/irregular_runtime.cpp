@@ -9,6 +9,7 @@
 // arrays are never modified: edges are uploaded once and turned into a
 // by-destination CSR on the device, node data stays resident, and results
 // come back in the caller's node order.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -220,9 +221,28 @@ struct IrState {
     // the plain CSR only) when its padding would exceed kSellMaxPad.  With
     // positions p4 (periodic box, sentinel layout) the slices follow the
     // row-class order computed from them.
+    // What the captured force launches take as arguments: buffers, kernel
+    // choice and launch shape.  A rebuild that leaves all of it unchanged
+    // keeps captured graphs valid: their kernels read the rebuilt contents
+    // in place.
+    std::array<int64_t, 20> launch_sig() const
+    {
+        return {(int64_t)(intptr_t)rowptr, (int64_t)(intptr_t)cols, (int64_t)(intptr_t)sptr, (int64_t)(intptr_t)scol,
+                (int64_t)(intptr_t)scol16, (int64_t)(intptr_t)hptr, (int64_t)(intptr_t)hidx, (int64_t)(intptr_t)perm,
+                (int64_t)(intptr_t)force, (int64_t)(intptr_t)vel, (int64_t)(intptr_t)pos[0], (int64_t)(intptr_t)pos[1],
+                lpr, variant, sell_kind, order_on, (int64_t)sell_ok, (int64_t)halo_ok, halo_max, N};
+    }
+
     int build_sell(const float *p4 = nullptr, float box = 0.f, int pbc = 0, float rc2 = 0.f)
     {
-        layout_gen++;
+        const auto before = launch_sig();
+        const int rc = build_sell_body(p4, box, pbc, rc2);
+        if (launch_sig() != before) layout_gen++;
+        return rc;
+    }
+
+    int build_sell_body(const float *p4, float box, int pbc, float rc2)
+    {
         order_on = 0;
         halo_ok = false;
         if (want_order && p4 && pbc && sentinel()) {
@@ -257,12 +277,20 @@ struct IrState {
         }
         if (rc != PAT_OK) return rc;
         sell_ok = true;
-        return want_halo ? build_halo() : PAT_OK;
+        return want_halo ? build_halo_body() : PAT_OK;
     }
 
     // Halo copy of the current sentinel layout; halo_ok false when some
     // block does not fit it (the global-gather kinds remain).
     int build_halo()
+    {
+        const auto before = launch_sig();
+        const int rc = build_halo_body();
+        if (launch_sig() != before) layout_gen++;
+        return rc;
+    }
+
+    int build_halo_body()
     {
         halo_ok = false;
         if (!sell_ok || !sentinel()) return PAT_OK;
@@ -585,6 +613,9 @@ struct pat_md {
     double cutneigh = 0.0;
     int64_t cap = 0;    // capacity of st.cols
     int64_t steps = 0;  // steps completed
+    // a run failed after its first launch was queued: positions, velocities
+    // and the step count no longer agree, so later calls refuse
+    bool failed = false;
     int64_t rebuilds = 0;
     float rebuild_ms = 0.0f;  // device time of all rebuilds (settled ones)
     pat::EventPair rb_ev;     // around the last rebuild's launches
@@ -639,13 +670,11 @@ struct pat_md {
     int run_binned(int nsteps, float *kernel_ms)
     {
         hipStream_t s = st.stream;
-        std::vector<hipEvent_t> ev;
-        if (kernel_ms) {
-            ev.resize(2 * (size_t)nsteps);
-            for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
-        }
+        pat::EventVec ev;
+        if (kernel_ms) PAT_HIP(ev.create(2 * (size_t)nsteps));
         int c = cur;
         int rc = st.begin_steps(st.pos[c], box, pbc, dt, s);
+        failed = rc != PAT_OK;
         for (int k = 0; k < nsteps && rc == PAT_OK; k++) {
             const bool last = k == nsteps - 1;
             if ((steps + k + 1) % every == 0) rc = rebuild(c);
@@ -664,8 +693,9 @@ struct pat_md {
                 for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
             cur = c;
             steps += nsteps;
+        } else {
+            failed = true;
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
         return rc;
     }
 
@@ -1145,9 +1175,11 @@ static int md_run_graph(pat_md *h, int nsteps, float *device_ms)
         PAT_HIP(ev.create());
         PAT_HIP(hipEventRecord(ev.a, h->st.stream));
     }
+    h->failed = true;  // until the replay has completed
     PAT_HIP(hipGraphLaunch(it->second, h->st.stream));
     if (device_ms) PAT_HIP(hipEventRecord(ev.b, h->st.stream));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
+    h->failed = false;
     if (device_ms) PAT_HIP(hipEventElapsedTime(device_ms, ev.a, ev.b));
     if (h->st.variant != PAT_IR_ATOMIC && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
     h->steps += nsteps;
@@ -1157,6 +1189,8 @@ static int md_run_graph(pat_md *h, int nsteps, float *device_ms)
 int pat_md_run(pat_md *h, int nsteps)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run: null handle");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_run: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run: negative step count");
     if (nsteps == 0) return PAT_OK;
     if (h->every > 0) return h->run_binned(nsteps, nullptr);
@@ -1166,6 +1200,8 @@ int pat_md_run(pat_md *h, int nsteps)
 int pat_md_run_graph_timed(pat_md *h, int nsteps, float *device_ms)
 {
     PAT_REQUIRE(h && device_ms, PAT_ERR_INVALID, "pat_md_run_graph_timed: null argument");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_run_graph_timed: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_md_run_graph_timed: need at least one step");
     PAT_REQUIRE(h->every == 0, PAT_ERR_UNSUPPORTED, "pat_md_run_graph_timed: binned rebuilds run outside the graph");
     return md_run_graph(h, nsteps, device_ms);
@@ -1174,6 +1210,8 @@ int pat_md_run_graph_timed(pat_md *h, int nsteps, float *device_ms)
 int pat_md_run_async(pat_md *h, int nsteps, void *stream)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_run_async: null handle");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_run_async: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(h->every == 0, PAT_ERR_UNSUPPORTED, "pat_md_run_async: binned rebuilds need blocking runs");
     PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_md_run_async: negative step count");
     if (nsteps == 0) return PAT_OK;
@@ -1181,7 +1219,10 @@ int pat_md_run_async(pat_md *h, int nsteps, void *stream)
     h->st.stream = stream ? (hipStream_t)stream : saved;
     int rc = h->enqueue(nsteps, h->cur);
     h->st.stream = saved;
-    if (rc != PAT_OK) return rc;
+    if (rc != PAT_OK) {
+        h->failed = true;
+        return rc;
+    }
     if (h->st.variant != PAT_IR_ATOMIC && (nsteps - 1) % 2 == 1) h->cur = 1 - h->cur;
     h->steps += nsteps;
     return PAT_OK;
@@ -1190,6 +1231,8 @@ int pat_md_run_async(pat_md *h, int nsteps, void *stream)
 int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
 {
     PAT_REQUIRE(h && total_ms, PAT_ERR_INVALID, "pat_md_run_timed: null argument");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_run_timed: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_md_run_timed: need at least one step");
     if (h->every > 0) {
         auto t0 = std::chrono::steady_clock::now();
@@ -1198,11 +1241,8 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
         return r;
     }
     hipStream_t s = h->st.stream;
-    std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 2);
-    for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
-    auto cleanup = [&]() {
-        for (auto &e : ev) (void)hipEventDestroy(e);
-    };
+    pat::EventVec ev;
+    PAT_HIP(ev.create(2 * (size_t)nsteps + 2));
     PAT_HIP(hipEventRecord(ev[0], s));
     int c = h->cur;
     int rc = PAT_OK;
@@ -1237,16 +1277,20 @@ int pat_md_run_timed(pat_md *h, int nsteps, float *total_ms, float *kernel_ms)
             for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[1 + 2 * k], ev[2 + 2 * k]);
         h->cur = c;
         h->steps += nsteps;
-    } else if (rc == PAT_ERR_HIP) {
+    } else {
+        h->failed = true;
+    }
+    if (rc == PAT_ERR_HIP) {
         pat::set_error("pat_md_run_timed: HIP call failed: %s", hipGetErrorString(hipGetLastError()));
     }
-    cleanup();
     return rc;
 }
 
 int pat_md_get(pat_md *h, float *pos, float *vel, float *force)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_get: null handle");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_get: an earlier run of this handle failed part-way; its state is undefined");
     PAT_HIP(hipStreamSynchronize(h->st.stream));
     if (pos) PAT_TRY(h->st.download3(pos, h->st.pos[h->cur]));
     if (vel) PAT_TRY(h->st.download3(vel, h->st.vel));
@@ -1257,6 +1301,8 @@ int pat_md_get(pat_md *h, float *pos, float *vel, float *force)
 int pat_md_energy(pat_md *h, double *epot, double *ekin)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_md_energy: null handle");
+    PAT_REQUIRE(!h->failed, PAT_ERR_STATE,
+                "pat_md_energy: an earlier run of this handle failed part-way; its state is undefined");
     if (epot) {
         PAT_TRY(pat::lj_energy_csr((const float4 *)h->st.pos[h->cur], h->st.rowptr, h->st.cols, h->st.N, h->box,
                                    h->pbc, h->rc2, h->st.node_e, h->st.stream));
@@ -1480,10 +1526,13 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
 int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_run: null handle");
+    PAT_REQUIRE(!h->md.failed, PAT_ERR_STATE,
+                "pat_mdb_run: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(nsteps >= 0, PAT_ERR_INVALID, "pat_mdb_run: negative step count");
     if (nsteps == 0) return PAT_OK;
     auto t0 = std::chrono::steady_clock::now();
     int rc = h->run(nsteps, kernel_ms);
+    if (rc != PAT_OK) h->md.failed = true;
     if (total_ms) *total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
@@ -1526,6 +1575,8 @@ int pat_mdb_set_slice_order(pat_mdb *h, int order)
 int pat_mdb_get(pat_mdb *h, int sys, float *pos, float *vel, float *force)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_mdb_get: null handle");
+    PAT_REQUIRE(!h->md.failed, PAT_ERR_STATE,
+                "pat_mdb_get: an earlier run of this handle failed part-way; its state is undefined");
     PAT_REQUIRE(sys >= 0 && sys < (int)h->box.size(), PAT_ERR_INVALID, "pat_mdb_get: system %d out of range", sys);
     const int64_t o = h->off[sys], n = h->off[sys + 1] - o;
     std::vector<float> buf(4 * (size_t)n);

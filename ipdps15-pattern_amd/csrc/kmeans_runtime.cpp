@@ -322,8 +322,8 @@ int pat_kmeans_run_timed(pat_kmeans *h, int iters, float *total_ms, float *assig
 {
     PAT_REQUIRE(h && total_ms, PAT_ERR_INVALID, "pat_kmeans_run_timed: null argument");
     PAT_REQUIRE(iters > 0, PAT_ERR_INVALID, "pat_kmeans_run_timed: need at least one iteration");
-    std::vector<hipEvent_t> ev(2 * (size_t)iters + 2);
-    for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
+    pat::EventVec ev;
+    PAT_HIP(ev.create(2 * (size_t)iters + 2));
     int rc = PAT_OK;
     (void)hipEventRecord(ev[0], h->stream);
     for (int i = 0; i < iters && rc == PAT_OK; i++) {
@@ -350,7 +350,6 @@ int pat_kmeans_run_timed(pat_kmeans *h, int iters, float *total_ms, float *assig
     } else if (rc == PAT_ERR_HIP) {
         pat::set_error("pat_kmeans_run_timed: HIP call failed");
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 

@@ -76,6 +76,33 @@ struct EventPair {
     }
 };
 
+// HIP events destroyed on every path out of the scope that owns them.
+struct EventVec {
+    std::vector<hipEvent_t> ev;
+    EventVec() = default;
+    EventVec(const EventVec &) = delete;
+    EventVec &operator=(const EventVec &) = delete;
+    ~EventVec()
+    {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    hipError_t create(size_t n)
+    {
+        ev.assign(n, nullptr);
+        for (hipEvent_t &e : ev) {
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                e = nullptr;
+                return r;
+            }
+        }
+        return hipSuccess;
+    }
+    hipEvent_t &operator[](size_t i) { return ev[i]; }
+    size_t size() const { return ev.size(); }
+};
+
 }  // namespace pat
 
 #define PAT_HIP(call)                                                                                      \

@@ -351,8 +351,12 @@ int pat_serve_submit(pat_serve *s, const pat_job *job, int64_t *id)
                     "pat_serve_submit: heat3D side %lld / steps %d out of range", (long long)j.size, j.steps);
         break;
     case PAT_JOB_LJ:
-        PAT_REQUIRE(j.size >= 2 && j.size <= 400 && j.steps >= 0, PAT_ERR_INVALID,
-                    "pat_serve_submit: LJ cells %lld / steps %d out of range", (long long)j.size, j.steps);
+        // 4 size^3 atoms with ~78 neighbour-list entries each (cutoff + skin
+        // 2.8 at density 0.8442) must stay below 2^31 entries: 160^3 cells
+        // list ~1.3e9.
+        PAT_REQUIRE(j.size >= 2 && j.size <= 160 && j.steps >= 0, PAT_ERR_INVALID,
+                    "pat_serve_submit: LJ cells %lld / steps %d out of range (2..160 cells per side)",
+                    (long long)j.size, j.steps);
         break;
     case PAT_JOB_KMEANS:
         PAT_REQUIRE(j.dim >= 1 && j.dim <= 128 && j.k >= 1 && j.k <= j.size && j.steps >= 1, PAT_ERR_INVALID,

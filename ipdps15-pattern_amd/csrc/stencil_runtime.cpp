@@ -233,8 +233,8 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     PAT_REQUIRE(s && total_ms, PAT_ERR_INVALID, "pat_stencil_run_timed: null argument");
     PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run_timed: no grid uploaded");
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_stencil_run_timed: need at least one step");
-    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
-    for (auto &e : ev) PAT_HIP(hipEventCreate(&e));
+    pat::EventVec ev;
+    PAT_HIP(ev.create(2 * (size_t)nsteps));
     pat::HeatTile t;
     PAT_TRY(s->resolve_tile(&t));
     const int v = s->kernel_variant();
@@ -276,7 +276,6 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
         }
         s->cur = c;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 

@@ -399,6 +399,59 @@ void orc_lj_force_f32_gather(int64_t N, int64_t E, const int32_t *edges, const f
 }
 
 /*
+ * FP64 form of orc_lj_force_f32_gather (same incidence order, pair_force_f64
+ * arithmetic): the FP64 shadow of the full-size runs, threaded over nodes.
+ */
+void orc_lj_force_f64_gather(int64_t N, int64_t E, const int32_t *edges, const double *pos, double L, int pbc,
+                             double rc2, double *f, int nthreads)
+{
+    const double hL = 0.5 * L;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    int64_t *start = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+    int64_t *inc = (int64_t *)malloc(sizeof(int64_t) * 2 * (size_t)(E > 0 ? E : 1));
+    for (int64_t e = 0; e < E; e++) {
+        start[edges[2 * e] + 1]++;
+        start[edges[2 * e + 1] + 1]++;
+    }
+    for (int64_t i = 0; i < N; i++) start[i + 1] += start[i];
+    {
+        int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N > 0 ? N : 1));
+        memcpy(cur, start, sizeof(int64_t) * N);
+        for (int64_t e = 0; e < E; e++) {
+            inc[cur[edges[2 * e]]++] = 2 * e;
+            inc[cur[edges[2 * e + 1]]++] = 2 * e + 1;
+        }
+        free(cur);
+    }
+    #pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+    for (int64_t r = 0; r < N; r++) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int64_t k = start[r]; k < start[r + 1]; k++) {
+            const int64_t e = inc[k] >> 1;
+            const int side = (int)(inc[k] & 1);
+            const int i = edges[2 * e], j = edges[2 * e + 1];
+            double fx, fy, fz;
+            if (pair_force_f64(pos + 3 * (int64_t)i, pos + 3 * (int64_t)j, L, hL, pbc, rc2, &fx, &fy, &fz, 0)) {
+                if (side == 0) {
+                    ax += fx;
+                    ay += fy;
+                    az += fz;
+                } else {
+                    ax -= fx;
+                    ay -= fy;
+                    az -= fz;
+                }
+            }
+        }
+        f[3 * r] = ax;
+        f[3 * r + 1] = ay;
+        f[3 * r + 2] = az;
+    }
+    free(start);
+    free(inc);
+}
+
+/*
  * Cell-list half pair list: every pair i<j whose minimum-image distance
  * squared (same float expression as orc_pairs_brute) is below cut2, in
  * (i, j) lexicographic order.  Cells are at least one cutoff wide (needs
@@ -573,6 +626,55 @@ static inline double wrap_f64(double x, double L)
     if (x < 0.0) x += L;
     else if (x >= L) x -= L;
     return x;
+}
+
+/*
+ * FP64 form of orc_md_run_rebuild_f32: the same velocity-Verlet loop and
+ * rebuild schedule, positions, velocities and forces in FP64.  The pair list
+ * is built from the positions rounded to FP32 with the neighbour cutoff
+ * (cutneigh > rc by the skin, so every pair within rc is listed); forces
+ * use the FP64 minimum image and cutoff test.
+ */
+double orc_md_run_rebuild_f64(int64_t N, double *pos, double *vel, double *f, double L, double rc2, double cutneigh,
+                              int every, double dt, int nsteps, int nthreads, int64_t *npairs_out)
+{
+    const double hdt = 0.5 * dt;
+    int64_t cap = 0, np = 0;
+    int32_t *edges = NULL;
+    float *p32 = (float *)malloc(sizeof(float) * 3 * (size_t)(N > 0 ? N : 1));
+    for (int s = 0; s <= nsteps; s++) {
+        if (s > 0) {
+            for (int64_t m = 0; m < 3 * N; m++) {
+                vel[m] = vel[m] + hdt * f[m];
+                pos[m] = wrap_f64(pos[m] + dt * vel[m], L);
+            }
+        }
+        if (s == 0 || s % every == 0) {
+            for (int64_t m = 0; m < 3 * N; m++) {
+                float x = (float)pos[m];
+                p32[m] = x >= (float)L ? 0.0f : x; /* rounding may reach L */
+            }
+            np = orc_pairs_cells(N, p32, (float)L, (float)cutneigh, NULL, 0);
+            if (np < 0) {
+                free(edges);
+                free(p32);
+                return NAN;
+            }
+            if (np > cap) {
+                free(edges);
+                cap = np;
+                edges = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(cap > 0 ? cap : 1));
+            }
+            orc_pairs_cells(N, p32, (float)L, (float)cutneigh, edges, cap);
+        }
+        orc_lj_force_f64_gather(N, np, edges, pos, L, 1, rc2, f, nthreads);
+        if (s > 0)
+            for (int64_t m = 0; m < 3 * N; m++) vel[m] = vel[m] + hdt * f[m];
+    }
+    if (npairs_out) *npairs_out = np;
+    free(edges);
+    free(p32);
+    return 0.0;
 }
 
 double orc_md_run_f64(int64_t N, int64_t E, const int32_t *edges, double *pos, double *vel, double *f, double L,

@@ -34,6 +34,9 @@ _SIG = {
     "orc_lj_force_f32_gather": (None, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp, c_int]),
     "orc_md_run_rebuild_f32": (c_double, [c_i64, c_vp, c_vp, c_vp, c_float, c_float, c_float, c_int, c_float, c_int,
                                           c_int, ctypes.POINTER(c_i64)]),
+    "orc_lj_force_f64_gather": (None, [c_i64, c_i64, c_vp, c_vp, c_double, c_int, c_double, c_vp, c_int]),
+    "orc_md_run_rebuild_f64": (c_double, [c_i64, c_vp, c_vp, c_vp, c_double, c_double, c_double, c_int, c_double,
+                                          c_int, c_int, ctypes.POINTER(c_i64)]),
     "orc_kmeans_assign_f32": (None, [c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int]),
     "orc_kmeans_assign_f64": (None, [c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int]),
     "orc_kmeans_fixed_shift": (c_int, [c_i64, c_float]),
@@ -166,6 +169,34 @@ def md_run_rebuild(pos, vel, box, rc, cutneigh, every, dt, nsteps, threads=0):
     if r != r:
         raise ValueError("box too small for the cell list")
     return p, v, f, npairs.value
+
+
+def md_run_rebuild_f64(pos, vel, box, rc, cutneigh, every, dt, nsteps, threads=0):
+    """FP64 shadow of md_run_rebuild (same schedule; pair lists built from the
+    positions rounded to FP32).  Returns (pos, vel, force, last pair count) in
+    FP64."""
+    n = pos.shape[0]
+    p = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    v = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    f = np.zeros((n, 3), dtype=np.float64)
+    npairs = c_i64(0)
+    r = lib().orc_md_run_rebuild_f64(n, _p(p), _p(v), _p(f), float(box), float(rc) ** 2, float(cutneigh), every,
+                                     float(dt), nsteps, threads or os.cpu_count(), ctypes.byref(npairs))
+    if r != r:
+        raise ValueError("box too small for the cell list")
+    return p, v, f, npairs.value
+
+
+def lj_force_f64_gather(pos, edges, box, rc, pbc=True, threads=0):
+    """FP64 forces over a half edge list, threaded (same sums as lj_force with
+    dtype float64)."""
+    n = pos.shape[0]
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    e = np.ascontiguousarray(edges, dtype=np.int32)
+    f = np.zeros((n, 3), dtype=np.float64)
+    lib().orc_lj_force_f64_gather(n, e.shape[0], _p(e), _p(p), float(box), int(pbc), float(rc) ** 2, _p(f),
+                                  threads or os.cpu_count())
+    return f
 
 
 def pairs_brute(pos, box, cut, pbc=True):

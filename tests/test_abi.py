@@ -39,3 +39,20 @@ def test_error_path_without_gpu(pa):
     interior = (ctypes.c_int64 * 3)(4, 4, 4)
     assert pa.lib().pat_stencil_create(2, interior, 1, 4, None, 0, ctypes.byref(h)) == -5
     assert pa.lib().pat_stencil_create(3, interior, 2, 4, None, 0, ctypes.byref(h)) == -5
+
+
+def test_neighbour_build_entry_total_guard(pa):
+    """The binned neighbour build refuses, before writing any row, a list
+    whose 64-bit entry total does not fit the int32 row pointers -- also when
+    the int32 scan has wrapped back to a positive number."""
+    import ctypes
+    f = pa.lib().pat_internal_nb_check_total
+    f.argtypes = [ctypes.c_ulonglong, ctypes.c_int]
+    f.restype = ctypes.c_int
+    assert f(0, 0) == 0
+    assert f(2 ** 31 - 1, 2 ** 31 - 1) == 0
+    assert f(2 ** 31, -2 ** 31) == -5  # one past: the int32 total is negative
+    assert b"int32" in pa.lib().pat_last_error()
+    assert f(2 ** 32 + 5, 5) == -5  # wrapped to a small positive total
+    assert f(3 * 2 ** 31, 2 ** 31 - 2 ** 31) == -5
+    assert f(1000, 999) == -1  # the two totals disagree

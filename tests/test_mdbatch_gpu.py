@@ -47,7 +47,7 @@ def test_batch_slice_orders_equal_alone(pa, gpu, order):
     b.close()
 
 
-def test_batch_workload_64_systems(pa, gpu):
+def test_batch_workload_64_systems(pa, orc, gpu):
     """64 systems of 8^3-32^3 FCC cells (log-uniform), T = 1.44, 100 steps
     with rebuilds every 20, as one batched launch per step."""
     systems = pa.batch_systems(64, 8, 32, seed=99)
@@ -63,6 +63,9 @@ def test_batch_workload_64_systems(pa, gpu):
         ref = alone(pa, pos, vel, box, 100, 1)
         for g, r in zip(got, ref):
             assert np.array_equal(g, r)
+        # and the oracle, rebuilding its cell-list pair list at the same steps
+        rp, rv, rf, _ = orc.md_run_rebuild(pos, vel, box, 2.5, 2.8, 20, 0.005, 100, threads=16)
+        assert np.array_equal(got[0], rp) and np.array_equal(got[1], rv) and np.array_equal(got[2], rf), s
         # momentum of each system is conserved
         assert np.max(np.abs(got[1].astype(np.float64).sum(0) - vel.astype(np.float64).sum(0))) < 1e-2
     b.close()

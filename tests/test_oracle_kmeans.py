@@ -123,3 +123,37 @@ def test_gmm_generator(pa):
     assert np.array_equal(x, y)
     assert x.dtype == np.float32 and x.shape == (10000, 4)
     assert np.max(np.abs(x)) < 16
+
+
+def test_update_is_the_cluster_mean(pa, orc):
+    # the update divides the fixed-point sums by the integer counts: within
+    # the fixed-point and FP32 rounding of the FP64 cluster means (a count
+    # off by one would miss by ~1/count)
+    x = pa.gen_gmm(30000, 6, 5, sigma=1.0, seed=11)
+    rng = np.random.default_rng(12)
+    k = 7
+    a = rng.integers(0, k - 1, len(x)).astype(np.int32)  # cluster k-1 stays empty
+    maxabs = float(np.max(np.abs(x)))
+    s = orc.kmeans_shift(len(x), maxabs)
+    sums, counts, _ = orc.kmeans_accumulate(x, a, k, s)
+    c0 = rng.normal(size=(k, 6)).astype(np.float32)
+    c = orc.kmeans_update(sums, counts, s, c0)
+    for j in range(k - 1):
+        m = x[a == j].astype(np.float64).mean(axis=0)
+        tol = np.abs(m) * 2.0 ** -24 + 2.0 ** (-s - 1) + 1e-300
+        assert np.all(np.abs(c[j] - m) <= tol), j
+        assert counts[j] == np.sum(a == j)
+    assert np.array_equal(c[k - 1], c0[k - 1])
+
+
+def test_lloyd_matches_fp64_lloyd(pa, orc):
+    # a separated mixture: FP32 Lloyd (fixed-point sums) and FP64 Lloyd take
+    # the same assignments and their centres agree to FP32 rounding
+    x = pa.gen_gmm(40000, 8, 6, sigma=0.5, lo=-10, hi=10, seed=21)
+    s = orc.kmeans_shift(len(x), float(np.max(np.abs(x))))
+    c32, a32 = orc.kmeans_lloyd(x, 6, 8, s)[:2]
+    c64, a64 = orc.kmeans_lloyd_f64(x, 6, 8)[:2]
+    assert np.array_equal(a32, a64)
+    rel = np.max(np.abs(c32 - c64)) / np.max(np.abs(c64))
+    print(f"Lloyd FP32 vs FP64 after 8 iterations: {rel:.2e} relative")
+    assert rel <= 1e-6

@@ -197,3 +197,32 @@ def test_md_rebuild_matches_fixed_list_between_rebuilds(pa, orc):
     b = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 100, 0.005, 15, threads=2)
     for x, y in zip(a[:3], b[:3]):
         assert np.array_equal(x, y)
+
+
+def test_f64_gather_equals_serial_scatter(pa, orc):
+    pos, box, edges = pa.moldyn_system(5, seed=13)
+    f_serial, _ = orc.lj_force(pos.astype(np.float64), edges, box, 2.5, dtype=np.float64)
+    for t in (1, 5):
+        assert np.array_equal(orc.lj_force_f64_gather(pos, edges, box, 2.5, threads=t), f_serial)
+
+
+def test_md_rebuild_f64_matches_fixed_list(pa, orc):
+    # one list build (every > steps): the FP64 loop with a fixed list of the
+    # pairs within the neighbour cutoff at t=0, bit for bit
+    pos, box = pa.gen_fcc(7, 7, 7, jitter=0.05, seed=19)
+    vel = pa.gen_velocities(len(pos), 1.44, seed=20)
+    e = orc.pairs_cells(pos, box[0], 2.8)
+    a = orc.md_run(pos.astype(np.float64), vel.astype(np.float64), e, box[0], 2.5, 0.005, 15, dtype=np.float64)
+    b = orc.md_run_rebuild_f64(pos, vel, box[0], 2.5, 2.8, 100, 0.005, 15, threads=3)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    # with rebuilds every 10 steps over 45 steps: the same trajectory as a
+    # fixed list wide enough that no pair enters the cutoff from outside it
+    e_wide = orc.pairs_cells(pos, box[0], 3.6)
+    c = orc.md_run(pos.astype(np.float64), vel.astype(np.float64), e_wide, box[0], 2.5, 0.005, 45, dtype=np.float64)
+    d = orc.md_run_rebuild_f64(pos, vel, box[0], 2.5, 2.8, 10, 0.005, 45, threads=2)
+    for x, y in zip(c[:3], d[:3]):
+        assert np.array_equal(x, y)
+    # and the FP32 loop stays close to it
+    f32 = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 10, 0.005, 45, threads=2)
+    assert np.max(np.abs(f32[0] - d[0])) < 1e-3

@@ -130,3 +130,76 @@ def test_serve_admission_respects_budget(pa, gpu):
         held += b
         peak = max(peak, held)
     assert peak <= 100 * MB
+
+
+def test_serving_mix_10000_jobs(pa, gpu):
+    """The serving workload's job list: 10,000 jobs, one third each of
+    kmeans (N 10k-4M, D in {8..128}, K 16-1024, 10 iterations), heat3D (side
+    32-384, 50 steps) and LJ (4^3-48^3 cells, 20 steps), all submitted to 8
+    slots under a 200 GB budget.  Every job succeeds; the jobs in flight
+    never hold more than the budget; a seeded sample of 30 jobs -- the
+    largest of each kind among them -- equals its standalone handle run bit
+    for bit."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    jobs = bench.serving_mix(10000, seed=5)
+    kinds = np.array([j[0] for j in jobs])
+    size = np.array([j[1] for j in jobs], dtype=np.float64)
+    work = np.array([j[1] * j[3] * j[4] if j[0] == pa.PAT_JOB_KMEANS else j[1] for j in jobs], dtype=np.float64)
+    pick = set()
+    for kind in (pa.PAT_JOB_HEAT3D, pa.PAT_JOB_LJ, pa.PAT_JOB_KMEANS):
+        idx = np.flatnonzero(kinds == kind)
+        pick.add(int(idx[np.argmax(work[idx] if kind == pa.PAT_JOB_KMEANS else size[idx])]))
+    rng = np.random.default_rng(17)
+    while len(pick) < 30:
+        pick.add(int(rng.integers(len(jobs))))
+    budget = 200 << 30
+    s = pa.Serve(budget, slots=8)
+    bufs, ids = {}, []
+    for i, (kind, n, steps, d, k) in enumerate(jobs):
+        out = None
+        if i in pick:
+            m = {pa.PAT_JOB_HEAT3D: n ** 3, pa.PAT_JOB_LJ: 4 * n ** 3 * 3, pa.PAT_JOB_KMEANS: d * k}[kind]
+            out = bufs[i] = np.zeros(m, dtype=np.float32)
+        ids.append(s.submit(pa.job(kind, n, steps, 7000 + i, dim=d, k=k, out=out), keep=out))
+    res = [s.wait(i) for i in ids]
+    s.close()
+    bad = [(i, r["status"]) for i, r in enumerate(res) if r["status"] != 0]
+    assert not bad, bad[:5]
+    ev = sorted([(r["start_ms"], r["device_bytes"]) for r in res] + [(r["end_ms"], -r["device_bytes"]) for r in res],
+                key=lambda t: (t[0], t[1]))
+    held = peak = 0
+    for _, b in ev:
+        held += b
+        peak = max(peak, held)
+    span = max(r["end_ms"] for r in res) - min(r["start_ms"] for r in res)
+    print(f"serving mix: 10,000 jobs in {span / 1e3:.1f} s ({1e4 / span * 1e3:.0f} jobs/s), peak admitted "
+          f"{peak / 2 ** 30:.1f} GiB of {budget / 2 ** 30:.0f}")
+    assert peak <= budget
+    for i in sorted(pick):
+        kind, n, steps, d, k = jobs[i]
+        seed = 7000 + i
+        if kind == pa.PAT_JOB_HEAT3D:
+            st = pa.Stencil(n, n, n, alpha=0.1)
+            st.upload(pa.gen_heat3d(n, n, n, seed=seed))
+            st.run(steps)
+            ref = st.download()[1:-1, 1:-1, 1:-1]
+            st.close()
+            assert np.array_equal(bufs[i].reshape(n, n, n), ref), (i, jobs[i])
+        elif kind == pa.PAT_JOB_LJ:
+            pos, box = pa.gen_fcc(n, n, n, rho=0.8442, jitter=0.05, seed=seed)
+            md = pa.MD(pos, None, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+            md.run(steps)
+            ref = md.get()[0]
+            md.close()
+            assert np.array_equal(bufs[i].reshape(-1, 3), ref), (i, jobs[i])
+        else:
+            x = pa.gen_gmm_dev(n, d, k, seed=seed)
+            km = pa.KMeans(x, k, mode=pa.PAT_KM_MFMA)
+            km.run(steps)
+            ref = km.centroids()
+            km.close()
+            x.free()
+            assert np.array_equal(bufs[i].reshape(k, d), ref), (i, jobs[i])
