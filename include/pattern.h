@@ -276,7 +276,8 @@ int pat_md_create(int64_t num_nodes, const float *pos, const float *vel, int64_t
  * positions by binning, and rebuilt before the force of every step whose
  * index is a multiple of 'every' -- miniMD's Neighbor::setup/build
  * (miniMD/neighbor.cpp:76-427) with its reneighbouring interval and skin
- * (miniMD/lj.in:9-10).  Needs box >= 3 * (cutoff + skin). */
+ * (miniMD/lj.in:9-10).  Needs box >= 3 * (cutoff + skin), or box >= 2 *
+ * (cutoff + skin) for at most 65,536 atoms (every pair is then tested). */
 int pat_md_create_binned(int64_t num_nodes, const float *pos, const float *vel, double box, double cutoff,
                          double skin, int every, double dt, pat_md **out);
 /* The same from device float4 arrays (vel4 may be NULL = 0), copied. */
@@ -492,8 +493,9 @@ int pat_read_xyz(const char *path, int64_t num_nodes, double *xyz);
  * per slot), their inputs generated on the device from the job's seed.
  * ---------------------------------------------------------------------- */
 #define PAT_JOB_HEAT3D 0 /* size = cube side; steps of u' = u + 0.1 (sum6 - 6u); interior U[0,1) */
-#define PAT_JOB_LJ 1     /* size = FCC cells per side (rho 0.8442, jitter 0.05); steps of velocity
-                            Verlet, cutoff 2.5, skin 0.3, rebuild every 20, dt 0.005 */
+#define PAT_JOB_LJ 1     /* size = FCC cells per side, 4..160 (rho 0.8442, jitter 0.05); steps of
+                            velocity Verlet, cutoff 2.5, skin 0.3, neighbour list rebuilt on the
+                            device every 20 steps, dt 0.005 */
 #define PAT_JOB_KMEANS 2 /* size = points, dim, k (also the mixture's components, sigma 1,
                             centres in [-10, 10]^dim); steps = Lloyd iterations */
 typedef struct pat_job {

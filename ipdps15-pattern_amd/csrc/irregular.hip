@@ -494,6 +494,38 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
     }
 }
 
+// Small boxes (two bins per side, L >= 2 cut: every pair has at most one
+// image within the cutoff): one wavefront per atom i tests every atom j in
+// increasing index order with the same distance test as nb_cells, so each
+// row comes out already sorted.  Count mode writes deg[i]; fill mode writes
+// the row at rowptr[i], never past rowptr[i + 1].
+constexpr int kNbBruteMax = 1 << 16;  // atoms: O(N^2) tests per build
+
+template <int MODE>
+__global__ __launch_bounds__(256) void nb_brute(const float4 *__restrict__ pos, int N, BinGeom g,
+                                                int *__restrict__ deg, const int *__restrict__ rowptr,
+                                                int *__restrict__ cols)
+{
+    const int i = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (i >= N) return;  // whole wavefronts
+    const float4 pi = pos[i];
+    const int w = MODE == kNbFill ? rowptr[i] : 0;
+    const int wend = MODE == kNbFill ? rowptr[i + 1] : 0;
+    int found = 0;
+    for (int j0 = 0; j0 < N; j0 += kWave) {
+        const int j = j0 + lane;
+        const bool hit = j < N && j != i && nb_within(pi, pos[min(j, N - 1)], g);
+        const unsigned long long m = __ballot(hit);
+        if (MODE == kNbFill && hit) {
+            const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
+            if (o < wend) cols[o] = j;
+        }
+        found += __popcll(m);
+    }
+    if (MODE == kNbCount && lane == 0) deg[i] = found;
+}
+
 // ------------------------------------------------------------------ force
 
 struct LJParams {
@@ -2077,6 +2109,7 @@ struct NbWork {
     NbSys *sys = nullptr;
     int *slot = nullptr, *flag = nullptr;  // one-pass rows: kNbSlot entries per atom; overflow flag
     unsigned long long *tot64 = nullptr;   // 64-bit sum of the row lengths
+    int64_t small_n = 0;                   // deg / ws capacity in atoms of the two-bin (all pairs) build
     int64_t slot_cap = 0;
     bool slot_fail = false;  // a row outgrew its slot: two passes from then on
     std::vector<NbSys> sys_h;  // source of the last upload (read before the build returns)
@@ -2129,6 +2162,65 @@ static int nb_alloc(int **p, int64_t n)
     return PAT_OK;
 }
 
+// The neighbour CSR of one periodic cube whose side holds two bins of the
+// cutoff (2 cut <= L < 3 cut, at most kNbBruteMax atoms): every pair tested
+// (nb_brute), rows sorted by construction.  Same outputs and growth rules as
+// neighbor_build_multi.
+static int neighbor_build_small(const float4 *pos, int64_t N, double L, double cut, NbWork *w, int32_t *rowptr,
+                                int32_t **cols, int64_t *cap, int64_t *total, hipStream_t st)
+{
+    if (!w->deg || !w->ws || std::max(w->N, w->small_n) < N) {
+        if (w->deg) (void)pat::dev_free(w->deg);
+        if (w->ws) (void)pat::dev_free(w->ws);
+        w->deg = w->ws = nullptr;
+        w->N = 0;  // a binned build on this workspace reallocates all of its buffers
+        const int64_t scan_ws = 2 * ((N + 1 + kScanBlock - 1) / kScanBlock) + 2 * kScanBlock + 4;
+        PAT_TRY(nb_alloc(&w->deg, N + 1));
+        PAT_TRY(nb_alloc(&w->ws, scan_ws));
+        w->small_n = N;
+    }
+    if (!w->tot64 && pat::dev_malloc((void **)&w->tot64, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipGetLastError();
+        w->tot64 = nullptr;
+        pat::set_error("neighbour build: hipMalloc of 8 bytes failed");
+        return PAT_ERR_NOMEM;
+    }
+    BinGeom g;
+    g.L = (float)L;
+    g.hL = 0.5f * g.L;
+    g.cut2 = (float)(cut * cut);
+    g.inv = 0.f;
+    g.nb = 2;
+    const unsigned grid = (unsigned)((N + 3) / 4);
+    hipLaunchKernelGGL(nb_brute<kNbCount>, dim3(grid), dim3(256), 0, st, pos, (int)N, g, w->deg, (const int *)nullptr,
+                       (int *)nullptr);
+    PAT_LAUNCH_CHECK();
+    PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
+    PAT_HIP(hipMemsetAsync(w->tot64, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(sum_nonneg_i64, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 1024)), dim3(256), 0, st,
+                       (const int *)w->deg, N, w->tot64);
+    PAT_LAUNCH_CHECK();
+    PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
+    int tot = 0;
+    unsigned long long tot64 = 0;
+    PAT_HIP(hipMemcpyAsync(&tot, rowptr + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipMemcpyAsync(&tot64, w->tot64, sizeof(tot64), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    PAT_TRY(nb_check_total(tot64, tot));
+    if (tot > *cap) {
+        const int64_t want = (int64_t)(tot * 1.25) + 1024;
+        if (*cols) (void)pat::dev_free(*cols);
+        *cols = nullptr;
+        PAT_TRY(nb_alloc(cols, want));
+        *cap = want;
+    }
+    hipLaunchKernelGGL(nb_brute<kNbFill>, dim3(grid), dim3(256), 0, st, pos, (int)N, g, (int *)nullptr,
+                       (const int *)rowptr, *cols);
+    PAT_LAUNCH_CHECK();
+    *total = tot;
+    return PAT_OK;
+}
+
 // Rebuilds the full, row-sorted neighbour CSR of nsys independent periodic
 // cubes at once: system s holds atoms [off[s], off[s+1]) of the float4
 // positions in a cube of side Ls[s]; pairs closer than cut (within one
@@ -2147,8 +2239,13 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
         const double L = Ls[s];
         PAT_REQUIRE(L > 0 && off[s + 1] > off[s], PAT_ERR_INVALID, "neighbour build: system %d is empty", s);
         const int nb = (int)std::floor(L / (cut * 1.0001));
-        PAT_REQUIRE(nb >= 3, PAT_ERR_UNSUPPORTED, "neighbour build: box %.3f holds fewer than 3 bins of %.3f", L,
-                    cut);
+        if (nb == 2 && nsys == 1 && off[1] <= kNbBruteMax) {
+            const int64_t n = off[1];
+            return neighbor_build_small(pos, n, L, cut, w, rowptr, cols, cap, total, st);
+        }
+        PAT_REQUIRE(nb >= 3, PAT_ERR_UNSUPPORTED,
+                    "neighbour build: box %.3f holds fewer than 3 bins of %.3f (a single system of at most %d "
+                    "atoms may hold 2)", L, cut, kNbBruteMax);
         PAT_REQUIRE(nb < 1024, PAT_ERR_INVALID, "neighbour build: too many bins");
         NbSys &S = w->sys_h[(size_t)s];
         S.L = (float)L;

@@ -959,8 +959,13 @@ int md_create_binned(int64_t num_nodes, const float *pos, const float *vel, bool
     PAT_REQUIRE(num_nodes > 1 && num_nodes < (1LL << 31) - 1, PAT_ERR_INVALID, "pat_md_create_binned: bad node count");
     PAT_REQUIRE(cutoff > 0 && skin >= 0 && dt > 0 && every > 0, PAT_ERR_INVALID,
                 "pat_md_create_binned: need cutoff > 0, skin >= 0, dt > 0, every > 0");
-    PAT_REQUIRE(box >= 3 * (cutoff + skin) * 1.0001, PAT_ERR_UNSUPPORTED,
-                "pat_md_create_binned: box %.3f must hold 3 bins of cutoff+skin %.3f", box, cutoff + skin);
+    // three bins of cutoff + skin per side, or two (every pair then has one
+    // image within the cutoff) for at most 65,536 atoms, whose rows are
+    // built by testing every pair
+    PAT_REQUIRE(box >= 3 * (cutoff + skin) * 1.0001 || (box >= 2 * (cutoff + skin) * 1.0001 && num_nodes <= 65536),
+                PAT_ERR_UNSUPPORTED,
+                "pat_md_create_binned: box %.3f must hold 3 bins of cutoff+skin %.3f (2 for at most 65,536 atoms)", box,
+                cutoff + skin);
     auto *h = new pat_md();
     h->box = (float)box;
     h->pbc = 1;

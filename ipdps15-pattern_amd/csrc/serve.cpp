@@ -174,25 +174,9 @@ struct pat_serve {
         rc = pat_gen_fcc_dev(c, c, c, kRho, kJitter, j.seed, pos4, box, st);
         if (rc == PAT_OK && hipStreamSynchronize(st) != hipSuccess) rc = PAT_ERR_HIP;
         pat_md *md = nullptr;
-        if (rc == PAT_OK) {
-            if (box[0] >= 3 * (kCut + kSkin) * 1.0001) {
-                // binned neighbour CSR rebuilt on the device every kEvery steps
-                rc = pat_md_create_binned_dev(n, pos4, nullptr, box[0], kCut, kSkin, kEvery, kDt, &md);
-            } else {
-                // box too small for three bins: a fixed list within the cutoff
-                std::vector<float> p4((size_t)n * 4), p3((size_t)n * 3);
-                if (hipMemcpyAsync(p4.data(), pos4, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                    hipStreamSynchronize(st) != hipSuccess)
-                    rc = PAT_ERR_HIP;
-                for (int64_t i = 0; i < n && rc == PAT_OK; i++)
-                    for (int q = 0; q < 3; q++) p3[3 * i + q] = p4[4 * i + q];
-                int64_t ne = 0;
-                if (rc == PAT_OK) rc = pat_gen_pairs(n, p3.data(), box[0], kCut, 0, 0, nullptr, &ne);
-                std::vector<int32_t> e((size_t)std::max<int64_t>(2 * ne, 2));
-                if (rc == PAT_OK) rc = pat_gen_pairs(n, p3.data(), box[0], kCut, 0, 0, e.data(), &ne);
-                if (rc == PAT_OK) rc = pat_md_create(n, p3.data(), nullptr, ne, e.data(), box[0], 1, kCut, kDt, &md);
-            }
-        }
+        // neighbour CSR rebuilt on the device every kEvery steps (binned, or
+        // every pair tested in boxes of two bins per side)
+        if (rc == PAT_OK) rc = pat_md_create_binned_dev(n, pos4, nullptr, box[0], kCut, kSkin, kEvery, kDt, &md);
         if (rc == PAT_OK) {
             int v = 0, l = 0, o = 0;
             if (pat_md_get_variant(md, &v, &l, &o) == PAT_OK) r->kernel = l;
@@ -354,8 +338,9 @@ int pat_serve_submit(pat_serve *s, const pat_job *job, int64_t *id)
         // 4 size^3 atoms with ~78 neighbour-list entries each (cutoff + skin
         // 2.8 at density 0.8442) must stay below 2^31 entries: 160^3 cells
         // list ~1.3e9.
-        PAT_REQUIRE(j.size >= 2 && j.size <= 160 && j.steps >= 0, PAT_ERR_INVALID,
-                    "pat_serve_submit: LJ cells %lld / steps %d out of range (2..160 cells per side)",
+        // at least 4 cells: the box must hold two neighbour bins of 2.8
+        PAT_REQUIRE(j.size >= 4 && j.size <= 160 && j.steps >= 0, PAT_ERR_INVALID,
+                    "pat_serve_submit: LJ cells %lld / steps %d out of range (4..160 cells per side)",
                     (long long)j.size, j.steps);
         break;
     case PAT_JOB_KMEANS:

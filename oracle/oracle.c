@@ -534,6 +534,22 @@ int64_t orc_pairs_cells(int64_t N, const float *pos, float L, float cut, int32_t
     return n;
 }
 
+int64_t orc_pairs_brute(int64_t N, const float *pos, float L, int pbc, float cut2, int32_t *edges, int64_t cap);
+
+/*
+ * The rebuild loops' pair list: the cell list where the box holds three
+ * cells of the cutoff, else (boxes of two cells, where every pair has at
+ * most one image within the cutoff) the O(N^2) list -- the same pairs, in
+ * the same (i, j) order.  Returns -1 for boxes under two cells.
+ */
+int64_t orc_pairs_any(int64_t N, const float *pos, float L, float cut, int32_t *edges, int64_t cap)
+{
+    const int nb = (int)floor((double)L / ((double)cut * 1.0001));
+    if (nb >= 3) return orc_pairs_cells(N, pos, L, cut, edges, cap);
+    if (nb < 2) return -1;
+    return orc_pairs_brute(N, pos, L, 1, cut * cut, edges, cap);
+}
+
 /*
  * Velocity-Verlet with unit mass and periodic wrap into [0, L), the
  * integrator the moldyn and miniMD workloads run around the force
@@ -600,7 +616,7 @@ double orc_md_run_rebuild_f32(int64_t N, float *pos, float *vel, float *f, float
             }
         }
         if (s == 0 || s % every == 0) {
-            np = orc_pairs_cells(N, pos, L, cutneigh, NULL, 0);
+            np = orc_pairs_any(N, pos, L, cutneigh, NULL, 0);
             if (np < 0) {
                 free(edges);
                 return NAN;
@@ -610,7 +626,7 @@ double orc_md_run_rebuild_f32(int64_t N, float *pos, float *vel, float *f, float
                 cap = np;
                 edges = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(cap > 0 ? cap : 1));
             }
-            orc_pairs_cells(N, pos, L, cutneigh, edges, cap);
+            orc_pairs_any(N, pos, L, cutneigh, edges, cap);
         }
         orc_lj_force_f32_gather(N, np, edges, pos, L, 1, rc2, f, nthreads);
         if (s > 0)
@@ -654,7 +670,7 @@ double orc_md_run_rebuild_f64(int64_t N, double *pos, double *vel, double *f, do
                 float x = (float)pos[m];
                 p32[m] = x >= (float)L ? 0.0f : x; /* rounding may reach L */
             }
-            np = orc_pairs_cells(N, p32, (float)L, (float)cutneigh, NULL, 0);
+            np = orc_pairs_any(N, p32, (float)L, (float)cutneigh, NULL, 0);
             if (np < 0) {
                 free(edges);
                 free(p32);
@@ -665,7 +681,7 @@ double orc_md_run_rebuild_f64(int64_t N, double *pos, double *vel, double *f, do
                 cap = np;
                 edges = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(cap > 0 ? cap : 1));
             }
-            orc_pairs_cells(N, p32, (float)L, (float)cutneigh, edges, cap);
+            orc_pairs_any(N, p32, (float)L, (float)cutneigh, edges, cap);
         }
         orc_lj_force_f64_gather(N, np, edges, pos, L, 1, rc2, f, nthreads);
         if (s > 0)

@@ -134,3 +134,30 @@ def test_minimd_workload(pa, orc, gpu):
     rp, rv, rf, rn = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 20, 0.005, 100, threads=16)
     assert npairs == rn
     assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
+
+
+@pytest.mark.parametrize("cells", [4, 5])
+def test_binned_small_box_vs_oracle(pa, orc, gpu, cells):
+    """Boxes of two neighbour bins per side (4^3 and 5^3 FCC cells: sides
+    6.72 and 8.40 against cutoff + skin 2.8): every pair is tested on the
+    device; across the rebuilds at steps 20 and 40 the run is bit-identical
+    to the oracle rebuilding its O(N^2) pair list at the same steps."""
+    pos, box = pa.gen_fcc(cells, cells, cells, rho=0.8442, jitter=0.05, seed=40 + cells)
+    assert 2 * 2.8 < box[0] < 3 * 2.8
+    vel = pa.gen_velocities(len(pos), 1.44, seed=50 + cells)
+    md = pa.MD(pos, vel, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+    md.run(23)
+    md.run(22)
+    p, v, f = md.get()
+    npairs, steps, rebuilds, _ = md.stats()
+    rp_, cl_ = md.csr()
+    md.close()
+    assert steps == 45 and rebuilds == 3
+    rp, rv, rf, rn = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 20, 0.005, 45)
+    assert npairs == rn
+    assert np.array_equal(p, rp) and np.array_equal(v, rv) and np.array_equal(f, rf)
+    # the last build's rows: every neighbour within 2.8 of the step-40 positions, sorted
+    assert np.all(np.diff(rp_) >= 0) and rp_[-1] == 2 * npairs
+    for r in range(0, len(pos), 37):
+        row = cl_[rp_[r]:rp_[r + 1]]
+        assert np.all(np.diff(row) > 0)
