@@ -136,12 +136,40 @@ int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, int64_t nz, i
 typedef struct pat_stencil pat_stencil;
 /* StencilCreator<T>(...) + CreateRuntime + StencilInit
  * (heat_3D/lib/stencil_creator.h:22-30, kmeans/lib/stencil_runtime.cu:34-67):
- * a device-resident double-buffered grid of the given interior size
- * (x, y, z), halo width 'width', element size unit_size (4 = FP32). */
+ * a device-resident double-buffered grid of ndims (2 or 3) with the given
+ * interior size (x, y[, z]), halo width 'width' (1..16), element size
+ * unit_size (4 = float, 8 = double).  The library's own engine runs 3-D FP32
+ * grids of width 1; the rest need an application table
+ * (pat_stencil_set_table). */
 int pat_stencil_create(int ndims, const int64_t *interior, int width, int unit_size, const void *param,
                        int64_t param_size, pat_stencil **out);
-/* set_stencil_idx (kmeans/lib/stencil_runtime.h:36). */
+/* set_stencil_idx (kmeans/lib/stencil_runtime.h:36): an index of the
+ * registered application table, or PAT_STENCIL_HEAT without one. */
 int pat_stencil_set_idx(pat_stencil *s, int idx);
+/* An application's stencil functor table compiled for the device: the
+ * stencil_function_table of its functor header (heat_3D/stencil.h:119,
+ * kmeans/stencil.h:251), built by hipcc with GPU_KERNEL defined through the
+ * templates of api/gpu/stencil_table.hpp, which also define
+ * pat_app_stencil_table() returning this struct.  Replaces the reference's
+ * __constant__ function-pointer call per cell (kmeans/lib/compute_cuda.cu:95). */
+typedef struct pat_stencil_table {
+    int count;             /* functors in the table */
+    const int *unit_size;  /* per functor: bytes per grid element it reads and writes */
+    const int *same_as;    /* per functor: the library functor giving the same bits
+                              (PAT_STENCIL_HEAT: the tuned engine runs it), or -1 */
+    /* One sweep of functor idx: out = f(in) at every interior cell of a
+     * halo-inclusive grid of extents size[0..ndims-1] (dimension 0 fastest)
+     * with halo width 'width'; param is a device pointer (may be NULL);
+     * queued on stream, not waited for. */
+    int (*sweep)(int idx, const void *in, void *out, const int32_t *size, int ndims, int width, const void *param,
+                 void *stream);
+} pat_stencil_table;
+/* Registers (or with NULL removes) the application's table for the handle.
+ * Its entries then run on grids of any halo width and of the element size
+ * they declare; entries declared the same as PAT_STENCIL_HEAT keep the tuned
+ * engine on 3-D FP32 grids of width 1 unless PAT_STENCIL_GENERIC is set.
+ * The table must outlive the handle. */
+int pat_stencil_set_table(pat_stencil *s, const pat_stencil_table *table);
 /* Selects the kernel variant and tile (0 = tuned). */
 int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc);
 /* Copies a halo-inclusive host grid in (the app writes grid()->data(),

@@ -4,20 +4,27 @@
 //   irmap_fp / irreduce_fp      irregular reduction   (kmeans/irregular.h:22-24)
 //   stencil_fp                  stencil with parameter (heat_3D/stencil.h:32)
 // and the grid accessors GET_FLOAT3 & co (kmeans/lib/macro.h:70-76), which
-// read an implicit 'size' array (halo-inclusive extents, dimension 0 fastest).
-// CPU_KERNEL selects host functors (namespaces FFCPU/FCPU, as the reference);
-// the GPU engines bind the library's compiled functors by table index.
+// read an implicit 'size' array (halo-inclusive extents, dimension 0 fastest;
+// here with 64-bit offsets, so grids past 2^31 cells address correctly).
+// An application header selects host functors with CPU_KERNEL (namespaces
+// FFCPU/FCPU, as the reference) or device functors with GPU_KERNEL (FFGPU/
+// FGPU, compiled by hipcc with the templates of api/gpu/, whose
+// PAT_*_TABLE macros register the table with the library's runtime); its
+// function tables are declared CONSTANT = __device__ constexpr there, so
+// every index dispatches at compile time.
 #ifndef PATTERN_API_H_
 #define PATTERN_API_H_
 
 #include <cstring>
 #include <vector>
 
-#define GET_FLOAT3(input, i, j, k) *((float *)input + size[0] * (j) + (i)*size[0] * size[1] + (k))
-#define GET_DOUBLE3(input, i, j, k) *((double *)input + size[0] * (j) + (i)*size[0] * size[1] + (k))
-#define GET_INT3(input, i, j, k) *((int *)input + size[0] * (j) + (i)*size[0] * size[1] + (k))
-#define GET_FLOAT2(input, i, j) *((float *)input + size[0] * (i) + j)
-#define GET_INT2(input, i, j) *((int *)input + size[0] * (i) + j)
+#define PAT_GRID_OFFSET3(i, j, k) \
+    ((long long)size[0] * (j) + (long long)(i) * size[0] * size[1] + (k))
+#define GET_FLOAT3(input, i, j, k) *((float *)input + PAT_GRID_OFFSET3(i, j, k))
+#define GET_DOUBLE3(input, i, j, k) *((double *)input + PAT_GRID_OFFSET3(i, j, k))
+#define GET_INT3(input, i, j, k) *((int *)input + PAT_GRID_OFFSET3(i, j, k))
+#define GET_FLOAT2(input, i, j) *((float *)input + (long long)size[0] * (i) + (j))
+#define GET_INT2(input, i, j) *((int *)input + (long long)size[0] * (i) + (j))
 
 // Host reduction object for the generalized-reduction functors: a dense
 // key -> value table with the insert(key, ksize, value, vsize, reduce) call

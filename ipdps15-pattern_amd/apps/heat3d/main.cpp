@@ -4,12 +4,15 @@
 // interior (default 128) with a Dirichlet-0 halo, uniform [0,1) from seed 42,
 // 100 steps of the 7-point rule with alpha = 0.1.  Prints the L-infinity norm
 // of the final field and checks one step of the GPU engine against the host
+// functor; then one step of heat_3D's own FP64 functor (table index 1,
+// StencilCreator<double>, as heat_3D/main.cpp:59 runs it) against the host
 // functor.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
 
+#define CPU_KERNEL
 #include "include/runtime.h"
 #include "lib/stencil_creator.h"
 #include "lib/stencil_runtime.h"
@@ -51,6 +54,38 @@ int main(int argc, char *argv[])
                         bad++;
                 }
         printf("heat3d: one step, %ld of %ld cells differ from the host functor\n", bad, (long)N * N * N);
+        sr->StencilFinalize();
+        delete sr;
+        if (bad) return 1;
+    }
+    // heat_3D's FP64 functor: one step on the GPU, every cell against the host functor
+    {
+        double params[5] = {0, 0, 0, 0, 0};
+        StencilCreator<double> sc(3, global_size, 3, proc_size, W, 1, false, false, params, sizeof(params));
+        StencilRuntime *sr = sc.CreateRuntime();
+        sr->set_stencil_idx(1);
+        sr->StencilInit();
+        double *g = (double *)sr->grid()->data();
+        std::vector<double> in(halo.size(), 0.0), out(halo.size(), 0.0);
+        for (int z = 0; z < N; z++)
+            for (int y = 0; y < N; y++)
+                for (int x = 0; x < N; x++) {
+                    g[((size_t)z * N + y) * N + x] = at(x + 1, y + 1, z + 1);
+                    in[((size_t)(z + 1) * (N + 2) + y + 1) * (N + 2) + x + 1] = at(x + 1, y + 1, z + 1);
+                }
+        sr->StencilBegin();
+        int size[3] = {N + 2, N + 2, N + 2};
+        long bad = 0;
+        for (int z = 1; z <= N; z++)
+            for (int y = 1; y <= N; y++)
+                for (int x = 1; x <= N; x++) {
+                    int off[3] = {x, y, z};
+                    FCPU::stencil_function_table[1](in.data(), out.data(), off, size, params);
+                    if (out[((size_t)z * (N + 2) + y) * (N + 2) + x] != g[((size_t)(z - 1) * N + (y - 1)) * N + (x - 1)])
+                        bad++;
+                }
+        printf("heat3d: heat_3D's FP64 functor, one step, %ld of %ld cells differ from the host functor\n", bad,
+               (long)N * N * N);
         sr->StencilFinalize();
         delete sr;
         if (bad) return 1;

@@ -7,6 +7,14 @@
 // holding the whole domain, the runtime keeps two halo-inclusive device grids,
 // ping-pongs them each step, and replays repeated step counts from a captured
 // hipGraph so that small grids are not bound by launch latency.
+//
+// Functors: the library's tuned heat engine runs PAT_STENCIL_HEAT on 3-D FP32
+// grids of halo width 1.  An application that compiles its own functor table
+// for the device (api/gpu/stencil_table.hpp) registers it with
+// pat_stencil_set_table; then any entry runs, on grids of any halo width and
+// element size the entry declares, through the table's one-thread-per-cell
+// kernels -- except entries declared equal to the tuned functor, which keep
+// the tuned engine unless PAT_STENCIL_GENERIC is selected.
 #include <cstring>
 #include <map>
 
@@ -23,15 +31,53 @@ struct pat_stencil {
     std::vector<unsigned char> param;
     int variant = PAT_STENCIL_AUTO;
     int bx = 0, by = 0, ry = 0, zc = 0;
-    float *buf[2] = {nullptr, nullptr};
+    float *buf[2] = {nullptr, nullptr};  // element type: 'unit' bytes (the tuned engine: float)
     int cur = 0;  // buffer holding the current field
+    const pat_stencil_table *table = nullptr;  // application functors compiled for the device
+    void *param_dev = nullptr;                  // the parameter bytes, for table functors
     bool loaded = false;
     hipStream_t stream = nullptr;
     // captured step sequences keyed by (nsteps, starting buffer), with the
     // buffer each leaves the result in
     std::map<std::pair<int, int>, std::pair<hipGraphExec_t, int>> graphs;
 
-    int64_t cells() const { return (n[0] + 2 * width) * (n[1] + 2 * width) * (n[2] + 2 * width); }
+    // halo-inclusive extent of dimension d (1 beyond ndims)
+    int64_t ext(int d) const { return d < ndims ? n[d] + 2 * width : 1; }
+    int64_t cells() const { return ext(0) * ext(1) * ext(2); }
+    size_t bytes() const { return (size_t)unit * (size_t)cells(); }
+
+    // The tuned heat engine runs this handle's functor.
+    bool tuned() const
+    {
+        const bool heat = table ? (table->same_as && table->same_as[idx] == PAT_STENCIL_HEAT) : idx == PAT_STENCIL_HEAT;
+        return heat && ndims == 3 && width == 1 && unit == 4 && !(table && variant == PAT_STENCIL_GENERIC);
+    }
+
+    // Whether the current functor can run on this grid at all.
+    int check_runnable() const
+    {
+        if (tuned()) return PAT_OK;
+        PAT_REQUIRE(table, PAT_ERR_UNSUPPORTED,
+                    "stencil: functor %d on a %d-D grid of %d-byte elements with halo width %d needs an application "
+                    "functor table (pat_stencil_set_table); the library's own engine runs the heat functor on 3-D "
+                    "FP32 grids of halo width 1",
+                    idx, ndims, unit, width);
+        PAT_REQUIRE(idx >= 0 && idx < table->count, PAT_ERR_UNSUPPORTED, "stencil: functor index %d not in the table",
+                    idx);
+        PAT_REQUIRE(!table->unit_size || table->unit_size[idx] == unit, PAT_ERR_UNSUPPORTED,
+                    "stencil: functor %d reads %d-byte elements, the grid holds %d-byte ones", idx,
+                    table->unit_size ? table->unit_size[idx] : 0, unit);
+        return PAT_OK;
+    }
+
+    // One sweep of the table functor from buffer c into 1 - c.
+    int table_sweep(int c, hipStream_t st) const
+    {
+        const int32_t size[3] = {(int32_t)ext(0), (int32_t)ext(1), (int32_t)ext(2)};
+        const int rc = table->sweep(idx, buf[c], buf[1 - c], size, ndims, width, param_dev, st);
+        PAT_REQUIRE(rc == PAT_OK, rc, "stencil: the table's sweep of functor %d failed (%d)", idx, rc);
+        return PAT_OK;
+    }
 
     void drop_graphs()
     {
@@ -73,6 +119,12 @@ struct pat_stencil {
     // steps per buffer swap)
     int enqueue(int nsteps, int start, int *end)
     {
+        if (!tuned()) {
+            int c = start;
+            for (int k = 0; k < nsteps; k++, c = 1 - c) PAT_TRY(table_sweep(c, stream));
+            *end = c;
+            return PAT_OK;
+        }
         pat::HeatTile t;
         PAT_TRY(resolve_tile(&t));
         return pat::heat3d_steps(buf, start, n[0], n[1], n[2], alpha(), t, kernel_variant(), nsteps, stream, end);
@@ -85,29 +137,48 @@ int pat_stencil_create(int ndims, const int64_t *interior, int width, int unit_s
                        int64_t param_size, pat_stencil **out)
 {
     PAT_REQUIRE(out && interior, PAT_ERR_INVALID, "pat_stencil_create: null argument");
-    PAT_REQUIRE(ndims == 3, PAT_ERR_UNSUPPORTED, "pat_stencil_create: only 3-D grids are supported (got %d)", ndims);
-    PAT_REQUIRE(width == 1, PAT_ERR_UNSUPPORTED, "pat_stencil_create: only halo width 1 is supported (got %d)", width);
-    PAT_REQUIRE(unit_size == 4, PAT_ERR_UNSUPPORTED, "pat_stencil_create: only FP32 grids are supported (got %d B)",
-                unit_size);
+    PAT_REQUIRE(ndims == 2 || ndims == 3, PAT_ERR_UNSUPPORTED, "pat_stencil_create: 2-D or 3-D grids (got %d)", ndims);
+    PAT_REQUIRE(width >= 1 && width <= 16, PAT_ERR_UNSUPPORTED, "pat_stencil_create: halo width %d not in 1..16",
+                width);
+    PAT_REQUIRE(unit_size == 4 || unit_size == 8, PAT_ERR_UNSUPPORTED,
+                "pat_stencil_create: element size %d B (4 = float, 8 = double)", unit_size);
     PAT_REQUIRE(param_size >= 0 && (param_size == 0 || param), PAT_ERR_INVALID, "pat_stencil_create: bad parameter");
-    for (int d = 0; d < 3; d++)
-        PAT_REQUIRE(interior[d] > 0 && interior[d] < (1 << 20), PAT_ERR_INVALID, "pat_stencil_create: bad extent");
+    for (int d = 0; d < ndims; d++)
+        PAT_REQUIRE(interior[d] > 0 && interior[d] + 2 * width < (1 << 20), PAT_ERR_INVALID,
+                    "pat_stencil_create: bad extent");
     auto *s = new pat_stencil();
     s->ndims = ndims;
-    for (int d = 0; d < 3; d++) s->n[d] = interior[d];
+    for (int d = 0; d < 3; d++) s->n[d] = d < ndims ? interior[d] : 1;
     s->width = width;
     s->unit = unit_size;
+    if (ndims == 3)
+        PAT_REQUIRE(s->n[2] <= 65535, PAT_ERR_UNSUPPORTED, "pat_stencil_create: at most 65,535 interior planes");
     if (param_size > 0) s->param.assign((const unsigned char *)param, (const unsigned char *)param + param_size);
-    const size_t bytes = sizeof(float) * (size_t)s->cells();
+    const size_t bytes = s->bytes();
     for (int b = 0; b < 2; b++) {
         if (pat::dev_malloc(&s->buf[b], bytes) != hipSuccess) {
+            (void)hipGetLastError();
             if (s->buf[0]) (void)pat::dev_free(s->buf[0]);
             delete s;
             pat::set_error("pat_stencil_create: hipMalloc of %zu bytes failed", bytes);
             return PAT_ERR_NOMEM;
         }
     }
+    if (param_size > 0) {
+        if (pat::dev_malloc(&s->param_dev, (size_t)param_size) != hipSuccess ||
+            hipMemcpy(s->param_dev, param, (size_t)param_size, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            if (s->param_dev) (void)pat::dev_free(s->param_dev);
+            (void)pat::dev_free(s->buf[0]);
+            (void)pat::dev_free(s->buf[1]);
+            delete s;
+            pat::set_error("pat_stencil_create: copying %lld parameter bytes to the device failed",
+                           (long long)param_size);
+            return PAT_ERR_NOMEM;
+        }
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        if (s->param_dev) (void)pat::dev_free(s->param_dev);
         (void)pat::dev_free(s->buf[0]);
         (void)pat::dev_free(s->buf[1]);
         delete s;
@@ -121,9 +192,27 @@ int pat_stencil_create(int ndims, const int64_t *interior, int width, int unit_s
 int pat_stencil_set_idx(pat_stencil *s, int idx)
 {
     PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_set_idx: null handle");
-    PAT_REQUIRE(idx == PAT_STENCIL_HEAT, PAT_ERR_UNSUPPORTED, "pat_stencil_set_idx: functor index %d not in the table",
-                idx);
+    if (s->table)
+        PAT_REQUIRE(idx >= 0 && idx < s->table->count, PAT_ERR_UNSUPPORTED,
+                    "pat_stencil_set_idx: functor index %d not in the table", idx);
+    else
+        PAT_REQUIRE(idx == PAT_STENCIL_HEAT, PAT_ERR_UNSUPPORTED,
+                    "pat_stencil_set_idx: functor index %d not in the library's table (register the application's "
+                    "with pat_stencil_set_table)", idx);
     s->idx = idx;
+    s->drop_graphs();
+    return PAT_OK;
+}
+
+int pat_stencil_set_table(pat_stencil *s, const pat_stencil_table *table)
+{
+    PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_set_table: null handle");
+    PAT_REQUIRE(!table || (table->count > 0 && table->sweep), PAT_ERR_INVALID, "pat_stencil_set_table: bad table");
+    if (table)
+        PAT_REQUIRE(s->idx < table->count, PAT_ERR_UNSUPPORTED,
+                    "pat_stencil_set_table: the handle's functor index %d is not in the table", s->idx);
+    s->table = table;
+    s->drop_graphs();
     return PAT_OK;
 }
 
@@ -164,7 +253,7 @@ int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
 int pat_stencil_upload(pat_stencil *s, const void *host)
 {
     PAT_REQUIRE(s && host, PAT_ERR_INVALID, "pat_stencil_upload: null argument");
-    const size_t bytes = sizeof(float) * (size_t)s->cells();
+    const size_t bytes = s->bytes();
     // Both buffers get the same field, so both carry the Dirichlet halo.
     PAT_HIP(hipMemcpyAsync(s->buf[0], host, bytes, hipMemcpyHostToDevice, s->stream));
     PAT_HIP(hipMemcpyAsync(s->buf[1], host, bytes, hipMemcpyHostToDevice, s->stream));
@@ -183,8 +272,11 @@ static int stencil_run_graph(pat_stencil *s, int nsteps, float *device_ms)
     auto key = std::make_pair(nsteps, s->cur);
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
-        pat::HeatTile t;
-        PAT_TRY(s->resolve_tile(&t));  // tunes outside the capture
+        PAT_TRY(s->check_runnable());
+        if (s->tuned()) {
+            pat::HeatTile t;
+            PAT_TRY(s->resolve_tile(&t));  // tunes outside the capture
+        }
         hipGraph_t g;
         int end = s->cur;
         PAT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -233,8 +325,27 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     PAT_REQUIRE(s && total_ms, PAT_ERR_INVALID, "pat_stencil_run_timed: null argument");
     PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_run_timed: no grid uploaded");
     PAT_REQUIRE(nsteps > 0, PAT_ERR_INVALID, "pat_stencil_run_timed: need at least one step");
+    PAT_TRY(s->check_runnable());
     pat::EventVec ev;
     PAT_HIP(ev.create(2 * (size_t)nsteps));
+    if (!s->tuned()) {
+        int c = s->cur, rc = PAT_OK;
+        for (int k = 0; k < nsteps && rc == PAT_OK; k++, c = 1 - c) {
+            (void)hipEventRecord(ev[2 * k], s->stream);
+            rc = s->table_sweep(c, s->stream);
+            (void)hipEventRecord(ev[2 * k + 1], s->stream);
+        }
+        if (rc == PAT_OK && hipStreamSynchronize(s->stream) != hipSuccess) {
+            pat::set_error("pat_stencil_run_timed: stream synchronisation failed");
+            rc = PAT_ERR_HIP;
+        }
+        if (rc != PAT_OK) return rc;
+        (void)hipEventElapsedTime(total_ms, ev[0], ev[2 * (size_t)nsteps - 1]);
+        if (kernel_ms)
+            for (int k = 0; k < nsteps; k++) (void)hipEventElapsedTime(&kernel_ms[k], ev[2 * k], ev[2 * k + 1]);
+        s->cur = c;
+        return PAT_OK;
+    }
     pat::HeatTile t;
     PAT_TRY(s->resolve_tile(&t));
     const int v = s->kernel_variant();
@@ -283,7 +394,7 @@ int pat_stencil_download(pat_stencil *s, void *host)
 {
     PAT_REQUIRE(s && host, PAT_ERR_INVALID, "pat_stencil_download: null argument");
     PAT_REQUIRE(s->loaded, PAT_ERR_STATE, "pat_stencil_download: no grid uploaded");
-    PAT_HIP(hipMemcpyAsync(host, s->buf[s->cur], sizeof(float) * (size_t)s->cells(), hipMemcpyDeviceToHost, s->stream));
+    PAT_HIP(hipMemcpyAsync(host, s->buf[s->cur], s->bytes(), hipMemcpyDeviceToHost, s->stream));
     PAT_HIP(hipStreamSynchronize(s->stream));
     return PAT_OK;
 }
@@ -291,6 +402,12 @@ int pat_stencil_download(pat_stencil *s, void *host)
 int pat_stencil_tile(pat_stencil *s, int *bx, int *by, int *zc)
 {
     PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_tile: null handle");
+    if (!s->tuned()) {  // the table kernels' fixed shape: 64 x 4 cells per workgroup, one plane deep
+        if (bx) *bx = 64;
+        if (by) *by = 4;
+        if (zc) *zc = 1;
+        return PAT_OK;
+    }
     pat::HeatTile t;
     PAT_TRY(s->resolve_tile(&t));
     if (bx) *bx = t.bx;
@@ -313,6 +430,7 @@ int pat_stencil_destroy(pat_stencil *s)
     s->drop_graphs();
     (void)pat::dev_free(s->buf[0]);
     (void)pat::dev_free(s->buf[1]);
+    if (s->param_dev) (void)pat::dev_free(s->param_dev);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return PAT_OK;

@@ -107,6 +107,7 @@ SIGNATURES = {
                                    c_intp, c_vp]),
     "pat_stencil_create": (c_int, [c_int, c_i64p, c_int, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "pat_stencil_set_idx": (c_int, [c_vp, c_int]),
+    "pat_stencil_set_table": (c_int, [c_vp, c_vp]),
     "pat_stencil_set_variant": (c_int, [c_vp, c_int, c_int, c_int, c_int]),
     "pat_stencil_upload": (c_int, [c_vp, c_vp]),
     "pat_stencil_run": (c_int, [c_vp, c_int]),
@@ -389,22 +390,50 @@ def heat3d_run_dev(a, b, nx, ny, nz, nsteps, alpha=0.1, variant=PAT_STENCIL_AUTO
     return b if res.value else a
 
 
-class Stencil:
-    """Handle mirror of StencilCreator/StencilRuntime (heat_3D/lib/stencil_creator.h)."""
+_app_libs = {}
 
-    def __init__(self, nx, ny, nz, alpha=0.1, width=1):
-        self.shape = (nz + 2, ny + 2, nx + 2)
-        self._param = np.array([alpha], dtype=np.float32)
+
+def app_table(app, kind="stencil"):
+    """Pointer to the device functor table the application 'app' compiles
+    (apps/<app>/gpu_table.hip, built as apps/bin/lib<app>_table.so); kind
+    names its pat_app_<kind>_table() entry point."""
+    lib_ = _app_libs.get(app)
+    if lib_ is None:
+        path = os.path.join(_HERE, "apps", "bin", f"lib{app}_table.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (make -C ipdps15-pattern_amd)")
+        lib_ = _app_libs[app] = ctypes.CDLL(path)
+    f = getattr(lib_, f"pat_app_{kind}_table")
+    f.restype = c_vp
+    f.argtypes = []
+    return f()
+
+
+class Stencil:
+    """Handle mirror of StencilCreator/StencilRuntime (heat_3D/lib/stencil_creator.h).
+    width: halo width; dtype: element type (float32 or float64); param: the
+    functor parameter bytes (default: alpha as float32); table: an
+    application table (app_table(...)) whose functor idx runs."""
+
+    def __init__(self, nx, ny, nz, alpha=0.1, width=1, dtype=np.float32, param=None, table=None, idx=0):
+        self.dtype = np.dtype(dtype)
+        self.shape = (nz + 2 * width, ny + 2 * width, nx + 2 * width)
+        self._param = np.array([alpha], dtype=np.float32) if param is None else np.ascontiguousarray(param)
         h = c_vp()
         interior = (c_i64 * 3)(nx, ny, nz)
-        call("pat_stencil_create", 3, interior, width, 4, ptr(self._param), 4, ctypes.byref(h))
+        call("pat_stencil_create", 3, interior, width, self.dtype.itemsize, ptr(self._param), self._param.nbytes,
+             ctypes.byref(h))
         self.h = h
+        if table is not None:
+            call("pat_stencil_set_table", self.h, table)
+        if table is not None or idx:
+            call("pat_stencil_set_idx", self.h, idx)
 
     def set_variant(self, variant, bx=0, by=0, zc=0):
         call("pat_stencil_set_variant", self.h, variant, bx, by, zc)
 
     def upload(self, grid):
-        g = np.ascontiguousarray(grid, dtype=np.float32)
+        g = np.ascontiguousarray(grid, dtype=self.dtype)
         assert g.shape == self.shape
         call("pat_stencil_upload", self.h, ptr(g))
 
@@ -426,7 +455,7 @@ class Stencil:
         return ms.value
 
     def download(self):
-        out = np.empty(self.shape, dtype=np.float32)
+        out = np.empty(self.shape, dtype=self.dtype)
         call("pat_stencil_download", self.h, ptr(out))
         return out
 

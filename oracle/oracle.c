@@ -113,6 +113,77 @@ int orc_heat3d_run_f64(double *a, double *b, int nx, int ny, int nz, int nsteps,
 }
 
 /*
+ * The W = 3 star of the reference's kmeans directory (stencil2,
+ * kmeans/stencil.h:93-163, W from :3): over a halo-inclusive FP32 grid of
+ * halo width w (interior nx x ny x nz, x fastest), each interior cell
+ * becomes the sum of the cells 1..w-1 away along each axis, accumulated
+ * from 0 in the order x+, x-, y+, y-, z+, z- (each side by increasing
+ * distance), divided by 36.  The halo is copied through unchanged.
+ */
+void orc_star_step_f32(const float *in, float *out, int nx, int ny, int nz, int w)
+{
+    const int64_t sx = nx + 2 * w, sy = ny + 2 * w;
+    const int64_t sxy = sx * sy;
+    memcpy(out, in, sizeof(float) * (size_t)(sxy * (nz + 2 * w)));
+    for (int z = w; z < nz + w; z++)
+        for (int y = w; y < ny + w; y++)
+            for (int x = w; x < nx + w; x++) {
+                const int64_t id = (int64_t)z * sxy + (int64_t)y * sx + x;
+                float t = 0.0f;
+                for (int m = 1; m < w; m++) t += in[id + m];
+                for (int m = 1; m < w; m++) t += in[id - m];
+                for (int m = 1; m < w; m++) t += in[id + m * sx];
+                for (int m = 1; m < w; m++) t += in[id - m * sx];
+                for (int m = 1; m < w; m++) t += in[id + m * sxy];
+                for (int m = 1; m < w; m++) t += in[id - m * sxy];
+                out[id] = t / 36.0f;
+            }
+}
+
+int orc_star_run_f32(float *a, float *b, int nx, int ny, int nz, int w, int nsteps)
+{
+    float *src = a, *dst = b;
+    for (int s = 0; s < nsteps; s++) {
+        orc_star_step_f32(src, dst, nx, ny, nz, w);
+        float *t = src; src = dst; dst = t;
+    }
+    return src == a ? 0 : 1;
+}
+
+/*
+ * heat_3D's own functor (heat_3D/stencil.h:94-117), FP64, halo width 1: each
+ * axis contributes c * (its two neighbours + c * u) with c = 0.234, and the
+ * contributions are added in the order y, x, z.
+ */
+void orc_heat_ref_step_f64(const double *in, double *out, int nx, int ny, int nz)
+{
+    const int64_t sx = nx + 2, sy = ny + 2;
+    const int64_t sxy = sx * sy;
+    const double c = 0.234;
+    memcpy(out, in, sizeof(double) * (size_t)(sxy * (nz + 2)));
+    for (int z = 1; z <= nz; z++)
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++) {
+                const int64_t id = (int64_t)z * sxy + (int64_t)y * sx + x;
+                const double u = in[id];
+                const double ty = c * (in[id - sx] + in[id + sx] + u * c);
+                const double tx = c * (in[id - 1] + in[id + 1] + u * c);
+                const double tz = c * (in[id - sxy] + in[id + sxy] + u * c);
+                out[id] = ty + tx + tz;
+            }
+}
+
+int orc_heat_ref_run_f64(double *a, double *b, int nx, int ny, int nz, int nsteps)
+{
+    double *src = a, *dst = b;
+    for (int s = 0; s < nsteps; s++) {
+        orc_heat_ref_step_f64(src, dst, nx, ny, nz);
+        double *t = src; src = dst; dst = t;
+    }
+    return src == a ? 0 : 1;
+}
+
+/*
  * Threaded CPU path in the shape of the reference's CPU stencil engine:
  * the interior is cut into 16^3 tiles (CPU tile size, kmeans/lib/common.h:17;
  * tiling in kmeans/lib/data_util.cpp:224-307) and the tile list is split

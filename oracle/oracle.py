@@ -23,6 +23,8 @@ _SIG = {
     "orc_heat3d_run_f32": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_float]),
     "orc_heat3d_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_double]),
     "orc_heat3d_run_f32_tiled": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_float, c_int]),
+    "orc_star_run_f32": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int]),
+    "orc_heat_ref_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int]),
     "orc_lj_force_f32": (c_double, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp]),
     "orc_lj_force_f64": (c_double, [c_i64, c_i64, c_vp, c_vp, c_double, c_int, c_double, c_vp]),
     "orc_lj_force_f32_threads": (None, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp, c_vp, c_int]),
@@ -86,6 +88,26 @@ def heat3d(grid, nsteps, alpha=0.1, dtype=np.float32, threads=0):
             r = lib().orc_heat3d_run_f32(_p(a), _p(b), nx, ny, nz, nsteps, alpha)
     else:
         r = lib().orc_heat3d_run_f64(_p(a), _p(b), nx, ny, nz, nsteps, alpha)
+    return b if r else a
+
+
+def star(grid, nsteps, w=3):
+    """nsteps of the W = w star (the kmeans directory's stencil2) on a
+    halo-inclusive (z, y, x) FP32 grid of halo width w."""
+    nz, ny, nx = (d - 2 * w for d in grid.shape)
+    a = np.ascontiguousarray(grid, dtype=np.float32).copy()
+    b = a.copy()
+    r = lib().orc_star_run_f32(_p(a), _p(b), nx, ny, nz, w, nsteps)
+    return b if r else a
+
+
+def heat_ref(grid, nsteps):
+    """nsteps of heat_3D's own FP64 functor on a halo-inclusive (z, y, x)
+    grid of halo width 1."""
+    nz, ny, nx = (d - 2 for d in grid.shape)
+    a = np.ascontiguousarray(grid, dtype=np.float64).copy()
+    b = a.copy()
+    r = lib().orc_heat_ref_run_f64(_p(a), _p(b), nx, ny, nz, nsteps)
     return b if r else a
 
 

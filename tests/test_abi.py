@@ -37,8 +37,9 @@ def test_error_path_without_gpu(pa):
     assert b"distinct" in pa.lib().pat_last_error()
     h = ctypes.c_void_p()
     interior = (ctypes.c_int64 * 3)(4, 4, 4)
-    assert pa.lib().pat_stencil_create(2, interior, 1, 4, None, 0, ctypes.byref(h)) == -5
-    assert pa.lib().pat_stencil_create(3, interior, 2, 4, None, 0, ctypes.byref(h)) == -5
+    assert pa.lib().pat_stencil_create(1, interior, 1, 4, None, 0, ctypes.byref(h)) == -5  # 2-D or 3-D only
+    assert pa.lib().pat_stencil_create(3, interior, 17, 4, None, 0, ctypes.byref(h)) == -5  # halo width 1..16
+    assert pa.lib().pat_stencil_create(3, interior, 1, 2, None, 0, ctypes.byref(h)) == -5  # float or double
 
 
 def test_neighbour_build_entry_total_guard(pa):

@@ -22,7 +22,15 @@ def run(name, *args):
 
 def test_heat3d_app(gpu):
     out = run("heat3d", 64, 20)
-    assert "0 of 262144 cells differ" in out and "Linf" in out
+    assert "one step, 0 of 262144 cells differ" in out and "Linf" in out
+    assert "FP64 functor, one step, 0 of 262144 cells differ" in out
+
+
+def test_stencil2_app(gpu):
+    """The kmeans directory's W = 3 star through StencilCreator/StencilRuntime,
+    its functor compiled for the device by the application."""
+    out = run("stencil2", 96, 10)
+    assert "one step, 0 of 884736 cells differ" in out and "Linf" in out
 
 
 def test_kmeans_app(gpu):

@@ -123,3 +123,82 @@ def test_generator_seeded(pa):
     inner = a[1:-1, 1:-1, 1:-1]
     assert inner.min() >= 0 and inner.max() < 1
     assert not a[0].any() and not a[:, :, -1].any()
+
+
+# ---- the W = 3 star (kmeans/stencil.h stencil2) and heat_3D's FP64 functor
+
+
+def test_star_point_source(orc):
+    """One step from a unit impulse: exactly the 12 cells 1 and 2 away along
+    the axes get 1/36 (the impulse itself and every other cell 0)."""
+    w, n = 3, 9
+    g = np.zeros((n + 2 * w,) * 3, dtype=np.float32)
+    c = w + n // 2
+    g[c, c, c] = 1.0
+    out = orc.star(g, 1, w=w)
+    expect = np.zeros_like(g)
+    for m in (1, 2):
+        for d in range(3):
+            for sgn in (1, -1):
+                idx = [c, c, c]
+                idx[d] += sgn * m
+                expect[tuple(idx)] = np.float32(1.0) / np.float32(36.0)
+    assert np.array_equal(out, expect)
+
+
+def test_star_constant_field(orc):
+    """A constant interior of ones: cells at least 2 from every face see 12
+    ones (12/36 rounded once); a cell on a face sees 10, an edge 8, a corner
+    6 (the halo is 0)."""
+    w, n = 3, 12
+    g = np.zeros((n + 2 * w,) * 3, dtype=np.float32)
+    g[w:-w, w:-w, w:-w] = 1.0
+    out = orc.star(g, 1, w=w)[w:-w, w:-w, w:-w]
+    assert np.all(out[2:-2, 2:-2, 2:-2] == np.float32(12.0) / np.float32(36.0))
+    assert out[0, 5, 5] == np.float32(10.0) / np.float32(36.0)
+    assert out[0, 0, 5] == np.float32(8.0) / np.float32(36.0)
+    assert out[0, 0, 0] == np.float32(6.0) / np.float32(36.0)
+    assert out[1, 1, 1] == np.float32(9.0) / np.float32(36.0)
+
+
+def test_star_linear_and_symmetric(orc):
+    rng = np.random.default_rng(3)
+    w = 3
+    a = np.zeros((14, 15, 16), dtype=np.float32)
+    a[w:-w, w:-w, w:-w] = rng.random((8, 9, 10))
+    b = np.zeros_like(a)
+    b[w:-w, w:-w, w:-w] = rng.random((8, 9, 10))
+    sa, sb, sab = orc.star(a, 2, w=w), orc.star(b, 2, w=w), orc.star(a + b, 2, w=w)
+    assert np.max(np.abs(sab - (sa + sb))) < 1e-6
+    # reflecting the grid along x reflects the result (the sum's order
+    # differs, so within rounding)
+    assert np.max(np.abs(orc.star(a[:, :, ::-1].copy(), 2, w=w) - sa[:, :, ::-1])) < 1e-6
+
+
+def test_heat_ref_point_source(orc):
+    """One step from a unit impulse: the cell itself gets 3 c^2 (c = 0.234,
+    one c^2 per axis), its 6 face neighbours c, all else 0."""
+    c = 0.234
+    g = np.zeros((9, 9, 9))
+    g[4, 4, 4] = 1.0
+    out = orc.heat_ref(g, 1)
+    assert out[4, 4, 4] == c * c + c * c + c * c
+    for idx in ((3, 4, 4), (5, 4, 4), (4, 3, 4), (4, 5, 4), (4, 4, 3), (4, 4, 5)):
+        assert abs(out[idx] - c) < 1e-16
+    out[4, 4, 4] = 0
+    for idx in ((3, 4, 4), (5, 4, 4), (4, 3, 4), (4, 5, 4), (4, 4, 3), (4, 4, 5)):
+        out[idx] = 0
+    assert not out.any()
+
+
+def test_heat_ref_eigenmode(orc):
+    """The discrete sine mode with the zero halo is an eigenvector: each step
+    multiplies it by 2c sum_d cos(pi p_d / (M_d + 1)) + 3 c^2."""
+    c = 0.234
+    M = (10, 12, 9)  # z, y, x interior
+    p = (2, 1, 3)
+    axes = [np.sin(np.pi * pd * np.arange(Md + 2) / (Md + 1)) for pd, Md in zip(p, M)]
+    g = axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
+    lam = 2 * c * sum(np.cos(np.pi * pd / (Md + 1)) for pd, Md in zip(p, M)) + 3 * c * c
+    out = orc.heat_ref(g, 5)
+    assert np.max(np.abs(out - lam ** 5 * g)) <= 1e-12 * np.max(np.abs(g))
