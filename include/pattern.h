@@ -208,11 +208,14 @@ int pat_stencil_destroy(pat_stencil *s);
                                 param = struct para {double side; double cutoffRadius;} */
 #define PAT_IR_LJ_OPEN 1     /* miniMD map1, miniMD/irregular.h:109-163: no images;
                                 param = double cutforcesq */
+#define PAT_IR_NO_FUNCTOR -1 /* pat_ir_create before pat_ir_set_table: no library functor */
 /* Kernel variants. */
 #define PAT_IR_CSR 0      /* by-destination CSR, each row summed in its stored order: bit-identical
                              to a serial scatter over the edge list in that order (default) */
 #define PAT_IR_ATOMIC 1   /* edge-parallel, float atomics into both endpoints */
 #define PAT_IR_CSR_TREE 2 /* CSR with butterfly (tree-order) row sums: FP32-tolerance parity */
+#define PAT_IR_TABLE_KERNEL 3 /* the registered application table's own per-node kernel, also for
+                                 functors a library kernel computes (pat_ir_set_table) */
 /* lanes_per_row value of PAT_IR_CSR selecting the sliced layout: one lane per
  * row over 64-row slices stored entry-major (SELL-64), so a wavefront's k-th
  * column read is one contiguous load.  Same sums, bit for bit, as every other
@@ -266,11 +269,44 @@ typedef struct pat_ir pat_ir;
 /* IrregularRuntime(num_edges, num_nodes, edges, ...) + IrregularInit
  * (irregular_runtime.h:71-111, irregular_runtime.cu:17-193): uploads the
  * half edge list and builds the CSR on the device.  The caller's arrays are
- * not modified (the reference permutes them in place). */
+ * not modified (the reference permutes them in place).  map_idx: a library
+ * functor with its parameter, or PAT_IR_NO_FUNCTOR (param may be NULL) when
+ * an application table follows. */
 int pat_ir_create(int64_t num_edges, int64_t num_nodes, const int32_t *edges, int map_idx, const void *param,
                   int64_t param_size, pat_ir **out);
-/* set_map_idx (irregular_runtime.h:110). */
+/* set_map_idx (irregular_runtime.h:110): a library functor (PAT_IR_LJ_*), or
+ * with an application table registered an index of that table. */
 int pat_ir_set_map_idx(pat_ir *h, int idx);
+/* set_reduce_idx (irregular_runtime.h:111): an index of the application
+ * table's reduce functors (the library's own functors add float3 values). */
+int pat_ir_set_reduce_idx(pat_ir *h, int idx);
+/* An application's irregular functor tables compiled for the device: the
+ * map_function_table / reduce_function_table of its functor header
+ * (kmeans/irregular.h:116-118, miniMD/irregular.h:172-174), built by hipcc
+ * with GPU_KERNEL defined through api/gpu/irregular_table.hpp, which also
+ * defines pat_app_ir_table() returning this struct.  Replaces the reference's
+ * per-partition kernel with its LDS/device hashes
+ * (kmeans/irregular/gpu_kernel.cu:21-103). */
+typedef struct pat_ir_table {
+    int map_count, reduce_count; /* functors in the two tables */
+    int value_size;              /* bytes of the reduction VALUE */
+    const int *same_as;          /* per map: the library functor giving the same bits, or -1 */
+    /* For every node r: map map_idx over the edges in row r of the CSR by
+     * destination (rowptr; eids = edge ids, increasing within a row) with
+     * part_index = identity and part_id = r, into node r's object (first
+     * insert copied, later ones folded by reduce reduce_idx; inserts for
+     * other nodes refused and flagged in *foreign), written to out[r]
+     * (value_size bytes; zero for a node nothing was inserted for).  Device
+     * pointers; queued on stream, not waited for. */
+    int (*reduce_rows)(int map_idx, int reduce_idx, const int32_t *rowptr, const int32_t *eids, int64_t num_nodes,
+                       const void *point_data, const int32_t *part_index, const void *edge_data, const void *param,
+                       void *out, int32_t *foreign, void *stream);
+} pat_ir_table;
+/* Registers (or with NULL removes) the application's tables; map and reduce
+ * indices restart at 0 and the node data must be set again.  Maps declared
+ * the same as a library functor run on its kernels unless
+ * PAT_IR_TABLE_KERNEL is selected.  The table must outlive the handle. */
+int pat_ir_set_table(pat_ir *h, const pat_ir_table *table);
 /* The functor parameter (the runtime's 'parameter' argument). */
 int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size);
 /* Kernel variant and lanes per row (0 = tuned for PAT_IR_CSR). */
@@ -280,13 +316,16 @@ int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row);
  * an order of its own, the edge order of its partitions
  * (kmeans/irregular/reorder.cpp:13-281). */
 int pat_ir_set_row_order(pat_ir *h, int order);
-/* reset_node_data (irregular_runtime.cu:427-439): node coordinates,
- * node_elm_size 12 (float3) or 24 (double3, moldyn's point_data). */
+/* reset_node_data (irregular_runtime.cu:427-439): node data of node_elm_size
+ * bytes per node: float3 (12) or double3 (24, moldyn's point_data) for the
+ * library's functors; any size, passed to the functors as given, with an
+ * application table. */
 int pat_ir_reset_node_data(pat_ir *h, const void *node_data, int node_elm_size);
 /* IrregularStart (irregular_runtime.cu:311-391): one reduction pass. */
 int pat_ir_start(pat_ir *h);
-/* get_reduction_result (irregular_runtime.cu:393-411): N float3 values
- * (VALUE, kmeans/irregular/parameters.h:19-28) in the caller's node order. */
+/* get_reduction_result (irregular_runtime.cu:393-411): N VALUEs in the
+ * caller's node order: float3 (kmeans/irregular/parameters.h:19-28), or the
+ * application table's value_size bytes each. */
 int pat_ir_get_result(pat_ir *h, float *out);
 int pat_ir_destroy(pat_ir *h);
 

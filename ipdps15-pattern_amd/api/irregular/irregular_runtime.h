@@ -3,13 +3,17 @@
 // selected by set_map_idx runs on the GPU through libpattern and results come
 // back in the caller's node order (the caller's arrays are never permuted).
 //
-// Each application compiles its own functor table (the reference builds one
-// binary per app directory).  An application selects its table by defining
-// PAT_IR_MAP_TABLE before including this header; the default is moldyn's
-// (kmeans/irregular.h:116, {map}: minimum-image LJ with
-// parameter struct para {double side; double cutoffRadius;}).  miniMD's table
-// (miniMD/irregular.h:172, {map1}: LJ without images, parameter double
-// cutforcesq) is {PAT_IR_LJ_OPEN}.
+// Each application compiles its own functor tables (the reference builds one
+// binary per app directory).  An application that links its tables compiled
+// for the device (api/gpu/irregular_table.hpp, which defines
+// pat_app_ir_table) has them registered at IrregularInit: its map and reduce
+// indices then select its own functors, which run on the GPU.  Without
+// them, PAT_IR_MAP_TABLE (defined before including this header) maps each map
+// index to the library functor with the same bits (-1: none); the default is
+// moldyn's {PAT_IR_LJ_MINIMAGE} (kmeans/irregular.h:116, minimum-image LJ,
+// parameter struct para {double side; double cutoffRadius;}); miniMD's
+// (miniMD/irregular.h:172, no images, parameter double cutforcesq) is
+// {PAT_IR_LJ_OPEN}.
 #ifndef IRREGULAR_IRREGULAR_RUNTIME_H_
 #define IRREGULAR_IRREGULAR_RUNTIME_H_
 
@@ -24,6 +28,9 @@
 #ifndef PAT_IR_MAP_TABLE
 #define PAT_IR_MAP_TABLE {PAT_IR_LJ_MINIMAGE}
 #endif
+
+// Defined by an application's device functor tables when they are linked in.
+extern "C" const pat_ir_table *pat_app_ir_table(void) __attribute__((weak));
 
 class IrregularRuntime {
   public:
@@ -68,10 +75,17 @@ class IrregularRuntime {
     // (the reference's reorder + partition + halo set-up).
     void IrregularInit()
     {
-        const int table[] = PAT_IR_MAP_TABLE;
-        if (map_idx_ < 0 || map_idx_ >= (int)(sizeof(table) / sizeof(table[0]))) fail("map index not in the table");
-        PAT_API_CALL(pat_ir_create(global_num_edges_, global_num_nodes_, (const int32_t *)global_edges_,
-                                   table[map_idx_], parameter_, parameter_size_, &handle_));
+        if (pat_app_ir_table) {
+            PAT_API_CALL(pat_ir_create(global_num_edges_, global_num_nodes_, (const int32_t *)global_edges_,
+                                       PAT_IR_NO_FUNCTOR, nullptr, 0, &handle_));
+            PAT_API_CALL(pat_ir_set_table(handle_, pat_app_ir_table()));
+            PAT_API_CALL(pat_ir_set_map_idx(handle_, map_idx_));
+            PAT_API_CALL(pat_ir_set_reduce_idx(handle_, reduce_idx_));
+            PAT_API_CALL(pat_ir_set_param(handle_, parameter_, parameter_size_));
+        } else {
+            PAT_API_CALL(pat_ir_create(global_num_edges_, global_num_nodes_, (const int32_t *)global_edges_,
+                                       library_functor(map_idx_), parameter_, parameter_size_, &handle_));
+        }
         if (global_node_data_) reset_node_data(global_node_data_);
     }
 
@@ -79,13 +93,15 @@ class IrregularRuntime {
     {
         map_idx_ = idx;
         if (handle_) {
-            const int table[] = PAT_IR_MAP_TABLE;
-            if (idx < 0 || idx >= (int)(sizeof(table) / sizeof(table[0]))) fail("map index not in the table");
-            PAT_API_CALL(pat_ir_set_map_idx(handle_, table[idx]));
+            PAT_API_CALL(pat_ir_set_map_idx(handle_, pat_app_ir_table ? idx : library_functor(idx)));
             PAT_API_CALL(pat_ir_set_param(handle_, parameter_, parameter_size_));
         }
     }
-    void set_reduce_idx(int idx) { reduce_idx_ = idx; }
+    void set_reduce_idx(int idx)
+    {
+        reduce_idx_ = idx;
+        if (handle_ && pat_app_ir_table) PAT_API_CALL(pat_ir_set_reduce_idx(handle_, idx));
+    }
 
     // One pass of the edge functor over every edge, reducing into nodes.
     void IrregularStart()
@@ -97,7 +113,7 @@ class IrregularRuntime {
 
     void IrregularBarrier() { PAT_API_CALL(pat_device_sync()); }
 
-    // VALUE (float3) per node in the caller's order.
+    // VALUE per node in the caller's order.
     void get_reduction_result(void *buffer) { PAT_API_CALL(pat_ir_get_result(handle_, (float *)buffer)); }
 
     // New node coordinates (node_data_elm_size bytes per node: 12 = float3,
@@ -108,6 +124,13 @@ class IrregularRuntime {
     }
 
   private:
+    static int library_functor(int idx)
+    {
+        const int table[] = PAT_IR_MAP_TABLE;
+        if (idx < 0 || idx >= (int)(sizeof(table) / sizeof(table[0]))) fail("map index not in the table");
+        if (table[idx] < 0) fail("this map has no library kernel: link the application's device table");
+        return table[idx];
+    }
     static void fail(const char *msg)
     {
         fprintf(stderr, "pattern error: IrregularRuntime: %s\n", msg);

@@ -4,14 +4,17 @@
 // on M^3 FCC cells (default 20: 32,000 atoms) at density 0.8442 with
 // neighbour cutoff 2.8 and force cutoff 2.5, or the sizes of an lj.in file
 // (-f path); then the thermo pair energy through RegularRuntime (map2).  As in the reference's miniMD
-// there are no periodic images in the force.  Every pass is checked against
-// the host map1 functor.
+// there are no periodic images in the force.  Node data is double[3] per
+// atom, as miniMD passes it.  The force is map 0 of the application's functor
+// table, compiled for the device (gpu_table.hip); every pass is checked
+// against the same functor on the host.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
+#define CPU_KERNEL
 #include "irregular.h"
 #include "include/runtime.h"
 #include "irregular/irregular_runtime.h"
@@ -48,13 +51,15 @@ int main(int argc, char *argv[])
     PAT_API_CALL(pat_gen_pairs(n, x.data(), box[0], cutneigh, 0, 0, (int32_t *)edges.data(), &ne));
     double cutforcesq = cutforce * cutforce;
 
-    IrregularRuntime runtime(ne, n, edges.data(), nullptr, x.data(), 0, 3 * sizeof(float), sizeof(VALUE), 3, nullptr,
-                             0, &cutforcesq, sizeof(double), 1, 5);
+    std::vector<double> xd(x.begin(), x.end());
+    IrregularRuntime runtime(ne, n, edges.data(), nullptr, xd.data(), 0, 3 * sizeof(double), sizeof(VALUE), 3,
+                             nullptr, 0, &cutforcesq, sizeof(double), 1, 5);
     runtime.IrregularInit();
     std::vector<VALUE> f(n);
-    FCPU::Cobject obj(n);
+    std::vector<int> part(n, 0);
+    Cobject obj(n);
     for (long e = 0; e < ne; e++)
-        FCPU::map_function_table[0](&obj, x.data(), nullptr, edges.data(), &cutforcesq, (unsigned)e, 0,
+        FCPU::map_function_table[0](&obj, xd.data(), part.data(), edges.data(), &cutforcesq, (unsigned)e, 0,
                                     FCPU::reduce_function_table[0]);
     for (int pass = 0; pass < 5; pass++) {
         runtime.IrregularStart();

@@ -1679,6 +1679,38 @@ static int scan_exclusive(const int *in, int *out, int64_t n, int *ws, hipStream
     return PAT_OK;
 }
 
+// The edge ids of the CSR by destination (row r: every edge with r as an
+// endpoint, in increasing edge id -- the serial scatter's order for r);
+// rowptr as csr_build writes it.  Same workspace as csr_build.
+int csr_edge_ids(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *eids, void *work,
+                 int64_t work_bytes, hipStream_t st)
+{
+    PAT_REQUIRE(N > 0 && N < (1LL << 31) - 1 && E >= 0 && 2 * E < (1LL << 31), PAT_ERR_INVALID,
+                "csr_edge_ids: sizes out of range (N=%lld, E=%lld)", (long long)N, (long long)E);
+    PAT_REQUIRE(work_bytes >= csr_workspace_bytes(N, E), PAT_ERR_INVALID, "csr_edge_ids: workspace too small");
+    const int64_t nb = (N + 1 + kScanBlock - 1) / kScanBlock;
+    int *deg = (int *)work;
+    int *ws = deg + (N + 1);
+    int *tmp = ws + 2 * nb + 2 * ((nb + kScanBlock - 1) / kScanBlock) + 2;
+    PAT_HIP(hipMemsetAsync(deg, 0, sizeof(int) * (N + 1), st));
+    const unsigned grid = (unsigned)std::min<int64_t>((E + 255) / 256 + 1, 65536);
+    if (E > 0) {
+        hipLaunchKernelGGL(csr_degree, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg);
+        PAT_LAUNCH_CHECK();
+    }
+    PAT_TRY(scan_exclusive(deg, rowptr, N + 1, ws, st));
+    PAT_HIP(hipMemcpyAsync(deg, rowptr, sizeof(int) * (N + 1), hipMemcpyDeviceToDevice, st));
+    if (E > 0) {
+        int *tmpe = tmp + 2 * E;
+        hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp, tmpe);
+        PAT_LAUNCH_CHECK();
+        const unsigned g2 = (unsigned)((N * kWave + 255) / 256);
+        hipLaunchKernelGGL(csr_row_sort_by_edge, dim3(g2), dim3(256), 0, st, rowptr, N, tmpe, tmpe, eids);
+        PAT_LAUNCH_CHECK();
+    }
+    return PAT_OK;
+}
+
 // order 0: rows in edge-list order; 1: rows sorted by neighbour index.
 int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st)

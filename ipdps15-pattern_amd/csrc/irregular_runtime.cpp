@@ -21,6 +21,8 @@
 
 namespace pat {
 int64_t csr_workspace_bytes(int64_t N, int64_t E);
+int csr_edge_ids(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, int32_t *eids, void *work,
+                 int64_t work_bytes, hipStream_t st);
 int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *rowptr, int32_t *cols, void *work,
               int64_t work_bytes, hipStream_t st);
 int sell_build(const int32_t *rowptr, const int32_t *cols, const int32_t *perm, int64_t N, double max_pad,
@@ -590,14 +592,101 @@ int check_sizes(int64_t num_edges, int64_t num_nodes, const int32_t *edges)
 
 }  // namespace
 
+// The irregular runtime handle.  map_idx is the functor index the caller
+// selected: a library functor (PAT_IR_LJ_*), or with an application table
+// registered an index of that table; builtin is the library functor that runs
+// it (-1: the table's own per-node kernel runs it).
 struct pat_ir {
     IrState st;
-    int map_idx = PAT_IR_LJ_MINIMAGE;
+    int map_idx = PAT_IR_LJ_MINIMAGE, reduce_idx = 0;
+    int builtin = PAT_IR_LJ_MINIMAGE;
     float box = 0.f;
     int pbc = 1;
     float rc2 = 0.f;
     bool have_nodes = false;
     bool have_result = false;
+    // application table path
+    const pat_ir_table *table = nullptr;
+    bool table_kernel = false;  // run the table's kernel even where a library functor has its bits
+    std::vector<unsigned char> param;
+    void *param_dev = nullptr, *node_raw = nullptr, *out_raw = nullptr;
+    int64_t node_raw_bytes = 0;
+    int32_t *eids = nullptr, *ident = nullptr, *foreign = nullptr;
+    bool last_table = false;  // the last start ran the table's kernel
+
+    int resolve()
+    {
+        if (!table) {
+            builtin = map_idx;
+        } else {
+            const int same = table->same_as ? table->same_as[map_idx] : -1;
+            builtin = table_kernel ? -1 : same;
+        }
+        return PAT_OK;
+    }
+
+    // Device copy of the parameter bytes for the table functors.
+    int upload_param()
+    {
+        if (param_dev) (void)pat::dev_free(param_dev);
+        param_dev = nullptr;
+        if (param.empty()) return PAT_OK;
+        if (pat::dev_malloc(&param_dev, param.size()) != hipSuccess) {
+            (void)hipGetLastError();
+            param_dev = nullptr;
+            pat::set_error("irregular: hipMalloc of %zu parameter bytes failed", param.size());
+            return PAT_ERR_NOMEM;
+        }
+        PAT_HIP(hipMemcpyAsync(param_dev, param.data(), param.size(), hipMemcpyHostToDevice, st.stream));
+        PAT_HIP(hipStreamSynchronize(st.stream));
+        return PAT_OK;
+    }
+
+    // The table kernel's inputs: edge ids per row, identity part index,
+    // output values and the refusal flag (built once).
+    int table_buffers()
+    {
+        if (eids) return PAT_OK;
+        auto am = [](void **p, size_t b) -> int {
+            if (pat::dev_malloc(p, b ? b : 16) != hipSuccess) {
+                (void)hipGetLastError();
+                *p = nullptr;
+                pat::set_error("irregular: hipMalloc of %zu bytes failed", b);
+                return PAT_ERR_NOMEM;
+            }
+            return PAT_OK;
+        };
+        const int64_t N = st.N, E = st.E;
+        int32_t *rp = nullptr;
+        PAT_TRY(am((void **)&eids, sizeof(int32_t) * 2 * (size_t)E));
+        PAT_TRY(am((void **)&ident, sizeof(int32_t) * (size_t)N));
+        PAT_TRY(am((void **)&foreign, sizeof(int32_t)));
+        PAT_TRY(am(&out_raw, (size_t)table->value_size * (size_t)N));
+        PAT_TRY(am((void **)&rp, sizeof(int32_t) * (size_t)(N + 1)));
+        const int64_t wb = pat::csr_workspace_bytes(N, E);
+        void *work = nullptr;
+        int rc = am(&work, (size_t)wb);
+        if (rc == PAT_OK) rc = pat::csr_edge_ids(st.edges, E, N, rp, eids, work, wb, st.stream);
+        std::vector<int32_t> iota((size_t)N);
+        for (int64_t i = 0; i < N; i++) iota[(size_t)i] = (int32_t)i;
+        if (rc == PAT_OK &&
+            hipMemcpyAsync(ident, iota.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice, st.stream) !=
+                hipSuccess)
+            rc = PAT_ERR_HIP;
+        if (hipStreamSynchronize(st.stream) != hipSuccess && rc == PAT_OK) rc = PAT_ERR_HIP;
+        if (work) (void)pat::dev_free(work);
+        (void)pat::dev_free(rp);  // equal to st.rowptr: a row's length does not depend on its order
+        if (rc == PAT_ERR_HIP) pat::set_error("irregular: building the edge-id rows failed");
+        return rc;
+    }
+
+    void release_table()
+    {
+        for (void *p : {param_dev, node_raw, out_raw, (void *)eids, (void *)ident, (void *)foreign})
+            if (p) (void)pat::dev_free(p);
+        param_dev = node_raw = out_raw = nullptr;
+        eids = ident = foreign = nullptr;
+    }
 };
 
 struct pat_md {
@@ -824,17 +913,40 @@ int pat_ir_create(int64_t num_edges, int64_t num_nodes, const int32_t *edges, in
 int pat_ir_set_map_idx(pat_ir *h, int idx)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_map_idx: null handle");
-    PAT_REQUIRE(idx == PAT_IR_LJ_MINIMAGE || idx == PAT_IR_LJ_OPEN, PAT_ERR_UNSUPPORTED,
-                "pat_ir_set_map_idx: functor index %d not in the table", idx);
+    if (idx == PAT_IR_NO_FUNCTOR && !h->table) {  // an application table follows
+        h->map_idx = h->builtin = idx;
+        return PAT_OK;
+    }
+    if (h->table)
+        PAT_REQUIRE(idx >= 0 && idx < h->table->map_count, PAT_ERR_UNSUPPORTED,
+                    "pat_ir_set_map_idx: functor index %d not in the application's table", idx);
+    else
+        PAT_REQUIRE(idx == PAT_IR_LJ_MINIMAGE || idx == PAT_IR_LJ_OPEN, PAT_ERR_UNSUPPORTED,
+                    "pat_ir_set_map_idx: functor index %d not in the library's table (register the application's "
+                    "with pat_ir_set_table)", idx);
     h->map_idx = idx;
-    h->pbc = idx == PAT_IR_LJ_MINIMAGE;
+    PAT_TRY(h->resolve());
+    if (h->builtin >= 0) h->pbc = h->builtin == PAT_IR_LJ_MINIMAGE;
+    h->have_result = false;
+    return PAT_OK;
+}
+
+int pat_ir_set_reduce_idx(pat_ir *h, int idx)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_reduce_idx: null handle");
+    const int n = h->table ? h->table->reduce_count : 1;
+    PAT_REQUIRE(idx >= 0 && idx < n, PAT_ERR_UNSUPPORTED, "pat_ir_set_reduce_idx: reduce index %d not in the table",
+                idx);
+    h->reduce_idx = idx;
+    h->have_result = false;
     return PAT_OK;
 }
 
 int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size)
 {
-    PAT_REQUIRE(h && param, PAT_ERR_INVALID, "pat_ir_set_param: null argument");
-    if (h->map_idx == PAT_IR_LJ_MINIMAGE) {
+    PAT_REQUIRE(h && (param || param_size == 0) && param_size >= 0, PAT_ERR_INVALID,
+                "pat_ir_set_param: bad argument");
+    if (h->builtin == PAT_IR_LJ_MINIMAGE) {
         // struct para {double side; double cutoffRadius;} (kmeans/moldyn.h:39-43)
         PAT_REQUIRE(param_size == 2 * (int64_t)sizeof(double), PAT_ERR_INVALID,
                     "pat_ir_set_param: moldyn parameter is {double side, double cutoffRadius}");
@@ -844,7 +956,7 @@ int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size)
         PAT_REQUIRE(side > 2 * rc && rc > 0, PAT_ERR_INVALID, "pat_ir_set_param: box must exceed twice the cutoff");
         h->box = (float)side;
         h->rc2 = (float)rc * (float)rc;
-    } else {
+    } else if (h->builtin == PAT_IR_LJ_OPEN) {
         // double cutforcesq (miniMD/irregular.h:111)
         PAT_REQUIRE(param_size == (int64_t)sizeof(double), PAT_ERR_INVALID,
                     "pat_ir_set_param: miniMD parameter is double cutforcesq");
@@ -854,12 +966,46 @@ int pat_ir_set_param(pat_ir *h, const void *param, int64_t param_size)
         h->box = 1.0f;
         h->rc2 = (float)c2;
     }
+    h->param.assign((const unsigned char *)param, (const unsigned char *)param + param_size);
+    if (h->table) PAT_TRY(h->upload_param());
+    h->have_result = false;
+    return PAT_OK;
+}
+
+int pat_ir_set_table(pat_ir *h, const pat_ir_table *table)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_table: null handle");
+    PAT_REQUIRE(!table || (table->map_count > 0 && table->reduce_count > 0 && table->reduce_rows &&
+                           table->value_size > 0 && table->value_size <= 1024),
+                PAT_ERR_INVALID, "pat_ir_set_table: bad table");
+    PAT_HIP(hipStreamSynchronize(h->st.stream));
+    h->release_table();
+    h->table = table;
+    h->map_idx = 0;
+    h->reduce_idx = 0;
+    h->table_kernel = false;
+    PAT_TRY(h->resolve());
+    if (h->builtin >= 0) h->pbc = h->builtin == PAT_IR_LJ_MINIMAGE;
+    if (table) PAT_TRY(h->upload_param());
+    h->have_nodes = false;
+    h->have_result = false;
     return PAT_OK;
 }
 
 int pat_ir_set_variant(pat_ir *h, int variant, int lanes_per_row)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_set_variant: null handle");
+    if (variant == PAT_IR_TABLE_KERNEL) {
+        PAT_REQUIRE(h->table, PAT_ERR_UNSUPPORTED, "pat_ir_set_variant: no application table registered");
+        h->table_kernel = true;
+        PAT_TRY(h->resolve());
+        h->have_result = false;
+        return PAT_OK;
+    }
+    if (h->table_kernel) {
+        h->table_kernel = false;
+        PAT_TRY(h->resolve());
+    }
     PAT_TRY(check_variant(variant, lanes_per_row));
     PAT_TRY(check_sliced(h->st, variant, lanes_per_row));
     h->st.variant = variant;
@@ -881,8 +1027,32 @@ int pat_ir_set_row_order(pat_ir *h, int order)
 
 int pat_ir_reset_node_data(pat_ir *h, const void *node_data, int node_elm_size)
 {
-    PAT_REQUIRE(h && node_data, PAT_ERR_INVALID, "pat_ir_reset_node_data: null argument");
-    PAT_TRY(h->st.upload3(h->st.pos[0], node_data, node_elm_size));
+    PAT_REQUIRE(h && node_data && node_elm_size > 0, PAT_ERR_INVALID, "pat_ir_reset_node_data: bad argument");
+    if (h->table) {
+        // the bytes as given, for the application's functors
+        const int64_t bytes = (int64_t)node_elm_size * h->st.N;
+        if (bytes != h->node_raw_bytes) {
+            if (h->node_raw) (void)pat::dev_free(h->node_raw);
+            h->node_raw = nullptr;
+            h->node_raw_bytes = 0;
+            if (pat::dev_malloc(&h->node_raw, (size_t)bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                h->node_raw = nullptr;
+                pat::set_error("pat_ir_reset_node_data: hipMalloc of %lld bytes failed", (long long)bytes);
+                return PAT_ERR_NOMEM;
+            }
+            h->node_raw_bytes = bytes;
+        }
+        PAT_HIP(hipMemcpyAsync(h->node_raw, node_data, (size_t)bytes, hipMemcpyHostToDevice, h->st.stream));
+        PAT_HIP(hipStreamSynchronize(h->st.stream));
+    }
+    if (node_elm_size == 12 || node_elm_size == 24) {
+        PAT_TRY(h->st.upload3(h->st.pos[0], node_data, node_elm_size));
+    } else {
+        PAT_REQUIRE(h->table, PAT_ERR_UNSUPPORTED,
+                    "pat_ir_reset_node_data: node element size %d needs an application table (the library's LJ "
+                    "functors read float3 or double3)", node_elm_size);
+    }
     h->have_nodes = true;
     h->have_result = false;
     return PAT_OK;
@@ -892,8 +1062,28 @@ int pat_ir_start(pat_ir *h)
 {
     PAT_REQUIRE(h, PAT_ERR_INVALID, "pat_ir_start: null handle");
     PAT_REQUIRE(h->have_nodes, PAT_ERR_STATE, "pat_ir_start: no node data (call pat_ir_reset_node_data)");
-    PAT_TRY(h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2));
+    PAT_REQUIRE(h->table || h->builtin >= 0, PAT_ERR_STATE, "pat_ir_start: no functor (register a table)");
+    if (h->builtin >= 0) {
+        PAT_TRY(h->st.force_eval(h->st.pos[0], h->box, h->pbc, h->rc2));
+        PAT_HIP(hipStreamSynchronize(h->st.stream));
+        h->last_table = false;
+        h->have_result = true;
+        return PAT_OK;
+    }
+    PAT_REQUIRE(h->node_raw, PAT_ERR_STATE, "pat_ir_start: node data was set before the table");
+    PAT_TRY(h->table_buffers());
+    int32_t refused = 0;
+    PAT_HIP(hipMemsetAsync(h->foreign, 0, sizeof(int32_t), h->st.stream));
+    const int rc = h->table->reduce_rows(h->map_idx, h->reduce_idx, h->st.rowptr, h->eids, h->st.N, h->node_raw,
+                                         h->ident, h->st.edges, h->param_dev, h->out_raw, h->foreign, h->st.stream);
+    PAT_REQUIRE(rc == PAT_OK, rc, "pat_ir_start: the table's kernel for map %d / reduce %d failed (%d)", h->map_idx,
+                h->reduce_idx, rc);
+    PAT_HIP(hipMemcpyAsync(&refused, h->foreign, sizeof(int32_t), hipMemcpyDeviceToHost, h->st.stream));
     PAT_HIP(hipStreamSynchronize(h->st.stream));
+    PAT_REQUIRE(!refused, PAT_ERR_UNSUPPORTED,
+                "pat_ir_start: map %d inserted a value for a node other than the partition's own (only an edge's "
+                "endpoints may be keys, each filtered by part_index[node] == part_id)", h->map_idx);
+    h->last_table = true;
     h->have_result = true;
     return PAT_OK;
 }
@@ -902,12 +1092,20 @@ int pat_ir_get_result(pat_ir *h, float *out)
 {
     PAT_REQUIRE(h && out, PAT_ERR_INVALID, "pat_ir_get_result: null argument");
     PAT_REQUIRE(h->have_result, PAT_ERR_STATE, "pat_ir_get_result: no result (call pat_ir_start)");
+    if (h->last_table) {
+        PAT_HIP(hipMemcpyAsync(out, h->out_raw, (size_t)h->table->value_size * (size_t)h->st.N, hipMemcpyDeviceToHost,
+                               h->st.stream));
+        PAT_HIP(hipStreamSynchronize(h->st.stream));
+        return PAT_OK;
+    }
     return h->st.download3(out, h->st.force);
 }
 
 int pat_ir_destroy(pat_ir *h)
 {
     if (!h) return PAT_OK;
+    if (h->st.stream) (void)hipStreamSynchronize(h->st.stream);
+    h->release_table();
     h->st.release();
     delete h;
     return PAT_OK;

@@ -25,6 +25,8 @@ PAT_IR_LJ_OPEN = 1
 PAT_IR_CSR = 0
 PAT_IR_ATOMIC = 1
 PAT_IR_CSR_TREE = 2
+PAT_IR_TABLE_KERNEL = 3
+PAT_IR_NO_FUNCTOR = -1
 PAT_IR_SLICED = 64
 PAT_IR_ORDER_EDGE = 0
 PAT_IR_ORDER_NODE = 1
@@ -123,6 +125,8 @@ SIGNATURES = {
     "pat_lj_force_atomic_dev": (c_int, [c_vp, c_vp, c_i64, c_i64, c_float, c_int, c_float, c_vp, c_vp]),
     "pat_ir_create": (c_int, [c_i64, c_i64, c_vp, c_int, c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "pat_ir_set_map_idx": (c_int, [c_vp, c_int]),
+    "pat_ir_set_reduce_idx": (c_int, [c_vp, c_int]),
+    "pat_ir_set_table": (c_int, [c_vp, c_vp]),
     "pat_ir_set_param": (c_int, [c_vp, c_vp, c_i64]),
     "pat_ir_set_variant": (c_int, [c_vp, c_int, c_int]),
     "pat_ir_set_row_order": (c_int, [c_vp, c_int]),
@@ -515,16 +519,30 @@ def to_float4(a3):
 
 
 class IrregularRuntime:
-    """Handle mirror of IrregularRuntime (kmeans/irregular/irregular_runtime.h)."""
+    """Handle mirror of IrregularRuntime (kmeans/irregular/irregular_runtime.h).
+    With table (app_table(app, "ir")) the application's map / reduce indices
+    select its own functors, compiled for the device."""
 
-    def __init__(self, edges, num_nodes, map_idx=PAT_IR_LJ_MINIMAGE, param=None):
+    def __init__(self, edges, num_nodes, map_idx=PAT_IR_LJ_MINIMAGE, param=None, table=None, reduce_idx=0):
         e = np.ascontiguousarray(edges, dtype=np.int32)
         self.n = num_nodes
         self._param = np.ascontiguousarray(param)
         h = c_vp()
-        call("pat_ir_create", e.shape[0], num_nodes, ptr(e), map_idx, ptr(self._param), self._param.nbytes,
-             ctypes.byref(h))
+        if table is None:
+            call("pat_ir_create", e.shape[0], num_nodes, ptr(e), map_idx, ptr(self._param), self._param.nbytes,
+                 ctypes.byref(h))
+        else:
+            call("pat_ir_create", e.shape[0], num_nodes, ptr(e), PAT_IR_NO_FUNCTOR, None, 0, ctypes.byref(h))
         self.h = h
+        if table is not None:
+            call("pat_ir_set_table", self.h, table)
+            call("pat_ir_set_map_idx", self.h, map_idx)
+            call("pat_ir_set_reduce_idx", self.h, reduce_idx)
+            call("pat_ir_set_param", self.h, ptr(self._param), self._param.nbytes)
+
+    def set_map_idx(self, idx):
+        call("pat_ir_set_map_idx", self.h, idx)
+        call("pat_ir_set_param", self.h, ptr(self._param), self._param.nbytes)
 
     def set_variant(self, variant, lanes_per_row=0):
         call("pat_ir_set_variant", self.h, variant, lanes_per_row)

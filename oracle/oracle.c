@@ -319,6 +319,65 @@ static inline int pair_force_f64(const double *pi, const double *pj, double L, d
 }
 
 /*
+ * The reference's LJ functors in their own arithmetic, applied serially over
+ * the edge list with the reduction object's semantics (the first value for a
+ * node copied, later ones added, in FP32): FP64 positions (N x 3), FP64 pair
+ * arithmetic, each force component rounded to float where it is inserted,
+ * the j side the negated float.
+ *   form 0: moldyn's map (kmeans/irregular.h:26-92): minimum image in a box
+ *           of side L, rd < rc^2 with rc^2 = cutoff^2, rrd = 1/rd,
+ *           rrd2 = rrd^2, rrd4 = rrd2^2, rrd6 = rrd2 rrd4, rrd7 = rrd6 rrd,
+ *           r148 = rrd7 - 0.5 rrd4, force = d r148 (no 48 eps prefactor);
+ *   form 1: miniMD's map1 (miniMD/irregular.h:109-163): no images,
+ *           rd < cutforcesq, rrd3 = rrd2 rrd, force = d (rrd3 (rrd3 - 0.5) rrd).
+ * cut2 is rc^2 (form 0) or cutforcesq (form 1).  f: N x 3 FP32 (zero where
+ * nothing was inserted).
+ */
+void orc_ir_lj_ref(int64_t N, int64_t E, const int32_t *edges, const double *pos, double L, double cut2, int form,
+                   float *f)
+{
+    const double hL = 0.5 * L;
+    unsigned char *used = (unsigned char *)calloc((size_t)(N > 0 ? N : 1), 1);
+    memset(f, 0, sizeof(float) * 3 * (size_t)N);
+    for (int64_t e = 0; e < E; e++) {
+        const int i = edges[2 * e], j = edges[2 * e + 1];
+        double d[3];
+        for (int c = 0; c < 3; c++) d[c] = pos[3 * (int64_t)i + c] - pos[3 * (int64_t)j + c];
+        if (form == 0) {
+            for (int c = 0; c < 3; c++)
+                if (d[c] < -hL) d[c] += L;
+            for (int c = 0; c < 3; c++)
+                if (d[c] > hL) d[c] -= L;
+        }
+        const double rd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(rd < cut2)) continue;
+        const double rrd = 1.0 / rd, rrd2 = rrd * rrd;
+        double s;
+        if (form == 0) {
+            const double rrd4 = rrd2 * rrd2, rrd6 = rrd2 * rrd4, rrd7 = rrd6 * rrd;
+            s = rrd7 - 0.5 * rrd4;
+        } else {
+            const double rrd3 = rrd2 * rrd;
+            s = rrd3 * (rrd3 - 0.5) * rrd;
+        }
+        float v[3];
+        for (int c = 0; c < 3; c++) v[c] = (float)(d[c] * s);
+        for (int side = 0; side < 2; side++) {
+            const int64_t n = side ? j : i;
+            if (side)
+                for (int c = 0; c < 3; c++) v[c] = -v[c];
+            if (!used[n]) {
+                for (int c = 0; c < 3; c++) f[3 * n + c] = v[c];
+                used[n] = 1;
+            } else {
+                for (int c = 0; c < 3; c++) f[3 * n + c] += v[c];
+            }
+        }
+    }
+    free(used);
+}
+
+/*
  * Serial scatter over a half edge list, f[i] += F*d, f[j] -= F*d, in edge
  * order: the serial semantics of miniMD Force::compute (miniMD/force.cpp:
  * 88-116) and of moldyn's commented-out scatter (kmeans/irregular.h:94-100),

@@ -25,6 +25,7 @@ _SIG = {
     "orc_heat3d_run_f32_tiled": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_float, c_int]),
     "orc_star_run_f32": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int]),
     "orc_heat_ref_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int]),
+    "orc_ir_lj_ref": (None, [c_i64, c_i64, c_vp, c_vp, c_double, c_double, c_int, c_vp]),
     "orc_lj_force_f32": (c_double, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp]),
     "orc_lj_force_f64": (c_double, [c_i64, c_i64, c_vp, c_vp, c_double, c_int, c_double, c_vp]),
     "orc_lj_force_f32_threads": (None, [c_i64, c_i64, c_vp, c_vp, c_float, c_int, c_float, c_vp, c_vp, c_int]),
@@ -123,6 +124,19 @@ def lj_force(pos, edges, box, rc, pbc=True, dtype=np.float32):
     ep = fn(n, e.shape[0], _p(e), _p(p), box, int(pbc), rc * rc if dtype != np.float32 else np.float32(rc) ** 2,
             _p(f))
     return f, ep
+
+
+def ir_lj_ref(pos, edges, box, cut2, form):
+    """The reference's LJ functors in their own FP64 arithmetic (form 0:
+    moldyn's map with the minimum image; 1: miniMD's map1, no images), serial
+    over the edge list with the reduction object's copy-then-add semantics;
+    returns N x 3 float32."""
+    n = pos.shape[0]
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    e = np.ascontiguousarray(edges, dtype=np.int32)
+    f = np.zeros((n, 3), dtype=np.float32)
+    lib().orc_ir_lj_ref(n, e.shape[0], _p(e), _p(p), float(box), float(cut2), int(form), _p(f))
+    return f
 
 
 def lj_force_threads(pos, edges, box, rc, pbc=True, threads=0):

@@ -41,6 +41,7 @@ def test_kmeans_app(gpu):
 def test_moldyn_app(gpu):
     out = run("moldyn", 8, 20)
     assert "first pass" in out and "kinetic energy" in out and "velocity statistics" in out
+    assert "reference functor (FP64, no 48)" in out and "0 nodes beyond 1e-5" in out
 
 
 def test_minimd_app(gpu):

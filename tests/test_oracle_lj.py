@@ -226,3 +226,25 @@ def test_md_rebuild_f64_matches_fixed_list(pa, orc):
     # and the FP32 loop stays close to it
     f32 = orc.md_run_rebuild(pos, vel, box[0], 2.5, 2.8, 10, 0.005, 45, threads=2)
     assert np.max(np.abs(f32[0] - d[0])) < 1e-3
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_reference_functor_forms(pa, orc, form):
+    """The reference functors' own arithmetic (no 48 eps prefactor): 48 x it
+    equals the FP64 force of the serial scatter to FP32 rounding; on two
+    particles it is the closed form r148 d (moldyn) / rrd3 (rrd3 - 1/2) rrd d
+    (miniMD), which are the same function."""
+    pos, box, edges = pa.moldyn_system(5, seed=17)
+    pos64 = pos.astype(np.float64)
+    ref = orc.ir_lj_ref(pos64, edges, box, 2.5 ** 2, form)
+    f64, _ = orc.lj_force(pos64, edges, box, 2.5, pbc=form == 0, dtype=np.float64)
+    scale = np.max(np.abs(f64))
+    assert np.max(np.abs(48.0 * ref.astype(np.float64) - f64)) < 1e-5 * scale
+    p2 = np.array([[3.0, 3.0, 3.0], [4.3, 3.0, 3.0]])
+    r = 1.3
+    one = orc.ir_lj_ref(p2, np.array([[0, 1]], dtype=np.int32), 20.0, 6.25, form)
+    expect = np.float32(-r * (r ** -14 - 0.5 * r ** -8))
+    assert abs(one[0, 0] - expect) <= 1e-6 * abs(expect) and one[1, 0] == -one[0, 0]
+    # a node with no pair inside the cutoff keeps zero
+    p3 = np.array([[3.0, 3.0, 3.0], [5.6, 3.0, 3.0]])
+    assert not orc.ir_lj_ref(p3, np.array([[0, 1]], dtype=np.int32), 20.0, 6.25, form).any()
