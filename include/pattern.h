@@ -513,6 +513,72 @@ int pat_gr_run(pat_gr *h);
 int pat_gr_get_output(pat_gr *h, int *num_keys, int *num_values, double *values, int64_t *emits);
 int pat_gr_destroy(pat_gr *h);
 
+/* Generalized reduction with an application's own map/reduce functors and
+ * keyed reduction objects: RegularRuntime(num_procs, input, input_size,
+ * offsets, n, params, param_size) + RegularInit + set_map_idx /
+ * set_reduce_idx + RegularStart + RegularGetOutput
+ * (kmeans/regular/regular_runtime.h:79-105) for any map_fp / reduce_fp
+ * (kmeans/regular.h:25-27) with any key and value bytes, their tables
+ * compiled for the device by hipcc with GPU_KERNEL defined through
+ * api/gpu/regular_table.hpp, which defines pat_app_gr_table().  Replaces the
+ * reference's regular compute_gpu with its LDS and device hash objects
+ * (kmeans/regular/gpu_kernel.cu:39-183, ros.cu:15-170, rog.cu:17-310).
+ * Each wavefront folds its units into an LDS hash table (lanes inserting in
+ * turn); a full table is merged into the wavefront's table in device memory
+ * and the refused units' maps run again; the wavefronts' tables are then
+ * folded in a fixed order into the final table, so every value is the same
+ * fold every run. */
+typedef struct pat_gr_object { /* device memory of one run, set up by the library */
+    const void *data;          /* the units' bytes */
+    const int64_t *offsets;    /* byte offset of each unit in data, or NULL: unit i at i * unit_bytes */
+    int64_t unit_bytes, num_units;
+    const void *param;         /* the functors' parameter (may be NULL) */
+    uint32_t *wave_tables;     /* waves tables of (table_buckets + 1 + table_pool_words) words */
+    uint32_t *final_table;     /* one table of that size */
+    int32_t *status;           /* [0] error bits (1 a wavefront's table full, 2 a unit's values do not fit
+                                  an empty LDS table, 4 a map's insert refused after another was taken,
+                                  8 the final table full), [1] LDS merges, [2] maps run again */
+    int32_t waves, table_buckets, table_pool_words, lds_buckets, lds_pool_words;
+} pat_gr_object;
+typedef struct pat_gr_table {
+    int map_count, reduce_count;
+    int max_lds_buckets, max_lds_pool_words; /* the kernels' LDS tables hold at most this much */
+    /* One pass of map map_idx over every unit with reduce reduce_idx into the
+     * final table; queued on stream, not waited for. */
+    int (*map_reduce)(int map_idx, int reduce_idx, const pat_gr_object *obj, void *stream);
+} pat_gr_table;
+typedef struct pat_regular pat_regular;
+/* RegularRuntime(..., input, input_size, offsets, n, ...) + RegularInit: the
+ * input (copied to the device) and its units: unit i is unit_bytes bytes at
+ * offsets[i] (Offset.offset, kmeans/regular/data_type.h:21-27), or at
+ * i * unit_bytes when offsets is NULL. */
+int pat_regular_create(const void *input, int64_t input_bytes, int64_t num_units, int64_t unit_bytes,
+                       const int64_t *offsets, pat_regular **out);
+/* The application's table (required before a start; must outlive the handle). */
+int pat_regular_set_table(pat_regular *h, const pat_gr_table *table);
+/* Capacity of the reduction objects: distinct keys and bytes of key + value
+ * per record (default 4096 keys of 256 bytes), and the keys an LDS table
+ * takes before it is merged (0: as many as fit; a small number forces the
+ * merge-and-run-again path). */
+int pat_regular_set_capacity(pat_regular *h, int64_t max_keys, int64_t max_record_bytes, int lds_keys);
+int pat_regular_set_map_idx(pat_regular *h, int idx);
+int pat_regular_set_reduce_idx(pat_regular *h, int idx);
+int pat_regular_set_param(pat_regular *h, const void *param, int64_t param_bytes);
+/* RegularStart: one map + reduce pass over every unit.  Blocking; fails with
+ * the status error bits named above. */
+int pat_regular_start(pat_regular *h);
+/* RegularGetOutput: the final object's records sorted by key bytes.  Call
+ * with NULL buffers for *num_keys and the byte totals; then keys (key bytes
+ * back to back), key_index (byte offset of each key), key_size, vals,
+ * val_index, val_size are filled (each array num_keys long). */
+int pat_regular_get_output(pat_regular *h, int64_t *num_keys, int64_t *key_bytes, int64_t *val_bytes, void *keys,
+                           uint32_t *key_index, uint32_t *key_size, void *vals, uint32_t *val_index,
+                           uint32_t *val_size);
+/* LDS merges and maps run again in the last start (the reference's
+ * merge-and-retry), and the wavefronts used. */
+int pat_regular_stats(pat_regular *h, int64_t *merges, int64_t *reruns, int *waves);
+int pat_regular_destroy(pat_regular *h);
+
 /* ------------------------------------------------------------------------
  * Reference input files (host only), so reference-shaped datasets replay.
  * pat_read_mold: moldyn's parameter file, which kmeans/moldyn.cu:193-201

@@ -26,39 +26,9 @@
 #define GET_FLOAT2(input, i, j) *((float *)input + (long long)size[0] * (i) + (j))
 #define GET_INT2(input, i, j) *((int *)input + (long long)size[0] * (i) + (j))
 
-// Host reduction object for the generalized-reduction functors: a dense
-// key -> value table with the insert(key, ksize, value, vsize, reduce) call
-// the functors use (kmeans/regular/roc.h:39).
-class CO {
-  public:
-    typedef void (*reduce_fn)(void *, unsigned short, void *, unsigned short);
-    CO(int max_keys, int value_size) : nkeys_(max_keys), vsize_(value_size), used_(max_keys, 0),
-                                       vals_((size_t)max_keys * value_size, 0) {}
-    bool insert(void *key, unsigned short ksize, void *value, unsigned short vsize, reduce_fn reduce)
-    {
-        unsigned int k = 0;
-        memcpy(&k, key, ksize < sizeof(k) ? ksize : sizeof(k));
-        if ((int)k >= nkeys_ || vsize != vsize_) return false;
-        char *slot = vals_.data() + (size_t)k * vsize_;
-        if (!used_[k]) {
-            memcpy(slot, value, vsize);
-            used_[k] = 1;
-        } else {
-            reduce(slot, vsize, value, vsize);
-        }
-        return true;
-    }
-    bool used(int k) const { return used_[k] != 0; }
-    const void *value(int k) const { return vals_.data() + (size_t)k * vsize_; }
-
-  private:
-    int nkeys_;
-    unsigned short vsize_;
-    std::vector<char> used_;
-    std::vector<char> vals_;
-};
-
-static inline void copy_val(void *dst, const void *src, int size) { memcpy(dst, src, size); }
+// The generalized-reduction objects (CO on the host, SO on the device) and
+// ANY_KEY / copy_val are in regular/objects.h; the irregular ones in
+// irregular/objects.h.
 
 // Host reduction object for the irregular functors: a dense node -> VALUE
 // array with the insert(&key, &value, reduce) call of

@@ -15,15 +15,26 @@
 // The reference spreads indices 0 and 2 over ANY_KEY buckets; here they are
 // one key (the buckets' sum).  RegularStart() is one map+reduce pass over
 // every unit.
+//
+// An application that links its functor tables compiled for the device
+// (api/gpu/regular_table.hpp, which defines pat_app_gr_table) runs its own
+// map/reduce pairs instead, any index, with keyed reduction objects
+// (pat_regular_*): the units are the input bytes at the Offset array's
+// offsets as given, and RegularGetOutput returns every key the object holds
+// with its value, sorted by key bytes.
 #ifndef REGULAR_RUNTIME_H_
 #define REGULAR_RUNTIME_H_
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "../pat_check.h"
 #include "data_type.h"
+
+// Defined by an application's device functor tables when they are linked in.
+extern "C" const pat_gr_table *pat_app_gr_table(void) __attribute__((weak));
 
 class RegularRuntime {
   public:
@@ -50,6 +61,7 @@ class RegularRuntime {
     {
         if (handle_) pat_kmeans_destroy(handle_);
         if (gr_) pat_gr_destroy(gr_);
+        if (reg_) pat_regular_destroy(reg_);
     }
 
     // Validates the units and uploads them.  Units of equal size laid out
@@ -57,6 +69,19 @@ class RegularRuntime {
     void RegularInit()
     {
         if (num_offsets_ == 0 || offsets_ == nullptr) fail("RegularInit: no units");
+        if (pat_app_gr_table) {
+            // the units as the Offset array places them, on the device
+            std::vector<int64_t> off(num_offsets_);
+            int64_t usz = 0;
+            for (size_t i = 0; i < num_offsets_; i++) {
+                off[i] = (int64_t)offsets_[i].offset;
+                usz = std::max<int64_t>(usz, (int64_t)offsets_[i].size);
+            }
+            PAT_API_CALL(pat_regular_create(input_, (int64_t)input_size_, (int64_t)num_offsets_, usz, off.data(),
+                                            &reg_));
+            PAT_API_CALL(pat_regular_set_table(reg_, pat_app_gr_table()));
+            return;
+        }
         const size_t usz = offsets_[0].size;
         if (usz == 0 || usz % sizeof(float)) fail("RegularInit: unit size must be a whole number of floats");
         for (size_t i = 0; i < num_offsets_; i++)
@@ -78,6 +103,13 @@ class RegularRuntime {
 
     void RegularStart()
     {
+        if (reg_) {
+            PAT_API_CALL(pat_regular_set_map_idx(reg_, map_idx_));
+            PAT_API_CALL(pat_regular_set_reduce_idx(reg_, reduce_idx_));
+            PAT_API_CALL(pat_regular_set_param(reg_, parameter_, parameter_size_));
+            PAT_API_CALL(pat_regular_start(reg_));
+            return;
+        }
         if (map_idx_ != reduce_idx_ || map_idx_ < 0 || map_idx_ > 2)
             fail("RegularStart: functor pairs 0/0, 1/1 and 2/2 have GPU engines");
         if (map_idx_ != 1) {
@@ -100,6 +132,7 @@ class RegularRuntime {
 
     struct output RegularGetOutput()
     {
+        if (reg_) return keyed_output();
         if (gr_) return additive_output();
         if (!handle_) fail("RegularGetOutput: RegularStart has not run");
         std::vector<int64_t> sums((size_t)k_ * dim_), counts(k_), dsums(k_);
@@ -130,6 +163,15 @@ class RegularRuntime {
         return out_;
     }
 
+    // Capacity of the keyed reduction objects (with device tables; not in the
+    // reference API, whose objects have compile-time sizes): distinct keys,
+    // bytes of key + value, keys an LDS object takes before it is merged.
+    void set_object_capacity(int64_t max_keys, int64_t record_bytes, int lds_keys)
+    {
+        if (!reg_) fail("set_object_capacity: RegularInit with a device table first");
+        PAT_API_CALL(pat_regular_set_capacity(reg_, max_keys, record_bytes, lds_keys));
+    }
+
     // Assignment of every unit from the last pass (not in the reference API;
     // the kmeans driver uses it to report cluster sizes).
     void get_keys(int *keys)
@@ -154,6 +196,27 @@ class RegularRuntime {
         else
             PAT_API_CALL(pat_gr_set_param(gr_, parameter_, parameter_size_));
         PAT_API_CALL(pat_gr_run(gr_));
+    }
+
+    struct output keyed_output()
+    {
+        int64_t nk = 0, kb = 0, vb = 0;
+        PAT_API_CALL(pat_regular_get_output(reg_, &nk, &kb, &vb, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+        kbytes_.resize((size_t)std::max<int64_t>(kb, 1));
+        vbytes_.resize((size_t)std::max<int64_t>(vb, 1));
+        kidx_.resize((size_t)nk);
+        vidx_.resize((size_t)nk);
+        std::vector<uint32_t> ksz((size_t)nk), vsz((size_t)nk);
+        PAT_API_CALL(pat_regular_get_output(reg_, &nk, &kb, &vb, kbytes_.data(), kidx_.data(), ksz.data(),
+                                            vbytes_.data(), vidx_.data(), vsz.data()));
+        out_.output_keys = kbytes_.data();
+        out_.output_vals = vbytes_.data();
+        out_.key_index = kidx_.data();
+        out_.val_index = vidx_.data();
+        out_.key_num = (unsigned int)nk;
+        out_.key_size = nk ? ksz[0] : 0;
+        out_.value_size = nk ? vsz[0] : 0;
+        return out_;
     }
 
     struct output additive_output()
@@ -194,6 +257,8 @@ class RegularRuntime {
     std::vector<float> vals_;
     std::vector<double> dvals_;
     pat_gr *gr_ = nullptr;
+    pat_regular *reg_ = nullptr;
+    std::vector<char> kbytes_, vbytes_;
     struct output out_;
 };
 

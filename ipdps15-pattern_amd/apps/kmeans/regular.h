@@ -10,6 +10,7 @@
 #include <cmath>
 
 #include "pattern_api.h"
+#include "regular/objects.h"
 
 namespace FFCPU {
 

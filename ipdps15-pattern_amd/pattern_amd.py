@@ -157,6 +157,16 @@ SIGNATURES = {
     "pat_gr_run": (c_int, [c_vp]),
     "pat_gr_get_output": (c_int, [c_vp, c_intp, c_intp, c_vp, c_vp]),
     "pat_gr_destroy": (c_int, [c_vp]),
+    "pat_regular_create": (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
+    "pat_regular_set_table": (c_int, [c_vp, c_vp]),
+    "pat_regular_set_capacity": (c_int, [c_vp, c_i64, c_i64, c_int]),
+    "pat_regular_set_map_idx": (c_int, [c_vp, c_int]),
+    "pat_regular_set_reduce_idx": (c_int, [c_vp, c_int]),
+    "pat_regular_set_param": (c_int, [c_vp, c_vp, c_i64]),
+    "pat_regular_start": (c_int, [c_vp]),
+    "pat_regular_get_output": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pat_regular_stats": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pat_regular_destroy": (c_int, [c_vp]),
     "pat_md_get_sliced_form": (c_int, [c_vp, c_intp]),
     "pat_md_set_slice_order": (c_int, [c_vp, c_int]),
     "pat_md_get_slice_order": (c_int, [c_vp, c_intp]),
@@ -997,6 +1007,64 @@ class GR:
 
 
 # ---------------------------------------------------------------- reference input files
+
+class Regular:
+    """Handle mirror of RegularRuntime with an application's device functor
+    tables and keyed reduction objects (pat_regular_*).  data: the input
+    (any numpy array, used as bytes); unit i is unit_bytes at offsets[i]
+    (bytes), or at i * unit_bytes."""
+
+    def __init__(self, data, num_units, unit_bytes, offsets=None, table=None):
+        self._data = np.ascontiguousarray(data)
+        self._off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.int64)
+        h = c_vp()
+        call("pat_regular_create", ptr(self._data), self._data.nbytes, int(num_units), int(unit_bytes),
+             ptr(self._off), ctypes.byref(h))
+        self.h = h
+        if table is not None:
+            call("pat_regular_set_table", self.h, table)
+
+    def set_capacity(self, max_keys, record_bytes, lds_keys=0):
+        call("pat_regular_set_capacity", self.h, int(max_keys), int(record_bytes), int(lds_keys))
+
+    def run(self, map_idx, reduce_idx=None, param=None):
+        """One map + reduce pass; returns {key bytes: value bytes}."""
+        call("pat_regular_set_map_idx", self.h, map_idx)
+        call("pat_regular_set_reduce_idx", self.h, map_idx if reduce_idx is None else reduce_idx)
+        self._param = None if param is None else np.ascontiguousarray(param)
+        call("pat_regular_set_param", self.h, ptr(self._param), 0 if param is None else self._param.nbytes)
+        call("pat_regular_start", self.h)
+        return self.output()
+
+    def output(self):
+        nk, kb, vb = c_i64(0), c_i64(0), c_i64(0)
+        call("pat_regular_get_output", self.h, ctypes.byref(nk), ctypes.byref(kb), ctypes.byref(vb), None, None,
+             None, None, None, None)
+        n = nk.value
+        keys = np.zeros(max(kb.value, 1), np.uint8)
+        vals = np.zeros(max(vb.value, 1), np.uint8)
+        ki, ks, vi, vs = (np.zeros(max(n, 1), np.uint32) for _ in range(4))
+        call("pat_regular_get_output", self.h, ctypes.byref(nk), ctypes.byref(kb), ctypes.byref(vb), ptr(keys),
+             ptr(ki), ptr(ks), ptr(vals), ptr(vi), ptr(vs))
+        return {keys[ki[i]:ki[i] + ks[i]].tobytes(): vals[vi[i]:vi[i] + vs[i]].tobytes() for i in range(n)}
+
+    def stats(self):
+        """(LDS merges, maps run again, wavefronts) of the last run."""
+        m, r, w = c_i64(0), c_i64(0), c_int(0)
+        call("pat_regular_stats", self.h, ctypes.byref(m), ctypes.byref(r), ctypes.byref(w))
+        return m.value, r.value, w.value
+
+    def close(self):
+        if self.h:
+            call("pat_regular_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class Mold(ctypes.Structure):
     _fields_ = [("box_cells", c_int), ("steps", c_int), ("neigh_update", c_int), ("cutoff", ctypes.c_double),

@@ -81,3 +81,12 @@ def test_moldyn_app_replays_a_reference_dataset(pa, gpu, tmp_path):
     (tmp_path / "mold").write_text(f"4\n10\n20\n2.5\ninput/g.mesh\ninput/g.xyz\n10\n0\n1\n")
     out = run("moldyn", "-f", str(tmp_path / "mold"))
     assert f"{n} molecules, {len(e)} pairs" in out and "first pass" in out and "velocity statistics" in out
+
+
+def test_regular_app(gpu):
+    """RegularRuntime with the application's own functors on the device
+    (keyed reduction objects): every pair agrees with the host functors."""
+    out = run("regular")
+    for line in ("map 0 velocity statistics", "map 1 kmeans D=3 K=40", "map 2 pair energy", "map 3 histogram"):
+        assert line in out
+    assert "DIFFER" not in out and out.count("match") == 5
