@@ -9,9 +9,10 @@
 //   2 miniMD pair energy,
 //   3 a 5,003-key histogram, its units placed by an Offset array with gaps,
 //     once with the LDS objects forced to merge after 4 keys.
-// Integer values (counts) must match exactly; FP32 sums to 1e-5 (of the sum,
-// or for coordinate sums that may cancel of count x 16) and FP64 sums to
-// 1e-10 relative (the device folds in another order).
+// Integer values (counts) must match exactly; FP32 sums to 1e-5 (kmeans:
+// 1e-4 of the sum, or of count x 16 for coordinate sums that may cancel,
+// since the host adds thousands of values one by one) and FP64 sums to 1e-10
+// relative (the device folds in another order).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -104,14 +105,23 @@ int main(int argc, char *argv[])
                 continue;
             }
             const float *a = (const float *)it->second.data(), *b = (const float *)e.second.data();
-            // counts exact; sums within FP32 summation error: 1e-5 of the
-            // count times the largest |coordinate| (16) for the coordinate
-            // sums, which may nearly cancel, and of the distance sum
+            // counts exact; sums of ~7,500 FP32 values, which the host adds
+            // one by one, within 1e-4 of the count times the largest
+            // |coordinate| (16) for the coordinate sums (they may nearly
+            // cancel) and of the distance sum
             ok = ok && a[3] == b[3];
-            for (int d = 0; d < 3; d++) ok = ok && std::fabs(a[d] - b[d]) <= 1e-5 * 16.0 * b[3];
-            ok = ok && close(a[4], b[4], 1e-5);
+            for (int d = 0; d < 3; d++) ok = ok && std::fabs(a[d] - b[d]) <= 1e-4 * 16.0 * b[3];
+            ok = ok && close(a[4], b[4], 1e-4);
         }
         printf("regular: map 1 kmeans D=3 K=40: %zu keys, counts and sums %s\n", dev.size(), ok ? "match" : "DIFFER");
+        if (!ok)
+            for (auto &e : host.entries()) {
+                auto it = dev.find(e.first);
+                if (it == dev.end()) continue;
+                const float *a = (const float *)it->second.data(), *b = (const float *)e.second.data();
+                printf("  key %u: device %g %g %g %g %g host %g %g %g %g %g\n", *(const unsigned *)e.first.data(),
+                       a[0], a[1], a[2], a[3], a[4], b[0], b[1], b[2], b[3], b[4]);
+            }
         bad += !ok;
     }
     // 2: miniMD pair energy over an 8^3-cell FCC lattice's pairs within 2.8
