@@ -55,7 +55,7 @@ __device__ bool ro_merge_into(RoTable &d, const RoTable &s, R reduce)
     for (int b = 0; b < s.nb; b++) {
         const uint32_t st = s.bk[b];
         if (st == 0u) continue;
-        const uint32_t *r = s.pool + (st - 1u);
+        const ro_word *r = s.pool + (st - 1u);
         const int ks = (int)(r[0] >> 16), vs = (int)(r[0] & 0xffffu);
         if (!d.insert(r + 1, ks, r + 1 + RoTable::key_words(ks), vs, reduce)) return false;
     }
@@ -65,14 +65,14 @@ __device__ bool ro_merge_into(RoTable &d, const RoTable &s, R reduce)
 template <class Tab, int MI, int RI>
 __global__ __launch_bounds__(256) void ro_map(pat_gr_object a)
 {
-    __shared__ uint32_t lds[kRoWaves * (kRoLdsBuckets + 1 + kRoLdsPool)];
+    __shared__ ro_word lds[kRoWaves * (kRoLdsBuckets + 1 + kRoLdsPool)];
     const int wl = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
     const int64_t w = (int64_t)blockIdx.x * kRoWaves + wl;
     if (w >= a.waves) return;  // whole wavefronts; no workgroup barrier below
     SO so;
     so.t = RoTable::at(lds + wl * (kRoLdsBuckets + 1 + kRoLdsPool), a.lds_buckets, a.lds_pool_words);
     so.t.clear(lane, 64);
-    RoTable wt = RoTable::at(a.wave_tables + (size_t)w * RoTable::words(a.table_buckets, a.table_pool_words),
+    RoTable wt = RoTable::at((ro_word *)a.wave_tables + (size_t)w * RoTable::words(a.table_buckets, a.table_pool_words),
                              a.table_buckets, a.table_pool_words);
     wt.clear(lane, 64);
     __builtin_amdgcn_wave_barrier();
@@ -128,30 +128,30 @@ __global__ __launch_bounds__(256) void ro_map(pat_gr_object a)
 // compare-and-swap; the inserts of one round carry distinct keys, so a
 // bucket another thread is filling (kRoPending) never holds this key.
 template <typename R>
-__device__ bool ro_insert_shared(RoTable &d, const uint32_t *rec, R reduce)
+__device__ bool ro_insert_shared(RoTable &d, const ro_word *rec, R reduce)
 {
     const int ks = (int)(rec[0] >> 16), vs = (int)(rec[0] & 0xffffu);
-    const uint32_t *key = rec + 1;
-    const uint32_t *val = rec + 1 + RoTable::key_words(ks);
+    const ro_word *key = rec + 1;
+    const ro_word *val = rec + 1 + RoTable::key_words(ks);
     const uint32_t h = ro_hash((const unsigned char *)key, ks);
     for (int p = 0; p < d.nb; p++) {
         const int b = (int)((h + (uint32_t)p) & (uint32_t)(d.nb - 1));
-        uint32_t st = __atomic_load_n(&d.bk[b], __ATOMIC_RELAXED);
+        uint32_t st = __atomic_load_n((uint32_t *)&d.bk[b], __ATOMIC_RELAXED);
         if (st == 0u) {
-            st = atomicCAS(&d.bk[b], 0u, kRoPending);
+            st = atomicCAS((uint32_t *)&d.bk[b], 0u, kRoPending);
             if (st == 0u) {
                 const int need = 1 + RoTable::key_words(ks) + ((vs + 3) >> 2);
-                const uint32_t at = atomicAdd(d.top, (uint32_t)need);
+                const uint32_t at = atomicAdd((uint32_t *)d.top, (uint32_t)need);
                 if ((int)at + need > d.pw) return false;  // the bucket stays claimed: the result is void anyway
-                uint32_t *r = d.pool + at;
+                ro_word *r = d.pool + at;
                 for (int i = 0; i < need; i++) r[i] = rec[i];
                 __threadfence_block();
-                atomicExch(&d.bk[b], at + 1u);
+                atomicExch((uint32_t *)&d.bk[b], at + 1u);
                 return true;
             }
         }
         if (st == kRoPending) continue;
-        uint32_t *r = d.pool + (st - 1u);
+        ro_word *r = d.pool + (st - 1u);
         if (r[0] == rec[0] && ro_equal(r + 1, key, ks)) {
             reduce((void *)(r + 1 + RoTable::key_words(ks)), (unsigned short)vs, (void *)val, (unsigned short)vs);
             return true;
@@ -163,13 +163,14 @@ __device__ bool ro_insert_shared(RoTable &d, const uint32_t *rec, R reduce)
 template <class Tab, int RI>
 __global__ __launch_bounds__(256) void ro_fold(pat_gr_object a)
 {
-    RoTable ft = RoTable::at(a.final_table, a.table_buckets, a.table_pool_words);
+    RoTable ft = RoTable::at((ro_word *)a.final_table, a.table_buckets, a.table_pool_words);
     ft.clear((int)threadIdx.x, (int)blockDim.x);
     __syncthreads();
     const auto reduce = Tab::template reduce<RI>();
     int err = 0;
     for (int w = 0; w < a.waves; w++) {
-        const RoTable wt = RoTable::at(a.wave_tables + (size_t)w * RoTable::words(a.table_buckets, a.table_pool_words),
+        const RoTable wt = RoTable::at((ro_word *)a.wave_tables +
+                                           (size_t)w * RoTable::words(a.table_buckets, a.table_pool_words),
                                        a.table_buckets, a.table_pool_words);
         for (int b = (int)threadIdx.x; b < wt.nb; b += (int)blockDim.x) {
             const uint32_t st = wt.bk[b];

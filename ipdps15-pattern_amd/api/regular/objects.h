@@ -20,6 +20,7 @@
 #define REGULAR_OBJECTS_H_
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -38,27 +39,50 @@ class CO {
         std::string k((const char *)key, ksize);
         auto it = vals_.find(k);
         if (it == vals_.end()) {
-            vals_.emplace(std::move(k), std::string((const char *)value, vsize));
+            Val v(vsize);
+            memcpy(v.p, value, vsize);
+            vals_.emplace(std::move(k), std::move(v));
         } else {
-            reduce(&it->second[0], (unsigned short)it->second.size(), value, vsize);
+            reduce(it->second.p, (unsigned short)it->second.n, value, vsize);
         }
         return true;
     }
-    // by key bytes
-    const std::map<std::string, std::string> &entries() const { return vals_; }
+    // key bytes -> value bytes
+    std::map<std::string, std::string> entries() const
+    {
+        std::map<std::string, std::string> r;
+        for (const auto &e : vals_) r.emplace(e.first, std::string((const char *)e.second.p, e.second.n));
+        return r;
+    }
     // by an unsigned int key
     bool used(unsigned int k) const { return vals_.count(ukey(k)) != 0; }
-    const void *value(unsigned int k) const { return vals_.at(ukey(k)).data(); }
+    const void *value(unsigned int k) const { return vals_.at(ukey(k)).p; }
 
   private:
+    // a value's bytes in malloc'd storage, which the functors read and write
+    // as their own types
+    struct Val {
+        void *p = nullptr;
+        size_t n = 0;
+        explicit Val(size_t bytes) : p(std::malloc(bytes ? bytes : 1)), n(bytes) {}
+        Val(Val &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; }
+        Val(const Val &) = delete;
+        Val &operator=(const Val &) = delete;
+        ~Val() { std::free(p); }
+    };
     static std::string ukey(unsigned int k) { return std::string((const char *)&k, sizeof(k)); }
-    std::map<std::string, std::string> vals_;
+    std::map<std::string, Val> vals_;
 };
 
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 
 namespace pat_gpu {
+
+// Words of the tables: keys and values of any type are stored into and read
+// out of them, so every access goes through a type that may alias the
+// functors' own (float, double, struct) accesses to the same bytes.
+typedef uint32_t __attribute__((__may_alias__)) ro_word;
 
 // FNV-1a over the key bytes.
 __device__ __forceinline__ uint32_t ro_hash(const unsigned char *k, int n)
@@ -71,7 +95,7 @@ __device__ __forceinline__ uint32_t ro_hash(const unsigned char *k, int n)
 __device__ __forceinline__ void ro_copy(void *dst, const void *src, int n)
 {
     if ((((uintptr_t)dst | (uintptr_t)src | (uintptr_t)n) & 3) == 0) {
-        for (int i = 0; i < n / 4; i++) ((uint32_t *)dst)[i] = ((const uint32_t *)src)[i];
+        for (int i = 0; i < n / 4; i++) ((ro_word *)dst)[i] = ((const ro_word *)src)[i];
     } else {
         for (int i = 0; i < n; i++) ((unsigned char *)dst)[i] = ((const unsigned char *)src)[i];
     }
@@ -89,10 +113,10 @@ __device__ __forceinline__ bool ro_equal(const void *a, const void *b, int n)
 // counter, then pw pool words.  Record: a header word (key_size << 16 |
 // value_size), the key and the value, each padded to whole words.
 struct RoTable {
-    uint32_t *bk, *top, *pool;
+    ro_word *bk, *top, *pool;
     int nb, pw;
 
-    __device__ static RoTable at(uint32_t *base, int nb, int pw)
+    __device__ static RoTable at(ro_word *base, int nb, int pw)
     {
         return RoTable{base, base + nb, base + nb + 1, nb, pw};
     }
@@ -119,7 +143,7 @@ struct RoTable {
                 const uint32_t rec = *top;
                 if ((int)rec + need > pw) return false;
                 *top = rec + (uint32_t)need;
-                uint32_t *r = pool + rec;
+                ro_word *r = pool + rec;
                 r[0] = hdr;
                 for (int w = 0; w < key_words(ks); w++) r[1 + w] = 0u;
                 ro_copy(r + 1, key, ks);
@@ -127,7 +151,7 @@ struct RoTable {
                 bk[b] = rec + 1u;
                 return true;
             }
-            uint32_t *r = pool + (st - 1u);
+            ro_word *r = pool + (st - 1u);
             if (r[0] == hdr && ro_equal(r + 1, key, ks)) {
                 reduce((void *)(r + 1 + key_words(ks)), (unsigned short)vs, const_cast<void *>(val),
                        (unsigned short)vs);
