@@ -236,6 +236,16 @@ int pat_ir_csr_workspace(int64_t num_nodes, int64_t num_edges, int64_t *bytes);
  * Reorder (kmeans/irregular/partitioner.cpp:20-499, reorder.cpp:13-281). */
 int pat_ir_csr_build_dev(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int order, int32_t *rowptr,
                          int32_t *cols, void *work, int64_t work_bytes, void *stream);
+/* pat_ir_csr_build_dev without allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The work
+ * space is a static 128 MiB device array that every GPU the library runs on
+ * holds until the process ends; PAT_ERR_UNSUPPORTED when the build needs more
+ * (pat_ir_csr_workspace bytes: about 16 per edge + 4 per node, so up to ~8.3M
+ * edges).  It calls no allocator and records no memset.  Calls that overlap
+ * on one GPU share the array and race: serialise them. */
+int pat_ir_csr_build_dev_noalloc(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int order,
+                                 int32_t *rowptr, int32_t *cols, void *stream);
 /* LJ forces from float4 positions (x, y, z, unused) into float4 forces over
  * the CSR; pbc selects the minimum image in a cubic box of side 'box'.
  * lanes_per_row 1, 4, 8, 16, 32: rows summed in CSR order (bit-identical for

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <mutex>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -43,6 +44,23 @@ namespace {
 using pat::kWave;
 
 // ------------------------------------------------------------------ CSR build
+
+// p[0..n) = v, and dst[0..n) = src[0..n): the CSR builds clear and copy
+// their counters with kernels rather than hipMemsetAsync/hipMemcpyAsync, so
+// a build recorded into a captured graph holds kernel nodes only.
+__global__ void fill_i32(int *__restrict__ p, int64_t n, int v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void copy_i32(int *__restrict__ dst, const int *__restrict__ src, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256 + 1, 4096); }
 
 __global__ void csr_degree(const int2 *__restrict__ edges, int64_t E, int *__restrict__ deg)
 {
@@ -1692,14 +1710,16 @@ int csr_edge_ids(const int32_t *edges, int64_t E, int64_t N, int32_t *rowptr, in
     int *deg = (int *)work;
     int *ws = deg + (N + 1);
     int *tmp = ws + 2 * nb + 2 * ((nb + kScanBlock - 1) / kScanBlock) + 2;
-    PAT_HIP(hipMemsetAsync(deg, 0, sizeof(int) * (N + 1), st));
+    hipLaunchKernelGGL(fill_i32, dim3(grid_for(N + 1)), dim3(256), 0, st, deg, N + 1, 0);
+    PAT_LAUNCH_CHECK();
     const unsigned grid = (unsigned)std::min<int64_t>((E + 255) / 256 + 1, 65536);
     if (E > 0) {
         hipLaunchKernelGGL(csr_degree, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg);
         PAT_LAUNCH_CHECK();
     }
     PAT_TRY(scan_exclusive(deg, rowptr, N + 1, ws, st));
-    PAT_HIP(hipMemcpyAsync(deg, rowptr, sizeof(int) * (N + 1), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(copy_i32, dim3(grid_for(N + 1)), dim3(256), 0, st, deg, (const int *)rowptr, N + 1);
+    PAT_LAUNCH_CHECK();
     if (E > 0) {
         int *tmpe = tmp + 2 * E;
         hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp, tmpe);
@@ -1723,7 +1743,8 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *ro
     int *deg = (int *)work;
     int *ws = deg + (N + 1);
     int *tmp = ws + 2 * nb + 2 * ((nb + kScanBlock - 1) / kScanBlock) + 2;
-    PAT_HIP(hipMemsetAsync(deg, 0, sizeof(int) * (N + 1), st));
+    hipLaunchKernelGGL(fill_i32, dim3(grid_for(N + 1)), dim3(256), 0, st, deg, N + 1, 0);
+    PAT_LAUNCH_CHECK();
     const unsigned grid = (unsigned)std::min<int64_t>((E + 255) / 256 + 1, 65536);
     if (E > 0) {
         hipLaunchKernelGGL(csr_degree, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg);
@@ -1731,7 +1752,8 @@ int csr_build(const int32_t *edges, int64_t E, int64_t N, int order, int32_t *ro
     }
     PAT_TRY(scan_exclusive(deg, rowptr, N + 1, ws, st));
     // cursor = rowptr (reuse deg)
-    PAT_HIP(hipMemcpyAsync(deg, rowptr, sizeof(int) * (N + 1), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(copy_i32, dim3(grid_for(N + 1)), dim3(256), 0, st, deg, (const int *)rowptr, N + 1);
+    PAT_LAUNCH_CHECK();
     if (E > 0) {
         int *tmpe = tmp + 2 * E;
         hipLaunchKernelGGL(csr_scatter, dim3(grid), dim3(256), 0, st, (const int2 *)edges, E, deg, tmp, tmpe);
@@ -2423,3 +2445,61 @@ int neighbor_build(const float4 *pos, int64_t N, double L, double cut, NbWork *w
 }
 
 }  // namespace pat
+
+// ------------------------------------------------------------------
+// CSR build without allocation: the work space of pat_ir_csr_build_dev comes
+// from a static device array, for callers that cannot allocate (code under
+// graph capture, serving loops, harnesses that forbid hipMalloc).  Every GPU
+// the library runs on holds its own copy until the process ends; calls that
+// overlap on one GPU share it and race.
+
+namespace {
+
+constexpr int64_t kCsrScratchBytes = 128LL << 20;  // the csr_build work space of ~8.3M edges
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kCsrScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+
+// The current device's address of 'scratch', looked up once per device.
+int scratch_address(void **out)
+{
+    int dev = 0;
+    PAT_HIP(hipGetDevice(&dev));
+    PAT_REQUIRE(dev >= 0 && dev < kScratchDevices, PAT_ERR_UNSUPPORTED, "scratch: device %d beyond %d", dev,
+                kScratchDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        PAT_REQUIRE(e == hipSuccess && p, PAT_ERR_HIP, "scratch: hipGetSymbolAddress failed on device %d: %s", dev,
+                    hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PAT_OK;
+}
+
+}  // namespace
+
+extern "C" int pat_ir_csr_build_dev_noalloc(const int32_t *edges, int64_t num_edges, int64_t num_nodes, int order,
+                                            int32_t *rowptr, int32_t *cols, void *stream)
+{
+    PAT_REQUIRE(rowptr && (num_edges == 0 || (edges && cols)), PAT_ERR_INVALID,
+                "pat_ir_csr_build_dev_noalloc: null argument");
+    PAT_REQUIRE(order == PAT_IR_ORDER_EDGE || order == PAT_IR_ORDER_NODE, PAT_ERR_INVALID,
+                "pat_ir_csr_build_dev_noalloc: row order %d not in {PAT_IR_ORDER_EDGE, PAT_IR_ORDER_NODE}", order);
+    PAT_REQUIRE(num_nodes > 0 && num_nodes < (1LL << 31) - 1 && num_edges >= 0 && 2 * num_edges < (1LL << 31),
+                PAT_ERR_INVALID, "pat_ir_csr_build_dev_noalloc: sizes out of range (N=%lld, E=%lld)",
+                (long long)num_nodes, (long long)num_edges);
+    const int64_t need = pat::csr_workspace_bytes(num_nodes, num_edges);
+    PAT_REQUIRE(need <= kCsrScratchBytes, PAT_ERR_UNSUPPORTED,
+                "pat_ir_csr_build_dev_noalloc: %lld nodes and %lld edges need %lld bytes of work space; the static "
+                "scratch holds %lld (use pat_ir_csr_build_dev with a work space of your own)",
+                (long long)num_nodes, (long long)num_edges, (long long)need, (long long)kCsrScratchBytes);
+    void *ws = nullptr;
+    PAT_TRY(scratch_address(&ws));
+    return pat::csr_build(edges, num_edges, num_nodes, order, rowptr, cols, ws, kCsrScratchBytes, (hipStream_t)stream);
+}

@@ -120,6 +120,7 @@ SIGNATURES = {
     "pat_stencil_device_ptr": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "pat_stencil_destroy": (c_int, [c_vp]),
     "pat_ir_csr_workspace": (c_int, [c_i64, c_i64, c_i64p]),
+    "pat_ir_csr_build_dev_noalloc": (c_int, [c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_vp]),
     "pat_ir_csr_build_dev": (c_int, [c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pat_lj_force_csr_dev": (c_int, [c_vp, c_vp, c_vp, c_i64, c_float, c_int, c_float, c_int, c_vp, c_vp]),
     "pat_lj_force_atomic_dev": (c_int, [c_vp, c_vp, c_i64, c_i64, c_float, c_int, c_float, c_vp, c_vp]),
