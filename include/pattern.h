@@ -449,7 +449,7 @@ int pat_md_destroy(pat_md *h);
 #define PAT_KM_MFMA 1  /* BF16 MFMA distances + exact FP32 rescoring inside the error margin */
 
 /* Fixed-point shift s used for the order-independent sums of n values of
- * magnitude below maxabs (n * maxabs * 2^s < 2^62). */
+ * magnitude below maxabs (maxabs * 2^s <= 2^30 and n * maxabs * 2^s < 2^62). */
 int pat_kmeans_fixed_shift(int64_t n, float maxabs, int *shift);
 
 typedef struct pat_kmeans pat_kmeans;

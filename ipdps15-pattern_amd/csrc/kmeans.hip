@@ -7,7 +7,8 @@
 // nearest centre and reduce1 adds values per key.  The reference adds FP32
 // values in lock-acquisition order, which is not reproducible.  Here each
 // coordinate (and min_dist) is rounded once to an integer multiple of 2^-s
-// (s chosen so the largest possible sum fits in 62 bits) and summed as int64:
+// (s chosen so every rounded value fits in 31 bits and the largest possible
+// sum in 62) and summed as int64:
 // integer addition is associative, so every summation order -- LDS atomics
 // within a workgroup, global atomics across workgroups, or the CPU oracle's
 // loop -- gives identical bits, and so identical centroids every iteration.
@@ -33,6 +34,7 @@
 // results decide (same tie rule).  A centre outside the margin provably has a
 // larger reference distance than the best one, so the assignment equals the
 // plain path's bit for bit.
+#include <algorithm>
 #include <cmath>
 
 #include "pat_common.hpp"
@@ -60,18 +62,13 @@ __host__ __device__ __forceinline__ int64_t frag_index(int k, int e, int ns)
 }
 
 // rint(v * 2^s) as a 64-bit two's-complement integer, with scale = 2^s an
-// FP32 power of two: v * scale is exact (no overflow: |v| 2^s < 2^62 by the
-// choice of s), rounding it to an integer is exact in FP32, and its
-// magnitude splits exactly into 32-bit halves.  The same integer as
+// FP32 power of two chosen so that |v| 2^s <= 2^30 for every value summed:
+// v * scale is exact, rounding it to an integer is exact in FP32, and the
+// integer converts exactly to int32.  The same integer as
 // rint((double)v * 2^s) in FP64.
 __device__ __forceinline__ unsigned long long to_fixed(float v, float scale)
 {
-    const float f = __builtin_rintf(v * scale);
-    const float m = fabsf(f);
-    const float hi = floorf(m * 0x1p-32f);
-    const float lo = m - hi * 0x1p32f;  // exact, in [0, 2^32)
-    const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) + (unsigned long long)(unsigned)lo;
-    return f < 0.0f ? 0ULL - u : u;
+    return (unsigned long long)(long long)(int)__builtin_rintf(v * scale);
 }
 
 // ------------------------------------------------------------ shared parts
@@ -331,10 +328,13 @@ constexpr float kFar = 3.0e38f;
 //
 // Per 32-point tile a wavefront (lanes l and l+32 hold halves of point l):
 // computes the approximate values of every centre with MFMAs, packs each
-// with its index (pack_key) and keeps the smallest and second-smallest keys.
-// The threshold is the minimum plus the error bound plus the packing error;
-// when only the smallest key is under it (the usual case) that one centre is
-// rescored, otherwise every centre under it is.  The exact FP32 results
+// with its index (pack_key) and keeps the three smallest keys of each half
+// (the two smallest per chunk when the index bits cannot tell the chunks
+// apart).  The threshold is the minimum plus the error bound plus the
+// packing error; when only the smallest key is under it (the usual case)
+// that one centre is rescored, when a half's third key lies above it its
+// kept keys under it are, and otherwise every centre under it is (the
+// values recomputed chunk by chunk).  The exact FP32 results
 // decide with map1's rule, and the point is added to the LDS reduction object
 // from the registers holding its half rows.
 // TAIL = false: D == 16*KS and the rows are 16-byte aligned, so every
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
     const int col = lane % 32;   // point within the tile
     const int half = lane / 32;  // which 8 of each 16 K elements this lane holds
     const int nchunks = (K_pad / 32 + KB - 1) / KB;
-    unsigned long long my_rescored = 0;
+    unsigned my_rescored = 0;  // this lane's rescored points (under 2^32 per lane)
 
     // approximate values of chunk ch for this lane's point: value i of block
     // b is centre row 8*(i/4) + 4*half + i%4 of the block (the 32x32 MFMA
@@ -515,13 +515,33 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         // packing error bound: |values| <= ||c||^2 + 2 ||x|| ||c||
         const float dp = 0x1p-16f * (cc_max + 2.0f * xn * cmax) * 1.01f + 0x1p-126f;
 
-        // smallest and second-smallest keys over every centre of this half
         f32x16 acc[KB];
-        float k1 = kFar, k2 = kFar;
+        // The three smallest keys of this half.  When the lane's values of all
+        // chunks fit the 7 index bits (gidx: at most 128 per lane), every key
+        // carries the lane's global value index (value i of block blk ->
+        // 16 blk + i) and one sorted triple runs over all chunks; otherwise
+        // each chunk's two smallest are merged with their chunk recorded and
+        // k3 stays unknown (kFar, never used).
+        const bool gidx = nchunks * KB <= 8;  // uniform
+        float k1 = kFar, k2 = kFar, k3 = kFar;
         int ch1 = 0;
         for (int ch = 0; ch < nchunks; ch++) {
             if (GC) stage_chunk(ch);
             chunk_values(ch, bx, acc);
+            if (gidx) {
+#pragma unroll
+                for (int b = 0; b < KB; b++) {
+                    if (ch * KB + b >= kblocks) break;  // uniform
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const float key = pack_key(acc[b][i], (ch * KB + b) * 16 + i);
+                        k3 = __builtin_amdgcn_fmed3f(k2, k3, key);
+                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                        k1 = __builtin_amdgcn_fmed3f(k1, key, -kFar);  // min (keys are finite)
+                    }
+                }
+                continue;
+            }
             float c1 = kFar, c2 = kFar;
 #pragma unroll
             for (int b = 0; b < KB; b++) {
@@ -549,21 +569,35 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         const float dhi = (xx + m1 + 4.0f * E + 4.0f * dp) * 1.001f + 1e-30f;
         const float thr = m1 + 2.0f * E + 2.0f * dp + 2.0f * bd.c_ref * dhi;
         const bool multi = !(m2 > thr);
+        // lanes whose candidates may lie beyond the keys they kept: every
+        // value under the threshold is then visited again (full)
+        const bool full = multi && (!gidx || !(k3 > thr));
 
         float best_d = kInitMin;
         int best_k = 0x7fffffff;
-        if (!multi && k1 < o1 && live) {
-            // one candidate, held by this half: rescore it
-            const int k = centre_of(ch1, __float_as_int(k1) & 127);
+        // exact distance to value idx of chunk ch (the pack_key index)
+        auto rescore = [&](int ch, int idx) {
+            const int k = centre_of(ch, idx);
             if (k < a.K) {
-                best_d = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
-                best_k = k;
+                const float dk = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
+                if (dk < best_d || (dk == best_d && k < best_k)) {
+                    best_d = dk;
+                    best_k = k;
+                }
             }
+        };
+        if (live && !full) {
+            // one candidate: rescored by the half that holds it; several
+            // (gidx only): each half rescores its kept keys under the
+            // threshold, which are all of its candidates since its third
+            // smallest lies above it
+            if (multi ? !(k1 > thr) : k1 < o1) rescore(gidx ? 0 : ch1, __float_as_int(k1) & 127);
+            if (multi && !(k2 > thr)) rescore(0, __float_as_int(k2) & 127);
         }
-        // Several candidates: rescore every key under the threshold.  Taken
-        // by the whole wavefront (a vote), because recomputing a chunk's
-        // values needs every lane's operands.
-        if (GC ? __syncthreads_or(multi && live) != 0 : __any(multi)) {
+        // Every value under the threshold for the full lanes.  Taken by the
+        // whole wavefront (a vote), because recomputing a chunk's values
+        // needs every lane's operands.
+        if (GC ? __syncthreads_or(full && live) != 0 : __any(full)) {
             for (int ch = 0; ch < nchunks; ch++) {
                 if (GC) {
                     stage_chunk(ch);
@@ -571,7 +605,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                 } else if (nchunks > 1) {
                     chunk_values(ch, bx, acc);
                 }
-                if (!multi) continue;
+                if (!full) continue;
                 uint64_t w0 = 0, w1 = 0;
 #pragma unroll
                 for (int b = 0; b < KB; b++) {
@@ -588,14 +622,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                     while (mw) {
                         const int bit = __builtin_ctzll(mw);
                         mw &= mw - 1;
-                        const int k = centre_of(ch, 64 * w + bit);
-                        if (k < a.K && live) {
-                            const float dk = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
-                            if (dk < best_d || (dk == best_d && k < best_k)) {
-                                best_d = dk;
-                                best_k = k;
-                            }
-                        }
+                        if (live) rescore(ch, 64 * w + bit);
                     }
                 }
             }
@@ -619,24 +646,32 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                 a.assign[p] = best_k;
                 if (a.mindist) a.mindist[p] = best_d;
                 if (multi) my_rescored++;
-                atomicAdd(&cnt_o[best_k], 1ULL);
-                atomicAdd(&dsum_o[best_k], to_fixed(best_d, a.dscale));
             }
-            // this half-wave's dimensions into the reduction object
-            unsigned long long *row = acc_o + (int64_t)best_k * a.rs;
-#pragma unroll
-            for (int s = 0; s < KS; s++)
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int d = 16 * s + 8 * half + e;
-                    if (!TAIL || d < D) atomicAdd(&row[d], to_fixed(xo[8 * s + e], a.scale));
+            // the point (count, distance, this half-wave's dimensions) into
+            // the reduction object; the LDS form addresses the shared array
+            // itself, so that its adds compile to LDS atomics
+            auto add_point = [&](unsigned long long *acc, unsigned long long *cnt, unsigned long long *dsum) {
+                if (half == 0) {
+                    atomicAdd(&cnt[best_k], 1ULL);
+                    atomicAdd(&dsum[best_k], to_fixed(best_d, a.dscale));
                 }
+                unsigned long long *row = acc + best_k * a.rs;
+#pragma unroll
+                for (int s = 0; s < KS; s++)
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const int d = 16 * s + 8 * half + e;
+                        if (!TAIL || d < D) atomicAdd(&row[d], to_fixed(xo[8 * s + e], a.scale));
+                    }
+            };
+            if (a.use_lds_acc) add_point(smem, smem + a.K * a.rs, smem + a.K * a.rs + a.K);
+            else add_point(a.gsum, a.gcnt, a.gdsum);
         }
     }
     // rescored-point counter
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) my_rescored += __shfl_xor(my_rescored, o, kWave);
-    if (lane == 0 && my_rescored) atomicAdd(rescored, my_rescored);
+    if (lane == 0 && my_rescored) atomicAdd(rescored, (unsigned long long)my_rescored);
     if (a.use_lds_acc) {
         __syncthreads();
         flush_lds_object(a, acc_o, cnt_o, dsum_o);
@@ -860,15 +895,16 @@ __global__ void km_zero_i32(int *__restrict__ p, int64_t n)
 
 namespace pat {
 
-// Fixed-point shift for n values of magnitude < maxabs: the largest possible
-// sum n * maxabs * 2^s stays below 2^62.  Same formula as the oracle's.
+// Fixed-point shift for n values of magnitude < maxabs: each value times
+// 2^s stays within 2^30 (so it converts to int32) and the largest possible
+// sum n * maxabs * 2^s below 2^62.  Same formula as the oracle's.
 int kmeans_fixed_shift(int64_t n, float maxabs)
 {
     int eN = 0, eM = 0;
     while ((1LL << eN) < n) eN++;
     double m = maxabs > 0 ? maxabs : 1.0f;
     frexp(m, &eM);
-    int s = 62 - eN - eM;
+    int s = std::min(62 - eN - eM, 30 - eM);
     if (s > 60) s = 60;
     return s;
 }
@@ -945,7 +981,7 @@ int kmeans_prepare_kernels()
         (const void *)km_mfma<4, 4, 1, false, false>, (const void *)km_mfma<8, 2, 1, false, false>,
         // three terms, rows in LDS
         (const void *)km_mfma<1, 4, 3, true, false, 768>, (const void *)km_mfma<1, 4, 3, false, false, 768>,
-        (const void *)km_mfma<2, 4, 3, true, false, 768>, (const void *)km_mfma<2, 4, 3, false, false, 768>,
+        (const void *)km_mfma<2, 1, 3, true, false, 768>, (const void *)km_mfma<2, 1, 3, false, false, 768>,
         (const void *)km_mfma<4, 2, 3, true, false, 768>, (const void *)km_mfma<4, 2, 3, false, false, 768>,
         (const void *)km_mfma<8, 2, 3, true, false>, (const void *)km_mfma<8, 2, 3, false, false>,
         // three terms, rows staged chunk by chunk
@@ -1152,7 +1188,7 @@ int kmeans_assign_mfma(const KmDev &d, hipStream_t st)
     case 4 * 4 + 1: PAT_KM(4, 4, 1); break;
     case 8 * 4 + 1: PAT_KM(8, 2, 1); break;
     case 1 * 4 + 3: PAT_KM_BS(1, 4, 3, false, 768); break;  // 3 wavefronts per SIMD
-    case 2 * 4 + 3: PAT_KM_BS(2, 4, 3, false, 768); break;  // 3 wavefronts per SIMD (168 VGPRs)
+    case 2 * 4 + 3: PAT_KM_BS(2, 1, 3, false, 768); break;  // 3 wavefronts per SIMD
     case 4 * 4 + 3: PAT_KM_BS(4, 2, 3, false, 768); break;  // 3 wavefronts per SIMD
     case 8 * 4 + 3: PAT_KM(8, 2, 3); break;
     default: pat::set_error("kmeans: D too large"); return PAT_ERR_UNSUPPORTED;

@@ -948,8 +948,9 @@ void orc_kmeans_assign_f64(int64_t N, int D, int K, const float *x, const double
  * reduce1 (kmeans/regular.h:121-136) adds FP32 values in whatever order the
  * locked hash inserts happen, which is not reproducible; here each coordinate
  * is rounded once to an integer multiple of 2^-s and summed as int64, so any
- * summation order gives the same bits.  s is chosen so that N * max|x| * 2^s
- * stays below 2^62.  Returns s.
+ * summation order gives the same bits.  s is chosen so that max|x| * 2^s
+ * stays within 2^30 (each rounded value fits in 31 bits) and N * max|x| * 2^s
+ * below 2^62.  Returns s.
  */
 int orc_kmeans_fixed_shift(int64_t N, float maxabs)
 {
@@ -958,6 +959,7 @@ int orc_kmeans_fixed_shift(int64_t N, float maxabs)
     double m = maxabs > 0 ? maxabs : 1.0f;
     frexp(m, &eM); /* m < 2^eM */
     int s = 62 - eN - eM;
+    if (30 - eM < s) s = 30 - eM;
     if (s > 60) s = 60;
     return s;
 }

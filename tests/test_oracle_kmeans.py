@@ -95,15 +95,19 @@ def test_fixed_point_sums(pa, orc):
     np.add.at(exact, a, x.astype(np.float64))
     assert np.max(np.abs(sums * 2.0 ** -s - exact)) <= len(x) * 2.0 ** (-s - 1)
     assert counts.tolist() == [12500] * 4
-    # the largest possible sum stays inside int64
-    assert len(x) * maxabs * 2.0 ** s < 2.0 ** 62
+    # every rounded value fits in 31 bits, the largest possible sum in int64
+    assert maxabs * 2.0 ** s <= 2.0 ** 30 and len(x) * maxabs * 2.0 ** s < 2.0 ** 62
 
 
 @pytest.mark.parametrize("n,m", [(1, 0.5), (1000, 1.0), (1 << 24, 14.7), ((1 << 24) + 1, 1e-3), (7, 3e5)])
 def test_shift_formula_shared(pa, orc, n, m):
     assert pa.kmeans_fixed_shift(n, m) == orc.kmeans_shift(n, m)
     s = orc.kmeans_shift(n, m)
-    assert n * m * 2.0 ** s < 2.0 ** 62
+    assert n * m * 2.0 ** s < 2.0 ** 62 and m * 2.0 ** s <= 2.0 ** 30
+    # the largest shift with both properties for a magnitude bound that is a
+    # power of two (m < 2^e): one more bit breaks one of them
+    e = int(np.frexp(np.float64(m))[1])
+    assert n * 2.0 ** (e + s + 1) >= 2.0 ** 62 or 2.0 ** (e + s + 1) > 2.0 ** 30 or s == 60
 
 
 def test_update_keeps_empty_cluster(orc):
