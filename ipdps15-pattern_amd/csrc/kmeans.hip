@@ -306,6 +306,45 @@ __device__ __forceinline__ float exact_dist(const float *__restrict__ xp, const 
     return sqrtf(dist);
 }
 
+// The same distance with the point row held by the wavefront's two halves
+// (lane l and l+32 hold the point's dimensions 16s + 8*half + e in xo):
+// v_permlane32_swap gives both lanes dimensions 16s + e in its first result
+// and 16s + 8 + e in its second, so the row is read in order from registers
+// instead of memory.  Called by every lane (the swap is cross-lane); the
+// row is only formed where D == 16*KS.
+template <int KS>
+__device__ __forceinline__ void point_row(const float (&xo)[8 * KS], float (&row)[16 * KS])
+{
+#pragma unroll
+    for (int s = 0; s < KS; s++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const unsigned v = __float_as_uint(xo[8 * s + e]);
+            const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+            row[16 * s + e] = __uint_as_float(r[0]);
+            row[16 * s + 8 + e] = __uint_as_float(r[1]);
+        }
+}
+
+template <int KS>
+__device__ __forceinline__ float exact_dist_row(const float (&row)[16 * KS], const float *__restrict__ ck)
+{
+    float dist = 0;
+#pragma unroll
+    for (int d = 0; d < 16 * KS; d += 4) {
+        const float4 cv = *(const float4 *)(ck + d);
+        float t = cv.x - row[d];
+        dist = dist + t * t;
+        t = cv.y - row[d + 1];
+        dist = dist + t * t;
+        t = cv.z - row[d + 2];
+        dist = dist + t * t;
+        t = cv.w - row[d + 3];
+        dist = dist + t * t;
+    }
+    return sqrtf(dist);
+}
+
 // Approximate value v with its position idx (0..127) in the low 7 bits of
 // the significand: a key that orders like v (up to 127 ulps) and carries its
 // index through a plain float minimum.  |key - v| <= 2^-16 |v| + 2^-126.
@@ -572,6 +611,11 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         // lanes whose candidates may lie beyond the keys they kept: every
         // value under the threshold is then visited again (full)
         const bool full = multi && (!gidx || !(k3 > thr));
+        // the whole point row in registers (full K-steps) where the form has
+        // the registers to spare (the others would spill: they reload it)
+        constexpr bool ROW = !TAIL && !GC && KS == 2 && KB == 1;
+        float row[16 * KS];
+        if (ROW) point_row<KS>(xo, row);
 
         float best_d = kInitMin;
         int best_k = 0x7fffffff;
@@ -579,7 +623,8 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         auto rescore = [&](int ch, int idx) {
             const int k = centre_of(ch, idx);
             if (k < a.K) {
-                const float dk = exact_dist(xp, a.c + (int64_t)k * D, D, vec_c && vec_x);
+                const float *ck = a.c + (int64_t)k * D;
+                const float dk = ROW && vec_c ? exact_dist_row<KS>(row, ck) : exact_dist(xp, ck, D, vec_c && vec_x);
                 if (dk < best_d || (dk == best_d && k < best_k)) {
                     best_d = dk;
                     best_k = k;
