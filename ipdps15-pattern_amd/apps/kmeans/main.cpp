@@ -3,7 +3,11 @@
 // RegularInit, set_map_idx(1), set_reduce_idx(1), RegularStart) followed by
 // the Lloyd update km.cu leaves out, on a Gaussian mixture (default
 // N = 2^20, D = 32, K = 256, 20 iterations).  The first pass's keys are
-// checked against the host map1 functor on a sample of points.
+// checked against the host map1 functor on a sample of points.  A sixth
+// argument gap > 0 stores the points gap floats apart (unit i at
+// i * (D + gap) floats), so the Offset array describes units that are not
+// packed back to back and the runtime gathers them.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -18,16 +22,24 @@ int main(int argc, char *argv[])
     const int D = argc > 2 ? atoi(argv[2]) : 32;
     const int K = argc > 3 ? atoi(argv[3]) : 256;
     const int iters = argc > 4 ? atoi(argv[4]) : 20;
+    const int gap = argc > 5 ? atoi(argv[5]) : 0;
     RuntimeInit(argc, argv);
 
-    std::vector<float> points((size_t)N * D);
-    PAT_API_CALL(pat_gen_gmm(N, D, 256, 1.0, -10.0, 10.0, 2024, points.data()));
+    std::vector<float> packed((size_t)N * D);
+    PAT_API_CALL(pat_gen_gmm(N, D, 256, 1.0, -10.0, 10.0, 2024, packed.data()));
+    const size_t stride = (size_t)D + (gap > 0 ? gap : 0);  // floats per unit slot
+    std::vector<float> points(gap > 0 ? (size_t)N * stride : 0, -1.0e30f);
+    if (gap > 0)
+        for (long i = 0; i < N; i++) std::copy_n(&packed[(size_t)i * D], D, &points[(size_t)i * stride]);
+    else
+        points.swap(packed);
     std::vector<Offset> offsets(N);
     for (long i = 0; i < N; i++) {
-        offsets[i].offset = (size_t)i * D * sizeof(float);
+        offsets[i].offset = (size_t)i * stride * sizeof(float);
         offsets[i].size = D * sizeof(float);
     }
-    std::vector<float> centers(points.begin(), points.begin() + (size_t)K * D);  // first K points
+    std::vector<float> centers(K * (size_t)D);  // first K points
+    for (int k = 0; k < K; k++) std::copy_n(&points[(size_t)k * stride], D, &centers[(size_t)k * D]);
 
     RegularRuntime rr(1, points.data(), points.size() * sizeof(float), offsets.data(), N, centers.data(),
                       K * D * (int)sizeof(float));
@@ -75,7 +87,8 @@ int main(int argc, char *argv[])
             return 1;
         }
     }
-    printf("kmeans: N=%ld D=%d K=%d, %d iterations, sum of min distances %.6e\n", N, D, K, iters, sse);
+    printf("kmeans: N=%ld D=%d K=%d, %d iterations (units %s), sum of min distances %.6e\n", N, D, K, iters,
+           gap > 0 ? "gathered" : "packed", sse);
     RuntimeFinalize();
     return 0;
 }

@@ -38,6 +38,17 @@ def test_kmeans_app(gpu):
     assert "0 of 1031 sampled keys differ" in out
 
 
+def test_kmeans_app_units_not_packed(gpu):
+    """Units placed 3 floats apart by their Offset array (the runtime
+    gathers them before the engine runs): the same keys as the host functor,
+    and the same result as the packed run."""
+    packed = run("kmeans", 20000, 16, 32, 3)
+    gathered = run("kmeans", 20000, 16, 32, 3, 3)
+    assert "0 of 207 sampled keys differ" in gathered and "units gathered" in gathered
+    tail = lambda s: s.strip().splitlines()[-1].split("sum of min distances")[1]
+    assert tail(packed) == tail(gathered)
+
+
 def test_moldyn_app(gpu):
     out = run("moldyn", 8, 20)
     assert "first pass" in out and "kinetic energy" in out and "velocity statistics" in out

@@ -119,10 +119,20 @@ def test_lloyd_20_iterations_midsize(pa, orc, gpu):
     s, _ = pa.kmeans_shifts(len(x), 32, maxabs)
     sums, counts, _ = orc.kmeans_accumulate(x, a20, 256, s)
     assert np.array_equal(res[1][0], orc.kmeans_update(sums, counts, s, c19))
+    # FP64 statement: the centres against the FP64 means of the same final
+    # clusters (a deterministic bound: fixed-point sums plus one FP32
+    # rounding), and, printed, against an FP64 Lloyd run from the same start
+    cnt = np.bincount(a20, minlength=256)
+    means = np.zeros((256, 32))
+    np.add.at(means, a20, x.astype(np.float64))
+    full = cnt > 0
+    means[full] /= cnt[full, None]
+    merr = np.max(np.abs(res[1][0][full] - means[full])) / np.max(np.abs(means[full]))
+    assert merr <= 1e-6
     c64, _ = orc.kmeans_lloyd_f64(x, 256, 20)
     err = np.max(np.abs(res[1][0] - c64)) / np.max(np.abs(c64))
     print(f"kmeans 200k x32 K256 x20: rescored {resc} point-passes ({resc / (20 * len(x)):.2%}); "
-          f"centre rel err vs FP64 Lloyd {err:.2e}")
+          f"centres vs FP64 means of their clusters {merr:.2e}, vs FP64 Lloyd {err:.2e}")
 
 
 def test_kmeans_workload_full_size(pa, orc, gpu):
