@@ -140,6 +140,8 @@ struct pat_regular {
                         "pat_regular_start: damaged final table");
             const uint32_t *r = fin.data() + nb + 1 + (s - 1u);
             const int ks = (int)(r[0] >> 16), vs = (int)(r[0] & 0xffffu);
+            PAT_REQUIRE((int64_t)s - 1 + 1 + (ks + 3) / 4 + (vs + 3) / 4 <= pw, PAT_ERR_INVALID,
+                        "pat_regular_start: damaged final table record");
             const char *kp = (const char *)(r + 1);
             const char *vp = (const char *)(r + 1 + (ks + 3) / 4);
             out.emplace_back(std::string(kp, (size_t)ks), std::string(vp, (size_t)vs));
@@ -157,13 +159,14 @@ int pat_regular_create(const void *input, int64_t input_bytes, int64_t num_units
 {
     PAT_REQUIRE(out && input && input_bytes > 0 && num_units > 0 && unit_bytes > 0, PAT_ERR_INVALID,
                 "pat_regular_create: bad arguments");
+    PAT_REQUIRE(unit_bytes <= input_bytes, PAT_ERR_INVALID, "pat_regular_create: units larger than the input");
     if (offsets) {
         for (int64_t i = 0; i < num_units; i++)
-            PAT_REQUIRE(offsets[i] >= 0 && offsets[i] + unit_bytes <= input_bytes, PAT_ERR_INVALID,
+            PAT_REQUIRE(offsets[i] >= 0 && offsets[i] <= input_bytes - unit_bytes, PAT_ERR_INVALID,
                         "pat_regular_create: unit %lld at %lld + %lld bytes lies outside the %lld-byte input",
                         (long long)i, (long long)offsets[i], (long long)unit_bytes, (long long)input_bytes);
     } else {
-        PAT_REQUIRE(num_units * unit_bytes <= input_bytes, PAT_ERR_INVALID,
+        PAT_REQUIRE(num_units <= input_bytes / unit_bytes, PAT_ERR_INVALID,
                     "pat_regular_create: %lld units of %lld bytes exceed the %lld-byte input", (long long)num_units,
                     (long long)unit_bytes, (long long)input_bytes);
     }

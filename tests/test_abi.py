@@ -4,6 +4,8 @@ import glob
 import os
 import re
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -40,6 +42,25 @@ def test_error_path_without_gpu(pa):
     assert pa.lib().pat_stencil_create(1, interior, 1, 4, None, 0, ctypes.byref(h)) == -5  # 2-D or 3-D only
     assert pa.lib().pat_stencil_create(3, interior, 17, 4, None, 0, ctypes.byref(h)) == -5  # halo width 1..16
     assert pa.lib().pat_stencil_create(3, interior, 1, 2, None, 0, ctypes.byref(h)) == -5  # float or double
+
+
+def test_regular_create_rejects_units_outside_the_input(pa):
+    """The keyed generalized reduction checks every unit against the input
+    before it allocates anything, without overflowing on extreme offsets."""
+    import ctypes
+    h = ctypes.c_void_p()
+    buf = np.zeros(64, np.uint8)
+    f = pa.lib().pat_regular_create
+    off = (ctypes.c_int64 * 2)(0, 2 ** 63 - 4)
+    assert f(buf.ctypes.data, 64, 2, 8, off, ctypes.byref(h)) == -1  # second unit far past the end
+    assert b"outside" in pa.lib().pat_last_error()
+    off = (ctypes.c_int64 * 2)(0, 57)
+    assert f(buf.ctypes.data, 64, 2, 8, off, ctypes.byref(h)) == -1  # one byte past the end
+    off = (ctypes.c_int64 * 2)(-8, 0)
+    assert f(buf.ctypes.data, 64, 2, 8, off, ctypes.byref(h)) == -1
+    assert f(buf.ctypes.data, 64, 2 ** 61, 8, None, ctypes.byref(h)) == -1  # count x size would overflow
+    assert f(buf.ctypes.data, 64, 9, 8, None, ctypes.byref(h)) == -1  # 72 bytes in a 64-byte input
+    assert f(buf.ctypes.data, 64, 1, 65, None, ctypes.byref(h)) == -1  # one unit larger than the input
 
 
 def test_neighbour_build_entry_total_guard(pa):
