@@ -431,6 +431,15 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
             int jj[kNbSlots];
 #pragma unroll
             for (int r = 0; r < kNbSlots; r++) {
+                jj[r] = -1;
+                pj[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            // slots past the last candidate stay empty and are not visited
+            // (uniform over the wavefront)
+            const int slots = min(kNbSlots, (total - base + kWave - 1) / kWave);
+#pragma unroll
+            for (int r = 0; r < kNbSlots; r++) {
+                if (r < slots) {
                 const int f = base + r * kWave + lane;
                 // largest k <= 26 with excl_k <= f (excl is nondecreasing,
                 // so bin k is non-empty and holds f when f < total)
@@ -441,11 +450,10 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                     k = (k + step <= 26 && e <= f) ? k + step : k;
                 }
                 const int ex = __shfl(excl, k, kWave), s0 = __shfl(ns, k, kWave);
-                jj[r] = -1;
-                pj[r] = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (f < total) {
                     jj[r] = catoms[s0 + f - ex];
                     pj[r] = pos[jj[r]];
+                }
                 }
             }
             // When one pass holds every candidate (the bin's own atoms among
@@ -490,14 +498,16 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                     int found = 0;
 #pragma unroll
                     for (int r = 0; r < kNbSlots; r++) {
-                        const bool hit = jj[r] >= 0 && jj[r] != i &&
-                                         (NEAR ? nb_within_near(pi, pj[r], g) : nb_within(pi, pj[r], g));
-                        const unsigned long long m = __ballot(hit);
-                        if (FILL && hit) {
-                            const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
-                            if (o < wend) tmp[o] = jj[r];
+                        if (r < slots) {
+                            const bool hit = jj[r] >= 0 && jj[r] != i &&
+                                             (NEAR ? nb_within_near(pi, pj[r], g) : nb_within(pi, pj[r], g));
+                            const unsigned long long m = __ballot(hit);
+                            if (FILL && hit) {
+                                const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
+                                if (o < wend) tmp[o] = jj[r];
+                            }
+                            found += __popcll(m);
                         }
-                        found += __popcll(m);
                     }
                     if (lane == a - a0) acc += found;
                 }
