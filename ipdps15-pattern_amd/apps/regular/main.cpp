@@ -183,6 +183,23 @@ int main(int argc, char *argv[])
             bad += !ok;
         }
     }
+    {
+        // map 4: 6-byte keys, 3-byte values added bytewise modulo 256, so
+        // the device object must equal the host one byte for byte
+        const long n = 200000;
+        std::vector<unsigned int> v(n);
+        for (long i = 0; i < n; i++) v[i] = (unsigned int)(i * 2654435761u);
+        std::vector<Offset> off(n);
+        for (long i = 0; i < n; i++) off[i] = Offset{(size_t)i * 4, 4};
+        CO host = host_run(4, v.data(), off, nullptr);
+        for (int lds : {0, 3}) {
+            auto dev = device_run(4, v.data(), v.size() * 4, off, nullptr, 0, lds);
+            const bool ok = dev == host.entries() && dev.size() == 231;
+            printf("regular: map 4 odd-sized records (6-byte keys, 3-byte values%s): %zu keys, bytes %s\n",
+                   lds ? ", LDS objects merged every 3 keys" : "", dev.size(), ok ? "match" : "DIFFER");
+            bad += !ok;
+        }
+    }
     RuntimeFinalize();
     return bad ? 1 : 0;
 }

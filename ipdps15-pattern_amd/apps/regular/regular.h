@@ -20,7 +20,11 @@
 //                      reduce2 adds;
 //   3  map3 / reduce3  a many-key histogram: unit = an unsigned value v, key
 //                      v mod 5003, value {long long count 1, double v/1000};
-//                      reduce3 adds both.
+//                      reduce3 adds both;
+//   4  map4 / reduce4  records of odd sizes: unit = an unsigned value v, key
+//                      the 6 bytes {v % 7, v / 7 % 11, v / 77 % 3} (231
+//                      keys), value the 3 bytes {1, v & 255, v >> 8 & 255};
+//                      reduce4 adds them byte by byte modulo 256.
 // The reference spreads one logical key over ANY_KEY's 41 buckets on the
 // device and uses key 0 on the host; a caller adds the buckets.
 #ifndef REDUCTION
@@ -154,8 +158,30 @@ DEVICE void reduce3(void *value1, unsigned short value1_size, void *value2, unsi
     ((HistValue *)value1)->sum += ((HistValue *)value2)->sum;
 }
 
-CONSTANT map_fp map_function_table[] = {map, map1, map2, map3};
-CONSTANT reduce_fp reduce_function_table[] = {reduce, reduce1, reduce2, reduce3};
+struct SmallKey {
+    unsigned short a, b, c;
+};
+
+DEVICE bool map4(OBJECT *object, void *global_data, unsigned long long index, void *parameter, int device_type,
+                 reduce_fp reduce_ptr)
+{
+    (void)parameter, (void)device_type;
+    const unsigned int v = *(const unsigned int *)((char *)global_data + index);
+    SmallKey key = {(unsigned short)(v % 7), (unsigned short)(v / 7 % 11), (unsigned short)(v / 77 % 3)};
+    unsigned char value[3] = {1, (unsigned char)(v & 255), (unsigned char)(v >> 8 & 255)};
+    return object->insert(&key, sizeof(key), value, sizeof(value), reduce_ptr);
+}
+
+DEVICE void reduce4(void *value1, unsigned short value1_size, void *value2, unsigned short value2_size)
+{
+    (void)value2_size;
+    unsigned char *a = (unsigned char *)value1;
+    const unsigned char *b = (const unsigned char *)value2;
+    for (int i = 0; i < value1_size; i++) a[i] = (unsigned char)(a[i] + b[i]);
+}
+
+CONSTANT map_fp map_function_table[] = {map, map1, map2, map3, map4};
+CONSTANT reduce_fp reduce_function_table[] = {reduce, reduce1, reduce2, reduce3, reduce4};
 
 }  // namespace FFCPU / FFGPU
 #endif

@@ -109,6 +109,32 @@ def test_many_key_histogram_offsets_and_merges(pa, gpu, tab, lds_keys):
         assert merges > waves and reruns > 0
 
 
+@pytest.mark.parametrize("lds_keys", [0, 3])
+def test_odd_sized_keys_and_values(pa, gpu, tab, lds_keys):
+    """6-byte keys and 3-byte values (the objects' byte-wise copy and
+    compare, keys padded to whole words): every key's three bytes equal the
+    byte-wise sums modulo 256 of its units, also with merges forced."""
+    n = 300000
+    v = np.random.default_rng(11).integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    r = pa.Regular(v, n, 4, table=tab)
+    r.set_capacity(512, 16, lds_keys)
+    out = r.run(4, reduce_idx=4)
+    assert r.run(4, reduce_idx=4) == out
+    r.close()
+    k = np.stack([v % 7, v // 7 % 11, v // 77 % 3], 1).astype(np.uint16)
+    flat = k[:, 0].astype(np.int64) * 33 + k[:, 1].astype(np.int64) * 3 + k[:, 2]
+    want = {}
+    for col, vals in ((0, np.ones(n, np.int64)), (1, (v & 255).astype(np.int64)), (2, (v >> 8 & 255).astype(np.int64))):
+        sums = np.bincount(flat, weights=vals, minlength=231).astype(np.int64) % 256
+        for f in np.unique(flat):
+            want.setdefault(int(f), [0, 0, 0])[col] = int(sums[f])
+    assert len(out) == len(want) == 231
+    for kb, vb in out.items():
+        a, b, c = np.frombuffer(kb, np.uint16)
+        assert len(kb) == 6 and len(vb) == 3
+        assert list(vb) == want[int(a) * 33 + int(b) * 3 + int(c)]
+
+
 def test_capacity_errors(pa, gpu, tab):
     vals = np.arange(100000, dtype=np.uint32)
     r = pa.Regular(vals, len(vals), 4, table=tab)
@@ -116,5 +142,5 @@ def test_capacity_errors(pa, gpu, tab):
     with pytest.raises(pa.PatternError, match="keys"):
         r.run(3)
     with pytest.raises(pa.PatternError, match="not in the table"):
-        pa.call("pat_regular_set_map_idx", r.h, 4)
+        pa.call("pat_regular_set_map_idx", r.h, 5)
     r.close()
