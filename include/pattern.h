@@ -90,7 +90,7 @@ int pat_gen_fcc(int64_t ncx, int64_t ncy, int64_t ncz, double rho, double jitter
  * sum v^2 / (3(N-1)) = t (miniMD create_velocity, miniMD/setup.cpp:150-190). */
 int pat_gen_velocities(int64_t n, double t, uint64_t seed, float *vel);
 /* Half list of all pairs i<j whose minimum-image distance is below cut
- * (cell-list build; the moldyn driver's RebuildNeighbor,
+ * (cell-list build; the moldyn program's RebuildNeighbor,
  * kmeans/moldyn.cu:552-594, is the O(N^2) form).  Pairs come out in (i, j)
  * order, or shuffled with shuffle_seed when shuffle != 0.  Call with
  * edges == NULL to get the count in *num_edges; otherwise *num_edges is the

@@ -3,7 +3,7 @@
 // centres: map1 emits key = nearest centre (FP32 difference form in
 // dimension order, sqrt, strict <, lowest index wins) with value
 // {x_0..x_{D-1}, 1, min_dist}; reduce1 adds values.  Table index 1 is the
-// GPU engine's kmeans pass.  Host versions, used by the driver's check.
+// GPU engine's kmeans pass.  Host versions, used by the program's host check.
 #ifndef REDUCTION
 #define REDUCTION
 

@@ -8,8 +8,8 @@
 //
 // The reference's RegularRuntime splits units over the CPU threads and GPUs
 // by a speed profile, feeds the GPU from mapped host memory and merges the
-// device objects on the host (regular_runtime.cu:98-286, where the merge is
-// never called, Appendix D3 of the survey); with one GPU holding the input,
+// device objects on the host (regular_runtime.cu:98-286; its merge_device is
+// never called, regular_runtime.cu:285); with one GPU holding the input,
 // one start is one device pass and the object comes back whole.
 #include <algorithm>
 #include <cstring>

@@ -843,7 +843,7 @@ double orc_md_run_f64(int64_t N, int64_t E, const int32_t *edges, double *pos, d
 /*
  * Brute-force O(N^2) half list: every pair i<j with minimum-image distance
  * squared below cut2, emitted in (i, j) lexicographic order.  The moldyn
- * driver's RebuildNeighbor (kmeans/moldyn.cu:552-594) is this O(N^2) loop.
+ * program's RebuildNeighbor (kmeans/moldyn.cu:552-594) is this O(N^2) loop.
  * Returns the pair count; writes at most cap pairs when edges is non-NULL.
  */
 int64_t orc_pairs_brute(int64_t N, const float *pos, float L, int pbc, float cut2, int32_t *edges, int64_t cap)
