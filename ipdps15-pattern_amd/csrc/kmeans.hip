@@ -193,14 +193,17 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
     extern __shared__ unsigned long long smem[];
     unsigned long long *acc, *cnt, *dsum;
     lds_object(a, smem, acc, cnt, dsum);
+    // centres in LDS: rows of D4 = D rounded up to 4 floats, zero-padded
     float *cs = (float *)(smem + (a.use_lds_acc ? object_words(a.K, a.D) : 0));
-    const float *C = centres_in_lds ? cs : a.c;
+    const int D = a.D, K = a.K, D4 = (D + 3) & ~3;
     if (a.use_lds_acc) zero_lds_object(a, acc, cnt, dsum);
     if (centres_in_lds)
-        for (int i = threadIdx.x; i < a.K * a.D; i += blockDim.x) cs[i] = a.c[i];
+        for (int i = threadIdx.x; i < K * D4; i += blockDim.x) {
+            const int k = i / D4, d = i - k * D4;
+            cs[i] = d < D ? a.c[(int64_t)k * D + d] : 0.0f;
+        }
     __syncthreads();
 
-    const int D = a.D, K = a.K;
     const int lane = threadIdx.x % kWave;
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -212,14 +215,31 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
         for (int d = 0; d < DMAX; d++) x[d] = (live && d < D) ? a.x[p * D + d] : 0.0f;
         unsigned int key = 0;
         float min_dist = kInitMin;
-        int i = 0;
-        for (; i + 4 <= K; i += 4) {
-            // four independent chains for ILP; compared in index order
-            float dv[4];
+        // dist = sum over d < D of (c_d - x_d)^2 in dimension order.  From
+        // the LDS rows: groups of four dimensions (one 16-byte read, the same
+        // address in every lane); past D both the row and x hold zeros, and
+        // adding (0 - 0)^2 = +0 to the non-negative sum changes no bit.
+        // From global memory: one guarded dimension at a time.
+        auto dist_of = [&](int k) {
+            float dist = 0;
+            if (centres_in_lds) {
+                const float *ci = cs + k * D4;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float *ci = C + (int64_t)(i + q) * D;
-                float dist = 0;
+                for (int g = 0; g < DMAX / 4; g++) {
+                    if (4 * g < D) {
+                        const float4 c4 = *(const float4 *)(ci + 4 * g);
+                        float t = c4.x - x[4 * g];
+                        dist = dist + t * t;
+                        t = c4.y - x[4 * g + 1];
+                        dist = dist + t * t;
+                        t = c4.z - x[4 * g + 2];
+                        dist = dist + t * t;
+                        t = c4.w - x[4 * g + 3];
+                        dist = dist + t * t;
+                    }
+                }
+            } else {
+                const float *ci = a.c + (int64_t)k * D;
 #pragma unroll
                 for (int d = 0; d < DMAX; d++) {
                     if (d < D) {
@@ -227,8 +247,15 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
                         dist = dist + t * t;
                     }
                 }
-                dv[q] = sqrtf(dist);
             }
+            return sqrtf(dist);
+        };
+        int i = 0;
+        for (; i + 4 <= K; i += 4) {
+            // four independent chains for ILP; compared in index order
+            float dv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) dv[q] = dist_of(i + q);
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 if (dv[q] < min_dist) {
@@ -237,16 +264,7 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
                 }
         }
         for (; i < K; i++) {
-            const float *ci = C + (int64_t)i * D;
-            float dist = 0;
-#pragma unroll
-            for (int d = 0; d < DMAX; d++) {
-                if (d < D) {
-                    const float t = ci[d] - x[d];
-                    dist = dist + t * t;
-                }
-            }
-            dist = sqrtf(dist);
+            const float dist = dist_of(i);
             if (dist < min_dist) {
                 min_dist = dist;
                 key = i;
@@ -257,10 +275,16 @@ __global__ __launch_bounds__(BS) void km_plain(KmArgs a, int centres_in_lds)
             if (a.mindist) a.mindist[p] = min_dist;
         }
         const int np = (int)min<int64_t>(kWave, a.N - base);
-        if (!a.defer_acc)
-            accumulate_points(
-                a, base, np, [&](int j) { return (int)__shfl((int)key, j, kWave); },
-                [&](int j) { return __shfl(min_dist, j, kWave); }, acc, cnt, dsum);
+        if (!a.defer_acc) {
+            auto key_of = [&](int j) { return (int)__shfl((int)key, j, kWave); };
+            auto dist_at = [&](int j) { return __shfl(min_dist, j, kWave); };
+            // the LDS object addressed from the shared array itself, so that
+            // its adds compile to LDS atomics
+            if (a.use_lds_acc)
+                accumulate_points(a, base, np, key_of, dist_at, smem, smem + K * a.rs, smem + K * a.rs + K);
+            else
+                accumulate_points(a, base, np, key_of, dist_at, a.gsum, a.gcnt, a.gdsum);
+        }
     }
     if (a.use_lds_acc) {
         __syncthreads();
@@ -1113,7 +1137,7 @@ int kmeans_finalize(const KmDev &d, int update, hipStream_t st)
 int kmeans_assign_plain(const KmDev &d, hipStream_t st)
 {
     KmArgs a = make_args(d);
-    const int64_t cbytes = 4LL * d.K * d.D;
+    const int64_t cbytes = 4LL * d.K * ((d.D + 3) & ~3);  // rows padded to 4 floats
     int64_t lds = a.use_lds_acc ? object_bytes(d.K, d.D) : 0;
     int in_lds = lds + cbytes <= g_lds_limit;
     if (in_lds) lds += cbytes;
