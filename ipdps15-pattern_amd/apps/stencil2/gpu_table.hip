@@ -6,6 +6,6 @@
 
 #include "gpu/stencil_table.hpp"
 
-static const int kUnit[] = {4};
-static const int kSameAs[] = {-1};
+static const int kUnit[] = {4, 4};
+static const int kSameAs[] = {-1, -1};
 PAT_STENCIL_TABLE(FGPU::stencil_function_table, kUnit, kSameAs)

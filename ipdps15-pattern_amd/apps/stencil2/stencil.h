@@ -2,10 +2,12 @@
 // 93-163, its stencil_function_table at :251), in the 4-argument stencil
 // shape (kmeans/stencil.h:32), compiled for the host (CPU_KERNEL, FCPU) or
 // the device (GPU_KERNEL, FGPU; see gpu_table.hip):
-//   0  star  FP32: the sum of the cells 1..W-1 away along each axis, taken
-//            x+, x-, y+, y-, z+, z- (each side by increasing distance) into a
-//            sum started at 0, divided by 36.
-// The halo is W cells wide; the functor reads W-1 of them.
+//   0  star     FP32: the sum of the cells 1..W-1 away along each axis,
+//               taken x+, x-, y+, y-, z+, z- (each side by increasing
+//               distance) into a sum started at 0, divided by 36.  The
+//               halo is W cells wide; the functor reads W-1 of them.
+//   1  cross2d  FP32 on a 2-D grid with a halo of 1 (not in the reference:
+//               the runtime's 2-D path): ((x+ + x-) + (y+ + y-)) * 0.25.
 #ifndef STENCIL
 #define STENCIL
 #define W 3
@@ -38,7 +40,15 @@ DEVICE void star(void *input, void *output, int *offset, int *size)
     GET_FLOAT3(output, i, j, k) = t;
 }
 
-CONSTANT stencil_fp stencil_function_table[] = {star};
+DEVICE void cross2d(void *input, void *output, int *offset, int *size)
+{
+    const int k = offset[0], j = offset[1];
+    const float a = GET_FLOAT3(input, 0, j, k + 1) + GET_FLOAT3(input, 0, j, k - 1);
+    const float b = GET_FLOAT3(input, 0, j + 1, k) + GET_FLOAT3(input, 0, j - 1, k);
+    GET_FLOAT3(output, 0, j, k) = (a + b) * 0.25f;
+}
+
+CONSTANT stencil_fp stencil_function_table[] = {star, cross2d};
 
 }  // namespace FCPU / FGPU
 #endif

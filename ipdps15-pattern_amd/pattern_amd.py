@@ -431,13 +431,16 @@ class Stencil:
     application table (app_table(...)) whose functor idx runs."""
 
     def __init__(self, nx, ny, nz, alpha=0.1, width=1, dtype=np.float32, param=None, table=None, idx=0):
+        """nz None: a 2-D grid (ny, nx)."""
         self.dtype = np.dtype(dtype)
-        self.shape = (nz + 2 * width, ny + 2 * width, nx + 2 * width)
+        ndims = 2 if nz is None else 3
+        self.shape = (ny + 2 * width, nx + 2 * width) if nz is None else (nz + 2 * width, ny + 2 * width,
+                                                                          nx + 2 * width)
         self._param = np.array([alpha], dtype=np.float32) if param is None else np.ascontiguousarray(param)
         h = c_vp()
-        interior = (c_i64 * 3)(nx, ny, nz)
-        call("pat_stencil_create", 3, interior, width, self.dtype.itemsize, ptr(self._param), self._param.nbytes,
-             ctypes.byref(h))
+        interior = (c_i64 * 3)(nx, ny, 1 if nz is None else nz)
+        call("pat_stencil_create", ndims, interior, width, self.dtype.itemsize, ptr(self._param),
+             self._param.nbytes, ctypes.byref(h))
         self.h = h
         if table is not None:
             call("pat_stencil_set_table", self.h, table)

@@ -68,6 +68,30 @@ def test_star_w3_vs_oracle(pa, orc, gpu, shape, steps):
     st.close()
 
 
+@pytest.mark.parametrize("shape,steps", [((37, 23), 1), ((130, 61), 4), ((1, 1), 2)])
+def test_cross2d_on_a_2d_grid(pa, gpu, shape, steps):
+    """The runtime's 2-D path: a 2-D table functor on (ny + 2) x (nx + 2)
+    grids (halo fixed at zero), several steps, against the same FP32
+    operations in NumPy (no contraction: each sum and the product rounded as
+    written)."""
+    nx, ny = shape
+    g = np.zeros((ny + 2, nx + 2), np.float32)
+    g[1:-1, 1:-1] = np.random.default_rng(9).standard_normal((ny, nx)).astype(np.float32)
+    ref = g.copy()
+    for _ in range(steps):
+        nxt = np.zeros_like(ref)
+        a = ref[1:-1, 2:] + ref[1:-1, :-2]
+        b = ref[2:, 1:-1] + ref[:-2, 1:-1]
+        nxt[1:-1, 1:-1] = (a + b) * np.float32(0.25)
+        ref = nxt
+    st = pa.Stencil(nx, ny, None, width=1, param=np.zeros(0, np.float32), table=pa.app_table("stencil2"), idx=1)
+    st.upload(g)
+    st.run(steps)
+    got = st.download()
+    st.close()
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+
+
 def test_table_errors(pa, gpu):
     # an index outside the table, a functor of another element type, and a
     # width the tuned engine cannot run without a table are refused
@@ -79,7 +103,7 @@ def test_table_errors(pa, gpu):
     call = pa.call
     call("pat_stencil_set_table", st.h, tab)
     with pytest.raises(pa.PatternError, match="not in the table"):
-        call("pat_stencil_set_idx", st.h, 1)
+        call("pat_stencil_set_idx", st.h, 2)
     st.close()
     st = pa.Stencil(8, 8, 8, width=1, param=np.zeros(1), table=pa.app_table("heat3d"), idx=1)  # FP64 functor
     st.upload(np.zeros(st.shape, np.float32))
