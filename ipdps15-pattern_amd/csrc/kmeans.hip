@@ -46,6 +46,16 @@ namespace {
 using pat::kWave;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+// (lo, hi) rounded to BF16 (nearest even, as a scalar conversion rounds)
+// in one packed conversion: lo in bits 0-15, hi in bits 16-31.
+__device__ __forceinline__ unsigned bf16_pair(float lo, float hi)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr float kInitMin = 65536 * 65;  // map1's initial min_dist (kmeans/regular.h:86)
@@ -372,9 +382,30 @@ __device__ __forceinline__ float exact_dist_row(const float (&row)[16 * KS], con
 // Approximate value v with its position idx (0..127) in the low 7 bits of
 // the significand: a key that orders like v (up to 127 ulps) and carries its
 // index through a plain float minimum.  |key - v| <= 2^-16 |v| + 2^-126.
-__device__ __forceinline__ float pack_key(float v, int idx)
+// keep: ~127 held in a vector register (key_mask), so that with idx in a
+// scalar register the packing is one v_and_or_b32 (an instruction reads at
+// most one scalar operand, and the mask as a literal would be a second).
+__device__ __forceinline__ float pack_key(float v, int idx, int keep)
 {
-    return __int_as_float((__float_as_int(v) & ~127) | idx);
+    return __int_as_float((__float_as_int(v) & keep) | idx);
+}
+
+// ~127 in a vector register, opaque to constant folding.
+__device__ __forceinline__ int key_mask()
+{
+    int m;
+    asm volatile("v_mov_b32 %0, 0xffffff80" : "=v"(m));
+    return m;
+}
+
+// base | i for a wavefront-uniform base, in one scalar register (a scalar
+// ALU operation, opaque to the compiler, which would otherwise fold the OR
+// into the packing as a second vector instruction).
+__device__ __forceinline__ int scalar_or(int base, int i)
+{
+    int r;
+    asm("s_or_b32 %0, %1, %2" : "=s"(r) : "s"(base), "s"(i));
+    return r;
 }
 
 // Stand-in for a value past the last centre row: finite (so its key is not
@@ -463,6 +494,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
     // KB*32) are computed from block 0's rows -- the block index is clamped
     // on the scalar unit -- and skipped by the scans.
     const int kblocks = K_pad / 32;
+    const int keep = key_mask();
     auto chunk_values = [&](int ch, const bf16x8 (&bx)[NS], f32x16 (&acc)[KB]) {
         constexpr int T = KB * NS, P = 3;
         const bf16x8 *ab = (const bf16x8 *)cfr + lane;
@@ -556,18 +588,25 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         // [x, 1, 1, 1, 0...] or [xh, xh, xl, 1, 1, 1, 0...]
         bf16x8 bx[NS];
         float xx = 0.0f;
+        // pairs of elements, one packed conversion each, straight into the
+        // operand's 32-bit words (xh = bf16(x), xl = bf16(x - xh))
 #pragma unroll
         for (int s = 0; s < KS; s++) {
+            u32x4 wh, wl;
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float v = xo[8 * s + e];
-                xx = __builtin_fmaf(v, v, xx);  // a bound input only (used with slack)
-                const __bf16 h = (__bf16)v;
-                bx[s][e] = h;
-                if (NT == 3) {
-                    bx[KS + s][e] = h;
-                    bx[2 * KS + s][e] = (__bf16)(v - (float)h);
-                }
+            for (int e = 0; e < 8; e += 2) {
+                const float v0 = xo[8 * s + e], v1 = xo[8 * s + e + 1];
+                xx = __builtin_fmaf(v0, v0, xx);  // a bound input only (used with slack)
+                xx = __builtin_fmaf(v1, v1, xx);
+                const unsigned h = bf16_pair(v0, v1);
+                wh[e / 2] = h;
+                if (NT == 3)
+                    wl[e / 2] = bf16_pair(v0 - __uint_as_float(h << 16), v1 - __uint_as_float(h & 0xffff0000u));
+            }
+            bx[s] = __builtin_bit_cast(bf16x8, wh);
+            if (NT == 3) {
+                bx[KS + s] = bx[s];
+                bx[2 * KS + s] = __builtin_bit_cast(bf16x8, wl);
             }
         }
 #pragma unroll
@@ -597,7 +636,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                     if (ch * KB + b >= kblocks) break;  // uniform
 #pragma unroll
                     for (int i = 0; i < 16; i++) {
-                        const float key = pack_key(acc[b][i], (ch * KB + b) * 16 + i);
+                        const float key = pack_key(acc[b][i], scalar_or((ch * KB + b) * 16, i), keep);
                         k3 = __builtin_amdgcn_fmed3f(k2, k3, key);
                         k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
                         k1 = __builtin_amdgcn_fmed3f(k1, key, -kFar);  // min (keys are finite)
@@ -611,7 +650,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                 if (ch * KB + b >= kblocks) break;  // uniform
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
-                    const float key = pack_key(acc[b][i], b * 16 + i);
+                    const float key = pack_key(acc[b][i], b * 16 + i, keep);
                     c2 = __builtin_amdgcn_fmed3f(c1, c2, key);
                     c1 = __builtin_amdgcn_fmed3f(c1, key, -kFar);  // min (keys are finite)
                 }
@@ -681,7 +720,7 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
                     if (ch * KB + b >= kblocks) break;  // uniform
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        if (pack_key(acc[b][i], b * 16 + i) <= thr) {
+                        if (pack_key(acc[b][i], b * 16 + i, keep) <= thr) {
                             if (b < 4) w0 |= 1ULL << (b * 16 + i);
                             else w1 |= 1ULL << ((b - 4) * 16 + i);
                         }
