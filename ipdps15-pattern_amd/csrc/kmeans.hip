@@ -612,7 +612,11 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
 #pragma unroll
         for (int e = 0; e < 8; e++) bx[NT * KS][e] = (__bf16)((half == 0 && e < 3) ? 1.0f : 0.0f);
         xx = xx + __shfl_xor(xx, 32, kWave);  // ||x||^2 of point p (both halves)
-        const float xn = sqrtf(xx) * 1.0001f;
+        // an upper bound of ||x|| only: the hardware square root (within an
+        // ulp or two for normal inputs, 0 for flushed subnormals, whose
+        // root lies below 2^-63) with relative and absolute slack, in place
+        // of the correctly rounded sequence
+        const float xn = __builtin_amdgcn_sqrtf(xx) * 1.0001f + 0x1p-40f;
         const float E = bd.c_dot * xn * cmax + bd.c_norm * (cc_max + xx);
         // packing error bound: |values| <= ||c||^2 + 2 ||x|| ||c||
         const float dp = 0x1p-16f * (cc_max + 2.0f * xn * cmax) * 1.01f + 0x1p-126f;
