@@ -1232,7 +1232,7 @@ __global__ __launch_bounds__(256) void lj_sell(const float4 *__restrict__ pos, c
                                                float4 *__restrict__ pos_out, int mode)
 {
     const int nsl = (N + kWave - 1) / kWave;
-    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + threadIdx.x / kWave;
+    const int s = pat::xcd_swizzle(blockIdx.x, gridDim.x) * 4 + pat::wave_in_block();
     if (s >= nsl) return;  // whole wavefronts
     const int lane = threadIdx.x % kWave;
     const int row = slice_row(perm, s * kWave + lane);
@@ -1497,7 +1497,7 @@ __global__ __launch_bounds__(256) void lj_sell_halo(const float4 *__restrict__ p
         for (int r = 0; r < 4; r++)
             if (i0 + 256 * r <= nh) hpos[i0 + 256 * r] = v[r];
     }
-    const int s = b * 4 + threadIdx.x / kWave;
+    const int s = b * 4 + pat::wave_in_block();
     const int lane = threadIdx.x % kWave;
     const int row = s < nsl ? slice_row(perm, s * kWave + lane) : N;
     const int self = min(row, N - 1);

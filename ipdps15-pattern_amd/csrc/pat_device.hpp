@@ -20,6 +20,14 @@ __device__ __forceinline__ int xcd_swizzle(int bid, int nblocks)
     return (bid % kXcds) * per + bid / kXcds;
 }
 
+// This thread's wavefront within its workgroup, as a wavefront-uniform value
+// (a scalar register), so that per-wavefront indices, loop bounds and
+// pointers derived from it stay on the scalar unit.
+__device__ __forceinline__ int wave_in_block()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 // Periodic wrap of one coordinate into [0, L) after a move shorter than L.
 __device__ __forceinline__ float wrap_coord(float x, float L)
 {
