@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                                                 int *__restrict__ overflow)
 {
     constexpr bool FILL = MODE != kNbCount;
-    const int q = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int q = blockIdx.x * 4 + pat::wave_in_block();
     const int lane = threadIdx.x % kWave;
     if (q >= ncell) return;  // whole wavefronts
     const NbSys S = sys[nb_sys_of<true>(sys, nsys, q)];
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
         if (lane >= d) incl += t;
     }
     const int excl = incl - nc;
-    const int total = __shfl(incl, 26, kWave);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 26, kWave));  // uniform
     BinGeom g;
     g.L = S.L;
     g.hL = S.hL;
@@ -481,35 +481,47 @@ __global__ __launch_bounds__(256) void nb_cells(const float4 *__restrict__ pos, 
                 }
                 near = hx - lx < g.hL && hy - ly < g.hL && hz - lz < g.hL;
             }
+            // Atom a's values are broadcast from lane a - a0 (a uniform
+            // index: scalar reads of one lane, no LDS exchange).
+            auto from_lane = [&](auto v, int l) {
+                if constexpr (std::is_same_v<decltype(v), float>)
+                    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+                else
+                    return __builtin_amdgcn_readlane(v, l);
+            };
             auto test_atoms = [&](auto near_c) {
                 constexpr bool NEAR = decltype(near_c)::value;
                 for (int a = a0; a < a0 + na; a++) {
-                    const int i = __shfl(my_i, a - a0, kWave);
-                    const float4 pi = make_float4(__shfl(my_p.x, a - a0, kWave), __shfl(my_p.y, a - a0, kWave),
-                                                  __shfl(my_p.z, a - a0, kWave), 0.f);
+                    const int l = a - a0;
+                    const int i = from_lane(my_i, l);
+                    const float4 pi = make_float4(from_lane(my_p.x, l), from_lane(my_p.y, l), from_lane(my_p.z, l), 0.f);
                     int w = 0, wend = 0;
                     if (MODE == kNbFill) {
-                        w = __shfl(my_w, a - a0, kWave) + __shfl(acc, a - a0, kWave);
-                        wend = __shfl(my_wend, a - a0, kWave);  // the counting pass sized the row; never write past it
+                        w = from_lane(my_w, l) + from_lane(acc, l);
+                        wend = from_lane(my_wend, l);  // the counting pass sized the row; never write past it
                     } else if (MODE == kNbSlotFill) {
-                        w = i * kNbSlot + __shfl(acc, a - a0, kWave);
+                        w = i * kNbSlot + from_lane(acc, l);
                         wend = (i + 1) * kNbSlot;
                     }
                     int found = 0;
 #pragma unroll
                     for (int r = 0; r < kNbSlots; r++) {
                         if (r < slots) {
-                            const bool hit = jj[r] >= 0 && jj[r] != i &&
+                            // all three conditions evaluated (no branch around the distance test)
+                            const bool hit = (jj[r] >= 0) & (jj[r] != i) &
                                              (NEAR ? nb_within_near(pi, pj[r], g) : nb_within(pi, pj[r], g));
                             const unsigned long long m = __ballot(hit);
                             if (FILL && hit) {
-                                const int o = w + found + __popcll(m & ((1ull << lane) - 1ull));
+                                // hits in lanes below this one
+                                const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                                const int o = w + found + below;
                                 if (o < wend) tmp[o] = jj[r];
                             }
                             found += __popcll(m);
                         }
                     }
-                    if (lane == a - a0) acc += found;
+                    if (lane == l) acc += found;
                 }
             };
             if (near)
