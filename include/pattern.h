@@ -126,6 +126,9 @@ int pat_gen_gmm_dev(int64_t n, int d, int ncomp, double sigma, double lo, double
 #define PAT_STENCIL_GENERIC 2 /* one thread per cell calling the functor */
 #define PAT_STENCIL_TEMPORAL2 3 /* two sweeps fused per pass (temporal blocking, 64-wide windows of 8,
                                    16 or 32 rows); an odd last step runs the LDS25D kernel.  Same bits */
+#define PAT_STENCIL_CUBES 4 /* several sweeps per launch on blocks held in LDS; bx = shape: 0, 1, 2 =
+                               16^3 blocks, 2, 3 or 4 sweeps; 3 = 32x16x8, 2 sweeps; 4 = 32x8x8, 2
+                               sweeps.  Steps left over run the LDS25D kernel.  Same bits */
 
 /* nsteps ping-pong heat sweeps over device grids a, b (halo-inclusive,
  * x fastest, both halos already 0).  *result_in_b = 1 when the final field

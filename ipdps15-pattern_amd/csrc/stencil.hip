@@ -300,6 +300,127 @@ __global__ __launch_bounds__(256) void heat_tb2(const float *__restrict__ in, fl
     }
 }
 
+// Several sweeps per launch on blocks of the grid held in LDS, for small
+// grids, whose single sweeps are bound by a few memory latencies per launch
+// (and the fused window by its halo work) rather than by bandwidth.  A
+// workgroup loads its BX x BY x BZ block of cells with a T-cell margin,
+// sweeps that copy T times between two LDS buffers (sweep s updates the
+// block grown by T - s cells: the cells whose remaining dependence stays
+// inside the copy) and writes the block back.  The grid's halo cells keep
+// their values; positions beyond the halo are read only by halo cells,
+// which are copied, not updated.  Each cell's arithmetic is the single
+// sweep's, so the result is bit-identical to T sweeps.
+constexpr int kCubeThreads = 1024;
+
+// Sweep S: the region grown by T - S cells, by (x, y) columns (split into
+// ZS segments of z when the columns are fewer than the threads), each
+// thread marching z with its column's z-1, z, z+1 values in registers (the
+// march unrolled, so the values rotate through fixed registers), so that a
+// cell reads four neighbours from LDS and the x-y bounds are tested once.
+template <int BX, int BY, int BZ, int T, int S>
+__device__ __forceinline__ void cube_sweep(const float *__restrict__ a, float *__restrict__ b, int gx0, int gy0,
+                                           int gz0, int nx, int ny, int nz, float alpha)
+{
+    constexpr int EX = BX + 2 * T, EXY = EX * (BY + 2 * T);
+    constexpr int RX = BX + 2 * (T - S), RY = BY + 2 * (T - S), RZ = BZ + 2 * (T - S);
+    constexpr int NC = RX * RY;
+    constexpr int ZS = NC >= kCubeThreads ? 1 : (kCubeThreads / NC < RZ ? kCubeThreads / NC : RZ);
+    constexpr int LZ = (RZ + ZS - 1) / ZS;
+#pragma unroll
+    for (int i = 0; i < (NC * ZS + kCubeThreads - 1) / kCubeThreads; i++) {
+        const int w = (int)threadIdx.x + i * kCubeThreads;
+        if (w < NC * ZS) {
+            const int col = w % NC, seg = ZS == 1 ? 0 : w / NC;  // one segment: z uniform over the wavefront
+            const int x = S + col % RX, y = S + col / RX;
+            const int z0 = S + seg * LZ, nzs = min(LZ, S + RZ - z0);
+            const int gx = gx0 + x, gy = gy0 + y;
+            const bool inxy = gx >= 1 && gx <= nx && gy >= 1 && gy <= ny;
+            int l = x + EX * y + EXY * z0;
+            float zm = a[l - EXY], zc = a[l];
+#pragma unroll
+            for (int k = 0; k < LZ; k++) {
+                if (k < nzs) {
+                    const float zp = a[l + EXY];
+                    const int gz = gz0 + z0 + k;
+                    // computed everywhere (every read lies in the copy) and
+                    // kept only inside the grid: a select, no branch
+                    float sum = a[l - 1] + a[l + 1];
+                    sum = sum + a[l - EX];
+                    sum = sum + a[l + EX];
+                    sum = sum + zm;
+                    sum = sum + zp;
+                    const float v = zc + alpha * (sum - 6.0f * zc);
+                    b[l] = inxy && gz >= 1 && gz <= nz ? v : zc;
+                    zm = zc;
+                    zc = zp;
+                    l += EXY;
+                }
+            }
+        }
+    }
+}
+
+// Sweeps S .. T: sweep S reads buffer (S - 1) % 2 and writes S % 2.
+template <int BX, int BY, int BZ, int T, int S = 1>
+__device__ __forceinline__ void cube_sweeps(float *c0, float *c1, int gx0, int gy0, int gz0, int nx, int ny, int nz,
+                                            float alpha)
+{
+    if constexpr (S <= T) {
+        cube_sweep<BX, BY, BZ, T, S>((S & 1) ? c0 : c1, (S & 1) ? c1 : c0, gx0, gy0, gz0, nx, ny, nz, alpha);
+        __syncthreads();
+        cube_sweeps<BX, BY, BZ, T, S + 1>(c0, c1, gx0, gy0, gz0, nx, ny, nz, alpha);
+    }
+}
+
+template <int BX, int BY, int BZ, int T>
+__global__ __launch_bounds__(kCubeThreads) void heat_cube(const float *__restrict__ in, float *__restrict__ out, int nx,
+                                                  int ny, int nz, float alpha, int tiles_x, int tiles_y, int tiles_z)
+{
+    constexpr int EX = BX + 2 * T, EY = BY + 2 * T, EZ = BZ + 2 * T, EXY = EX * EY, E3 = EXY * EZ;
+    extern __shared__ float cube[];  // two buffers of E3 cells
+    float *c0 = cube, *c1 = cube + E3;
+
+    const int nblocks = tiles_x * tiles_y * tiles_z;
+    const int t = xcd_swizzle(blockIdx.x, nblocks);
+    const int bx = t % tiles_x;
+    const int by = (t / tiles_x) % tiles_y;
+    const int bz = t / (tiles_x * tiles_y);
+    // array coordinates (halo-inclusive) of the copy's first cell
+    const int gx0 = bx * BX + 1 - T, gy0 = by * BY + 1 - T, gz0 = bz * BZ + 1 - T;
+    const int sx = nx + 2;
+    const int64_t sxy = (int64_t)sx * (ny + 2);
+
+    // the copy, every load issued before the first LDS write; cells beyond
+    // the array read as 0 (never used by a written cell)
+    constexpr int NL = (E3 + kCubeThreads - 1) / kCubeThreads;
+    float v[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        const int c = (int)threadIdx.x + i * kCubeThreads;
+        const int gx = gx0 + c % EX, gy = gy0 + (c / EX) % EY, gz = gz0 + c / EXY;
+        v[i] = 0.0f;
+        if (c < E3 && gx >= 0 && gx <= nx + 1 && gy >= 0 && gy <= ny + 1 && gz >= 0 && gz <= nz + 1)
+            v[i] = in[gx + (int64_t)sx * gy + sxy * gz];
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        const int c = (int)threadIdx.x + i * kCubeThreads;
+        if (c < E3) c0[c] = v[i];
+    }
+    __syncthreads();
+    cube_sweeps<BX, BY, BZ, T>(c0, c1, gx0, gy0, gz0, nx, ny, nz, alpha);
+    const float *r = (T & 1) ? c1 : c0;
+    // the block's cells inside the grid
+#pragma unroll
+    for (int i = 0; i < (BX * BY * BZ + kCubeThreads - 1) / kCubeThreads; i++) {
+        const int c = (int)threadIdx.x + i * kCubeThreads;
+        const int x = T + c % BX, y = T + (c / BX) % BY, z = T + c / (BX * BY);
+        const int gx = gx0 + x, gy = gy0 + y, gz = gz0 + z;
+        if (c < BX * BY * BZ && gx <= nx && gy <= ny && gz <= nz)
+            out[gx + (int64_t)sx * gy + sxy * gz] = r[x + EX * y + EXY * z];
+    }
+}
+
 // One thread per interior cell, all seven loads from global memory (through
 // the caches): the shape of the reference's per-cell kernel, kept as the
 // plain baseline and as an independent check of the tiled kernel.
@@ -350,6 +471,52 @@ HeatKernel heat_kernel_for(int bx, int by, int ry)
     PAT_SHAPE(128, 4, 1)
 #undef PAT_SHAPE
     return nullptr;
+}
+
+// LDS cube shapes (block cells per axis, sweeps per launch); two buffers of
+// (BX+2T)(BY+2T)(BZ+2T) floats each, 41-111 KB.
+struct CubeShape {
+    int bx, by, bz, t;
+};
+constexpr CubeShape kCubes[] = {{16, 16, 16, 2}, {16, 16, 16, 3}, {16, 16, 16, 4}, {32, 16, 8, 2}, {32, 8, 8, 2}};
+constexpr int kNumCubes = sizeof(kCubes) / sizeof(kCubes[0]);
+
+using CubeKernel = void (*)(const float *, float *, int, int, int, float, int, int, int);
+
+CubeKernel cube_kernel(int shape)
+{
+    switch (shape) {
+    case 0: return heat_cube<16, 16, 16, 2>;
+    case 1: return heat_cube<16, 16, 16, 3>;
+    case 2: return heat_cube<16, 16, 16, 4>;
+    case 3: return heat_cube<32, 16, 8, 2>;
+    case 4: return heat_cube<32, 8, 8, 2>;
+    default: return nullptr;
+    }
+}
+
+size_t cube_lds_bytes(int shape)
+{
+    const CubeShape &c = kCubes[shape];
+    return 2 * sizeof(float) * (size_t)(c.bx + 2 * c.t) * (c.by + 2 * c.t) * (c.bz + 2 * c.t);
+}
+
+// Lets each cube kernel use its dynamic LDS (above the default 64 KB);
+// once per process, before any launch or capture.
+int cube_prepare()
+{
+    static std::once_flag once;
+    static int rc = PAT_OK;
+    std::call_once(once, [] {
+        for (int s = 0; s < kNumCubes; s++)
+            if (hipFuncSetAttribute((const void *)cube_kernel(s), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)cube_lds_bytes(s)) != hipSuccess) {
+                (void)hipGetLastError();
+                pat::set_error("heat3d: LDS size for cube shape %d not accepted", s);
+                rc = PAT_ERR_HIP;
+            }
+    });
+    return rc;
 }
 
 // Default tile for a grid: 64-wide rows, 4x2 thread rows, and a z-chunk that
@@ -422,10 +589,31 @@ int heat3d_sweep2(const float *in, float *out, int64_t nx, int64_t ny, int64_t n
     return PAT_OK;
 }
 
-// One launch of tile t: a single sweep (fused == 0) or two (fused == 1).
+// kCubes[shape].t sweeps in -> out on LDS cubes.
+int heat3d_cube(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, int shape,
+                hipStream_t stream)
+{
+    PAT_REQUIRE(shape >= 0 && shape < kNumCubes, PAT_ERR_UNSUPPORTED, "heat3d: cube shape %d not instantiated",
+                shape);
+    PAT_TRY(cube_prepare());
+    const CubeShape &c = kCubes[shape];
+    const int tiles_x = (int)((nx + c.bx - 1) / c.bx);
+    const int tiles_y = (int)((ny + c.by - 1) / c.by);
+    const int tiles_z = (int)((nz + c.bz - 1) / c.bz);
+    const int64_t nblocks = (int64_t)tiles_x * tiles_y * tiles_z;
+    PAT_REQUIRE(nblocks < (1LL << 31), PAT_ERR_INVALID, "heat3d: grid too large");
+    hipLaunchKernelGGL(cube_kernel(shape), dim3((unsigned)nblocks), dim3(kCubeThreads), cube_lds_bytes(shape), stream, in,
+                       out, (int)nx, (int)ny, (int)nz, alpha, tiles_x, tiles_y, tiles_z);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
+}
+
+// One launch of tile t: a single sweep (fused == 0), two (fused == 1) or
+// t.ry on LDS cubes (fused == 2).
 int heat3d_launch(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz, float alpha, const HeatTile &t,
                   int variant, hipStream_t stream)
 {
+    if (t.fused == 2) return heat3d_cube(in, out, nx, ny, nz, alpha, t.bx, stream);
     if (t.fused) return heat3d_sweep2(in, out, nx, ny, nz, alpha, t.ry, t.zc, stream);
     return heat3d_sweep(in, out, nx, ny, nz, alpha, variant, t.bx, t.by, t.ry, t.zc, stream);
 }
@@ -438,15 +626,16 @@ int heat3d_steps(float *const buf[2], int start, int64_t nx, int64_t ny, int64_t
 {
     int c = start;
     int k = 0;
-    if (t.fused) {
-        for (; k + 2 <= nsteps; k += 2) {
-            PAT_TRY(heat3d_sweep2(buf[c], buf[1 - c], nx, ny, nz, alpha, t.ry, t.zc, stream));
+    const int spl = heat3d_steps_per_launch(t);
+    if (spl > 1) {
+        for (; k + spl <= nsteps; k += spl) {
+            PAT_TRY(heat3d_launch(buf[c], buf[1 - c], nx, ny, nz, alpha, t, variant, stream));
             c = 1 - c;
         }
     }
     if (k < nsteps) {
         HeatTile p = t;
-        if (t.fused) {  // the odd step runs on the plain kernel
+        if (t.fused) {  // the steps left over run on the plain kernel
             p.fused = 0;
             default_tile(nx, ny, nz, &p.bx, &p.by, &p.ry, &p.zc);
         }
@@ -494,18 +683,21 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
             if (z > nz && z != 2) continue;
             cands.push_back(HeatTile{0, sh.bx, sh.by, sh.ry, (int)std::min<int64_t>(z, nz)});
         }
-    if (allow_fused)
+    if (allow_fused) {
         for (int r : {2, 4, 8})
             for (int z : {2, 4, 8, 16, 32, 64, 128}) {
                 if (z > nz && z != 2) continue;
                 cands.push_back(HeatTile{1, 64, 4, r, (int)std::min<int64_t>(z, nz)});
             }
+        PAT_TRY(cube_prepare());  // before any capture
+        for (int c = 0; c < kNumCubes; c++) cands.push_back(HeatTile{2, c, 0, kCubes[c].t, 0});
+    }
     pat::EventPair ev;
     PAT_HIP(ev.create());
     hipEvent_t a = ev.a, b = ev.b;
     // a few milliseconds of warm-up on the default tile (clocks ramp up
     // under load), then rounds that time every candidate in turn; each
-    // candidate keeps its best round, fused ones counted per step
+    // candidate keeps its best round, multi-sweep ones counted per step
     const HeatTile def = *t;
     auto w0 = std::chrono::steady_clock::now();
     for (int i = 0; i < 2000; i++) {
@@ -515,23 +707,60 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
             if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(5)) break;
         }
     }
-    // Each timing covers kReps back-to-back launches of the same sweep, so
-    // that on small grids the fixed cost of an event pair around one launch
-    // (several microseconds) does not hide the kernels' own differences:
-    // steps run back to back (graph replays) in use.
+    // Each timing covers kReps launches of the same sweep.  They run as a
+    // captured graph where the stream allows it (steps run as graph
+    // replays in use), since back-to-back launches from the host cost
+    // several microseconds each, which hides every kernel faster than that;
+    // the null stream and a stream already capturing time back-to-back
+    // launches instead.
     constexpr int kReps = 8;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (st) PAT_HIP(hipStreamIsCapturing(st, &cap));
+    const bool by_graph = st != nullptr && cap == hipStreamCaptureStatusNone;
+    struct Execs {  // one graph per candidate, destroyed on every path
+        std::vector<hipGraphExec_t> v;
+        ~Execs()
+        {
+            for (hipGraphExec_t e : v)
+                if (e) (void)hipGraphExecDestroy(e);
+        }
+    } ex;
+    ex.v.assign(cands.size(), nullptr);
+    auto build = [&](size_t c) -> int {
+        hipGraph_t g = nullptr;
+        PAT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = PAT_OK;
+        for (int i = 0; i < kReps && rc == PAT_OK; i++)
+            rc = heat3d_launch(in, out, nx, ny, nz, alpha, cands[c], PAT_STENCIL_LDS25D, st);
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc != PAT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PAT_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&ex.v[c], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PAT_HIP(ei);
+        PAT_HIP(hipGraphLaunch(ex.v[c], st));  // the first replay carries one-time costs
+        return PAT_OK;
+    };
     std::vector<float> tm(cands.size(), 1e30f);
     for (int r = 0; r < 3; r++)
         for (size_t c = 0; c < cands.size(); c++) {
+            if (by_graph && !ex.v[c]) PAT_TRY(build(c));
             PAT_HIP(hipEventRecord(a, st));
-            for (int i = 0; i < kReps; i++)
-                PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, cands[c], PAT_STENCIL_LDS25D, st));
+            if (by_graph) {
+                PAT_HIP(hipGraphLaunch(ex.v[c], st));
+            } else {
+                for (int i = 0; i < kReps; i++)
+                    PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, cands[c], PAT_STENCIL_LDS25D, st));
+            }
             PAT_HIP(hipEventRecord(b, st));
             PAT_HIP(hipEventSynchronize(b));
             float ms = 0;
             PAT_HIP(hipEventElapsedTime(&ms, a, b));
             ms /= kReps;
-            tm[c] = std::min(tm[c], cands[c].fused ? ms / 2 : ms);
+            tm[c] = std::min(tm[c], ms / heat3d_steps_per_launch(cands[c]));
         }
     size_t bi = 0;
     for (size_t c = 1; c < cands.size(); c++)
@@ -559,6 +788,14 @@ int heat3d_tuned_tile(const float *in, float *out, int64_t nx, int64_t ny, int64
     return PAT_OK;
 }
 
+int heat3d_cube_tile(int shape, HeatTile *t)
+{
+    PAT_REQUIRE(shape >= 0 && shape < kNumCubes, PAT_ERR_UNSUPPORTED, "heat3d: cube shape %d not instantiated",
+                shape);
+    *t = HeatTile{2, shape, 0, kCubes[shape].t, 0};
+    return PAT_OK;
+}
+
 int heat3d_shapes(const int **triples)
 {
     static int flat[sizeof(kShapes) / sizeof(kShapes[0]) * 3];
@@ -580,12 +817,14 @@ extern "C" int pat_heat3d_run_dev(float *a, float *b, int64_t nx, int64_t ny, in
     PAT_REQUIRE(a && b && a != b, PAT_ERR_INVALID, "pat_heat3d_run_dev: need two distinct device grids");
     PAT_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nsteps >= 0, PAT_ERR_INVALID, "pat_heat3d_run_dev: bad sizes");
     PAT_REQUIRE((nx + 2) * (ny + 2) * (nz + 2) < (1LL << 40), PAT_ERR_INVALID, "pat_heat3d_run_dev: grid too large");
-    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_TEMPORAL2, PAT_ERR_UNSUPPORTED,
+    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_CUBES, PAT_ERR_UNSUPPORTED,
                 "pat_heat3d_run_dev: unknown variant %d", variant);
     hipStream_t st = (hipStream_t)stream;
     pat::HeatTile t;
     if (variant == PAT_STENCIL_AUTO && bx == 0 && by == 0 && zc == 0) {
         PAT_TRY(pat::heat3d_tuned(a, b, nx, ny, nz, alpha, st, true, &t));
+    } else if (variant == PAT_STENCIL_CUBES) {
+        PAT_TRY(pat::heat3d_cube_tile(bx, &t));
     } else if (variant == PAT_STENCIL_TEMPORAL2) {
         PAT_REQUIRE(bx == 0 || bx == 64, PAT_ERR_UNSUPPORTED, "pat_heat3d_run_dev: fused windows are 64 wide");
         PAT_REQUIRE(by == 0 || by == 8 || by == 16 || by == 32, PAT_ERR_UNSUPPORTED,

@@ -100,6 +100,7 @@ struct pat_stencil {
             PAT_TRY(pat::heat3d_tuned(buf[cur], buf[1 - cur], n[0], n[1], n[2], alpha(), stream, true, t));
             return PAT_OK;
         }
+        if (variant == PAT_STENCIL_CUBES) return pat::heat3d_cube_tile(bx, t);
         if (variant == PAT_STENCIL_TEMPORAL2) {
             *t = pat::HeatTile{1, 64, 4, ry > 0 ? ry : 8, zc > 0 ? zc : (int)std::min<int64_t>(64, n[2])};
             return PAT_OK;
@@ -219,10 +220,13 @@ int pat_stencil_set_table(pat_stencil *s, const pat_stencil_table *table)
 int pat_stencil_set_variant(pat_stencil *s, int variant, int bx, int by, int zc)
 {
     PAT_REQUIRE(s, PAT_ERR_INVALID, "pat_stencil_set_variant: null handle");
-    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_TEMPORAL2, PAT_ERR_UNSUPPORTED,
+    PAT_REQUIRE(variant >= PAT_STENCIL_AUTO && variant <= PAT_STENCIL_CUBES, PAT_ERR_UNSUPPORTED,
                 "pat_stencil_set_variant: unknown variant %d", variant);
     int tby = 0, tr = 0;
-    if (variant == PAT_STENCIL_TEMPORAL2) {
+    if (variant == PAT_STENCIL_CUBES) {
+        pat::HeatTile t;
+        PAT_TRY(pat::heat3d_cube_tile(bx, &t));
+    } else if (variant == PAT_STENCIL_TEMPORAL2) {
         PAT_REQUIRE((bx == 0 || bx == 64) && (by == 0 || by == 8 || by == 16 || by == 32) && zc >= 0,
                     PAT_ERR_UNSUPPORTED, "pat_stencil_set_variant: fused windows are 64 wide and 8, 16 or 32 rows");
         tby = by > 0 ? 4 : 0;
@@ -349,10 +353,11 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     pat::HeatTile t;
     PAT_TRY(s->resolve_tile(&t));
     const int v = s->kernel_variant();
-    // one launch per step, or per pair of steps for a fused tile (then each
-    // step of the pair is reported as half the launch); an odd last step
-    // of a fused run uses the plain kernel
+    // one launch per step, or per group of steps for a multi-sweep tile
+    // (then each step of the group is reported as its share of the
+    // launch); steps left over after the last group use the plain kernel
     pat::HeatTile plain = t;
+    const int spl = pat::heat3d_steps_per_launch(t);
     if (t.fused) {
         plain.fused = 0;
         pat::heat3d_default_tile(s->n[0], s->n[1], s->n[2], &plain.bx, &plain.by, &plain.ry, &plain.zc);
@@ -362,13 +367,13 @@ int pat_stencil_run_timed(pat_stencil *s, int nsteps, float *total_ms, float *ke
     int c = s->cur;
     int launches = 0;
     for (int k = 0; k < nsteps && rc == PAT_OK; launches++) {
-        const bool two = t.fused && k + 2 <= nsteps;
+        const bool group = spl > 1 && k + spl <= nsteps;
         (void)hipEventRecord(ev[2 * launches], s->stream);
-        rc = pat::heat3d_launch(s->buf[c], s->buf[1 - c], s->n[0], s->n[1], s->n[2], s->alpha(), two ? t : plain, v,
-                                s->stream);
+        rc = pat::heat3d_launch(s->buf[c], s->buf[1 - c], s->n[0], s->n[1], s->n[2], s->alpha(), group ? t : plain,
+                                v, s->stream);
         (void)hipEventRecord(ev[2 * launches + 1], s->stream);
-        steps_of.push_back(two ? 2 : 1);
-        k += two ? 2 : 1;
+        steps_of.push_back(group ? spl : 1);
+        k += group ? spl : 1;
         c = 1 - c;
     }
     if (rc == PAT_OK && hipStreamSynchronize(s->stream) != hipSuccess) {

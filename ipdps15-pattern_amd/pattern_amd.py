@@ -18,6 +18,7 @@ PAT_STENCIL_AUTO = 0
 PAT_STENCIL_LDS25D = 1
 PAT_STENCIL_GENERIC = 2
 PAT_STENCIL_TEMPORAL2 = 3
+PAT_STENCIL_CUBES = 4
 PAT_GR_VSTATS = 0
 PAT_GR_LJ_ENERGY = 2
 PAT_IR_LJ_MINIMAGE = 0

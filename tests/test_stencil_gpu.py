@@ -67,6 +67,39 @@ def test_temporal2_handle_replay(pa, orc, gpu):
     assert np.array_equal(got, orc.heat3d(g, 23))
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 4), (15, 17, 16), (16, 16, 16), (33, 9, 31), (61, 13, 17),
+                                   (130, 29, 2)])
+@pytest.mark.parametrize("steps", [1, 3, 4, 9])
+def test_cubes_bitexact_ragged(pa, orc, gpu, shape, steps):
+    """Several sweeps per launch on LDS blocks: every block edge, grids
+    smaller than one block, step counts that leave sweeps over for the
+    plain kernel."""
+    nx, ny, nz = shape
+    g = pa.gen_heat3d(nx, ny, nz, seed=23)
+    ref = orc.heat3d(g, steps)
+    for cube in range(5):
+        got = run_dev(pa, g, steps, variant=pa.PAT_STENCIL_CUBES, bx=cube)
+        assert np.array_equal(got, ref), cube
+
+
+def test_cubes_handle_replay_and_errors(pa, orc, gpu):
+    nx, ny, nz = 70, 50, 40
+    g = pa.gen_heat3d(nx, ny, nz, seed=16)
+    s = pa.Stencil(nx, ny, nz)
+    with pytest.raises(RuntimeError):
+        s.set_variant(pa.PAT_STENCIL_CUBES, 5, 0, 0)  # no such shape
+    s.set_variant(pa.PAT_STENCIL_CUBES, 2, 0, 0)  # 16^3 blocks, 4 sweeps per launch
+    s.upload(g)
+    s.run(8)
+    s.run(8)  # replays the captured graph from the other buffer
+    s.run(5)
+    tot, kms = s.run_timed(6)
+    got = s.download()
+    s.close()
+    assert len(kms) == 6 and np.all(kms > 0)
+    assert np.array_equal(got, orc.heat3d(g, 27))
+
+
 def test_naive_bitexact(pa, orc, gpu):
     g = pa.gen_heat3d(45, 30, 20, seed=13)
     ref = orc.heat3d(g, 4)
