@@ -1684,13 +1684,44 @@ int64_t csr_workspace_bytes(int64_t N, int64_t E)
 // Sum of n non-negative ints in 64 bits (into *out, which starts at 0):
 // the neighbour build checks the total entry count with it before the int32
 // row pointers are used, since an int32 scan of more than 2^31 entries wraps.
-__global__ void sum_nonneg_i64(const int *__restrict__ a, int64_t n, unsigned long long *__restrict__ out)
+// *out += sum of n non-negative ints, in 64 bits (a 16-byte aligned; 256
+// threads per workgroup).  Four 16-byte loads in flight per thread, and one
+// atomic per workgroup: atomics from every wavefront to one word serialise.
+__global__ __launch_bounds__(256) void sum_nonneg_i64(const int *__restrict__ a, int64_t n,
+                                                      unsigned long long *__restrict__ out)
 {
+    __shared__ unsigned long long part[4];
+    auto add4 = [](const int4 v) {
+        return (unsigned long long)(unsigned)v.x + (unsigned)v.y + (unsigned long long)(unsigned)v.z + (unsigned)v.w;
+    };
+    const int4 *__restrict__ a4 = (const int4 *)a;
+    const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long s = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        s += (unsigned)a[i];
+    int64_t i = gid;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const int4 v0 = a4[i], v1 = a4[i + stride], v2 = a4[i + 2 * stride], v3 = a4[i + 3 * stride];
+        s += add4(v0) + add4(v1) + add4(v2) + add4(v3);
+    }
+    for (; i < n4; i += stride) s += add4(a4[i]);
+    if (gid < n - 4 * n4) s += (unsigned)a[4 * n4 + gid];  // the last n % 4 ints
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long b = part[0] + part[1] + part[2] + part[3];
+        if (b) atomicAdd(out, b);
+    }
+}
+
+// Launches sum_nonneg_i64 over n ints at a (a 16-byte aligned device array).
+static int sum_nonneg(const int *a, int64_t n, unsigned long long *out, hipStream_t st)
+{
+    PAT_REQUIRE(((uintptr_t)a & 15) == 0, PAT_ERR_INVALID, "sum_nonneg: array not 16-byte aligned");
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n / 4 + 255) / 256, 256));
+    hipLaunchKernelGGL(sum_nonneg_i64, dim3((unsigned)blocks), dim3(256), 0, st, a, n, out);
+    PAT_LAUNCH_CHECK();
+    return PAT_OK;
 }
 
 // Exclusive scan of n ints (n <= kScanBlock^3) using ws for block sums.
@@ -2273,9 +2304,7 @@ static int neighbor_build_small(const float4 *pos, int64_t N, double L, double c
     PAT_LAUNCH_CHECK();
     PAT_HIP(hipMemsetAsync(w->deg + N, 0, sizeof(int), st));
     PAT_HIP(hipMemsetAsync(w->tot64, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(sum_nonneg_i64, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 1024)), dim3(256), 0, st,
-                       (const int *)w->deg, N, w->tot64);
-    PAT_LAUNCH_CHECK();
+    PAT_TRY(sum_nonneg((const int *)w->deg, N, w->tot64, st));
     PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
     int tot = 0;
     unsigned long long tot64 = 0;
@@ -2415,9 +2444,7 @@ int neighbor_build_multi(const float4 *pos, int nsys, const double *Ls, const in
         return PAT_ERR_NOMEM;
     }
     PAT_HIP(hipMemsetAsync(w->tot64, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(sum_nonneg_i64, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 1024)), dim3(256), 0, st,
-                       (const int *)w->deg, N, w->tot64);
-    PAT_LAUNCH_CHECK();
+    PAT_TRY(sum_nonneg((const int *)w->deg, N, w->tot64, st));
     PAT_TRY(scan_exclusive(w->deg, rowptr, N + 1, w->ws, st));
     int tot = 0, over = 0;
     unsigned long long tot64 = 0;
