@@ -858,14 +858,39 @@ __global__ __launch_bounds__(256) void km_finalize(float *__restrict__ c, int K,
     }
 }
 
-__global__ void km_maxabs(const float *__restrict__ x, int64_t n, unsigned int *__restrict__ out)
+// 256 threads per workgroup.  Sixteen-byte loads, four in flight per thread,
+// when x is 16-byte aligned (else one float at a time); the workgroup's
+// maximum reaches *out by one atomic (atomics from every wavefront to one
+// word serialise).  The maximum does not depend on the order.
+__global__ __launch_bounds__(256) void km_maxabs(const float *__restrict__ x, int64_t n,
+                                                 unsigned int *__restrict__ out)
 {
+    __shared__ float part[4];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, fabsf(x[i]));
+    if (((uintptr_t)x & 15) == 0) {
+        auto mx = [](const float4 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))); };
+        const float4 *__restrict__ x4 = (const float4 *)x;
+        const int64_t n4 = n / 4;
+        int64_t i = gid;
+        for (; i + 3 * stride < n4; i += 4 * stride) {
+            const float4 v0 = x4[i], v1 = x4[i + stride], v2 = x4[i + 2 * stride], v3 = x4[i + 3 * stride];
+            m = fmaxf(m, fmaxf(fmaxf(mx(v0), mx(v1)), fmaxf(mx(v2), mx(v3))));
+        }
+        for (; i < n4; i += stride) m = fmaxf(m, mx(x4[i]));
+        if (gid < n - 4 * n4) m = fmaxf(m, fabsf(x[4 * n4 + gid]));  // the last n % 4 values
+    } else {
+        for (int64_t i = gid; i < n; i += stride) m = fmaxf(m, fabsf(x[i]));
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
-    if (threadIdx.x % kWave == 0) atomicMax(out, __float_as_uint(m));
+    if (threadIdx.x % kWave == 0) part[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        if (b > 0.0f) atomicMax(out, __float_as_uint(b));
+    }
 }
 
 // ------------------------------------------------------------ bucketed reduction
