@@ -44,6 +44,42 @@ def test_single_pass_bitexact(pa, orc, gpu, mode, n, d, k):
     assert np.array_equal(c1, orc.kmeans_update(rsums, rcounts, s, c0))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_points_at_unaligned_device_address(pa, orc, gpu, mode):
+    """Points read in place from a device address that is 4-byte but not
+    16-byte aligned: the set-up's max |x| pass and the assignment kernels
+    take their unaligned forms.  Same bits as the oracle (the sums agree
+    only if both found the same max |x|, hence the same fixed-point shift)."""
+    n, d, k = 5001, 32, 64
+    x = pa.gen_gmm(n, d, 32, sigma=1.0, seed=77)
+    x[1234, 5] = -40.0  # the largest |x|, inside the array
+
+    class View(pa.DeviceArray):  # points at an offset into base; base owns the memory
+        def __init__(self, base, offset, shape):
+            self.shape, self.dtype = shape, np.dtype(np.float32)
+            self.nbytes = int(np.prod(shape)) * 4
+            self.ptr = base.ptr + offset
+            self._base = base
+
+        def free(self):
+            self.ptr = None
+
+    base = pa.DeviceArray((n * d + 4,), np.float32)
+    pa.call("pat_memcpy_h2d", base.ptr + 4, x.ctypes.data, x.nbytes, None)
+    pa.call("pat_stream_sync", None)
+    view = View(base, 4, (n, d))
+    km = pa.KMeans(view, k, mode=mode)
+    km.run(1)
+    c1, a, md = km.get()
+    sums, counts, dsums, s, ds = km.get_object()
+    km.close()
+    c0, ra, rmd, rsums, rcounts, rdsum = next(oracle_iters(orc, pa, x, k, 1))
+    assert np.array_equal(a, ra)
+    assert np.array_equal(md, rmd)
+    assert np.array_equal(counts, rcounts) and np.array_equal(sums, rsums) and np.array_equal(dsums, rdsum)
+    assert np.array_equal(c1, orc.kmeans_update(rsums, rcounts, s, c0))
+
+
 def adversarial(kind, rng):
     if kind == "duplicates":  # repeated centres -> exact ties
         x = rng.normal(size=(4000, 32)).astype(np.float32)
