@@ -450,7 +450,7 @@ def bench_serving(njobs, rate, slots=8):
     jobs = serving_mix(njobs, seed=5)
     _, _, sat_wall = run_serving(jobs, slots, False)  # saturated: capacity of the loop
     capacity = njobs / sat_wall
-    lat, _, wall = run_serving(jobs, slots, False, rate=rate, seed=6)
+    lat, serv, wall = run_serving(jobs, slots, False, rate=rate, seed=6)
     plat, pserv, pwall = run_serving(jobs, 1, True, rate=rate, seed=6)
     plain_capacity = njobs / (np.sum(pserv) * 1e-3)
     out = {"jobs": njobs, "slots": slots, "mix": "1/3 kmeans (N 10k-4M log, D 8-128, K 16-1024 log, 10 it), "
@@ -461,6 +461,18 @@ def bench_serving(njobs, rate, slots=8):
            "plain_one_at_a_time": {"p50_ms": float(np.percentile(plat, 50)), "p99_ms": float(np.percentile(plat, 99)),
                                    "jobs_per_s": njobs / pwall, "capacity_jobs_per_s": plain_capacity},
            "capacity_over_plain": capacity / plain_capacity}
+    # the rated run by job kind: what makes the tail (latency = queueing + service)
+    kinds = np.array([j[0] for j in jobs])
+    out["by_kind"] = {}
+    for kind, name in ((pa.PAT_JOB_KMEANS, "kmeans"), (pa.PAT_JOB_HEAT3D, "heat3d"), (pa.PAT_JOB_LJ, "lj")):
+        m = kinds == kind
+        out["by_kind"][name] = {"jobs": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                                "p99_ms": float(np.percentile(lat[m], 99)),
+                                "service_p50_ms": float(np.percentile(serv[m], 50)),
+                                "service_p99_ms": float(np.percentile(serv[m], 99))}
+    log("[serve] by kind at the fixed rate: " + "; ".join(
+        f"{n} p50 {v['p50_ms']:.1f} p99 {v['p99_ms']:.1f} ms (service p99 {v['service_p99_ms']:.1f} ms)"
+        for n, v in out["by_kind"].items()))
     log(f"[serve] {njobs} jobs, {slots} slots, Poisson {rate:.0f} jobs/s: p50 {out['p50_ms']:.1f} ms p99 "
         f"{out['p99_ms']:.1f} ms (capacity {capacity:.1f} jobs/s); plain one-at-a-time at the same arrivals: p50 "
         f"{out['plain_one_at_a_time']['p50_ms']:.1f} ms p99 {out['plain_one_at_a_time']['p99_ms']:.1f} ms "

@@ -700,20 +700,29 @@ int heat3d_tuned(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz
     // candidate keeps its best round, multi-sweep ones counted per step
     const HeatTile def = *t;
     auto w0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < 2000; i++) {
+    int warm = 0;
+    for (; warm < 2000; warm++) {
         PAT_TRY(heat3d_launch(in, out, nx, ny, nz, alpha, def, PAT_STENCIL_LDS25D, st));
-        if (i % 8 == 7) {
+        if (warm % 8 == 7) {
             PAT_HIP(hipStreamSynchronize(st));
-            if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(5)) break;
+            if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(5)) {
+                warm++;
+                break;
+            }
         }
     }
+    const double launch_us =
+        std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - w0).count() / warm;
     // Each timing covers kReps launches of the same sweep.  They run as a
     // captured graph where the stream allows it (steps run as graph
     // replays in use), since back-to-back launches from the host cost
     // several microseconds each, which hides every kernel faster than that;
     // the null stream and a stream already capturing time back-to-back
     // launches instead.
-    constexpr int kReps = 8;
+    // kReps covers ~100 us (1 to 8 launches): enough to hide the timing's
+    // own cost on small grids without making large grids' tuning, which a
+    // serving job waits for, eight times longer.
+    const int kReps = std::max(1, std::min(8, (int)std::ceil(100.0 / std::max(launch_us, 1.0))));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (st) PAT_HIP(hipStreamIsCapturing(st, &cap));
     const bool by_graph = st != nullptr && cap == hipStreamCaptureStatusNone;
