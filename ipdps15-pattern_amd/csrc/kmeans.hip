@@ -566,6 +566,8 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
     // of more than 64 dimensions or in workgroups over 512 threads (the
     // extra registers would spill).
     constexpr bool PF = !GC && KS <= 4 && BS <= 512;
+    // software-pipelined chunk scan (one more accumulator of KB x 16 values)
+    constexpr bool PIPE = !GC && KB == 1;
     const int64_t tile0 = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     float xnext[8 * KS];
     if (PF && tile0 < ntiles) load_half_row(tile0, xnext);
@@ -631,21 +633,47 @@ __global__ __launch_bounds__(BS) void km_mfma(KmArgs a, const __bf16 *__restrict
         const bool gidx = nchunks * KB <= 8;  // uniform
         float k1 = kFar, k2 = kFar, k3 = kFar;
         int ch1 = 0;
-        for (int ch = 0; ch < nchunks; ch++) {
+        auto scan3 = [&](int ch, const f32x16 (&v)[KB]) {
+#pragma unroll
+            for (int b = 0; b < KB; b++) {
+                if (b > 0 && ch * KB + b >= kblocks) break;  // uniform; block 0 of a chunk always exists
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const float key = pack_key(v[b][i], scalar_or((ch * KB + b) * 16, i), keep);
+                    k3 = __builtin_amdgcn_fmed3f(k2, k3, key);
+                    k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                    k1 = __builtin_amdgcn_fmed3f(k1, key, -kFar);  // min (keys are finite)
+                }
+            }
+        };
+        if (PIPE && gidx) {
+            // the next chunk's MFMAs are issued before this chunk's keys are
+            // scanned, so one wavefront keeps the matrix cores and the vector
+            // ALU busy together; two accumulators take the roles in turn
+            // (no branch between a chunk's MFMAs and the other's scan, so the
+            // scheduler can interleave them)
+            f32x16 acc2[KB];
+            chunk_values(0, bx, acc);
+            int ch = 0;
+            for (; ch + 2 < nchunks; ch += 2) {
+                chunk_values(ch + 1, bx, acc2);
+                scan3(ch, acc);
+                chunk_values(ch + 2, bx, acc);
+                scan3(ch + 1, acc2);
+            }
+            if (ch + 1 < nchunks) {
+                chunk_values(ch + 1, bx, acc2);
+                scan3(ch, acc);
+                scan3(ch + 1, acc2);
+            } else {
+                scan3(ch, acc);
+            }
+        }
+        for (int ch = 0; ch < (PIPE && gidx ? 0 : nchunks); ch++) {
             if (GC) stage_chunk(ch);
             chunk_values(ch, bx, acc);
             if (gidx) {
-#pragma unroll
-                for (int b = 0; b < KB; b++) {
-                    if (ch * KB + b >= kblocks) break;  // uniform
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const float key = pack_key(acc[b][i], scalar_or((ch * KB + b) * 16, i), keep);
-                        k3 = __builtin_amdgcn_fmed3f(k2, k3, key);
-                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                        k1 = __builtin_amdgcn_fmed3f(k1, key, -kFar);  // min (keys are finite)
-                    }
-                }
+                scan3(ch, acc);
                 continue;
             }
             float c1 = kFar, c2 = kFar;
