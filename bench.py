@@ -420,7 +420,7 @@ def run_serving(jobs, slots, plain, rate=None, seed=0, budget=200 << 30):
     """Submits the jobs (all at once, or as a Poisson stream at 'rate' jobs/s)
     and waits for them; returns latencies (ms, arrival to result on the host),
     the jobs' device times (start to end, ms) and the wall time."""
-    s = pa.Serve(budget, slots=slots, plain=plain)
+    s = pa.Serve(budget, slots=slots, plain=plain, fuse=not plain)  # the plain path runs one job at a time
     arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, len(jobs))) if rate else None
     ids = []
     t0 = time.perf_counter()
@@ -439,7 +439,7 @@ def run_serving(jobs, slots, plain, rate=None, seed=0, budget=200 << 30):
     if bad:
         raise RuntimeError(f"{len(bad)} serving jobs failed, first status {bad[0]['status']}")
     service = np.array([r["end_ms"] - r["start_ms"] for r in res])
-    return np.array([r["latency_ms"] for r in res]), service, wall
+    return np.array([r["latency_ms"] for r in res]), service, wall, np.array([r["batch"] for r in res])
 
 
 def bench_serving(njobs, rate, slots=8):
@@ -448,10 +448,10 @@ def bench_serving(njobs, rate, slots=8):
     arrivals.  The loop's capacity comes from a saturated run (all jobs
     queued at once); the plain path's from its jobs' summed service times."""
     jobs = serving_mix(njobs, seed=5)
-    _, _, sat_wall = run_serving(jobs, slots, False)  # saturated: capacity of the loop
+    _, _, sat_wall, sat_batch = run_serving(jobs, slots, False)  # saturated: capacity of the loop
     capacity = njobs / sat_wall
-    lat, serv, wall = run_serving(jobs, slots, False, rate=rate, seed=6)
-    plat, pserv, pwall = run_serving(jobs, 1, True, rate=rate, seed=6)
+    lat, serv, wall, batch = run_serving(jobs, slots, False, rate=rate, seed=6)
+    plat, pserv, pwall, _ = run_serving(jobs, 1, True, rate=rate, seed=6)
     plain_capacity = njobs / (np.sum(pserv) * 1e-3)
     out = {"jobs": njobs, "slots": slots, "mix": "1/3 kmeans (N 10k-4M log, D 8-128, K 16-1024 log, 10 it), "
                                                  "1/3 heat3D (side 32-384, 50 steps), 1/3 LJ (4^3-48^3 cells, 20 steps)",
@@ -461,6 +461,13 @@ def bench_serving(njobs, rate, slots=8):
            "plain_one_at_a_time": {"p50_ms": float(np.percentile(plat, 50)), "p99_ms": float(np.percentile(plat, 99)),
                                    "jobs_per_s": njobs / pwall, "capacity_jobs_per_s": plain_capacity},
            "capacity_over_plain": capacity / plain_capacity}
+    # LJ jobs fused into MD batches (jobs waiting with the same step count)
+    lj = np.array([j[0] for j in jobs]) == pa.PAT_JOB_LJ
+    out["lj_fused"] = {"saturated_share": float(np.mean(sat_batch[lj] > 1)),
+                       "saturated_mean_batch": float(np.mean(sat_batch[lj])),
+                       "rated_share": float(np.mean(batch[lj] > 1))}
+    log(f"[serve] LJ jobs fused: {out['lj_fused']['saturated_share']:.0%} when saturated (mean batch "
+        f"{out['lj_fused']['saturated_mean_batch']:.1f}), {out['lj_fused']['rated_share']:.0%} at {rate:.0f} jobs/s")
     # the rated run by job kind: what makes the tail (latency = queueing + service)
     kinds = np.array([j[0] for j in jobs])
     out["by_kind"] = {}

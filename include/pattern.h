@@ -660,9 +660,18 @@ typedef struct pat_job_result {
     double checksum;                          /* FP64 sum of the result (for a heat3D grid, of every cell) */
     double t_submit_ms, t_start_ms, t_end_ms; /* since pat_serve_start; end = result on the host */
     int64_t device_bytes;                     /* admission estimate held while it ran */
+    int batch; /* jobs that ran fused with this one, itself included (1 = alone) */
 } pat_job_result;
 typedef struct pat_serve pat_serve;
 #define PAT_SERVE_PLAIN 1 /* flag: kmeans on the plain FP32 kernel */
+/* flag: every LJ job runs on its own handle.  Without it, a slot that admits
+ * an LJ job also admits the LJ jobs waiting with the same step count (boxes
+ * of at least three neighbour bins, up to 16 jobs and 2^21 atoms, each within
+ * the budget) and advances them as one batch (pat_mdb: one fused force and
+ * update launch per step over every system, the steps between neighbour
+ * rebuilds replayed from a captured hipGraph).  Each job's result is the
+ * same bits either way; the batch's jobs share its start and end times. */
+#define PAT_SERVE_NO_FUSE 2
 /* Starts 'slots' workers over an arena of hbm_budget device bytes. */
 int pat_serve_start(int64_t hbm_budget, int slots, int flags, pat_serve **out);
 /* Queues a job (non-blocking); *id identifies it. */

@@ -148,6 +148,43 @@ __global__ __launch_bounds__(256) void sum_f32_k(const float *__restrict__ p, in
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// The same over the x, y, z of n float4 (w left out), in a fixed order.
+__global__ __launch_bounds__(256) void sum_xyz_k(const float4 *__restrict__ p, int64_t n, double *__restrict__ part)
+{
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        acc += (double)v.x;
+        acc += (double)v.y;
+        acc += (double)v.z;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void set_w_k(float4 *__restrict__ p, int64_t n, float w)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i].w = w;
+}
+
+int sum_parts(const double *part, double *out, hipStream_t st)
+{
+    double h[kSumBlocks];
+    PAT_HIP(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st));
+    PAT_HIP(hipStreamSynchronize(st));
+    double t = 0.0;
+    for (double v : h) t += v;
+    *out = t;
+    return PAT_OK;
+}
+
 }  // namespace
 
 namespace pat {
@@ -160,12 +197,22 @@ int sum_f32_dev(const float *p, int64_t n, double *part, double *out, hipStream_
 {
     hipLaunchKernelGGL(sum_f32_k, dim3(kSumBlocks), dim3(256), 0, st, p, n, part);
     PAT_LAUNCH_CHECK();
-    double h[kSumBlocks];
-    PAT_HIP(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st));
-    PAT_HIP(hipStreamSynchronize(st));
-    double t = 0.0;
-    for (double v : h) t += v;
-    *out = t;
+    return sum_parts(part, out, st);
+}
+
+// Sum of the x, y, z of n float4 at p (device), as sum_f32_dev.
+int sum_xyz_dev(const float *p, int64_t n, double *part, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(sum_xyz_k, dim3(kSumBlocks), dim3(256), 0, st, (const float4 *)p, n, part);
+    PAT_LAUNCH_CHECK();
+    return sum_parts(part, out, st);
+}
+
+// w = value for n float4 at p (device), queued on st.
+int set_w_dev(float *p, int64_t n, float w, hipStream_t st)
+{
+    hipLaunchKernelGGL(set_w_k, dim3(grid_for(n)), dim3(256), 0, st, (float4 *)p, n, w);
+    PAT_LAUNCH_CHECK();
     return PAT_OK;
 }
 

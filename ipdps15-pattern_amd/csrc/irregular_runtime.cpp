@@ -1661,8 +1661,15 @@ struct pat_mdb {
 
 extern "C" {
 
-int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const float *vel, const double *box,
-                   double cutoff, double skin, int every, double dt, pat_mdb **out)
+} // extern "C"
+
+namespace {
+
+// pat_mdb_create for host positions (pos: N x 3) or, with dev, device
+// positions (pos: N float4 with each node's box side in w, copied as they
+// are); velocities from the host, or zero.
+int mdb_create(int nsys, const int64_t *offsets, const float *pos, bool dev, const float *vel, const double *box,
+               double cutoff, double skin, int every, double dt, pat_mdb **out)
 {
     PAT_REQUIRE(out && offsets && pos && box && nsys > 0, PAT_ERR_INVALID, "pat_mdb_create: null argument");
     PAT_REQUIRE(cutoff > 0 && skin >= 0 && dt > 0 && every > 0, PAT_ERR_INVALID, "pat_mdb_create: bad parameters");
@@ -1686,7 +1693,16 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     m.cutneigh = cutoff + skin;
     m.nbw = pat::nb_create();
     int rc = m.st.alloc(N, 0, nullptr, true, false);
-    if (rc == PAT_OK) {
+    if (rc == PAT_OK && dev) {
+        (void)pat::dev_free(m.st.cols);
+        m.st.cols = nullptr;
+        if (hipMemcpyAsync(m.st.pos[0], pos, sizeof(float) * 4 * N, hipMemcpyDeviceToDevice, m.st.stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(m.st.stream) != hipSuccess) {
+            pat::set_error("pat_mdb_create: position copy failed");
+            rc = PAT_ERR_HIP;
+        }
+    } else if (rc == PAT_OK) {
         (void)pat::dev_free(m.st.cols);
         m.st.cols = nullptr;
         // positions with each node's box side in w
@@ -1724,6 +1740,38 @@ int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const flo
     }
     *out = h;
     return PAT_OK;
+}
+
+}  // namespace
+
+namespace pat {
+
+int mdb_create_dev(int nsys, const int64_t *offsets, const float *pos4, const double *box, double cutoff,
+                   double skin, int every, double dt, pat_mdb **out)
+{
+    return mdb_create(nsys, offsets, pos4, true, nullptr, box, cutoff, skin, every, dt, out);
+}
+
+int mdb_positions(pat_mdb *h, int sys, const float **pos4, int64_t *n)
+{
+    PAT_REQUIRE(h && pos4 && n, PAT_ERR_INVALID, "mdb_positions: null argument");
+    PAT_REQUIRE(!h->md.failed, PAT_ERR_STATE, "mdb_positions: an earlier run of this handle failed part-way");
+    PAT_REQUIRE(sys >= 0 && sys < (int)h->box.size(), PAT_ERR_INVALID, "mdb_positions: system %d out of range", sys);
+    *pos4 = (const float *)h->md.st.pos[h->md.cur] + 4 * h->off[sys];
+    *n = h->off[sys + 1] - h->off[sys];
+    return PAT_OK;
+}
+
+int mdb_lanes(pat_mdb *h) { return h ? h->md.st.lpr : 0; }
+
+}  // namespace pat
+
+extern "C" {
+
+int pat_mdb_create(int nsys, const int64_t *offsets, const float *pos, const float *vel, const double *box,
+                   double cutoff, double skin, int every, double dt, pat_mdb **out)
+{
+    return mdb_create(nsys, offsets, pos, false, vel, box, cutoff, skin, every, dt, out);
 }
 
 int pat_mdb_run(pat_mdb *h, int nsteps, float *total_ms, float *kernel_ms)

@@ -16,6 +16,14 @@
 // consecutive admissions that passed over the oldest job, only that job may
 // start, so a large job is never starved.  A job whose allocation fails
 // while others hold memory is queued again and then runs alone.
+//
+// Uneven LJ instances are fused: a slot that admits an LJ job also admits
+// the LJ jobs waiting with the same step count (within the budget, up to 16
+// jobs and 2^21 atoms) and runs them as one MD batch (pat_mdb): every step
+// is one force + velocity-Verlet launch over all their systems, and the
+// steps between neighbour rebuilds replay a captured hipGraph.  Each
+// system's sums keep their own order, so every job gets the bits it gets
+// alone.
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -24,6 +32,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <vector>
 
 #include "kmeans_dev.hpp"
 #include "pat_common.hpp"
@@ -31,6 +40,12 @@
 namespace pat {
 int sum_blocks();
 int sum_f32_dev(const float *p, int64_t n, double *part, double *out, hipStream_t st);
+int sum_xyz_dev(const float *p, int64_t n, double *part, double *out, hipStream_t st);
+int set_w_dev(float *p, int64_t n, float w, hipStream_t st);
+int mdb_create_dev(int nsys, const int64_t *offsets, const float *pos4, const double *box, double cutoff,
+                   double skin, int every, double dt, pat_mdb **out);
+int mdb_positions(pat_mdb *h, int sys, const float **pos4, int64_t *n);
+int mdb_lanes(pat_mdb *h);
 }  // namespace pat
 
 namespace {
@@ -39,6 +54,19 @@ constexpr int kMaxBypass = 64;
 constexpr float kHeatAlpha = 0.1f;  // heat3D update u' = u + 0.1 (sum6 - 6u)
 constexpr double kRho = 0.8442, kJitter = 0.05, kCut = 2.5, kSkin = 0.3, kDt = 0.005;
 constexpr int kEvery = 20;
+// LJ jobs advanced as one batch: at most this many, and atoms in all
+constexpr int kFuseMax = 16;
+constexpr int64_t kFuseAtoms = int64_t(1) << 21;
+
+int64_t lj_atoms(const pat_job &j) { return 4 * j.size * j.size * j.size; }
+
+// An LJ job whose box holds three neighbour bins of cutoff + skin per side
+// (the batch's binned build needs them; smaller boxes test every pair).
+bool lj_fusable(const pat_job &j)
+{
+    const double box = (double)j.size * std::pow(4.0 / kRho, 1.0 / 3.0);  // the FCC generator's side
+    return j.kind == PAT_JOB_LJ && box >= 3 * (kCut + kSkin) * 1.0001;
+}
 
 struct Job {
     int64_t id;
@@ -119,6 +147,26 @@ struct pat_serve {
         return false;
     }
 
+    // The LJ jobs waiting with the same step count as g[0], admitted with it
+    // while the budget, the batch limits and the bypass guard allow (under mu).
+    void take_partners(std::vector<Job> &g)
+    {
+        int64_t atoms = lj_atoms(g[0].d);
+        for (size_t i = 0; i < pending.size() && (int)g.size() < kFuseMax && bypass < kMaxBypass;) {
+            const Job &q = pending[i];
+            if (q.attempts == 0 && q.d.steps == g[0].d.steps && lj_fusable(q.d) &&
+                atoms + lj_atoms(q.d) <= kFuseAtoms && reserved + q.est <= budget) {
+                atoms += lj_atoms(q.d);
+                reserved += q.est;
+                g.push_back(q);
+                pending.erase(pending.begin() + (long)i);
+                bypass = i > 0 ? bypass + 1 : 0;
+            } else {
+                i++;
+            }
+        }
+    }
+
     int run_heat(const pat_job &j, hipStream_t st, double *part, pat_job_result *r)
     {
         const int64_t n = j.size, cells = (n + 2) * (n + 2) * (n + 2);
@@ -185,7 +233,7 @@ struct pat_serve {
         if (rc == PAT_OK) {
             void *p = nullptr;
             rc = pat_md_device_buffers(md, &p, nullptr, nullptr);
-            if (rc == PAT_OK) rc = pat::sum_f32_dev((const float *)p, 4 * n, part, &r->checksum, st);
+            if (rc == PAT_OK) rc = pat::sum_xyz_dev((const float *)p, n, part, &r->checksum, st);
         }
         if (rc == PAT_OK && j.out) {
             if (j.out_len < 3 * n) {
@@ -197,6 +245,59 @@ struct pat_serve {
         }
         if (md) (void)pat_md_destroy(md);
         (void)pat::dev_free(pos4);
+        return rc;
+    }
+
+    // Several LJ jobs as one pat_mdb batch: each system generated on the
+    // device at its offset (its box side in w), one fused launch per step
+    // over all of them.  Sets every result's status (the batch's own error,
+    // or the job's) and returns the batch's.
+    int run_lj_batch(const std::vector<Job> &g, hipStream_t st, double *part, std::vector<pat_job_result> &res)
+    {
+        const int ns = (int)g.size();
+        std::vector<int64_t> off(ns + 1, 0);
+        std::vector<double> box(ns, 0.0);
+        for (int s = 0; s < ns; s++) off[s + 1] = off[s] + lj_atoms(g[s].d);
+        const int64_t N = off[ns];
+        float *pos4 = nullptr;
+        int rc = PAT_OK;
+        if (pat::dev_malloc(&pos4, sizeof(float) * 4 * N) != hipSuccess) {
+            pat::set_error("serve: LJ batch: no device memory for %lld atoms", (long long)N);
+            rc = PAT_ERR_NOMEM;
+        }
+        for (int s = 0; s < ns && rc == PAT_OK; s++) {
+            const int64_t c = g[s].d.size;
+            double b3[3];
+            rc = pat_gen_fcc_dev(c, c, c, kRho, kJitter, g[s].d.seed, pos4 + 4 * off[s], b3, st);
+            box[s] = b3[0];
+            if (rc == PAT_OK) rc = pat::set_w_dev(pos4 + 4 * off[s], off[s + 1] - off[s], (float)b3[0], st);
+        }
+        if (rc == PAT_OK && hipStreamSynchronize(st) != hipSuccess) rc = PAT_ERR_HIP;
+        pat_mdb *h = nullptr;
+        if (rc == PAT_OK) rc = pat::mdb_create_dev(ns, off.data(), pos4, box.data(), kCut, kSkin, kEvery, kDt, &h);
+        if (pos4) (void)pat::dev_free(pos4);  // the batch holds its own copy
+        if (rc == PAT_OK) rc = pat_mdb_run(h, g[0].d.steps, nullptr, nullptr);
+        const int lanes = pat::mdb_lanes(h);
+        for (int s = 0; s < ns; s++) {
+            pat_job_result &r = res[s];
+            r.kernel = lanes;
+            r.batch = ns;
+            int rs = rc;
+            const float *p = nullptr;
+            int64_t n = 0;
+            if (rs == PAT_OK) rs = pat::mdb_positions(h, s, &p, &n);
+            if (rs == PAT_OK) rs = pat::sum_xyz_dev(p, n, part, &r.checksum, st);
+            if (rs == PAT_OK && g[s].d.out) {
+                if (g[s].d.out_len < 3 * n) {
+                    pat::set_error("serve: LJ job needs %lld output floats", (long long)(3 * n));
+                    rs = PAT_ERR_INVALID;
+                } else {
+                    rs = pat_mdb_get(h, s, g[s].d.out, nullptr, nullptr);
+                }
+            }
+            r.status = rs;
+        }
+        if (h) (void)pat_mdb_destroy(h);
         return rc;
     }
 
@@ -245,40 +346,54 @@ struct pat_serve {
         const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
                         pat::dev_malloc(&part, sizeof(double) * pat::sum_blocks()) == hipSuccess;
         for (;;) {
-            Job j;
+            std::vector<Job> g;  // the job, and the LJ jobs fused with it
             {
-                std::unique_lock<std::mutex> g(mu);
-                cv_work.wait(g, [&] { return pick_ready() || (stopping && pending.empty()); });
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return pick_ready() || (stopping && pending.empty()); });
+                Job j;
                 if (!pick(&j)) break;  // stopping with nothing left
                 reserved += j.est;
                 running++;
+                g.push_back(j);
+                if (!(flags & PAT_SERVE_NO_FUSE) && j.attempts == 0 && lj_fusable(j.d)) take_partners(g);
             }
-            pat_job_result r;
-            memset(&r, 0, sizeof(r));
-            r.t_submit_ms = j.t_submit;
-            r.t_start_ms = ms_since(t0);
-            r.device_bytes = j.est;
+            std::vector<pat_job_result> res(g.size());
+            const double t_start = ms_since(t0);
+            for (size_t s = 0; s < g.size(); s++) {
+                memset(&res[s], 0, sizeof(pat_job_result));
+                res[s].t_submit_ms = g[s].t_submit;
+                res[s].t_start_ms = t_start;
+                res[s].device_bytes = g[s].est;
+                res[s].batch = 1;
+                res[s].status = PAT_ERR_STATE;
+            }
             int rc = PAT_ERR_STATE;
-            if (ok) {
-                switch (j.d.kind) {
-                case PAT_JOB_HEAT3D: rc = run_heat(j.d, st, part, &r); break;
-                case PAT_JOB_LJ: rc = run_lj(j.d, st, part, &r); break;
-                case PAT_JOB_KMEANS: rc = run_kmeans(j.d, st, &r); break;
+            if (ok && g.size() > 1) {
+                rc = run_lj_batch(g, st, part, res);
+            } else if (ok) {
+                switch (g[0].d.kind) {
+                case PAT_JOB_HEAT3D: rc = run_heat(g[0].d, st, part, &res[0]); break;
+                case PAT_JOB_LJ: rc = run_lj(g[0].d, st, part, &res[0]); break;
+                case PAT_JOB_KMEANS: rc = run_kmeans(g[0].d, st, &res[0]); break;
                 default: rc = PAT_ERR_INVALID;
                 }
+                res[0].status = rc;
             }
-            r.t_end_ms = ms_since(t0);
-            r.status = rc;
+            const double t_end = ms_since(t0);
+            for (auto &r : res) r.t_end_ms = t_end;
             {
-                std::lock_guard<std::mutex> g(mu);
-                reserved -= j.est;
+                std::lock_guard<std::mutex> lk(mu);
+                for (const Job &q : g) reserved -= q.est;
                 running--;
-                if (rc == PAT_ERR_NOMEM && j.attempts == 0 && running > 0) {
-                    j.attempts = 1;  // run again once it can have the arena alone
-                    pending.push_front(j);
+                if (rc == PAT_ERR_NOMEM && g[0].attempts == 0 && running > 0) {
+                    // run again once each can have the arena alone
+                    for (size_t s = g.size(); s-- > 0;) {
+                        g[s].attempts = 1;
+                        pending.push_front(g[s]);
+                    }
                 } else {
-                    done[j.id] = r;
-                    finished++;
+                    for (size_t s = 0; s < g.size(); s++) done[g[s].id] = res[s];
+                    finished += (int64_t)g.size();
                     cv_done.notify_all();
                 }
             }

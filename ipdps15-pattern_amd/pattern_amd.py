@@ -37,6 +37,7 @@ PAT_JOB_HEAT3D = 0
 PAT_JOB_LJ = 1
 PAT_JOB_KMEANS = 2
 PAT_SERVE_PLAIN = 1
+PAT_SERVE_NO_FUSE = 2
 
 _lib = None
 
@@ -64,7 +65,7 @@ class JobResult(ctypes.Structure):
     """struct pat_job_result."""
     _fields_ = [("status", ctypes.c_int), ("kernel", ctypes.c_int), ("checksum", ctypes.c_double),
                 ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double), ("t_end_ms", ctypes.c_double),
-                ("device_bytes", ctypes.c_int64)]
+                ("device_bytes", ctypes.c_int64), ("batch", ctypes.c_int)]
 
 
 # name -> (restype, argtypes) for every symbol include/pattern.h declares
@@ -916,16 +917,17 @@ def job(kind, size, steps, seed, dim=0, k=0, out=None):
 def result_dict(r):
     return {"status": r.status, "kernel": r.kernel, "checksum": r.checksum, "submit_ms": r.t_submit_ms,
             "start_ms": r.t_start_ms, "end_ms": r.t_end_ms, "latency_ms": r.t_end_ms - r.t_submit_ms,
-            "device_bytes": r.device_bytes}
+            "device_bytes": r.device_bytes, "batch": r.batch}
 
 
 class Serve:
     """The serving loop (pat_serve_*): jobs admitted by device-memory budget
     and run concurrently on 'slots' streams."""
 
-    def __init__(self, budget_bytes, slots=8, plain=False):
+    def __init__(self, budget_bytes, slots=8, plain=False, fuse=True):
         h = c_vp()
-        call("pat_serve_start", int(budget_bytes), slots, PAT_SERVE_PLAIN if plain else 0, ctypes.byref(h))
+        flags = (PAT_SERVE_PLAIN if plain else 0) | (0 if fuse else PAT_SERVE_NO_FUSE)
+        call("pat_serve_start", int(budget_bytes), slots, flags, ctypes.byref(h))
         self.h = h
         self._keep = {}
 

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -78,3 +79,31 @@ def test_neighbour_build_entry_total_guard(pa):
     assert f(2 ** 32 + 5, 5) == -5  # wrapped to a small positive total
     assert f(3 * 2 ** 31, 2 ** 31 - 2 ** 31) == -5
     assert f(1000, 999) == -1  # the two totals disagree
+
+
+def test_serving_structs_match_the_header(pa, tmp_path):
+    """The ctypes mirrors of pat_job and pat_job_result have the header's
+    sizes and field offsets (compiled with the host C compiler)."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "layout.c"
+    fields = {"pat_job": [f[0] for f in pa.Job._fields_], "pat_job_result": [f[0] for f in pa.JobResult._fields_]}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pattern.h"', "int main(void) {"]
+    for t, fs in fields.items():
+        lines.append(f'printf("{t} %zu\\n", sizeof({t}));')
+        lines += [f'printf("{t}.{f} %zu\\n", offsetof({t}, {f}));' for f in fs]
+    lines += ["return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for t, cls in (("pat_job", pa.Job), ("pat_job_result", pa.JobResult)):
+        assert int(got[t]) == ctypes.sizeof(cls), t
+        for f in fields[t]:
+            assert int(got[f"{t}.{f}"]) == getattr(cls, f).offset, (t, f)

@@ -113,6 +113,47 @@ def test_serve_equals_handles(pa, gpu):
     km.close()
 
 
+def test_serve_fuses_waiting_lj_jobs(pa, gpu):
+    """LJ jobs waiting with the same step count are admitted together and run
+    as one MD batch (one fused launch per step over every system).  Each
+    result equals the same job with fusing off and the standalone handle,
+    bit for bit."""
+    specs = [(6, 31), (9, 32), (7, 33), (12, 34), (6, 35), (10, 36)]  # (cells per side, seed)
+    steps = 21  # a neighbour rebuild at step 20
+
+    def serve(fuse):
+        s = pa.Serve(2 << 30, slots=1, fuse=fuse)
+        # a heat3D job holds the one slot while the LJ jobs queue behind it
+        first = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 200, 50, 5))
+        bufs, ids = [], []
+        for c, seed in specs:
+            b = np.zeros(4 * c ** 3 * 3, dtype=np.float32)
+            bufs.append(b)
+            ids.append(s.submit(pa.job(pa.PAT_JOB_LJ, c, steps, seed, out=b), keep=b))
+        res = [s.wait(i) for i in ids]
+        assert s.wait(first)["status"] == 0
+        s.close()
+        return bufs, res
+
+    fb, fr = serve(True)
+    ab, ar = serve(False)
+    print("batch sizes fused:", [r["batch"] for r in fr], "unfused:", [r["batch"] for r in ar])
+    assert all(r["status"] == 0 for r in fr + ar)
+    assert all(r["batch"] == 1 for r in ar)
+    assert all(r["batch"] == len(specs) for r in fr)
+    assert len({(r["start_ms"], r["end_ms"]) for r in fr}) == 1  # one batch, one start and end
+    for (c, seed), f, a, x, y in zip(specs, fb, ab, fr, ar):
+        assert np.array_equal(f, a), c
+        assert x["checksum"] == y["checksum"], c
+        assert np.isclose(x["checksum"], f.astype(np.float64).sum(), rtol=1e-9)
+    for (c, seed), f in list(zip(specs, fb))[:2]:
+        pos, box = pa.gen_fcc(c, c, c, rho=0.8442, jitter=0.05, seed=seed)
+        md = pa.MD(pos, None, None, box[0], 2.5, 0.005, skin=0.3, every=20)
+        md.run(steps)
+        assert np.array_equal(f.reshape(-1, 3), md.get()[0]), c
+        md.close()
+
+
 def test_serve_admission_respects_budget(pa, gpu):
     # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
     s = pa.Serve(100 * MB, slots=4)
