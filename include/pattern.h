@@ -664,13 +664,17 @@ typedef struct pat_job_result {
 } pat_job_result;
 typedef struct pat_serve pat_serve;
 #define PAT_SERVE_PLAIN 1 /* flag: kmeans on the plain FP32 kernel */
-/* flag: every LJ job runs on its own handle.  Without it, a slot that admits
- * an LJ job also admits the LJ jobs waiting with the same step count (boxes
- * of at least three neighbour bins, up to 16 jobs and 2^21 atoms, each within
- * the budget) and advances them as one batch (pat_mdb: one fused force and
- * update launch per step over every system, the steps between neighbour
- * rebuilds replayed from a captured hipGraph).  Each job's result is the
- * same bits either way; the batch's jobs share its start and end times. */
+/* flag: every job runs alone.  Without it, a slot that admits an LJ or a
+ * small heat3D job also admits the jobs of that kind waiting with the same
+ * step count (each within the budget, up to 16 jobs) and advances them as
+ * one batch:
+ *   LJ (boxes of at least three neighbour bins, up to 2^21 atoms): pat_mdb,
+ *   one fused force and update launch per step over every system, the steps
+ *   between neighbour rebuilds replayed from a captured hipGraph;
+ *   heat3D (sides up to 128, up to 2^23 cells): two sweeps per launch over
+ *   every grid, the launches captured as one hipGraph.
+ * Each job's result is the same bits either way; the batch's jobs share its
+ * start and end times. */
 #define PAT_SERVE_NO_FUSE 2
 /* Starts 'slots' workers over an arena of hbm_budget device bytes. */
 int pat_serve_start(int64_t hbm_budget, int slots, int flags, pat_serve **out);

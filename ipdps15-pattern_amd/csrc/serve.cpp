@@ -17,13 +17,17 @@
 // start, so a large job is never starved.  A job whose allocation fails
 // while others hold memory is queued again and then runs alone.
 //
-// Uneven LJ instances are fused: a slot that admits an LJ job also admits
-// the LJ jobs waiting with the same step count (within the budget, up to 16
-// jobs and 2^21 atoms) and runs them as one MD batch (pat_mdb): every step
-// is one force + velocity-Verlet launch over all their systems, and the
-// steps between neighbour rebuilds replay a captured hipGraph.  Each
-// system's sums keep their own order, so every job gets the bits it gets
-// alone.
+// Uneven instances are fused: a slot that admits an LJ job or a small
+// heat3D job also admits the jobs of that kind waiting with the same step
+// count (within the budget and the batch limits) and runs them as one
+// batch.  LJ jobs become one MD batch (pat_mdb): every step is one force +
+// velocity-Verlet launch over all their systems, and the steps between
+// neighbour rebuilds replay a captured hipGraph; each system's sums keep
+// their own order.  Heat3D grids are swept two steps per launch over every
+// grid, the launches captured as one hipGraph; any tiling gives the serial
+// sweep's values.  So every job gets the bits it gets alone.  kmeans jobs
+// run alone (their D and K select different kernels), their iterations
+// replayed from the handle's captured hipGraph.
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -36,6 +40,7 @@
 
 #include "kmeans_dev.hpp"
 #include "pat_common.hpp"
+#include "stencil_dev.hpp"
 
 namespace pat {
 int sum_blocks();
@@ -54,19 +59,27 @@ constexpr int kMaxBypass = 64;
 constexpr float kHeatAlpha = 0.1f;  // heat3D update u' = u + 0.1 (sum6 - 6u)
 constexpr double kRho = 0.8442, kJitter = 0.05, kCut = 2.5, kSkin = 0.3, kDt = 0.005;
 constexpr int kEvery = 20;
-// LJ jobs advanced as one batch: at most this many, and atoms in all
+// Jobs of one kind advanced as one batch: at most this many, and LJ atoms
+// or heat3D cells in all; heat3D grids up to this side (larger ones fill
+// the GPU alone, and their tuned tiles beat the batch's fixed one)
 constexpr int kFuseMax = 16;
-constexpr int64_t kFuseAtoms = int64_t(1) << 21;
+constexpr int64_t kFuseAtoms = int64_t(1) << 21, kFuseCells = int64_t(1) << 23;
+constexpr int64_t kFuseHeatSide = 128;
 
 int64_t lj_atoms(const pat_job &j) { return 4 * j.size * j.size * j.size; }
 
 // An LJ job whose box holds three neighbour bins of cutoff + skin per side
-// (the batch's binned build needs them; smaller boxes test every pair).
-bool lj_fusable(const pat_job &j)
+// (the batch's binned build needs them; smaller boxes test every pair), or
+// a small heat3D grid.
+bool fusable(const pat_job &j)
 {
+    if (j.kind == PAT_JOB_HEAT3D) return j.size <= kFuseHeatSide;
     const double box = (double)j.size * std::pow(4.0 / kRho, 1.0 / 3.0);  // the FCC generator's side
     return j.kind == PAT_JOB_LJ && box >= 3 * (kCut + kSkin) * 1.0001;
 }
+
+// What a batch's limit counts: atoms (LJ) or interior cells (heat3D).
+int64_t fuse_units(const pat_job &j) { return j.kind == PAT_JOB_LJ ? lj_atoms(j) : j.size * j.size * j.size; }
 
 struct Job {
     int64_t id;
@@ -147,16 +160,19 @@ struct pat_serve {
         return false;
     }
 
-    // The LJ jobs waiting with the same step count as g[0], admitted with it
-    // while the budget, the batch limits and the bypass guard allow (under mu).
+    // The jobs of g[0]'s kind waiting with the same step count, admitted
+    // with it while the budget, the batch limits and the bypass guard allow
+    // (under mu).
     void take_partners(std::vector<Job> &g)
     {
-        int64_t atoms = lj_atoms(g[0].d);
+        const int kind = g[0].d.kind;
+        const int64_t limit = kind == PAT_JOB_LJ ? kFuseAtoms : kFuseCells;
+        int64_t units = fuse_units(g[0].d);
         for (size_t i = 0; i < pending.size() && (int)g.size() < kFuseMax && bypass < kMaxBypass;) {
             const Job &q = pending[i];
-            if (q.attempts == 0 && q.d.steps == g[0].d.steps && lj_fusable(q.d) &&
-                atoms + lj_atoms(q.d) <= kFuseAtoms && reserved + q.est <= budget) {
-                atoms += lj_atoms(q.d);
+            if (q.attempts == 0 && q.d.kind == kind && q.d.steps == g[0].d.steps && fusable(q.d) &&
+                units + fuse_units(q.d) <= limit && reserved + q.est <= budget) {
+                units += fuse_units(q.d);
                 reserved += q.est;
                 g.push_back(q);
                 pending.erase(pending.begin() + (long)i);
@@ -183,8 +199,19 @@ struct pat_serve {
         if (rc == PAT_OK && hipMemsetAsync(b, 0, sizeof(float) * cells, st) != hipSuccess) rc = PAT_ERR_HIP;
         if (rc == PAT_OK)
             rc = pat_heat3d_run_dev(a, b, n, n, n, j.steps, kHeatAlpha, PAT_STENCIL_AUTO, 0, 0, 0, &in_b, st);
-        const float *res = in_b ? b : a;
-        if (rc == PAT_OK) rc = pat::sum_f32_dev(res, cells, part, &r->checksum, st);
+        if (rc == PAT_OK) rc = finish_heat(j, in_b ? b : a, st, part, r);
+        (void)hipStreamSynchronize(st);
+        (void)pat::dev_free(a);
+        (void)pat::dev_free(b);
+        return rc;
+    }
+
+    // A heat3D job's checksum (every cell of the final grid) and, if asked,
+    // its interior on the host.
+    int finish_heat(const pat_job &j, const float *res, hipStream_t st, double *part, pat_job_result *r)
+    {
+        const int64_t n = j.size, cells = (n + 2) * (n + 2) * (n + 2);
+        int rc = pat::sum_f32_dev(res, cells, part, &r->checksum, st);
         if (rc == PAT_OK && j.out) {
             if (j.out_len < n * n * n) {
                 pat::set_error("serve: heat3D job needs %lld output floats", (long long)(n * n * n));
@@ -203,9 +230,68 @@ struct pat_serve {
                 }
             }
         }
+        return rc;
+    }
+
+    // Several heat3D jobs as one batch: their sweeps run two per launch over
+    // every grid (the fused-window kernel, each workgroup finding its grid
+    // in a device table), captured as one hipGraph.  Same bits as each job
+    // alone: every tiling gives the serial sweep's results.
+    int run_heat_batch(const std::vector<Job> &g, hipStream_t st, double *part, std::vector<pat_job_result> &res)
+    {
+        const int ng = (int)g.size();
+        std::vector<float *> a(ng, nullptr), b(ng, nullptr);
+        std::vector<int64_t> side(ng);
+        pat::HeatBatchGrid *table = nullptr;
+        int rc = PAT_OK;
+        for (int i = 0; i < ng && rc == PAT_OK; i++) {
+            side[i] = g[i].d.size;
+            const int64_t cells = (side[i] + 2) * (side[i] + 2) * (side[i] + 2);
+            if (pat::dev_malloc(&a[i], sizeof(float) * cells) != hipSuccess ||
+                pat::dev_malloc(&b[i], sizeof(float) * cells) != hipSuccess) {
+                pat::set_error("serve: heat3D batch: no device memory for 2 x %lld cells", (long long)cells);
+                rc = PAT_ERR_NOMEM;
+                break;
+            }
+            rc = pat_gen_heat3d_dev(side[i], side[i], side[i], g[i].d.seed, a[i], st);
+            if (rc == PAT_OK && hipMemsetAsync(b[i], 0, sizeof(float) * cells, st) != hipSuccess) rc = PAT_ERR_HIP;
+        }
+        if (rc == PAT_OK && pat::dev_malloc(&table, sizeof(pat::HeatBatchGrid) * 2 * ng) != hipSuccess) {
+            pat::set_error("serve: heat3D batch: no device memory for the grid table");
+            rc = PAT_ERR_NOMEM;
+        }
+        int nblocks = 0, in_b = 0;
+        if (rc == PAT_OK)
+            rc = pat::heat3d_batch_table(ng, a.data(), b.data(), side.data(), side.data(), side.data(), table, st,
+                                         &nblocks);
+        if (rc == PAT_OK) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = PAT_ERR_HIP;
+            } else {
+                rc = pat::heat3d_batch_enqueue(ng, a.data(), b.data(), side.data(), side.data(), side.data(),
+                                               kHeatAlpha, g[0].d.steps, table, nblocks, st, &in_b);
+                const hipError_t ec = hipStreamEndCapture(st, &graph);
+                if (rc == PAT_OK && ec != hipSuccess) rc = PAT_ERR_HIP;
+            }
+            if (rc == PAT_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = PAT_ERR_HIP;
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc == PAT_OK && (hipGraphLaunch(exec, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
+                rc = PAT_ERR_HIP;
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (rc == PAT_ERR_HIP) pat::set_error("serve: heat3D batch: graph capture or replay failed");
+        }
+        for (int i = 0; i < ng; i++) {
+            res[i].batch = ng;
+            res[i].status = rc == PAT_OK ? finish_heat(g[i].d, in_b ? b[i] : a[i], st, part, &res[i]) : rc;
+        }
         (void)hipStreamSynchronize(st);
-        (void)pat::dev_free(a);
-        (void)pat::dev_free(b);
+        for (int i = 0; i < ng; i++) {
+            if (a[i]) (void)pat::dev_free(a[i]);
+            if (b[i]) (void)pat::dev_free(b[i]);
+        }
+        if (table) (void)pat::dev_free(table);
         return rc;
     }
 
@@ -314,8 +400,7 @@ struct pat_serve {
         r->kernel = mode;
         pat_kmeans *km = nullptr;
         if (rc == PAT_OK) rc = pat_kmeans_create(x, j.size, j.dim, j.k, mode, 1, &km);
-        float tot = 0.f;
-        if (rc == PAT_OK) rc = pat_kmeans_run_timed(km, j.steps, &tot, nullptr);
+        if (rc == PAT_OK) rc = pat_kmeans_run(km, j.steps);  // the iterations replay one captured hipGraph
         std::vector<float> c((size_t)j.k * j.dim);
         if (rc == PAT_OK) rc = pat_kmeans_get(km, c.data(), nullptr, nullptr);
         if (rc == PAT_OK) {
@@ -355,7 +440,7 @@ struct pat_serve {
                 reserved += j.est;
                 running++;
                 g.push_back(j);
-                if (!(flags & PAT_SERVE_NO_FUSE) && j.attempts == 0 && lj_fusable(j.d)) take_partners(g);
+                if (!(flags & PAT_SERVE_NO_FUSE) && j.attempts == 0 && fusable(j.d)) take_partners(g);
             }
             std::vector<pat_job_result> res(g.size());
             const double t_start = ms_since(t0);
@@ -369,7 +454,7 @@ struct pat_serve {
             }
             int rc = PAT_ERR_STATE;
             if (ok && g.size() > 1) {
-                rc = run_lj_batch(g, st, part, res);
+                rc = g[0].d.kind == PAT_JOB_LJ ? run_lj_batch(g, st, part, res) : run_heat_batch(g, st, part, res);
             } else if (ok) {
                 switch (g[0].d.kind) {
                 case PAT_JOB_HEAT3D: rc = run_heat(g[0].d, st, part, &res[0]); break;

@@ -27,6 +27,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "pat_common.hpp"
 #include "stencil_dev.hpp"
@@ -176,19 +177,14 @@ __global__ __launch_bounds__(BX *BY) void heat_lds25d(const float *__restrict__ 
 // below.  Level 1 at the grid's halo cells is the fixed zero.  Each cell's
 // arithmetic is the single sweep's, so two passes' worth of output is
 // bit-identical to two sweeps; the field crosses HBM once per two steps.
+// The window (bx, by) of z-chunk bz, for a 64 x 4 workgroup.
 template <int RY>
-__global__ __launch_bounds__(256) void heat_tb2(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
-                                                int nz, int zc, float alpha, int tiles_x, int tiles_y, int tiles_z)
+__device__ __forceinline__ void tb2_window(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                           int nz, int zc, float alpha, int bx, int by, int bz)
 {
     constexpr int WX = 64, WY = 4 * RY, OX = WX - 4, OY = WY - 4;
     __shared__ float p0[WY][WX];     // level-0 plane z
     __shared__ float p1[2][WY][WX];  // level-1 planes (ping-pong)
-
-    const int nblocks = tiles_x * tiles_y * tiles_z;
-    const int t = xcd_swizzle(blockIdx.x, nblocks);
-    const int bx = t % tiles_x;
-    const int by = (t / tiles_x) % tiles_y;
-    const int bz = t / (tiles_x * tiles_y);
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int sx = nx + 2;
@@ -298,6 +294,34 @@ __global__ __launch_bounds__(256) void heat_tb2(const float *__restrict__ in, fl
         if (z + 3 > z1) break;
         step(z + 3, A[3], A[0], A[1], B[1], B[0], 1);
     }
+}
+
+template <int RY>
+__global__ __launch_bounds__(256) void heat_tb2(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                                int nz, int zc, float alpha, int tiles_x, int tiles_y, int tiles_z)
+{
+    const int nblocks = tiles_x * tiles_y * tiles_z;
+    const int t = xcd_swizzle(blockIdx.x, nblocks);
+    tb2_window<RY>(in, out, nx, ny, nz, zc, alpha, t % tiles_x, (t / tiles_x) % tiles_y, t / (tiles_x * tiles_y));
+}
+
+// The same two fused sweeps over several grids in one launch: workgroup t
+// takes window t - first of the last grid whose first workgroup is <= t.
+template <int RY>
+__global__ __launch_bounds__(256) void heat_tb2_batch(const pat::HeatBatchGrid *__restrict__ grids, int ngrids,
+                                                      int nblocks, float alpha)
+{
+    const int t = xcd_swizzle(blockIdx.x, nblocks);
+    int lo = 0, hi = ngrids - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (grids[mid].first <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const pat::HeatBatchGrid g = grids[lo];
+    const int u = t - g.first;
+    tb2_window<RY>(g.in, g.out, g.nx, g.ny, g.nz, g.zc, alpha, u % g.tiles_x, (u / g.tiles_x) % g.tiles_y,
+                   u / (g.tiles_x * g.tiles_y));
 }
 
 // Several sweeps per launch on blocks of the grid held in LDS, for small
@@ -621,6 +645,66 @@ int heat3d_launch(const float *in, float *out, int64_t nx, int64_t ny, int64_t n
 // nsteps sweeps ping-ponging buf[start] -> buf[1-start] ...: fused pairs
 // while two or more steps remain, a plain sweep for an odd last one.
 // Returns the buffer index holding the result in *end.
+int heat3d_batch_table(int n, float *const *a, float *const *b, const int64_t *nx, const int64_t *ny,
+                       const int64_t *nz, HeatBatchGrid *table, hipStream_t st, int *nblocks)
+{
+    PAT_REQUIRE(n > 0 && a && b && nx && ny && nz && table && nblocks, PAT_ERR_INVALID, "heat3d batch: bad arguments");
+    constexpr int OX = 60, OY = 12;  // output window of the 64 x 16 fused window (4 rows per thread)
+    std::vector<HeatBatchGrid> h(2 * (size_t)n);
+    int64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        PAT_REQUIRE(a[i] && b[i] && a[i] != b[i] && nx[i] > 0 && ny[i] > 0 && nz[i] > 0 &&
+                        (nx[i] + 2) * (ny[i] + 2) * (nz[i] + 2) < (1LL << 40),
+                    PAT_ERR_INVALID, "heat3d batch: grid %d has bad buffers or sizes", i);
+        HeatBatchGrid g;
+        g.in = a[i];
+        g.out = b[i];
+        g.nx = (int)nx[i];
+        g.ny = (int)ny[i];
+        g.nz = (int)nz[i];
+        g.zc = (int)std::min<int64_t>(8, nz[i]);
+        g.tiles_x = (int)((nx[i] + OX - 1) / OX);
+        g.tiles_y = (int)((ny[i] + OY - 1) / OY);
+        g.first = (int)total;
+        g.pad = 0;
+        total += (int64_t)g.tiles_x * g.tiles_y * ((nz[i] + g.zc - 1) / g.zc);
+        PAT_REQUIRE(total < (1LL << 31), PAT_ERR_INVALID, "heat3d batch: too many workgroups");
+        h[i] = g;
+        g.in = b[i];
+        g.out = a[i];
+        h[n + i] = g;
+    }
+    PAT_HIP(hipMemcpyAsync(table, h.data(), sizeof(HeatBatchGrid) * h.size(), hipMemcpyHostToDevice, st));
+    PAT_HIP(hipStreamSynchronize(st));  // the host copy goes out of scope
+    *nblocks = (int)total;
+    return PAT_OK;
+}
+
+int heat3d_batch_enqueue(int n, float *const *a, float *const *b, const int64_t *nx, const int64_t *ny,
+                         const int64_t *nz, float alpha, int nsteps, const HeatBatchGrid *table, int nblocks,
+                         hipStream_t st, int *in_b)
+{
+    PAT_REQUIRE(n > 0 && nblocks > 0 && nsteps >= 0 && table && in_b, PAT_ERR_INVALID, "heat3d batch: bad arguments");
+    int c = 0;
+    for (int k = 0; k + 2 <= nsteps; k += 2) {
+        hipLaunchKernelGGL(heat_tb2_batch<4>, dim3((unsigned)nblocks), dim3(64, 4), 0, st, table + (size_t)c * n, n,
+                           nblocks, alpha);
+        PAT_LAUNCH_CHECK();
+        c = 1 - c;
+    }
+    if (nsteps % 2) {
+        for (int i = 0; i < n; i++) {
+            int bx, by, ry, zc;
+            default_tile(nx[i], ny[i], nz[i], &bx, &by, &ry, &zc);
+            PAT_TRY(heat3d_sweep(c ? b[i] : a[i], c ? a[i] : b[i], nx[i], ny[i], nz[i], alpha, PAT_STENCIL_LDS25D, bx,
+                                 by, ry, zc, st));
+        }
+        c = 1 - c;
+    }
+    *in_b = c;
+    return PAT_OK;
+}
+
 int heat3d_steps(float *const buf[2], int start, int64_t nx, int64_t ny, int64_t nz, float alpha, const HeatTile &t,
                  int variant, int nsteps, hipStream_t stream, int *end)
 {

@@ -14,6 +14,30 @@ struct HeatTile {
                          // fused 2: cube shape index, 0, sweeps per launch, 0
 };
 
+// One grid of a fused batch: an entry of the device table that the batched
+// two-sweep kernel reads (one table per direction, a -> b and b -> a).
+struct HeatBatchGrid {
+    const float *in;
+    float *out;
+    int nx, ny, nz, zc;
+    int tiles_x, tiles_y;
+    int first;  // first workgroup of this grid in the launch
+    int pad;
+};
+
+// Fused sweeps of n grids.  heat3d_batch_table writes the two tables (2n
+// entries at 'table', device memory) for grids a[i] <-> b[i] and returns
+// the workgroups of one launch; heat3d_batch_enqueue then queues nsteps
+// sweeps of every grid from a[i] -- two per launch over all of them, an odd
+// last sweep one launch per grid -- and sets *in_b when the results end in
+// b (the same for every grid).  The enqueue makes no allocation and no
+// memset, so it may be captured into a graph.
+int heat3d_batch_table(int n, float *const *a, float *const *b, const int64_t *nx, const int64_t *ny,
+                       const int64_t *nz, HeatBatchGrid *table, hipStream_t st, int *nblocks);
+int heat3d_batch_enqueue(int n, float *const *a, float *const *b, const int64_t *nx, const int64_t *ny,
+                         const int64_t *nz, float alpha, int nsteps, const HeatBatchGrid *table, int nblocks,
+                         hipStream_t st, int *in_b);
+
 // Sweeps that one launch of tile t advances.
 inline int heat3d_steps_per_launch(const HeatTile &t) { return t.fused == 0 ? 1 : (t.fused == 1 ? 2 : t.ry); }
 

@@ -154,6 +154,46 @@ def test_serve_fuses_waiting_lj_jobs(pa, gpu):
         md.close()
 
 
+def test_serve_fuses_waiting_heat_jobs(pa, gpu):
+    """Small heat3D jobs waiting with the same step count run as one batch
+    (two sweeps per launch over every grid, an odd last sweep per grid).
+    Each result equals the job with fusing off and the standalone handle,
+    bit for bit."""
+    specs = [(20, 41), (37, 42), (64, 43), (48, 44), (100, 45), (128, 46), (33, 47)]  # (side, seed)
+    steps = 9
+
+    def serve(fuse):
+        s = pa.Serve(2 << 30, slots=1, fuse=fuse)
+        # a grid too large to fuse holds the one slot while the others queue
+        first = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 300, 20, 6))
+        bufs, ids = [], []
+        for n, seed in specs:
+            b = np.zeros(n ** 3, dtype=np.float32)
+            bufs.append(b)
+            ids.append(s.submit(pa.job(pa.PAT_JOB_HEAT3D, n, steps, seed, out=b), keep=b))
+        res = [s.wait(i) for i in ids]
+        r0 = s.wait(first)
+        s.close()
+        assert r0["status"] == 0 and r0["batch"] == 1
+        return bufs, res
+
+    fb, fr = serve(True)
+    ab, ar = serve(False)
+    print("batch sizes fused:", [r["batch"] for r in fr], "unfused:", [r["batch"] for r in ar])
+    assert all(r["status"] == 0 for r in fr + ar)
+    assert all(r["batch"] == 1 for r in ar)
+    assert all(r["batch"] == len(specs) for r in fr)
+    for (n, seed), f, a, x, y in zip(specs, fb, ab, fr, ar):
+        assert np.array_equal(f, a), n
+        assert x["checksum"] == y["checksum"], n
+    for (n, seed), f in list(zip(specs, fb))[:3]:
+        st = pa.Stencil(n, n, n, alpha=0.1)
+        st.upload(pa.gen_heat3d(n, n, n, seed=seed))
+        st.run(steps)
+        assert np.array_equal(f.reshape(n, n, n), st.download()[1:-1, 1:-1, 1:-1]), n
+        st.close()
+
+
 def test_serve_admission_respects_budget(pa, gpu):
     # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
     s = pa.Serve(100 * MB, slots=4)
