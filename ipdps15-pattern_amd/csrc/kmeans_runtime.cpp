@@ -318,6 +318,27 @@ int pat_kmeans_run(pat_kmeans *h, int iters)
     return PAT_OK;
 }
 
+}  // extern "C"
+
+namespace pat {
+
+// pat_kmeans_run's iterations launched one by one on the handle's stream
+// (no graph): for a handle that runs once, where instantiating the graph
+// would cost more than it saves.
+int kmeans_run_eager(pat_kmeans *h, int iters)
+{
+    PAT_REQUIRE(h, PAT_ERR_INVALID, "kmeans_run_eager: null handle");
+    PAT_REQUIRE(iters >= 0, PAT_ERR_INVALID, "kmeans_run_eager: negative iteration count");
+    for (int i = 0; i < iters; i++) PAT_TRY(h->enqueue_iter(h->stream));
+    PAT_HIP(hipStreamSynchronize(h->stream));
+    h->iters_done += iters;
+    return PAT_OK;
+}
+
+}  // namespace pat
+
+extern "C" {
+
 int pat_kmeans_run_timed(pat_kmeans *h, int iters, float *total_ms, float *assign_ms)
 {
     PAT_REQUIRE(h && total_ms, PAT_ERR_INVALID, "pat_kmeans_run_timed: null argument");

@@ -26,8 +26,7 @@
 // their own order.  Heat3D grids are swept two steps per launch over every
 // grid, the launches captured as one hipGraph; any tiling gives the serial
 // sweep's values.  So every job gets the bits it gets alone.  kmeans jobs
-// run alone (their D and K select different kernels), their iterations
-// replayed from the handle's captured hipGraph.
+// run alone (their D and K select different kernels).
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -51,6 +50,7 @@ int mdb_create_dev(int nsys, const int64_t *offsets, const float *pos4, const do
                    double skin, int every, double dt, pat_mdb **out);
 int mdb_positions(pat_mdb *h, int sys, const float **pos4, int64_t *n);
 int mdb_lanes(pat_mdb *h);
+int kmeans_run_eager(pat_kmeans *h, int iters);
 }  // namespace pat
 
 namespace {
@@ -400,7 +400,9 @@ struct pat_serve {
         r->kernel = mode;
         pat_kmeans *km = nullptr;
         if (rc == PAT_OK) rc = pat_kmeans_create(x, j.size, j.dim, j.k, mode, 1, &km);
-        if (rc == PAT_OK) rc = pat_kmeans_run(km, j.steps);  // the iterations replay one captured hipGraph
+        // launched one by one: a graph of the iterations, used once, costs more
+        // to instantiate than it saves (kmeans p50 at 60 jobs/s: 2.4 vs 2.0 ms)
+        if (rc == PAT_OK) rc = pat::kmeans_run_eager(km, j.steps);
         std::vector<float> c((size_t)j.k * j.dim);
         if (rc == PAT_OK) rc = pat_kmeans_get(km, c.data(), nullptr, nullptr);
         if (rc == PAT_OK) {
