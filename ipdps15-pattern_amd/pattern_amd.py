@@ -38,6 +38,9 @@ PAT_JOB_LJ = 1
 PAT_JOB_KMEANS = 2
 PAT_SERVE_PLAIN = 1
 PAT_SERVE_NO_FUSE = 2
+# status codes (include/pattern.h)
+PAT_OK, PAT_ERR_INVALID, PAT_ERR_HIP, PAT_ERR_NOMEM, PAT_ERR_BUDGET, PAT_ERR_UNSUPPORTED, PAT_ERR_STATE = \
+    0, -1, -2, -3, -4, -5, -6
 
 _lib = None
 

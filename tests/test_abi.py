@@ -107,3 +107,15 @@ def test_serving_structs_match_the_header(pa, tmp_path):
         assert int(got[t]) == ctypes.sizeof(cls), t
         for f in fields[t]:
             assert int(got[f"{t}.{f}"]) == getattr(cls, f).offset, (t, f)
+
+
+def test_python_constants_match_the_header(pa):
+    """Every integer #define of include/pattern.h that the Python binding
+    mirrors has the header's value; the status codes are all mirrored."""
+    with open(os.path.join(ROOT, "include", "pattern.h")) as f:
+        defs = dict(re.findall(r"#define (PAT_\w+) (-?\d+)\b", f.read()))
+    assert all(hasattr(pa, n) for n in defs if n == "PAT_OK" or n.startswith("PAT_ERR_"))
+    mirrored = [n for n in defs if hasattr(pa, n)]
+    assert len(mirrored) >= 20
+    for n in mirrored:
+        assert getattr(pa, n) == int(defs[n]), n

@@ -194,6 +194,33 @@ def test_serve_fuses_waiting_heat_jobs(pa, gpu):
         st.close()
 
 
+def test_serve_fused_member_error_stays_its_own(pa, gpu):
+    """A job whose output buffer is too small fails with PAT_ERR_INVALID
+    inside a fused batch; the other jobs of the batch finish with the same
+    bits as alone."""
+    for kind, sizes, steps, per in ((pa.PAT_JOB_LJ, [6, 7, 8], 5, lambda n: 4 * n ** 3 * 3),
+                                    (pa.PAT_JOB_HEAT3D, [24, 40, 31], 4, lambda n: n ** 3)):
+        def serve(fuse):
+            s = pa.Serve(1 << 30, slots=1, fuse=fuse)
+            first = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 250, 20, 8))
+            bufs, ids = [], []
+            for i, n in enumerate(sizes):
+                b = np.zeros(per(n) - (1 if i == 1 else 0), dtype=np.float32)  # the second is one float short
+                bufs.append(b)
+                ids.append(s.submit(pa.job(kind, n, steps, 90 + i, out=b), keep=b))
+            res = [s.wait(i) for i in ids]
+            assert s.wait(first)["status"] == 0
+            s.close()
+            return bufs, res
+
+        fb, fr = serve(True)
+        ab, ar = serve(False)
+        assert [r["batch"] for r in fr] == [3, 3, 3], kind
+        assert [r["status"] for r in fr] == [r["status"] for r in ar] == [0, pa.PAT_ERR_INVALID, 0], kind
+        for i in (0, 2):
+            assert np.array_equal(fb[i], ab[i]) and fr[i]["checksum"] == ar[i]["checksum"], (kind, i)
+
+
 def test_serve_admission_respects_budget(pa, gpu):
     # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
     s = pa.Serve(100 * MB, slots=4)
