@@ -221,6 +221,37 @@ def test_serve_fused_member_error_stays_its_own(pa, gpu):
             assert np.array_equal(fb[i], ab[i]) and fr[i]["checksum"] == ar[i]["checksum"], (kind, i)
 
 
+def test_serve_fusion_respects_budget(pa, gpu):
+    """A batch takes waiting jobs only while their estimates fit the budget:
+    with room for five 8^3-cell LJ jobs (about 7 MB each) in 36 MB, eight
+    waiting jobs run as a batch of five and then one of three, the bytes
+    held never above the budget, every result as alone."""
+    budget = 36 << 20
+
+    def serve(fuse):
+        s = pa.Serve(budget, slots=1, fuse=fuse)
+        first = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 100, 2000, 9))  # holds the slot while the others queue
+        bufs, ids = [], []
+        for i in range(8):
+            b = np.zeros(4 * 8 ** 3 * 3, dtype=np.float32)
+            bufs.append(b)
+            ids.append(s.submit(pa.job(pa.PAT_JOB_LJ, 8, 6, 300 + i, out=b), keep=b))
+        res = [s.wait(i) for i in ids]
+        assert s.wait(first)["status"] == 0
+        s.close()
+        return bufs, res
+
+    fb, fr = serve(True)
+    ab, ar = serve(False)
+    assert all(r["status"] == 0 for r in fr + ar)
+    assert [r["batch"] for r in fr] == [5] * 5 + [3] * 3
+    # each batch's jobs are admitted together and released together
+    assert sum(r["device_bytes"] for r in fr[:5]) <= budget
+    assert fr[4]["end_ms"] <= fr[5]["start_ms"]
+    for f, a, x, y in zip(fb, ab, fr, ar):
+        assert np.array_equal(f, a) and x["checksum"] == y["checksum"]
+
+
 def test_serve_admission_respects_budget(pa, gpu):
     # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
     s = pa.Serve(100 * MB, slots=4)
