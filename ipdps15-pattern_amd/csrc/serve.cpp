@@ -15,7 +15,8 @@
 // Admission is first-fit over the queue in arrival order; after 64
 // consecutive admissions that passed over the oldest job, only that job may
 // start, so a large job is never starved.  A job whose allocation fails
-// while others hold memory is queued again and then runs alone.
+// while others hold memory, or a fused batch whose allocation fails, is
+// queued again and then runs alone (each job of the batch on its own).
 //
 // Uneven instances are fused: a slot that admits an LJ job or a small
 // heat3D job also admits the jobs of that kind waiting with the same step
@@ -433,7 +434,7 @@ struct pat_serve {
         const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
                         pat::dev_malloc(&part, sizeof(double) * pat::sum_blocks()) == hipSuccess;
         for (;;) {
-            std::vector<Job> g;  // the job, and the LJ jobs fused with it
+            std::vector<Job> g;  // the job, and the jobs of its kind fused with it
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return pick_ready() || (stopping && pending.empty()); });
@@ -472,8 +473,9 @@ struct pat_serve {
                 std::lock_guard<std::mutex> lk(mu);
                 for (const Job &q : g) reserved -= q.est;
                 running--;
-                if (rc == PAT_ERR_NOMEM && g[0].attempts == 0 && running > 0) {
-                    // run again once each can have the arena alone
+                if (rc == PAT_ERR_NOMEM && g[0].attempts == 0 && (running > 0 || g.size() > 1)) {
+                    // run again once each can have the arena alone (a batch's
+                    // jobs one at a time, whatever else was running)
                     for (size_t s = g.size(); s-- > 0;) {
                         g[s].attempts = 1;
                         pending.push_front(g[s]);
