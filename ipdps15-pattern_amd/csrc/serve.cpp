@@ -265,7 +265,7 @@ struct pat_serve {
         if (rc == PAT_OK)
             rc = pat::heat3d_batch_table(ng, a.data(), b.data(), side.data(), side.data(), side.data(), table, st,
                                          &nblocks);
-        if (rc == PAT_OK) {
+        if (rc == PAT_OK && g[0].d.steps > 0) {  // zero steps: the generated grids are the results
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {

@@ -252,6 +252,30 @@ def test_serve_fusion_respects_budget(pa, gpu):
         assert np.array_equal(f, a) and x["checksum"] == y["checksum"]
 
 
+def test_serve_fused_zero_step_jobs(pa, gpu):
+    """Jobs of zero steps fuse too and return their generated inputs: heat3D
+    grids and LJ positions equal the host generators' values."""
+    s = pa.Serve(1 << 30, slots=1)
+    first = s.submit(pa.job(pa.PAT_JOB_HEAT3D, 250, 20, 8))
+    jobs = [(pa.PAT_JOB_HEAT3D, 30, 61), (pa.PAT_JOB_LJ, 6, 62), (pa.PAT_JOB_HEAT3D, 45, 63), (pa.PAT_JOB_LJ, 7, 64)]
+    bufs, ids = [], []
+    for kind, n, seed in jobs:
+        b = np.zeros(n ** 3 if kind == pa.PAT_JOB_HEAT3D else 4 * n ** 3 * 3, dtype=np.float32)
+        bufs.append(b)
+        ids.append(s.submit(pa.job(kind, n, 0, seed, out=b), keep=b))
+    res = [s.wait(i) for i in ids]
+    assert s.wait(first)["status"] == 0
+    s.close()
+    assert [r["status"] for r in res] == [0] * 4 and [r["batch"] for r in res] == [2] * 4
+    for (kind, n, seed), b in zip(jobs, bufs):
+        if kind == pa.PAT_JOB_HEAT3D:
+            ref = pa.gen_heat3d(n, n, n, seed=seed)[1:-1, 1:-1, 1:-1]
+            assert np.array_equal(b.reshape(n, n, n), ref), n
+        else:
+            pos, _ = pa.gen_fcc(n, n, n, rho=0.8442, jitter=0.05, seed=seed)
+            assert np.array_equal(b.reshape(-1, 3), pos), n
+
+
 def test_serve_admission_respects_budget(pa, gpu):
     # heat3D 200^3 holds about 2 x 32.5 MB: a 100 MB budget fits one at a time
     s = pa.Serve(100 * MB, slots=4)
