@@ -191,6 +191,14 @@ namespace pat {
 
 int sum_blocks() { return kSumBlocks; }
 
+// Loads this file's code object now rather than at a first launch.
+int gen_preload()
+{
+    hipFuncAttributes a;
+    PAT_HIP(hipFuncGetAttributes(&a, (const void *)sum_f32_k));
+    return PAT_OK;
+}
+
 // Sum of n floats at p (device) into *out, using part (kSumBlocks doubles of
 // device memory) on stream st; waits for the stream.
 int sum_f32_dev(const float *p, int64_t n, double *part, double *out, hipStream_t st)

@@ -52,6 +52,8 @@ int mdb_create_dev(int nsys, const int64_t *offsets, const float *pos4, const do
 int mdb_positions(pat_mdb *h, int sys, const float **pos4, int64_t *n);
 int mdb_lanes(pat_mdb *h);
 int kmeans_run_eager(pat_kmeans *h, int iters);
+int irregular_preload();
+int gen_preload();
 }  // namespace pat
 
 namespace {
@@ -518,6 +520,11 @@ int pat_serve_start(int64_t hbm_budget, int slots, int flags, pat_serve **out)
     s->device = pat::current_device();
     PAT_HIP(hipSetDevice(s->device));
     PAT_TRY(pat::kmeans_prepare_kernels());  // attribute calls stay off the workers' path
+    // every engine's code object loaded now: at a first launch the loading
+    // (~55 ms for the LJ kernels) would land on that job's latency
+    PAT_TRY(pat::stencil_preload());
+    PAT_TRY(pat::irregular_preload());
+    PAT_TRY(pat::gen_preload());
     s->arena = pat::arena_create((size_t)hbm_budget);
     if (!s->arena) {
         delete s;

@@ -38,6 +38,9 @@ int heat3d_batch_enqueue(int n, float *const *a, float *const *b, const int64_t 
                          const int64_t *nz, float alpha, int nsteps, const HeatBatchGrid *table, int nblocks,
                          hipStream_t st, int *in_b);
 
+// Loads the stencil kernels' code object now rather than at a first launch.
+int stencil_preload();
+
 // Sweeps that one launch of tile t advances.
 inline int heat3d_steps_per_launch(const HeatTile &t) { return t.fused == 0 ? 1 : (t.fused == 1 ? 2 : t.ry); }
 
